@@ -1,0 +1,115 @@
+/*
+ * omx -- MI355X (gfx950) plane-wave Maxwell eigensolver, C ABI.
+ *
+ * Drop-in backend for OptiMode.jl's MaxwellEigenmodes `solve_ω²` / `solve_k` and the
+ * `solve_k` adjoint rrule.  Every entry point takes plain host pointers in the
+ * reference's column-major layouts and returns an `int` status (OMX_OK = 0); the text of
+ * the last error on a handle is available from omx_last_error().
+ *
+ * Layouts (0-based flat indices, complex values interleaved (re, im) as Julia's
+ * ComplexF64):
+ *   epsinv  (3,3,Nx,Ny) Float64      : a + 3*b + 9*(ix + Nx*iy)
+ *   evec    (2,Nx,Ny)   ComplexF64   : c + 2*(ix + Nx*iy), i.e. 2N complex = 4N doubles
+ *   blocks of m vectors              : vector j starts at j * 4N doubles
+ *
+ * Threading: calls on different handles may run concurrently (one HIP stream per
+ * handle); calls on one handle are serialised by a per-handle mutex.
+ */
+#ifndef OMX_H
+#define OMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMX_ABI_VERSION 1
+
+/* status codes */
+#define OMX_OK 0
+#define OMX_ERR_NO_DEVICE 1      /* no usable gfx950 device */
+#define OMX_ERR_BAD_SHAPE 2      /* grid / block sizes not supported or inconsistent */
+#define OMX_ERR_NOT_CONVERGED 3  /* eigensolve or Newton iteration hit its iteration cap */
+#define OMX_ERR_CHOLESKY 4       /* orthogonalisation failed even after shifting */
+#define OMX_ERR_INVALID_ARG 5    /* null handle/pointer, non-finite or out-of-range value */
+#define OMX_ERR_HIP 6            /* HIP runtime failure */
+#define OMX_ERR_INTERNAL 7
+#define OMX_ERR_UNSUPPORTED 8    /* valid input outside what this backend implements */
+
+/* arithmetic of an operator application */
+#define OMX_PREC_FP32 0
+#define OMX_PREC_FP64 1
+
+typedef struct omx_solver omx_solver; /* one grid + one eps^-1 field + one HIP stream */
+
+/* per-solve statistics (LOBPCG counters follow DFTK's n_matvec / resid_history,
+   lib/MaxwellEigenmodes/src/solvers/dftk.jl:288-295) */
+typedef struct omx_info {
+  int32_t converged;         /* 1 when every requested mode met k_tol / tol */
+  int32_t n_newton;          /* Newton steps (eigensolves), summed over bands */
+  int32_t lobpcg_iters_fp32; /* LOBPCG iterations in FP32, summed over bands and steps */
+  int32_t lobpcg_iters_fp64; /* LOBPCG iterations in FP64 (refinement) */
+  int64_t n_matvec;          /* operator applications (vectors) */
+  int64_t n_precon;          /* preconditioner applications (vectors) */
+  double max_residual;       /* final FP64 residual norm ||M x - w2 x||, max over modes */
+  double device_ms;          /* GPU time of the solve (HIP events) */
+} omx_info;
+
+/* library */
+const char* omx_version(void);
+int omx_device_count(int* count);
+const char* omx_status_string(int status);
+
+/* Handle: grid (Dx x Dy um, Nx x Ny pixels; Grid, lib/DielectricSmoothing/src/grid.jl:46-71)
+   and eps^-1 (symmetric 3x3 per pixel), uploaded once.  Replaces the HelmholtzMap /
+   ModeSolver construction (lib/MaxwellEigenmodes/src/maxwell.jl:922-972). */
+int omx_create(omx_solver** out, int device, int nx, int ny, double Dx, double Dy, const double* epsinv);
+int omx_destroy(omx_solver* h);
+/* update_ε⁻¹ (maxwell.jl:1114-1136) */
+int omx_set_epsinv(omx_solver* h, const double* epsinv);
+const char* omx_last_error(const omx_solver* h);
+/* tuning knobs: "fp32_tol", "fp32_switch", "maxiter_fp64", "guard", "seed" */
+int omx_set_option(omx_solver* h, const char* key, double value);
+
+/* HelmholtzMap * H for m vectors at k_vec = (0,0,k): _M! (maxwell.jl:471-481) */
+int omx_apply_helmholtz(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision);
+/* HelmholtzPreconditioner * H: _P! (maxwell.jl:459-469) */
+int omx_apply_preconditioner(omx_solver* h, double k, const double* H_in, double* H_out, int m,
+                             int precision);
+/* HMₖH (maxwell.jl:267-275) per vector: out[j] = -Re<H_j, kx_ct(ifft(eps^-1 fft(zx_tc H_j)))> */
+int omx_hmkh(omx_solver* h, double k, const double* H, int m, double* out, int precision);
+
+/* solve_ω²(ms, solver; nev, tol, maxiter) (src/solve.jl:38-63, solvers/dftk.jl:32-37):
+   lowest nev eigenpairs of M(k).  H_io: in = start block (2N x nev complex) or NULL for a
+   seeded random start; out = eigenvectors (written when non-NULL). */
+int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, double* H_io,
+                     double* omega2_out, omx_info* info);
+
+/* solve_k(ω, ε⁻¹, grid, solver; nev, k_tol, eig_tol, max_eigsolves, maxiter, kguess, Hguess)
+   (src/solve.jl:182-196, :104-115; solvers/gpu.jl:213-242): per band, Newton on
+   ω²(k) - ω² = 0 with slope 2 HMₖH; eigenvectors unit-norm and phase-canonicalised
+   (fields.jl:258-260).  kguess may be NULL (k_guess, MaxwellEigenmodes.jl:63); Hguess may
+   be NULL or hold nguess start vectors.  evecs_out may be NULL: the eigenvectors then stay
+   on the device (fetch them with omx_get_evecs). */
+int omx_solve_k(omx_solver* h, double omega, int nev, double k_tol, double eig_tol, int max_eigsolves,
+                int maxiter, const double* kguess, const double* Hguess, int nguess, double* kmags_out,
+                double* evecs_out, omx_info* info);
+/* eigenvectors of the last omx_solve_k on this handle (2N x nev complex) */
+int omx_get_evecs(omx_solver* h, double* evecs_out);
+
+/* Pull-back of rrule(solve_k) (src/solve.jl:213-289, solvers/gpu.jl:292-365): given the
+   primal (kmags, evecs) and cotangents kbar (nev, NULL = zero) and evbar (2N x nev complex,
+   NULL = zero; evbar_mask[j] = 0 marks band j's cotangent as zero), returns
+   omegabar and epsinvbar (3,3,Nx,Ny) with the 2D mirrored off-diagonal convention
+   (grads/solve.jl:490-515). */
+int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+                        const double* kbar, const double* evbar, const uint8_t* evbar_mask,
+                        double* omegabar_out, double* epsinvbar_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OMX_H */

@@ -1,0 +1,140 @@
+// extern "C" entry points of libomx (declared in include/omx.h).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "solver.hpp"
+
+struct omx_solver {
+  std::unique_ptr<omx::Solver> s;
+  std::string create_error;
+};
+
+namespace {
+
+thread_local std::string g_error;  // errors raised before a handle exists
+
+template <typename F> int guarded(omx_solver* h, F&& f) {
+  if (!h || !h->s) return OMX_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  try {
+    h->s->last_error.clear();
+    f(*h->s);
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    h->s->last_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    h->s->last_error = e.what();
+    return OMX_ERR_INTERNAL;
+  } catch (...) {
+    h->s->last_error = "unknown error";
+    return OMX_ERR_INTERNAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* omx_version(void) { return "omx 0.1 (gfx950)"; }
+
+int omx_device_count(int* count) {
+  if (!count) return OMX_ERR_INVALID_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return OMX_OK;
+}
+
+const char* omx_status_string(int status) {
+  switch (status) {
+    case OMX_OK: return "ok";
+    case OMX_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case OMX_ERR_BAD_SHAPE: return "unsupported or inconsistent shape";
+    case OMX_ERR_NOT_CONVERGED: return "iteration did not converge";
+    case OMX_ERR_CHOLESKY: return "orthogonalisation failed after shifting";
+    case OMX_ERR_INVALID_ARG: return "invalid argument";
+    case OMX_ERR_HIP: return "HIP runtime error";
+    case OMX_ERR_INTERNAL: return "internal error";
+    case OMX_ERR_UNSUPPORTED: return "unsupported input";
+    default: return "unknown status";
+  }
+}
+
+int omx_create(omx_solver** out, int device, int nx, int ny, double Dx, double Dy, const double* epsinv) {
+  if (!out) return OMX_ERR_INVALID_ARG;
+  *out = nullptr;
+  try {
+    auto h = std::make_unique<omx_solver>();
+    h->s = std::make_unique<omx::Solver>(device, nx, ny, Dx, Dy, epsinv);
+    *out = h.release();
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  }
+}
+
+int omx_destroy(omx_solver* h) {
+  if (!h) return OMX_ERR_INVALID_ARG;
+  delete h;
+  return OMX_OK;
+}
+
+const char* omx_last_error(const omx_solver* h) {
+  if (!h || !h->s) return g_error.c_str();
+  return h->s->last_error.c_str();
+}
+
+int omx_set_epsinv(omx_solver* h, const double* epsinv) {
+  return guarded(h, [&](omx::Solver& s) { s.set_epsinv(epsinv); });
+}
+
+int omx_set_option(omx_solver* h, const char* key, double value) {
+  if (!key) return OMX_ERR_INVALID_ARG;
+  return guarded(h, [&](omx::Solver& s) { s.set_option(key, value); });
+}
+
+int omx_apply_helmholtz(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision) {
+  return guarded(h, [&](omx::Solver& s) { s.apply(omx::OP_M, precision, k, H_in, H_out, m, nullptr); });
+}
+
+int omx_apply_preconditioner(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision) {
+  return guarded(h, [&](omx::Solver& s) { s.apply(omx::OP_P, precision, k, H_in, H_out, m, nullptr); });
+}
+
+int omx_hmkh(omx_solver* h, double k, const double* H, int m, double* out, int precision) {
+  return guarded(h, [&](omx::Solver& s) { s.apply(omx::OP_MKH, precision, k, H, nullptr, m, out); });
+}
+
+int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, double* H_io, double* omega2_out,
+                     omx_info* info) {
+  return guarded(h, [&](omx::Solver& s) { s.solve_omega2(k, nev, tol, maxiter, H_io, omega2_out, info); });
+}
+
+int omx_solve_k(omx_solver* h, double omega, int nev, double k_tol, double eig_tol, int max_eigsolves, int maxiter,
+                const double* kguess, const double* Hguess, int nguess, double* kmags_out, double* evecs_out,
+                omx_info* info) {
+  return guarded(h, [&](omx::Solver& s) {
+    s.solve_k(omega, nev, k_tol, eig_tol, max_eigsolves, maxiter, kguess, Hguess, nguess, kmags_out, evecs_out, info);
+  });
+}
+
+int omx_get_evecs(omx_solver* h, double* evecs_out) {
+  return guarded(h, [&](omx::Solver& s) { s.get_evecs(evecs_out); });
+}
+
+int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+                        const double* kbar, const double* evbar, const uint8_t* evbar_mask, double* omegabar_out,
+                        double* epsinvbar_out) {
+  return guarded(h, [&](omx::Solver& s) {
+    s.adjoint(omega, nev, kmags, evecs, kbar, evbar, evbar_mask, omegabar_out, epsinvbar_out);
+  });
+}
+
+}  // extern "C"

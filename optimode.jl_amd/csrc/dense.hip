@@ -1,0 +1,710 @@
+// Small dense FP64 kernels (see dense.hpp).  One workgroup per instance; matrices live
+// in LDS, row-major, element (r, c) at M[r*ld + c].
+#include "dense.hpp"
+#include "omx_errors.hpp"
+
+namespace omx {
+
+namespace {
+
+constexpr int DBLK = 256;
+constexpr int DMAX = 48;       // max Y columns (3 x 16)
+constexpr int DLD = DMAX + 1;  // padded LDS row stride of Y-sized matrices
+constexpr int SLD = 17;        // LDS row stride of matrices with <= 16 columns
+
+__device__ __forceinline__ c64 cadd(c64 a, c64 b) { return c64{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ c64 csub(c64 a, c64 b) { return c64{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ c64 cmul(c64 a, c64 b) { return c64{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ c64 ccj(c64 a) { return c64{a.x, -a.y}; }
+__device__ __forceinline__ c64 cscale(double s, c64 a) { return c64{s * a.x, s * a.y}; }
+
+// deterministic pseudo-random normal (used by drop!, dftk.jl:224-234)
+__device__ __forceinline__ double hash_normal(uint64_t key) {
+  uint64_t z = key + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  uint64_t y = z + 0x9e3779b97f4a7c15ull;
+  y = (y ^ (y >> 30)) * 0xbf58476d1ce4e5b9ull;
+  y = (y ^ (y >> 27)) * 0x94d049bb133111ebull;
+  y ^= y >> 31;
+  const double u1 = (double(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = double(y >> 11) * (1.0 / 9007199254740992.0);
+  return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
+
+// ----------------------------------------------------------------------------------------
+// block-wide helpers
+// ----------------------------------------------------------------------------------------
+
+// out (ca x cb) = A^H B, A rows x ca, B rows x cb
+__device__ void d_ahb(const c64* A, int lda, const c64* B, int ldb, int rows, int ca, int cb, c64* out, int ldo) {
+  for (int e = threadIdx.x; e < ca * cb; e += blockDim.x) {
+    const int a = e / cb, b = e - (e / cb) * cb;
+    double sx = 0, sy = 0;
+    for (int r = 0; r < rows; ++r) {
+      const c64 x = A[r * lda + a], y = B[r * ldb + b];
+      sx += x.x * y.x + x.y * y.y;
+      sy += x.x * y.y - x.y * y.x;
+    }
+    out[a * ldo + b] = c64{sx, sy};
+  }
+  __syncthreads();
+}
+
+// X (rows x cx) -= A (rows x k) * B (k x cx)
+__device__ void d_sub_ab(c64* X, int ldx, const c64* A, int lda, const c64* B, int ldb, int rows, int k, int cx) {
+  for (int e = threadIdx.x; e < rows * cx; e += blockDim.x) {
+    const int r = e / cx, c = e - (e / cx) * cx;
+    c64 s{0, 0};
+    for (int l = 0; l < k; ++l) s = cadd(s, cmul(A[r * lda + l], B[l * ldb + c]));
+    X[r * ldx + c] = csub(X[r * ldx + c], s);
+  }
+  __syncthreads();
+}
+
+// X (rows x c) <- X * T (c x c), using scratch W (rows x c)
+__device__ void d_rmul(c64* X, int ldx, const c64* Tm, int ldt, int rows, int c, c64* W, int ldw) {
+  for (int e = threadIdx.x; e < rows * c; e += blockDim.x) {
+    const int r = e / c, q = e - (e / c) * c;
+    c64 s{0, 0};
+    for (int l = 0; l < c; ++l) s = cadd(s, cmul(X[r * ldx + l], Tm[l * ldt + q]));
+    W[r * ldw + q] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < rows * c; e += blockDim.x) {
+    const int r = e / c, q = e - (e / c) * c;
+    X[r * ldx + q] = W[r * ldw + q];
+  }
+  __syncthreads();
+}
+
+__shared__ double s_red[DBLK];
+
+__device__ double d_sum(double v) {
+  s_red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = DBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = s_red[0];
+  __syncthreads();
+  return r;
+}
+
+// Upper Cholesky O = U^H U in place (O becomes U, lower part zeroed).  Returns true on
+// success (all pivots positive and finite).
+__device__ bool d_chol(c64* O, int ld, int n) {
+  __shared__ int ok;
+  if (threadIdx.x == 0) ok = 1;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) {
+      const double d = O[k * ld + k].x;
+      if (!(d > 0.0) || !isfinite(d)) {
+        ok = 0;
+      } else {
+        O[k * ld + k] = c64{sqrt(d), 0.0};
+      }
+    }
+    __syncthreads();
+    if (!ok) break;
+    const double inv = 1.0 / O[k * ld + k].x;
+    for (int j = k + 1 + threadIdx.x; j < n; j += blockDim.x) O[k * ld + j] = cscale(inv, O[k * ld + j]);
+    __syncthreads();
+    // trailing: O[i][j] -= conj(U[k][i]) U[k][j], i, j > k, upper part only
+    const int m = n - k - 1;
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+      const int i = k + 1 + e / m, j = k + 1 + (e - (e / m) * m);
+      if (j >= i) O[i * ld + j] = csub(O[i * ld + j], cmul(ccj(O[k * ld + i]), O[k * ld + j]));
+    }
+    __syncthreads();
+  }
+  const int res = ok;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - (e / n) * n;
+    if (j < i) O[i * ld + j] = c64{0, 0};
+  }
+  __syncthreads();
+  return res != 0;
+}
+
+// Inverse of upper-triangular U (n x n) into W.
+__device__ void d_triinv(const c64* U, int ldu, c64* W, int ldw, int n) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    for (int i = 0; i < n; ++i) W[i * ldw + j] = c64{0, 0};
+    // solve U x = e_j, x[i] = 0 for i > j
+    for (int i = j; i >= 0; --i) {
+      c64 s = (i == j) ? c64{1, 0} : c64{0, 0};
+      for (int l = i + 1; l <= j; ++l) s = csub(s, cmul(U[i * ldu + l], W[l * ldw + j]));
+      // divide by U[i][i] (real positive)
+      W[i * ldw + j] = cscale(1.0 / U[i * ldu + i].x, s);
+    }
+  }
+  __syncthreads();
+}
+
+// normest (dftk.jl:146): max |diag| + Frobenius norm of the off-diagonal part
+__device__ double d_normest(const c64* M, int ld, int n) {
+  double off = 0, dmax = 0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - (e / n) * n;
+    const c64 v = M[i * ld + j];
+    if (i == j)
+      dmax = fmax(dmax, sqrt(v.x * v.x + v.y * v.y));
+    else
+      off += v.x * v.x + v.y * v.y;
+  }
+  // max via sum of per-thread maxima is not a max: reduce max separately
+  s_red[threadIdx.x] = dmax;
+  __syncthreads();
+  for (int s = DBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) s_red[threadIdx.x] = fmax(s_red[threadIdx.x], s_red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const double mx = s_red[0];
+  __syncthreads();
+  return mx + sqrt(d_sum(off));
+}
+
+// ortho!(X; tol) (dftk.jl:151-221) on X (rows x c) in LDS.  Returns growth factor, or a
+// negative value on failure.  O, W: c x c scratch.
+__device__ double d_ortho(c64* X, int ldx, int rows, int c, double eps, double tol, c64* O, c64* W, c64* Wr) {
+  double growth = 1.0;
+  int nchol = 0;
+  for (;;) {
+    d_ahb(X, ldx, X, ldx, rows, c, c, O, SLD);
+    double tr = 0;
+    for (int i = 0; i < c; ++i) tr += O[i * SLD + i].x;
+    // keep a copy for shifting
+    for (int e = threadIdx.x; e < c * c; e += blockDim.x) W[(e / c) * SLD + e % c] = O[(e / c) * SLD + e % c];
+    __syncthreads();
+    bool success = d_chol(O, SLD, c);
+    if (success) {
+      ++nchol;
+    } else {
+      double alpha = 100.0;
+      int nbad = 0;
+      for (;;) {
+        for (int e = threadIdx.x; e < c * c; e += blockDim.x) {
+          const int i = e / c, j = e - (e / c) * c;
+          O[i * SLD + j] = W[i * SLD + j];
+          if (i == j) O[i * SLD + j].x += alpha * eps * tr;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < c; e += blockDim.x) W[e * SLD + e].x += alpha * eps * tr;
+        __syncthreads();
+        alpha *= 10;
+        if (d_chol(O, SLD, c)) {
+          ++nchol;
+          break;
+        }
+        if (++nbad > 10) return -1.0;
+      }
+    }
+    d_triinv(O, SLD, W, SLD, c);
+    d_rmul(X, ldx, W, SLD, rows, c, Wr, SLD);
+    const double ninv = d_normest(W, SLD, c);
+    growth *= ninv;
+    const double condR = d_normest(O, SLD, c) * ninv;
+    if (success && eps * condR * condR < tol) break;
+    if (nchol > 10) return -1.0;
+  }
+  return growth;
+}
+
+// ortho!(X, Y, BY=Y; tol) (dftk.jl:238-285) with X rows x cx, Y rows x cy.
+__device__ int d_ortho_against(c64* X, int rows, int cx, const c64* Y, int cy, double eps, double tol, c64* B,
+                               c64* O, c64* W, c64* Wr, uint64_t seed) {
+  // normalise columns
+  for (int c = threadIdx.x; c < cx; c += blockDim.x) {
+    double s = 0;
+    for (int r = 0; r < rows; ++r) s += X[r * SLD + c].x * X[r * SLD + c].x + X[r * SLD + c].y * X[r * SLD + c].y;
+    const double inv = 1.0 / sqrt(s);
+    for (int r = 0; r < rows; ++r) X[r * SLD + c] = cscale(inv, X[r * SLD + c]);
+  }
+  __syncthreads();
+  __shared__ int dropped[DMAX];
+  __shared__ int ndropped;
+  for (int niter = 1;; ++niter) {
+    d_ahb(Y, SLD, X, SLD, rows, cy, cx, B, SLD);   // BYX = Y^H X
+    d_sub_ab(X, SLD, Y, SLD, B, SLD, rows, cy, cx);  // X -= Y BYX
+    // drop!: columns with norm <= tol are replaced by random vectors
+    if (threadIdx.x == 0) ndropped = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < cx; c += blockDim.x) {
+      double s = 0;
+      for (int r = 0; r < rows; ++r) s += X[r * SLD + c].x * X[r * SLD + c].x + X[r * SLD + c].y * X[r * SLD + c].y;
+      dropped[c] = sqrt(s) <= tol ? 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int c = 0; c < cx; ++c) ndropped += dropped[c];
+    __syncthreads();
+    if (ndropped) {
+      for (int e = threadIdx.x; e < rows * cx; e += blockDim.x) {
+        const int r = e / cx, c = e - (e / cx) * cx;
+        if (dropped[c]) {
+          const uint64_t key = seed * 7919ull + uint64_t(niter) * 104729ull + uint64_t(r * DMAX + c) * 2;
+          X[r * SLD + c] = c64{hash_normal(key), hash_normal(key + 1)};
+        }
+      }
+      __syncthreads();
+      // X[:, dropped] -= Y (Y^H X[:, dropped]) -- projecting every column again is
+      // equivalent for the kept columns up to O(eps) and keeps the code uniform
+      d_ahb(Y, SLD, X, SLD, rows, cy, cx, O, SLD);
+      for (int e = threadIdx.x; e < rows * cx; e += blockDim.x) {
+        const int r = e / cx, c = e - (e / cx) * cx;
+        if (dropped[c]) {
+          c64 s{0, 0};
+          for (int l = 0; l < cy; ++l) s = cadd(s, cmul(Y[r * SLD + l], O[l * SLD + c]));
+          X[r * SLD + c] = csub(X[r * SLD + c], s);
+        }
+      }
+      __syncthreads();
+    }
+    double nb = 0;
+    for (int e = threadIdx.x; e < cy * cx; e += blockDim.x) {
+      const c64 v = B[(e / cx) * SLD + e % cx];
+      nb += v.x * v.x + v.y * v.y;
+    }
+    nb = sqrt(d_sum(nb));
+    if (nb < tol && niter > 1) break;
+    const double growth = d_ortho(X, SLD, rows, cx, eps, tol, O, W, Wr);
+    if (growth < 0) return OMX_ERR_CHOLESKY;
+    if (growth * eps < tol) break;
+    if (niter > 10) return OMX_ERR_CHOLESKY;
+  }
+  return 0;
+}
+
+// Parallel cyclic Jacobi for Hermitian A (n x n, in LDS, destroyed): eigenvalues on the
+// diagonal, eigenvectors in the columns of V.  Round-robin pairing: n/2 disjoint
+// rotations per round, rows then columns.
+__device__ void d_jacobi(c64* A, c64* V, int n) {
+  __shared__ int rp[DMAX / 2 + 1], rq[DMAX / 2 + 1];
+  __shared__ double rc[DMAX / 2 + 1];
+  __shared__ c64 rs[DMAX / 2 + 1];
+  __shared__ int nrot;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - (e / n) * n;
+    V[i * DLD + j] = c64{i == j ? 1.0 : 0.0, 0.0};
+  }
+  __syncthreads();
+  const int m = n + (n & 1);
+  const int npair = m / 2;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (threadIdx.x == 0) nrot = 0;
+    __syncthreads();
+    for (int r = 0; r < m - 1; ++r) {
+      if (threadIdx.x < npair) {
+        const int t = threadIdx.x;
+        const int ia = (t == 0) ? 0 : 1 + (t - 1 + r) % (m - 1);
+        const int tb = m - 1 - t;
+        const int ib = 1 + (tb - 1 + r) % (m - 1);
+        const int p = ia < ib ? ia : ib, q = ia < ib ? ib : ia;
+        double c = 1.0;
+        c64 s{0.0, 0.0};
+        if (q < n) {
+          const double app = A[p * DLD + p].x, aqq = A[q * DLD + q].x;
+          const c64 apq = A[p * DLD + q];
+          const double a = sqrt(apq.x * apq.x + apq.y * apq.y);
+          if (a > 1e-300 && a > 1e-16 * sqrt(fabs(app * aqq))) {
+            const c64 ph = c64{apq.x / a, apq.y / a};
+            const double zeta = (aqq - app) / (2.0 * a);
+            const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + tt * tt);
+            s = cscale(c * tt, ph);
+            atomicAdd(&nrot, 1);
+          }
+        }
+        rp[t] = p;
+        rq[t] = q < n ? q : p;
+        rc[t] = c;
+        rs[t] = s;
+      }
+      __syncthreads();
+      // rows: A[p][j] = c A[p][j] - s A[q][j];  A[q][j] = conj(s) A[p][j] + c A[q][j]
+      for (int e = threadIdx.x; e < npair * n; e += blockDim.x) {
+        const int t = e / n, j = e - (e / n) * n;
+        const int p = rp[t], q = rq[t];
+        if (p == q) continue;
+        const double c = rc[t];
+        const c64 s = rs[t];
+        const c64 ap = A[p * DLD + j], aq = A[q * DLD + j];
+        A[p * DLD + j] = csub(cscale(c, ap), cmul(s, aq));
+        A[q * DLD + j] = cadd(cmul(ccj(s), ap), cscale(c, aq));
+      }
+      __syncthreads();
+      // columns: A[j][p] = c A[j][p] - conj(s) A[j][q];  A[j][q] = s A[j][p] + c A[j][q]
+      for (int e = threadIdx.x; e < npair * n; e += blockDim.x) {
+        const int t = e / n, j = e - (e / n) * n;
+        const int p = rp[t], q = rq[t];
+        if (p == q) continue;
+        const double c = rc[t];
+        const c64 s = rs[t];
+        const c64 ap = A[j * DLD + p], aq = A[j * DLD + q];
+        A[j * DLD + p] = csub(cscale(c, ap), cmul(ccj(s), aq));
+        A[j * DLD + q] = cadd(cmul(s, ap), cscale(c, aq));
+        const c64 vp = V[j * DLD + p], vq = V[j * DLD + q];
+        V[j * DLD + p] = csub(cscale(c, vp), cmul(ccj(s), vq));
+        V[j * DLD + q] = cadd(cmul(s, vp), cscale(c, vq));
+      }
+      __syncthreads();
+    }
+    if (nrot == 0) break;
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int shp(const int* shape, int f, int ninst, int i) { return shape[f * ninst + i]; }
+
+// ----------------------------------------------------------------------------------------
+// kernels
+// ----------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(DBLK) void k_dense_rr(DenseLaunch dl, LState* st, int* shape, const c64* gp,
+                                                   double* Cx, double* lam, int diag_mode) {
+  __shared__ c64 A[DMAX * DLD];
+  __shared__ c64 V[DMAX * DLD];
+  __shared__ int rank[DMAX];
+  const int i = blockIdx.x, ninst = dl.ninst, pm = dl.pm;
+  const int nact = shp(shape, F_NACT, ninst, i);
+  if (nact == 0) return;
+  const int np = shp(shape, F_NP, ninst, i);
+  const int py = diag_mode ? nact : 2 * nact + np;
+  const long esz = long(pm) * pm;
+  for (int e = threadIdx.x; e < py * py; e += blockDim.x) {
+    const int a = e / py, b = e - (e / py) * py;
+    double sx = 0, sy = 0;
+    const c64* p = gp + long(i) * dl.nchunk * esz + a * pm + b;
+    for (int c = 0; c < dl.nchunk; ++c) {
+      sx += p[c * esz].x;
+      sy += p[c * esz].y;
+    }
+    A[a * DLD + b] = c64{sx, sy};
+  }
+  __syncthreads();
+  // Hermitian part
+  for (int e = threadIdx.x; e < py * py; e += blockDim.x) {
+    const int a = e / py, b = e - (e / py) * py;
+    if (a < b) {
+      const c64 u = A[a * DLD + b], v = A[b * DLD + a];
+      const c64 h = c64{0.5 * (u.x + v.x), 0.5 * (u.y - v.y)};
+      A[a * DLD + b] = h;
+      A[b * DLD + a] = ccj(h);
+    } else if (a == b) {
+      A[a * DLD + a].y = 0.0;
+    }
+  }
+  __syncthreads();
+  double* Ci = Cx + 2 * long(i) * esz;
+  double* li = lam + long(i) * pm;
+  if (diag_mode) {
+    for (int e = threadIdx.x; e < py * nact; e += blockDim.x) {
+      const int a = e / nact, q = e - (e / nact) * nact;
+      Ci[2 * (a * pm + q)] = (a == q) ? 1.0 : 0.0;
+      Ci[2 * (a * pm + q) + 1] = 0.0;
+    }
+    for (int q = threadIdx.x; q < nact; q += blockDim.x) li[q] = A[q * DLD + q].x;
+    return;
+  }
+  d_jacobi(A, V, py);
+  for (int j = threadIdx.x; j < py; j += blockDim.x) {
+    const double wj = A[j * DLD + j].x;
+    int r = 0;
+    for (int l = 0; l < py; ++l) {
+      const double wl = A[l * DLD + l].x;
+      if (wl < wj || (wl == wj && l < j)) ++r;
+    }
+    rank[j] = r;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < py * py; e += blockDim.x) {
+    const int a = e / py, j = e - (e / py) * py;
+    const int q = rank[j];
+    if (q < nact) {
+      Ci[2 * (a * pm + q)] = V[a * DLD + j].x;
+      Ci[2 * (a * pm + q) + 1] = V[a * DLD + j].y;
+    }
+  }
+  for (int j = threadIdx.x; j < py; j += blockDim.x)
+    if (rank[j] < nact) li[rank[j]] = A[j * DLD + j].x;
+}
+
+__global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st, int* shape, const double* npart,
+                                                     const double* Cx, double* Cp, double* rhist) {
+  __shared__ c64 cX[DMAX * SLD];
+  __shared__ c64 cP[DMAX * SLD];
+  __shared__ c64 B[16 * SLD];
+  __shared__ c64 O[16 * SLD];
+  __shared__ c64 W[16 * SLD];
+  __shared__ c64 Wr[DMAX * SLD];
+  __shared__ double nrm[16];
+  __shared__ int s_newly, s_finished;
+  const int i = blockIdx.x, ninst = dl.ninst, pm = dl.pm;
+  LState& s = st[i];
+  const int nact = shp(shape, F_NACT, ninst, i);
+  if (nact == 0) return;
+  for (int q = threadIdx.x; q < nact; q += blockDim.x) {
+    double acc = 0;
+    for (int b = 0; b < dl.nblk; ++b) acc += npart[(long(i) * dl.nblk + b) * dl.qmax + q];
+    nrm[q] = sqrt(acc);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int M = s.M;
+    const int prev = s.prev_nlocked;
+    if (s.niter <= dl.maxiter)
+      for (int q = 0; q < nact; ++q) rhist[(long(i) * (dl.maxiter + 1) + s.niter) * 16 + prev + q] = nrm[q];
+    int nl = prev;
+    if (s.niter >= 1) {
+      for (int q = 0; q < nact; ++q) {
+        if (nrm[q] < s.tol)
+          ++nl;
+        else
+          break;
+      }
+    }
+    s.nlocked = nl;
+    s.newly_locked = nl - prev;
+    s.converged = nl >= s.ncheck ? 1 : 0;
+    const int finished = s.converged || s.niter >= dl.maxiter;
+    if (finished && !s.converged) s.status = OMX_ERR_NOT_CONVERGED;
+    s.finished = finished;
+    shape[F_NLOCK * ninst + i] = nl;
+    shape[F_NEWLY * ninst + i] = nl - prev;
+    shape[F_NNEW * ninst + i] = finished ? 0 : M - nl;
+    shape[F_NCOPY * ninst + i] = finished ? nact : nl - prev;
+    shape[F_NPN * ninst + i] = (finished || s.niter == 0) ? 0 : M - nl;
+    s_newly = nl - prev;
+    s_finished = finished;
+  }
+  __syncthreads();
+  const int newly = s_newly;
+  if (s_finished || s.niter == 0) return;
+  // cP = (cX - e)[:, xn], xn = newly .. nact-1 (dftk.jl:426-438)
+  const int np = s.np;
+  const int py = 2 * nact + np;
+  const int nn = nact - newly;
+  const long esz = long(pm) * pm;
+  const double* Ci = Cx + 2 * long(i) * esz;
+  for (int e = threadIdx.x; e < py * nact; e += blockDim.x) {
+    const int a = e / nact, q = e - (e / nact) * nact;
+    cX[a * SLD + q] = c64{Ci[2 * (a * pm + q)], Ci[2 * (a * pm + q) + 1]};
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < py * nn; e += blockDim.x) {
+    const int a = e / nn, jp = e - (e / nn) * nn;
+    const int col = newly + jp;
+    c64 v = cX[a * SLD + col];
+    if (col < nn && a == newly + col) v.x -= 1.0;
+    cP[a * SLD + jp] = v;
+  }
+  __syncthreads();
+  const int rc = d_ortho_against(cP, py, nn, cX, nact, s.eps, s.ortho_tol, B, O, W, Wr,
+                                 uint64_t(i) * 1000003ull + uint64_t(s.niter));
+  if (rc != 0 && threadIdx.x == 0) s.status = rc;
+  double* Pi = Cp + 2 * long(i) * esz;
+  for (int e = threadIdx.x; e < py * nn; e += blockDim.x) {
+    const int a = e / nn, q = e - (e / nn) * nn;
+    Pi[2 * (a * pm + q)] = cP[a * SLD + q].x;
+    Pi[2 * (a * pm + q) + 1] = cP[a * SLD + q].y;
+  }
+}
+
+__global__ __launch_bounds__(DBLK) void k_dense_chol(DenseLaunch dl, LState* st, const int* shape, int cnt_field,
+                                                     const c64* gp, double* C) {
+  __shared__ c64 O[16 * SLD];
+  __shared__ c64 O0[16 * SLD];
+  __shared__ c64 W[16 * SLD];
+  const int i = blockIdx.x, ninst = dl.ninst, pm = dl.pm;
+  const int n = shp(shape, cnt_field, ninst, i);
+  if (n == 0) return;
+  const long esz = long(pm) * pm;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int a = e / n, b = e - (e / n) * n;
+    double sx = 0, sy = 0;
+    const c64* p = gp + long(i) * dl.nchunk * esz + a * pm + b;
+    for (int c = 0; c < dl.nchunk; ++c) {
+      sx += p[c * esz].x;
+      sy += p[c * esz].y;
+    }
+    O0[a * SLD + b] = c64{sx, sy};
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int a = e / n, b = e - (e / n) * n;
+    const c64 u = O0[a * SLD + b], v = O0[b * SLD + a];
+    O[a * SLD + b] = c64{0.5 * (u.x + v.x), 0.5 * (u.y - v.y)};
+  }
+  __syncthreads();
+  double tr = 0;
+  for (int a = 0; a < n; ++a) tr += O[a * SLD + a].x;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) O0[(e / n) * SLD + e % n] = O[(e / n) * SLD + e % n];
+  __syncthreads();
+  bool ok = d_chol(O, SLD, n);
+  double alpha = 100.0;
+  int nbad = 0;
+  while (!ok && nbad <= 10) {
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      const int a = e / n, b = e - (e / n) * n;
+      O[a * SLD + b] = O0[a * SLD + b];
+      if (a == b) O[a * SLD + b].x += alpha * st[i].eps * tr;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) O0[e * SLD + e].x += alpha * st[i].eps * tr;
+    __syncthreads();
+    alpha *= 10;
+    ok = d_chol(O, SLD, n);
+    ++nbad;
+  }
+  double* Ci = C + 2 * long(i) * esz;
+  if (!ok) {
+    if (threadIdx.x == 0) st[i].status = OMX_ERR_CHOLESKY;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      const int a = e / n, b = e - (e / n) * n;
+      Ci[2 * (a * pm + b)] = a == b ? 1.0 : 0.0;
+      Ci[2 * (a * pm + b) + 1] = 0.0;
+    }
+    return;
+  }
+  d_triinv(O, SLD, W, SLD, n);
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int a = e / n, b = e - (e / n) * n;
+    Ci[2 * (a * pm + b)] = W[a * SLD + b].x;
+    Ci[2 * (a * pm + b) + 1] = W[a * SLD + b].y;
+  }
+}
+
+__global__ void k_advance(DenseLaunch dl, LState* st, int* shape) {
+  const int ninst = dl.ninst;
+  for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
+    LState& s = st[i];
+    if (shape[F_NACT * ninst + i] == 0) continue;
+    const int nnew = shape[F_NNEW * ninst + i];
+    const int niter_before = s.niter;
+    if (nnew == 0) {
+      // finished this iteration: idle from now on
+      shape[F_NACT * ninst + i] = 0;
+      shape[F_NP * ninst + i] = 0;
+      continue;
+    }
+    s.prev_nlocked = s.nlocked;
+    s.nact = nnew;
+    s.np = niter_before >= 1 ? nnew : 0;
+    s.niter = niter_before + 1;
+    shape[F_XOFF * ninst + i] = s.nlocked;
+    shape[F_NACT * ninst + i] = nnew;
+    shape[F_ROFF * ninst + i] = shape[F_NEWLY * ninst + i];
+    shape[F_NP * ninst + i] = s.np;
+  }
+}
+
+__global__ void k_lobpcg_init(DenseLaunch dl, LState* st, int* shape, const int* active, const int* band) {
+  const int ninst = dl.ninst;
+  for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
+    LState& s = st[i];
+    const int on = active[i];
+    s.active = on;
+    s.niter = 0;
+    s.nlocked = s.prev_nlocked = 0;
+    s.newly_locked = 0;
+    s.nact = s.M;
+    s.np = 0;
+    s.converged = 0;
+    s.finished = on ? 0 : 1;
+    s.status = 0;
+    shape[F_XOFF * ninst + i] = 0;
+    shape[F_NACT * ninst + i] = on ? s.M : 0;
+    shape[F_ROFF * ninst + i] = 0;
+    shape[F_NP * ninst + i] = 0;
+    shape[F_NLOCK * ninst + i] = 0;
+    shape[F_NNEW * ninst + i] = on ? s.M : 0;
+    shape[F_NEWLY * ninst + i] = 0;
+    shape[F_NCOPY * ninst + i] = 0;
+    shape[F_NPN * ninst + i] = 0;
+    shape[F_M * ninst + i] = on ? s.M : 0;
+    shape[F_ONE * ninst + i] = on ? 1 : 0;
+    shape[F_EIG * ninst + i] = band[i];
+  }
+}
+
+__global__ void k_newton(int ninst, NewtonState* ns, LState* st, const c64* rq, int rq_nchunk, int rq_pm,
+                         const double* hp, int hnblk, double* kinst) {
+  for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
+    NewtonState& n = ns[i];
+    if (n.done || !st[i].active) continue;
+    double lam = 0;
+    const long esz = long(rq_pm) * rq_pm;
+    for (int c = 0; c < rq_nchunk; ++c) lam += rq[(long(i) * rq_nchunk + c) * esz].x;
+    double h = 0;
+    const long v = long(i) * st[i].M + n.band;
+    for (int b = 0; b < hnblk; ++b) h += hp[v * hnblk + b];
+    const double f = lam - n.omega2;
+    const double slope = 2.0 * h;
+    n.lam = lam;
+    n.f = f;
+    n.slope = slope;
+    n.nsolves += 1;
+    if (fabs(f) <= n.k_tol) {
+      n.done = 1;
+    } else {
+      const double knew = n.k - f / slope;
+      if (!isfinite(knew)) {
+        n.done = 2;
+      } else if (fabs(knew - n.k) <= 4.0 * 2.220446049250313e-16 * fmax(fabs(n.k), 1.0)) {
+        n.done = 1;
+      } else if (n.nsolves >= n.max_solves) {
+        n.done = 2;
+      } else {
+        n.k = knew;
+      }
+    }
+    kinst[i] = n.k;
+  }
+}
+
+}  // namespace
+
+void launch_dense_rr(const DenseLaunch& dl, LState* st, int* shape, const c64* gram_partial, double* Cx,
+                     double* lam, int diag_mode, hipStream_t stream) {
+  hipLaunchKernelGGL(k_dense_rr, dim3(dl.ninst), dim3(DBLK), 0, stream, dl, st, shape, gram_partial, Cx, lam,
+                     diag_mode);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dense_lock(const DenseLaunch& dl, LState* st, int* shape, const double* norm_partial, const double* Cx,
+                       double* Cp, double* resid_hist, hipStream_t stream) {
+  hipLaunchKernelGGL(k_dense_lock, dim3(dl.ninst), dim3(DBLK), 0, stream, dl, st, shape, norm_partial, Cx, Cp,
+                     resid_hist);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dense_chol(const DenseLaunch& dl, LState* st, const int* shape, int cnt_field, const c64* gram_partial,
+                       double* C, hipStream_t stream) {
+  hipLaunchKernelGGL(k_dense_chol, dim3(dl.ninst), dim3(DBLK), 0, stream, dl, st, shape, cnt_field, gram_partial,
+                     C);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_advance(const DenseLaunch& dl, LState* st, int* shape, hipStream_t stream) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, dl, st, shape);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_lobpcg_init(const DenseLaunch& dl, LState* st, int* shape, const int* active, const int* band,
+                        hipStream_t stream) {
+  hipLaunchKernelGGL(k_lobpcg_init, dim3(1), dim3(64), 0, stream, dl, st, shape, active, band);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_newton(int ninst, NewtonState* ns, LState* st, const c64* rq_partial, int rq_nchunk, int rq_pm,
+                   const double* hmkh_partial, int hmkh_nblk, double* kinst, hipStream_t stream) {
+  hipLaunchKernelGGL(k_newton, dim3(1), dim3(64), 0, stream, ninst, ns, st, rq_partial, rq_nchunk, rq_pm,
+                     hmkh_partial, hmkh_nblk, kinst);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace omx

@@ -1,0 +1,391 @@
+// Fused plane-wave Helmholtz operator kernels for gfx950 (see operator.hpp).
+#include "operator.hpp"
+#include "omx_errors.hpp"
+
+namespace omx {
+
+namespace {
+
+constexpr int OP_BLOCK = 256;
+
+template <typename T> struct Basis {
+  T mag, m0, m1, m2, n0, n1, n2;
+};
+
+// Transverse basis at w = (0,0,k) - G: mag = |w|, n = normalize(z x w) (y at the gimbal
+// point), m = normalize(n x w)   (maxwell.jl:646-660).
+template <typename T>
+__device__ __forceinline__ Basis<T> basis_at(int ix, int iy, int nx, int ny, double invdx, double invdy, T k) {
+  const T gx = T(double(fft_index(ix, nx)) * invdx);
+  const T gy = T(double(fft_index(iy, ny)) * invdy);
+  const T wx = -gx, wy = -gy, wz = k;
+  Basis<T> b;
+  b.mag = sqrt(wx * wx + wy * wy + wz * wz);
+  const T rho2 = wx * wx + wy * wy;
+  if (rho2 > T(0)) {
+    const T ir = T(1) / sqrt(rho2);
+    b.n0 = -wy * ir;
+    b.n1 = wx * ir;
+  } else {
+    b.n0 = T(0);
+    b.n1 = T(1);
+  }
+  b.n2 = T(0);
+  const T cx = b.n1 * wz - b.n2 * wy;
+  const T cy = b.n2 * wx - b.n0 * wz;
+  const T cz = b.n0 * wy - b.n1 * wx;
+  const T ic = T(1) / sqrt(cx * cx + cy * cy + cz * cz);
+  b.m0 = cx * ic;
+  b.m1 = cy * ic;
+  b.m2 = cz * ic;
+  return b;
+}
+
+__device__ __forceinline__ bool vec_selected(const VecSel& sel, int inst, int col) {
+  if (sel.col0 == nullptr) return true;
+  const int c0 = sel.col0[inst];
+  return col >= c0 && col < c0 + sel.ncol[inst];
+}
+
+template <int OP> struct Dirs;
+template <> struct Dirs<OP_M> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
+template <> struct Dirs<OP_MKH> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
+template <> struct Dirs<OP_P> { static constexpr int x1 = +1, y1 = +1, y2 = -1, x2 = -1; };
+
+// ---------------------------------------------------------------------------------------
+// pass 1: k-space curl (kx_tc / kxinv_tc / zx_tc) + FFT along x
+// ---------------------------------------------------------------------------------------
+template <typename T, int OP, int VPT>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass1(OpPlan<T> plan, BView<T> X, cplx<T>* D, VecSel sel,
+                                                    const double* kinst) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, RB = plan.rows_per_block;
+  cplx<T>* buf = tw + nx;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int iy0 = blockIdx.x * RB;
+  for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = plan.twx[t];
+  const T k = T(kinst[inst]);
+  const cplx<T>* Xv = X.base + inst * X.istride + col * X.ld;
+  for (int e = threadIdx.x; e < RB * nx; e += blockDim.x) {
+    const int r = e / nx;
+    const int ix = e - r * nx;
+    const int iy = iy0 + r;
+    const long p = ix + long(nx) * iy;
+    const cplx<T> h1 = Xv[2 * p], h2 = Xv[2 * p + 1];
+    const Basis<T> b = basis_at<T>(ix, iy, nx, ny, plan.invdx, plan.invdy, k);
+    cplx<T> d0, d1, d2;
+    if (OP == OP_M) {
+      d0 = (h2 * b.m0 - h1 * b.n0) * b.mag;
+      d1 = (h2 * b.m1 - h1 * b.n1) * b.mag;
+      d2 = (h2 * b.m2 - h1 * b.n2) * b.mag;
+    } else if (OP == OP_P) {
+      const T im = T(1) / b.mag;
+      d0 = (h1 * b.n0 - h2 * b.m0) * im;
+      d1 = (h1 * b.n1 - h2 * b.m1) * im;
+      d2 = (h1 * b.n2 - h2 * b.m2) * im;
+    } else {
+      d0 = T(-1) * (h1 * b.m1 + h2 * b.n1);
+      d1 = h1 * b.m0 + h2 * b.n0;
+      d2 = cplx<T>{T(0), T(0)};
+    }
+    buf[(r * 3 + 0) * nx + ix] = d0;
+    buf[(r * 3 + 1) * nx + ix] = d1;
+    buf[(r * 3 + 2) * nx + ix] = d2;
+  }
+  __syncthreads();
+  fft_lds<T, Dirs<OP>::x1, VPT>(buf, 3 * RB, nx, 1, plan.fx, tw);
+  cplx<T>* Dv = D + long(v) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < RB * nx * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int r = q / nx;
+    const int ix = q - r * nx;
+    Dv[(long(iy0 + r) * nx + ix) * 3 + c] = buf[(r * 3 + c) * nx + ix];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// pass 2: FFT along y, pointwise eps^-1 (or eps_ave), inverse FFT along y
+// ---------------------------------------------------------------------------------------
+template <typename T, int OP, int VPT>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass2(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, TX = plan.cols_per_block;
+  cplx<T>* buf = tw + ny;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int ix0 = blockIdx.x * TX;
+  for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = plan.twy[t];
+  cplx<T>* Dv = D + long(v) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < ny * TX * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int iy = q / TX;
+    const int t = q - iy * TX;
+    buf[(t * 3 + c) * ny + iy] = Dv[(long(iy) * nx + ix0 + t) * 3 + c];
+  }
+  __syncthreads();
+  fft_lds<T, Dirs<OP>::y1, VPT>(buf, 3 * TX, ny, 1, plan.fy, tw);
+  for (int e = threadIdx.x; e < ny * TX; e += blockDim.x) {
+    const int iy = e / TX;
+    const int t = e - iy * TX;
+    const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+    const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
+    cplx<T>* b0 = buf + (t * 3 + 0) * ny + iy;
+    cplx<T>* b1 = buf + (t * 3 + 1) * ny + iy;
+    cplx<T>* b2 = buf + (t * 3 + 2) * ny + iy;
+    const cplx<T> d0 = *b0, d1 = *b1, d2 = *b2;
+    if (OP == OP_P) {
+      const T ea = T(3) / (exx + eyy + ezz);
+      *b0 = d0 * ea;
+      *b1 = d1 * ea;
+      *b2 = d2 * ea;
+    } else {
+      *b0 = d0 * exx + d1 * exy + d2 * exz;
+      *b1 = d0 * exy + d1 * eyy + d2 * eyz;
+      *b2 = d0 * exz + d1 * eyz + d2 * ezz;
+    }
+  }
+  __syncthreads();
+  fft_lds<T, Dirs<OP>::y2, VPT>(buf, 3 * TX, ny, 1, plan.fy, tw);
+  for (int e = threadIdx.x; e < ny * TX * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int iy = q / TX;
+    const int t = q - iy * TX;
+    Dv[(long(iy) * nx + ix0 + t) * 3 + c] = buf[(t * 3 + c) * ny + iy];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// pass 3: inverse FFT along x + curl back (kx_ct / kxinv_ct), scaled by 1/N; for OP_MKH
+// the result is dotted with the input instead of stored.
+// ---------------------------------------------------------------------------------------
+template <typename T, int OP, int VPT>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass3(OpPlan<T> plan, BView<T> Hin, BView<T> Y, cplx<T>* D,
+                                                    VecSel sel, const double* kinst, double* dot_partial,
+                                                    int dps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, RB = plan.rows_per_block;
+  cplx<T>* buf = tw + nx;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int iy0 = blockIdx.x * RB;
+  for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = plan.twx[t];
+  const cplx<T>* Dv = D + long(v) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < RB * nx * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int r = q / nx;
+    const int ix = q - r * nx;
+    buf[(r * 3 + c) * nx + ix] = Dv[(long(iy0 + r) * nx + ix) * 3 + c];
+  }
+  __syncthreads();
+  fft_lds<T, Dirs<OP>::x2, VPT>(buf, 3 * RB, nx, 1, plan.fx, tw);
+  const T k = T(kinst[inst]);
+  const T scale = T(1.0 / (double(nx) * double(ny)));
+  double acc = 0.0;
+  const cplx<T>* Hv = Hin.base + inst * Hin.istride + col * Hin.ld;
+  cplx<T>* Yv = (OP == OP_MKH) ? nullptr : Y.base + inst * Y.istride + col * Y.ld;
+  for (int e = threadIdx.x; e < RB * nx; e += blockDim.x) {
+    const int r = e / nx;
+    const int ix = e - r * nx;
+    const int iy = iy0 + r;
+    const Basis<T> b = basis_at<T>(ix, iy, nx, ny, plan.invdx, plan.invdy, k);
+    const cplx<T> e0 = buf[(r * 3 + 0) * nx + ix];
+    const cplx<T> e1 = buf[(r * 3 + 1) * nx + ix];
+    const cplx<T> e2 = buf[(r * 3 + 2) * nx + ix];
+    const cplx<T> en = e0 * b.n0 + e1 * b.n1 + e2 * b.n2;
+    const cplx<T> em = e0 * b.m0 + e1 * b.m1 + e2 * b.m2;
+    cplx<T> h1, h2;
+    if (OP == OP_P) {
+      const T s = scale / b.mag;
+      h1 = en * s;
+      h2 = em * (-s);
+    } else {
+      const T s = b.mag * scale;
+      h1 = en * (-s);
+      h2 = em * s;
+    }
+    const long p = ix + long(nx) * iy;
+    if (OP == OP_MKH) {
+      const cplx<T> a1 = Hv[2 * p], a2 = Hv[2 * p + 1];
+      acc += double(a1.x) * double(h1.x) + double(a1.y) * double(h1.y) + double(a2.x) * double(h2.x) +
+             double(a2.y) * double(h2.y);
+    } else {
+      Yv[2 * p] = h1;
+      Yv[2 * p + 1] = h2;
+    }
+  }
+  if (OP == OP_MKH) {
+    // block reduction of acc (FP64), deterministic order
+    __shared__ double red[OP_BLOCK];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dot_partial[long(v) * dps + blockIdx.x] = -red[0];
+  }
+}
+
+// E field of the mode: pass 1 (kx_tc + FFT along x) then FFT along y and eps^-1 . d,
+// left in D as E[v][iy][ix][c] -- i.e. fields.jl:197 ``ε⁻¹_dot(fft(kx_tc(H)))``.
+template <typename T, int VPT>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass2_efield(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, TX = plan.cols_per_block;
+  cplx<T>* buf = tw + ny;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int ix0 = blockIdx.x * TX;
+  for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = plan.twy[t];
+  cplx<T>* Dv = D + long(v) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < ny * TX * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int iy = q / TX;
+    const int t = q - iy * TX;
+    buf[(t * 3 + c) * ny + iy] = Dv[(long(iy) * nx + ix0 + t) * 3 + c];
+  }
+  __syncthreads();
+  fft_lds<T, -1, VPT>(buf, 3 * TX, ny, 1, plan.fy, tw);
+  for (int e = threadIdx.x; e < ny * TX; e += blockDim.x) {
+    const int iy = e / TX;
+    const int t = e - iy * TX;
+    const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+    const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
+    const cplx<T> d0 = buf[(t * 3 + 0) * ny + iy], d1 = buf[(t * 3 + 1) * ny + iy], d2 = buf[(t * 3 + 2) * ny + iy];
+    const long o = (long(iy) * nx + ix0 + t) * 3;
+    Dv[o + 0] = d0 * exx + d1 * exy + d2 * exz;
+    Dv[o + 1] = d0 * exy + d1 * eyy + d2 * eyz;
+    Dv[o + 2] = d0 * exz + d1 * eyz + d2 * ezz;
+  }
+}
+
+template <typename T, int VPT>
+void launch_efield_vpt(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst,
+                       hipStream_t stream) {
+  const int nvec = sel.ninst * sel.mper;
+  const size_t cs = sizeof(cplx<T>);
+  dim3 g1(plan.ny / plan.rows_per_block, nvec);
+  dim3 g2(plan.nx / plan.cols_per_block, nvec);
+  const size_t lds13 = cs * (size_t(plan.nx) + size_t(plan.rows_per_block) * 3 * plan.nx);
+  const size_t lds2 = cs * (size_t(plan.ny) + size_t(plan.cols_per_block) * 3 * plan.ny);
+  hipLaunchKernelGGL((k_pass1<T, OP_M, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
+  hipLaunchKernelGGL((k_pass2_efield<T, VPT>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel);
+}
+
+template <typename T, int OP, int VPT>
+void launch_op_vpt(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+                   const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  const int nvec = sel.ninst * sel.mper;
+  const size_t cs = sizeof(cplx<T>);
+  dim3 g1(plan.ny / plan.rows_per_block, nvec);
+  dim3 g2(plan.nx / plan.cols_per_block, nvec);
+  const size_t lds13 = cs * (size_t(plan.nx) + size_t(plan.rows_per_block) * 3 * plan.nx);
+  const size_t lds2 = cs * (size_t(plan.ny) + size_t(plan.cols_per_block) * 3 * plan.ny);
+  hipLaunchKernelGGL((k_pass1<T, OP, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
+  hipLaunchKernelGGL((k_pass2<T, OP, VPT>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel);
+  hipLaunchKernelGGL((k_pass3<T, OP, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, Y, D, sel, kinst,
+                     dot_partial, dps);
+}
+
+template <typename T, int OP>
+void launch_op(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+               const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  switch (plan.vpt) {
+    case 8: launch_op_vpt<T, OP, 8>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+    case 16: launch_op_vpt<T, OP, 16>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+    case 32: launch_op_vpt<T, OP, 32>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+    default: throw Error(OMX_ERR_INTERNAL, "operator plan has an unsupported values-per-thread bound");
+  }
+}
+
+}  // namespace
+
+template <typename T>
+void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+                     const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  switch (op) {
+    case OP_M: launch_op<T, OP_M>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+    case OP_P: launch_op<T, OP_P>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+    case OP_MKH: launch_op<T, OP_MKH>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+  }
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_efield(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst,
+                   hipStream_t stream) {
+  switch (plan.vpt) {
+    case 8: launch_efield_vpt<T, 8>(plan, X, D, sel, kinst, stream); break;
+    case 16: launch_efield_vpt<T, 16>(plan, X, D, sel, kinst, stream); break;
+    case 32: launch_efield_vpt<T, 32>(plan, X, D, sel, kinst, stream); break;
+    default: throw Error(OMX_ERR_INTERNAL, "operator plan has an unsupported values-per-thread bound");
+  }
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
+template void launch_efield<double>(const OpPlan<double>&, BView<double>, cplx<double>*, const VecSel&, const double*,
+                                    hipStream_t);
+template void launch_operator<float>(OpKind, const OpPlan<float>&, BView<float>, BView<float>, cplx<float>*,
+                                     const VecSel&, const double*, double*, int, hipStream_t);
+template void launch_operator<double>(OpKind, const OpPlan<double>&, BView<double>, BView<double>,
+                                      cplx<double>*, const VecSel&, const double*, double*, int, hipStream_t);
+
+bool fft_factor(int L, FFTFactors& f) {
+  f.L = L;
+  f.n = 0;
+  int rem = L;
+  int p2 = 0;
+  while (rem % 2 == 0) {
+    rem /= 2;
+    ++p2;
+  }
+  int others[FFT_MAX_RADICES];
+  int no = 0;
+  for (int r : {3, 5}) {
+    while (rem % r == 0) {
+      if (no >= FFT_MAX_RADICES) return false;
+      others[no++] = r;
+      rem /= r;
+    }
+  }
+  if (rem != 1) return false;
+  int pow2[FFT_MAX_RADICES];
+  int np = 0;
+  int e = p2;
+  if (e == 1) {
+    pow2[np++] = 2;
+    e = 0;
+  } else if (e % 3 == 1) {
+    pow2[np++] = 4;
+    pow2[np++] = 4;
+    e -= 4;
+  } else if (e % 3 == 2) {
+    pow2[np++] = 4;
+    e -= 2;
+  }
+  while (e > 0) {
+    pow2[np++] = 8;
+    e -= 3;
+  }
+  if (np + no > FFT_MAX_RADICES) return false;
+  // larger radices first
+  for (int i = np - 1; i >= 0; --i) f.r[f.n++] = pow2[i];
+  for (int i = 0; i < no; ++i) f.r[f.n++] = others[i];
+  return true;
+}
+
+}  // namespace omx

@@ -1,0 +1,68 @@
+// Fused plane-wave Helmholtz operator family (host-side launch interface).
+//
+//   M:   H -> kx_ct( bfft( eps^-1 . fft( kx_tc(H) ) ) ) / N            (maxwell.jl:471-481)
+//   P:   H -> kxinv_ct( fft( eps_ave . bfft( kxinv_tc(H) ) ) ) / N       (maxwell.jl:459-469)
+//   MkH: H -> kx_ct( ifft( eps^-1 . fft( zx_tc(H) ) ) )                  (maxwell.jl:267-275)
+//
+// Every application is three kernels over a batch of vectors:
+//   pass 1 (rows):    pointwise curl in k-space + 1D FFT along x, all 3 components
+//   pass 2 (columns): 1D FFT along y, 3x3 eps^-1 (or eps_ave) per pixel, inverse FFT along y
+//   pass 3 (rows):    inverse 1D FFT along x + curl back onto the (m, n) basis
+// with the intermediate 3-component field in a scratch buffer (L2/MALL-resident for the
+// grid sizes of interest).  The basis (m, n, |k+G|) is recomputed on the fly from k and
+// the integer reciprocal-lattice indices, so no per-k basis array is ever stored.
+#pragma once
+#include "omx_common.hpp"
+#include "fft_lds.hpp"
+
+namespace omx {
+
+enum OpKind : int { OP_M = 0, OP_P = 1, OP_MKH = 2 };
+
+// Batched block view: column c of instance i starts at base + i*istride + c*ld.
+template <typename T> struct BView {
+  cplx<T>* base;
+  long ld;
+  long istride;
+};
+
+// Per-launch description of the vectors to transform.  Vector v of the launch is
+// column (v % mper) of instance (v / mper); it is skipped unless
+// col0[i] <= column < col0[i] + ncol[i] (both arrays may be null: all columns).
+struct VecSel {
+  int ninst;
+  int mper;
+  const int* col0;   // device, per instance, may be null
+  const int* ncol;   // device, per instance, may be null
+};
+
+template <typename T> struct OpPlan {
+  int nx, ny;
+  double invdx, invdy;            // 1/Dx, 1/Dy: G = (fft_index(ix)/Dx, fft_index(iy)/Dy)
+  FFTFactors fx, fy;
+  const cplx<T>* twx;             // device: exp(-2 pi i t/nx), t < nx
+  const cplx<T>* twy;             // device: exp(-2 pi i t/ny)
+  const T* eps6;                  // device: (ny, nx, 6) = xx, yy, zz, xy, xz, yz
+  int rows_per_block;             // pass 1/3
+  int cols_per_block;             // pass 2
+  int vpt;                        // values per thread bound (8, 16, 32)
+};
+
+// Launch one operator application on `stream`.
+//   X: input block, Y: output block (OP_M/OP_P), D: scratch of 3*N complex per vector
+//   (column v of the launch at D + v*3N), kinst: per-instance k (device, FP64).
+//   For OP_MKH, Y is unused and dot_out[v] receives -Re<H_v, MkH H_v> (FP64, device).
+template <typename T>
+void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D,
+                     const VecSel& sel, const double* kinst, double* dot_partial, int dot_partial_stride,
+                     hipStream_t stream);
+
+// E = eps^-1 . fft2(kx_tc(H)) per vector into D (layout [v][iy][ix][c], 3N per vector).
+template <typename T>
+void launch_efield(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst,
+                   hipStream_t stream);
+
+// Host helper: factor L into radices {8,4,3,5,2} (no lone factor 2 unless L is 2*odd).
+bool fft_factor(int L, FFTFactors& f);
+
+}  // namespace omx

@@ -1,0 +1,655 @@
+// Host-side engine: plans, batched LOBPCG (DFTK algorithm, lib/MaxwellEigenmodes/src/
+// solvers/dftk.jl:304-506) over device-resident blocks, Newton solve_k (src/solve.jl:77-115,
+// solvers/gpu.jl:213-242) in an FP32 phase followed by an FP64 refinement phase.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace omx {
+
+template <> PrecisionData<float>& Solver::pd<float>() { return p32; }
+template <> PrecisionData<double>& Solver::pd<double>() { return p64; }
+
+namespace {
+
+constexpr double EPS32 = 1.1920928955078125e-07;
+constexpr double EPS64 = 2.220446049250313e-16;
+
+template <typename T> constexpr double eps_of() { return sizeof(T) == 4 ? EPS32 : EPS64; }
+
+int choose_vpt(int nb, const FFTFactors& f) {
+  for (int v : {8, 16, 32}) {
+    bool ok = true;
+    for (int p = 0; p < f.n; ++p) {
+      const int R = f.r[p];
+      const int nbf = nb * (f.L / R);
+      const int need = (nbf + 255) / 256;
+      if (need > (v + R - 1) / R) ok = false;
+    }
+    if (ok) return v;
+  }
+  return -1;
+}
+
+template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
+  OpPlan<T>& pl = P.plan;
+  pl.nx = S.nx;
+  pl.ny = S.ny;
+  pl.invdx = 1.0 / S.Dx;
+  pl.invdy = 1.0 / S.Dy;
+  if (!fft_factor(S.nx, pl.fx) || !fft_factor(S.ny, pl.fy))
+    throw Error(OMX_ERR_BAD_SHAPE, "grid sizes must factor into 2, 3 and 5 (got " + std::to_string(S.nx) + " x " +
+                                       std::to_string(S.ny) + ")");
+  auto twiddles = [](int L) {
+    std::vector<cplx<T>> tw(L);
+    for (int t = 0; t < L; ++t) {
+      const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)L;
+      tw[t] = cplx<T>{T(std::cos(a)), T(std::sin(a))};
+    }
+    return tw;
+  };
+  auto twx = twiddles(S.nx), twy = twiddles(S.ny);
+  P.twx.alloc(sizeof(cplx<T>) * S.nx);
+  P.twy.alloc(sizeof(cplx<T>) * S.ny);
+  OMX_HIP_CHECK(hipMemcpy(P.twx.template as<void>(), twx.data(), sizeof(cplx<T>) * S.nx, hipMemcpyHostToDevice));
+  OMX_HIP_CHECK(hipMemcpy(P.twy.template as<void>(), twy.data(), sizeof(cplx<T>) * S.ny, hipMemcpyHostToDevice));
+  pl.twx = P.twx.template as<cplx<T>>();
+  pl.twy = P.twy.template as<cplx<T>>();
+  // rows per block (passes 1/3) and columns per block (pass 2): ~3k complex values in LDS
+  const int budget = sizeof(T) == 4 ? 3072 : 1536;
+  pl.rows_per_block = 1;
+  for (int rb : {8, 4, 2}) {
+    if (S.ny % rb == 0 && 3 * rb * S.nx <= budget) {
+      pl.rows_per_block = rb;
+      break;
+    }
+  }
+  pl.cols_per_block = 1;
+  for (int tx : {16, 8, 4, 2}) {
+    if (S.nx % tx == 0 && 3 * tx * S.ny <= budget) {
+      pl.cols_per_block = tx;
+      break;
+    }
+  }
+  const int v1 = choose_vpt(3 * pl.rows_per_block, pl.fx);
+  const int v2 = choose_vpt(3 * pl.cols_per_block, pl.fy);
+  if (v1 < 0 || v2 < 0) throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
+  pl.vpt = std::max(v1, v2);
+  const size_t lds13 = sizeof(cplx<T>) * (size_t(S.nx) + 3 * size_t(pl.rows_per_block) * S.nx);
+  const size_t lds2 = sizeof(cplx<T>) * (size_t(S.ny) + 3 * size_t(pl.cols_per_block) * S.ny);
+  if (lds13 > 160 * 1024 || lds2 > 160 * 1024) throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
+}
+
+template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
+  const long N = S.N;
+  std::vector<T> e6(size_t(6) * N);
+  const double* E = S.epsinv_host.data();
+  for (int iy = 0; iy < S.ny; ++iy)
+    for (int ix = 0; ix < S.nx; ++ix) {
+      const long p = ix + long(S.nx) * iy;
+      const double* e = E + 9 * p;
+      T* o = e6.data() + 6 * p;
+      o[0] = T(e[0]);  // xx
+      o[1] = T(e[4]);  // yy
+      o[2] = T(e[8]);  // zz
+      o[3] = T(e[3]);  // (0,1)
+      o[4] = T(e[6]);  // (0,2)
+      o[5] = T(e[7]);  // (1,2)
+    }
+  P.eps6.ensure(sizeof(T) * 6 * N);
+  OMX_HIP_CHECK(hipMemcpy(P.eps6.template as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
+  P.plan.eps6 = P.eps6.template as<T>();
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched LOBPCG over ninst instances with block size M each, vectors of precision T.
+// ---------------------------------------------------------------------------------------
+template <typename T> class Batch {
+ public:
+  Solver& S;
+  int ninst, M;
+  long K, ld, istride;
+  DevMem X[2], AX[2], R[2], AR, P[2], AP[2], D;
+  DevMem st, shape, gramp, red, normp, Cx, Cp, Cw, lam, rhist, dotp, active, band, kd, ns;
+  DenseLaunch dl{};
+  int pm, nchunk, nblk, qmax, dps;
+  int cur = 0;
+  int maxiter_alloc;
+  std::vector<LState> hst;
+
+  Batch(Solver& s, int ninst_, int M_, int maxiter) : S(s), ninst(ninst_), M(M_), maxiter_alloc(maxiter) {
+    K = S.K;
+    ld = S.ld;
+    istride = long(M) * ld;
+    const size_t blk = sizeof(cplx<T>) * size_t(ninst) * istride;
+    for (int b = 0; b < 2; ++b) {
+      X[b].alloc(blk);
+      AX[b].alloc(blk);
+      R[b].alloc(blk);
+      P[b].alloc(blk);
+      AP[b].alloc(blk);
+    }
+    AR.alloc(blk);
+    D.alloc(sizeof(cplx<T>) * size_t(ninst) * M * 3 * S.N);
+    pm = (3 * M <= 16) ? 16 : (3 * M <= 32 ? 32 : 48);
+    qmax = M <= 4 ? 4 : (M <= 8 ? 8 : 16);
+    nchunk = std::max(1, std::min<int>(int((K + 1023) / 1024), std::max(1, 512 / ninst)));
+    nblk = std::max(1, std::min<int>(int((K + 255) / 256), std::max(1, 1024 / ninst)));
+    dps = S.ny / S.pd<T>().plan.rows_per_block;
+    st.alloc(sizeof(LState) * ninst);
+    shape.alloc(sizeof(int) * F_NFIELDS * ninst);
+    OMX_HIP_CHECK(hipMemset(shape.as<void>(), 0, sizeof(int) * F_NFIELDS * ninst));
+    gramp.alloc(sizeof(c64) * size_t(ninst) * nchunk * pm * pm);
+    red.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
+    normp.alloc(sizeof(double) * size_t(ninst) * nblk * qmax);
+    Cx.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
+    Cp.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
+    Cw.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
+    lam.alloc(sizeof(double) * size_t(ninst) * pm);
+    rhist.alloc(sizeof(double) * size_t(ninst) * (maxiter + 1) * 16);
+    dotp.alloc(sizeof(double) * size_t(ninst) * M * dps);
+    active.alloc(sizeof(int) * ninst);
+    band.alloc(sizeof(int) * ninst);
+    kd.alloc(sizeof(double) * ninst);
+    ns.alloc(sizeof(NewtonState) * ninst);
+    dl.ninst = ninst;
+    dl.pm = pm;
+    dl.nchunk = nchunk;
+    dl.nblk = nblk;
+    dl.qmax = qmax;
+    hst.resize(ninst);
+  }
+
+  cplx<T>* p(DevMem& m) { return m.as<cplx<T>>(); }
+  int* f(int field) { return shape.as<int>() + field * ninst; }
+  Part<T> part(DevMem& m, const int* off, const int* cnt, int foff = 0, int fcnt = 0) {
+    return Part<T>{m.as<cplx<T>>(), ld, istride, ShapeRef{off, cnt, foff, fcnt}};
+  }
+  OutView<T> out(DevMem& m, const int* off, int foff = 0) { return OutView<T>{m.as<cplx<T>>(), ld, istride, off, foff}; }
+  static ColList<T> list(std::initializer_list<Part<T>> ps) {
+    ColList<T> L{};
+    L.n = 0;
+    for (const auto& q : ps) L.p[L.n++] = q;
+    return L;
+  }
+  CombineCoef coef(DevMem& C, const int* qcnt, const double* lamp = nullptr) {
+    return CombineCoef{C.as<double>(), long(pm) * pm, pm, qcnt, 0, lamp, pm};
+  }
+
+  void apply(OpKind op, DevMem& in, DevMem& outm, const int* col0, const int* ncol, double* dots = nullptr) {
+    VecSel sel{ninst, M, col0, ncol};
+    BView<T> Xv{p(in), ld, istride}, Yv{p(outm), ld, istride};
+    launch_operator<T>(op, S.pd<T>().plan, Xv, Yv, p(D), sel, kd.as<double>(), dots, dps, S.stream);
+  }
+  void gram(const ColList<T>& A, const ColList<T>& B, int pmv = -1) {
+    launch_gram<T>(A, B, ninst, K, nchunk, pmv < 0 ? pm : pmv, gramp.as<c64>(), S.stream);
+  }
+  // R <- R U^-1 twice (CholeskyQR2), R = list of one part with offset/count fields
+  void cholqr(DevMem& Rm, const int* off, int cnt_field) {
+    ColList<T> Rl = list({part(Rm, off, f(cnt_field))});
+    ColList<T> none{};
+    for (int pass = 0; pass < 2; ++pass) {
+      gram(Rl, Rl);
+      launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, gramp.as<c64>(), Cw.as<double>(), S.stream);
+      launch_combine<T>(CMB_PLAIN, Rl, none, coef(Cw, f(cnt_field)), qmax, out(Rm, off), out(Rm, off), out(Rm, off),
+                        ninst, K, nblk, nullptr, S.stream);
+    }
+  }
+  // R <- R - Z (Z^H R), twice (classical Gram-Schmidt with reorthogonalisation)
+  void project(DevMem& Rm, const int* off, int cnt_field, const ColList<T>& Z) {
+    ColList<T> Rl = list({part(Rm, off, f(cnt_field))});
+    ColList<T> none{};
+    for (int pass = 0; pass < 2; ++pass) {
+      gram(Z, Rl);
+      launch_reduce(gramp.as<c64>(), ninst, nchunk, pm * pm, red.as<c64>(), S.stream);
+      launch_combine<T>(CMB_SUBTRACT, Z, none, coef(red, f(cnt_field)), qmax, out(Rm, off), out(Rm, off),
+                        out(Rm, off), ninst, K, nblk, nullptr, S.stream);
+    }
+  }
+
+  // Run LOBPCG on every active instance until all finished.  Returns iterations summed
+  // over instances; per-instance status in hst.
+  int run(const std::vector<int>& act, const std::vector<int>& bands, const std::vector<int>& ncheck, double tol,
+          int maxiter, SolveStats& stats) {
+    if (maxiter > maxiter_alloc) maxiter = maxiter_alloc;
+    dl.maxiter = maxiter;
+    std::vector<LState> h(ninst);
+    for (int i = 0; i < ninst; ++i) {
+      LState& s = h[i];
+      std::memset(&s, 0, sizeof(s));
+      s.M = M;
+      s.ncheck = ncheck[i];
+      s.tol = tol;
+      s.eps = eps_of<T>();
+      s.ortho_tol = 2 * eps_of<T>();
+    }
+    hipStream_t sm = S.stream;
+    OMX_HIP_CHECK(hipMemcpyAsync(st.as<void>(), h.data(), sizeof(LState) * ninst, hipMemcpyHostToDevice, sm));
+    OMX_HIP_CHECK(hipMemcpyAsync(active.as<void>(), act.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
+    OMX_HIP_CHECK(hipMemcpyAsync(band.as<void>(), bands.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
+    launch_lobpcg_init(dl, st.as<LState>(), shape.as<int>(), active.as<int>(), band.as<int>(), sm);
+    int nact_inst = 0;
+    for (int i = 0; i < ninst; ++i) nact_inst += act[i] ? 1 : 0;
+    // X = ortho!(X); AX = A X   (dftk.jl:317, :326)
+    cholqr(X[cur], nullptr, F_M);
+    apply(OP_M, X[cur], AX[cur], f(F_XOFF), f(F_M));
+    stats.matvec += long(nact_inst) * M;
+    ColList<T> none{};
+    std::vector<int> prev_nact(ninst, M);
+    int iters = 0;
+    for (int it = 0;; ++it) {
+      const int c = cur, n = cur ^ 1;
+      if (it > 0) apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
+      ColList<T> Y, AY;
+      if (it == 0) {
+        Y = list({part(X[c], f(F_XOFF), f(F_NACT))});
+        AY = list({part(AX[c], f(F_XOFF), f(F_NACT))});
+      } else {
+        Y = list({part(X[c], f(F_XOFF), f(F_NACT)), part(R[c], f(F_ROFF), f(F_NACT)), part(P[c], nullptr, f(F_NP))});
+        AY = list({part(AX[c], f(F_XOFF), f(F_NACT)), part(AR, f(F_ROFF), f(F_NACT)), part(AP[c], nullptr, f(F_NP))});
+      }
+      gram(Y, AY);
+      launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), gramp.as<c64>(), Cx.as<double>(), lam.as<double>(),
+                      it == 0 ? 1 : 0, sm);
+      launch_combine<T>(CMB_RESID, Y, AY, coef(Cx, f(F_NACT), lam.as<double>()), qmax, out(X[n], f(F_XOFF)),
+                        out(AX[n], f(F_XOFF)), out(R[n], nullptr, 0), ninst, K, nblk, normp.as<double>(), sm);
+      launch_dense_lock(dl, st.as<LState>(), shape.as<int>(), normp.as<double>(), Cx.as<double>(), Cp.as<double>(),
+                        rhist.as<double>(), sm);
+      apply(OP_P, R[n], R[n], f(F_NEWLY), f(F_NNEW));
+      if (it > 0)
+        launch_combine<T>(CMB_TWO, Y, AY, coef(Cp, f(F_NNEW)), qmax, out(P[n], nullptr, 0), out(AP[n], nullptr, 0),
+                          out(AP[n], nullptr, 0), ninst, K, nblk, nullptr, sm);
+      launch_copy_cols<T>(p(X[n]), p(X[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm);
+      launch_copy_cols<T>(p(AX[n]), p(AX[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm);
+      ColList<T> Z = list({part(X[n], nullptr, f(F_M)), part(P[n], nullptr, f(F_NPN))});
+      project(R[n], f(F_NEWLY), F_NNEW, Z);
+      cholqr(R[n], f(F_NEWLY), F_NNEW);
+      launch_advance(dl, st.as<LState>(), shape.as<int>(), sm);
+      cur = n;
+      OMX_HIP_CHECK(hipMemcpyAsync(hst.data(), st.as<void>(), sizeof(LState) * ninst, hipMemcpyDeviceToHost, sm));
+      OMX_HIP_CHECK(hipStreamSynchronize(sm));
+      bool all = true;
+      for (int i = 0; i < ninst; ++i) {
+        if (!act[i]) continue;
+        if (it > 0 && prev_nact[i] > 0) stats.matvec += prev_nact[i];
+        if (prev_nact[i] > 0) stats.precon += prev_nact[i];
+        if (!hst[i].finished) all = false;
+        prev_nact[i] = hst[i].finished ? 0 : hst[i].nact;
+      }
+      ++iters;
+      if (all) break;
+      if (it > maxiter + 2) throw Error(OMX_ERR_INTERNAL, "LOBPCG driver did not terminate");
+    }
+    int tot = 0;
+    for (int i = 0; i < ninst; ++i)
+      if (act[i]) tot += hst[i].niter;
+    for (int i = 0; i < ninst; ++i)
+      if (act[i] && hst[i].status == OMX_ERR_CHOLESKY)
+        throw Error(OMX_ERR_CHOLESKY, "Cholesky orthogonalisation failed even after shifting");
+    (void)iters;
+    return tot;
+  }
+
+  // One Newton update for every active instance from its band's Rayleigh quotient and HMkH.
+  void newton(std::vector<NewtonState>& hns) {
+    hipStream_t sm = S.stream;
+    OMX_HIP_CHECK(hipMemcpyAsync(ns.as<void>(), hns.data(), sizeof(NewtonState) * ninst, hipMemcpyHostToDevice, sm));
+    ColList<T> a = list({part(X[cur], f(F_EIG), f(F_ONE))});
+    ColList<T> b = list({part(AX[cur], f(F_EIG), f(F_ONE))});
+    gram(a, b, 16);
+    apply(OP_MKH, X[cur], X[cur], f(F_EIG), f(F_ONE), dotp.as<double>());
+    launch_newton(ninst, ns.as<NewtonState>(), st.as<LState>(), gramp.as<c64>(), nchunk, 16, dotp.as<double>(), dps,
+                  kd.as<double>(), sm);
+    OMX_HIP_CHECK(hipMemcpyAsync(hns.data(), ns.as<void>(), sizeof(NewtonState) * ninst, hipMemcpyDeviceToHost, sm));
+    OMX_HIP_CHECK(hipStreamSynchronize(sm));
+  }
+
+  // Rayleigh quotients diag(X^H A X) of all M columns of every instance (final_retval).
+  std::vector<double> rayleigh() {
+    ColList<T> a = list({part(X[cur], nullptr, nullptr, 0, M)});
+    ColList<T> b = list({part(AX[cur], nullptr, nullptr, 0, M)});
+    gram(a, b);
+    launch_reduce(gramp.as<c64>(), ninst, nchunk, pm * pm, red.as<c64>(), S.stream);
+    std::vector<c64> h(size_t(ninst) * pm * pm);
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), red.as<void>(), sizeof(c64) * h.size(), hipMemcpyDeviceToHost, S.stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+    std::vector<double> out(size_t(ninst) * M);
+    for (int i = 0; i < ninst; ++i)
+      for (int j = 0; j < M; ++j) out[size_t(i) * M + j] = h[size_t(i) * pm * pm + j * pm + j].x;
+    return out;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Solver
+// ---------------------------------------------------------------------------------------
+
+Solver::Solver(int dev, int nx_, int ny_, double Dx_, double Dy_, const double* epsinv)
+    : device(dev), nx(nx_), ny(ny_), Dx(Dx_), Dy(Dy_) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw Error(OMX_ERR_NO_DEVICE, "no HIP device available");
+  if (dev < 0 || dev >= count) throw Error(OMX_ERR_NO_DEVICE, "device index out of range");
+  hipDeviceProp_t prop;
+  OMX_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    throw Error(OMX_ERR_NO_DEVICE, std::string("omx is built for gfx950, device is ") + prop.gcnArchName);
+  if (nx < 2 || ny < 2) throw Error(OMX_ERR_BAD_SHAPE, "grid must be at least 2 x 2");
+  if (!(Dx > 0) || !(Dy > 0) || !std::isfinite(Dx) || !std::isfinite(Dy))
+    throw Error(OMX_ERR_INVALID_ARG, "cell size must be positive and finite");
+  OMX_HIP_CHECK(hipSetDevice(dev));
+  OMX_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  N = long(nx) * ny;
+  K = 2 * N;
+  ld = (K + 63) / 64 * 64;
+  build_plan<float>(*this, p32);
+  build_plan<double>(*this, p64);
+  set_epsinv(epsinv);
+}
+
+Solver::~Solver() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+void Solver::set_epsinv(const double* epsinv) {
+  if (!epsinv) throw Error(OMX_ERR_INVALID_ARG, "epsinv is NULL");
+  const long n9 = 9 * N;
+  double amax = 0;
+  for (long i = 0; i < n9; ++i) {
+    if (!std::isfinite(epsinv[i])) throw Error(OMX_ERR_INVALID_ARG, "epsinv has non-finite entries");
+    amax = std::max(amax, std::fabs(epsinv[i]));
+  }
+  for (long p = 0; p < N; ++p) {
+    const double* e = epsinv + 9 * p;
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b)
+        if (std::fabs(e[a + 3 * b] - e[b + 3 * a]) > 1e-12 * amax)
+          throw Error(OMX_ERR_UNSUPPORTED,
+                      "epsinv must be symmetric per pixel (the reference's eid! and _eid_g! agree only then)");
+    if (!(e[0] + e[4] + e[8] > 0)) throw Error(OMX_ERR_INVALID_ARG, "epsinv trace must be positive");
+  }
+  epsinv_host.assign(epsinv, epsinv + n9);
+  OMX_HIP_CHECK(hipSetDevice(device));
+  upload_eps<float>(*this, p32);
+  upload_eps<double>(*this, p64);
+}
+
+void Solver::set_option(const std::string& key, double v) {
+  if (key == "fp32_tol")
+    opt.fp32_tol_rel = v;
+  else if (key == "fp32_switch")
+    opt.fp32_switch = v;
+  else if (key == "maxiter_fp64")
+    opt.maxiter_fp64 = int(v);
+  else if (key == "guard")
+    opt.guard = int(v);
+  else if (key == "seed")
+    opt.seed = uint64_t(v);
+  else if (key == "fp64_only")
+    opt.fp64_only = int(v);
+  else
+    throw Error(OMX_ERR_INVALID_ARG, "unknown option '" + key + "'");
+}
+
+// Host FP64 blocks (m vectors of K complex, contiguous) -> operator -> host.
+void Solver::apply(OpKind op, int prec, double k, const double* Hin, double* Hout, int m, double* dots) {
+  if (m <= 0) throw Error(OMX_ERR_INVALID_ARG, "m must be positive");
+  if (!Hin || (op != OP_MKH && !Hout) || (op == OP_MKH && !dots)) throw Error(OMX_ERR_INVALID_ARG, "NULL buffer");
+  if (!std::isfinite(k)) throw Error(OMX_ERR_INVALID_ARG, "k must be finite");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  const size_t nbytes64 = sizeof(c64) * size_t(m) * K;
+  scratch_a.ensure(nbytes64);
+  scratch_b.ensure(nbytes64);
+  DevMem kbuf(sizeof(double));
+  OMX_HIP_CHECK(hipMemcpyAsync(kbuf.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
+  OMX_HIP_CHECK(hipMemcpyAsync(scratch_a.as<void>(), Hin, nbytes64, hipMemcpyHostToDevice, stream));
+  const int dps32 = ny / p32.plan.rows_per_block, dps64 = ny / p64.plan.rows_per_block;
+  const int dps = prec == OMX_PREC_FP32 ? dps32 : dps64;
+  DevMem dotbuf(sizeof(double) * size_t(m) * dps);
+  VecSel sel{1, m, nullptr, nullptr};
+  if (prec == OMX_PREC_FP64) {
+    DevMem Dm(sizeof(c64) * size_t(m) * 3 * N);
+    BView<double> X{scratch_a.as<c64>(), K, 0}, Y{scratch_b.as<c64>(), K, 0};
+    launch_operator<double>(op, p64.plan, X, Y, Dm.as<c64>(), sel, kbuf.as<double>(), dotbuf.as<double>(), dps, stream);
+  } else if (prec == OMX_PREC_FP32) {
+    DevMem x32(sizeof(c32) * size_t(m) * K), y32(sizeof(c32) * size_t(m) * K), Dm(sizeof(c32) * size_t(m) * 3 * N);
+    launch_convert<double, float>(scratch_a.as<c64>(), K, 0, x32.as<c32>(), K, 0, 1, m, K, stream);
+    BView<float> X{x32.as<c32>(), K, 0}, Y{y32.as<c32>(), K, 0};
+    launch_operator<float>(op, p32.plan, X, Y, Dm.as<c32>(), sel, kbuf.as<double>(), dotbuf.as<double>(), dps, stream);
+    if (op != OP_MKH) launch_convert<float, double>(y32.as<c32>(), K, 0, scratch_b.as<c64>(), K, 0, 1, m, K, stream);
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  } else {
+    throw Error(OMX_ERR_INVALID_ARG, "precision must be OMX_PREC_FP32 or OMX_PREC_FP64");
+  }
+  if (op == OP_MKH) {
+    std::vector<double> h(size_t(m) * dps);
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotbuf.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+    for (int j = 0; j < m; ++j) {
+      double s = 0;
+      for (int b = 0; b < dps; ++b) s += h[size_t(j) * dps + b];
+      dots[j] = s;
+    }
+  } else {
+    OMX_HIP_CHECK(hipMemcpyAsync(Hout, scratch_b.as<void>(), nbytes64, hipMemcpyDeviceToHost, stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  }
+}
+
+namespace {
+
+template <typename T>
+void upload_block(Solver& S, Batch<T>& b, const double* H, int ncols, int inst_from, int inst_to) {
+  // H: ncols host vectors (K complex FP64 each) -> columns 0..ncols-1 of instances [from, to)
+  DevMem tmp(sizeof(c64) * size_t(ncols) * S.K);
+  OMX_HIP_CHECK(hipMemcpyAsync(tmp.as<void>(), H, sizeof(c64) * size_t(ncols) * S.K, hipMemcpyHostToDevice, S.stream));
+  for (int i = inst_from; i < inst_to; ++i)
+    launch_convert<double, T>(tmp.as<c64>(), S.K, 0, b.p(b.X[b.cur]) + i * b.istride, b.ld, 0, 1, ncols, S.K,
+                              S.stream);
+  OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+}
+
+}  // namespace
+
+void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_io, double* w2_out, omx_info* info) {
+  if (nev < 1 || nev > 16) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..16");
+  if (3 * nev > K) throw Error(OMX_ERR_BAD_SHAPE, "problem too small for the requested block size");
+  if (!w2_out) throw Error(OMX_ERR_INVALID_ARG, "omega2_out is NULL");
+  if (!(tol > 0) || maxiter < 1 || !std::isfinite(k)) throw Error(OMX_ERR_INVALID_ARG, "bad tol/maxiter/k");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  hipEvent_t e0, e1;
+  OMX_HIP_CHECK(hipEventCreate(&e0));
+  OMX_HIP_CHECK(hipEventCreate(&e1));
+  OMX_HIP_CHECK(hipEventRecord(e0, stream));
+  SolveStats stats;
+  std::vector<int> act{1}, bands{0}, nchk{nev};
+  Batch<double> b64(*this, 1, nev, std::max(maxiter, opt.maxiter_fp64));
+  OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
+  if (!opt.fp64_only) {
+    Batch<float> b32(*this, 1, nev, maxiter);
+    OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
+    if (H_io)
+      upload_block<float>(*this, b32, H_io, nev, 0, 1);
+    else
+      launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, nev, K, opt.seed, stream);
+    // FP32 phase: locking threshold at the FP32 accuracy floor (relative to the Ritz values)
+    const double tol32 = std::max(tol, opt.fp32_tol_rel * std::max(1e-300, std::fabs(k * k)));
+    stats.it32 += b32.run(act, bands, nchk, tol32, maxiter, stats);
+    launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, 1, nev, K,
+                                  stream);
+  } else {
+    if (H_io)
+      upload_block<double>(*this, b64, H_io, nev, 0, 1);
+    else
+      launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, nev, K, opt.seed, stream);
+  }
+  stats.it64 += b64.run(act, bands, nchk, tol, std::max(maxiter, opt.maxiter_fp64), stats);
+  const int status = b64.hst[0].status;
+  auto lamv = b64.rayleigh();
+  for (int j = 0; j < nev; ++j) w2_out[j] = lamv[j];
+  last_evecs.ensure(sizeof(c64) * size_t(nev) * ld);
+  OMX_HIP_CHECK(hipMemcpyAsync(last_evecs.as<void>(), b64.p(b64.X[b64.cur]), sizeof(c64) * size_t(nev) * ld,
+                               hipMemcpyDeviceToDevice, stream));
+  last_nev = nev;
+  OMX_HIP_CHECK(hipEventRecord(e1, stream));
+  if (H_io) {
+    for (int j = 0; j < nev; ++j)
+      OMX_HIP_CHECK(hipMemcpyAsync(H_io + 2 * size_t(j) * K, last_evecs.as<c64>() + size_t(j) * ld, sizeof(c64) * K,
+                                   hipMemcpyDeviceToHost, stream));
+  }
+  OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  float ms = 0;
+  OMX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (info) {
+    info->converged = status == 0 ? 1 : 0;
+    info->n_newton = 0;
+    info->lobpcg_iters_fp32 = stats.it32;
+    info->lobpcg_iters_fp64 = stats.it64;
+    info->n_matvec = stats.matvec;
+    info->n_precon = stats.precon;
+    info->max_residual = 0;
+    info->device_ms = ms;
+  }
+  if (status == OMX_ERR_NOT_CONVERGED) last_error = "LOBPCG reached maxiter before converging";
+}
+
+void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int max_eigsolves, int maxiter,
+                     const double* kguess, const double* Hguess, int nguess, double* kmags, double* evecs,
+                     omx_info* info) {
+  if (nev < 1 || nev > 16) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..16");
+  if (3 * nev > K) throw Error(OMX_ERR_BAD_SHAPE, "problem too small for the requested block size");
+  if (!kmags) throw Error(OMX_ERR_INVALID_ARG, "kmags_out is NULL");
+  if (!(omega > 0) || !std::isfinite(omega) || !(k_tol > 0) || !(eig_tol > 0) || max_eigsolves < 1 || maxiter < 1)
+    throw Error(OMX_ERR_INVALID_ARG, "bad omega/k_tol/eig_tol/max_eigsolves/maxiter");
+  if (Hguess && nguess < 0) throw Error(OMX_ERR_INVALID_ARG, "nguess must be >= 0");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  hipEvent_t e0, e1;
+  OMX_HIP_CHECK(hipEventCreate(&e0));
+  OMX_HIP_CHECK(hipEventCreate(&e1));
+  OMX_HIP_CHECK(hipEventRecord(e0, stream));
+  SolveStats stats;
+  const int ninst = nev, M = nev;
+  // k_guess (MaxwellEigenmodes.jl:63): omega * sqrt(1 / min_a min eps^-1[a,a,:,:])
+  double k0;
+  if (kguess) {
+    k0 = kguess[0];
+    if (!std::isfinite(k0)) throw Error(OMX_ERR_INVALID_ARG, "kguess must be finite");
+  } else {
+    double mn = 1e300;
+    for (long p = 0; p < N; ++p)
+      for (int a = 0; a < 3; ++a) mn = std::min(mn, epsinv_host[9 * p + 4 * a]);
+    k0 = omega * std::sqrt(1.0 / mn);
+  }
+  const double w2 = omega * omega;
+  std::vector<NewtonState> hns(ninst);
+  for (int i = 0; i < ninst; ++i) {
+    NewtonState& s = hns[i];
+    std::memset(&s, 0, sizeof(s));
+    s.k = k0;
+    s.omega2 = w2;
+    s.band = i;
+    s.max_solves = max_eigsolves;
+  }
+  std::vector<int> bands(ninst), nchk(ninst), act(ninst, 1);
+  for (int i = 0; i < ninst; ++i) {
+    bands[i] = i;
+    nchk[i] = i + 1;  // band i needs the i+1 lowest pairs locked; the rest are guard vectors
+  }
+  std::vector<double> kh(ninst, k0);
+  const int maxit64 = std::max(maxiter, opt.maxiter_fp64);
+  Batch<double> b64(*this, ninst, M, maxit64);
+  int newton_total = 0;
+  int nguess_used = (Hguess && nguess > 0) ? std::min(nguess, M) : 0;
+  if (!opt.fp64_only) {
+    Batch<float> b32(*this, ninst, M, maxiter);
+    launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
+    for (int i = 1; i < ninst; ++i)
+      OMX_HIP_CHECK(hipMemcpyAsync(b32.p(b32.X[0]) + i * b32.istride, b32.p(b32.X[0]), sizeof(c32) * b32.istride,
+                                   hipMemcpyDeviceToDevice, stream));
+    if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, ninst);
+    OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, stream));
+    const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
+    for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
+    for (int step = 0; step < max_eigsolves; ++step) {
+      for (int i = 0; i < ninst; ++i) act[i] = hns[i].done ? 0 : 1;
+      if (std::none_of(act.begin(), act.end(), [](int a) { return a; })) break;
+      stats.it32 += b32.run(act, bands, nchk, tol32, maxiter, stats);
+      b32.newton(hns);
+      ++newton_total;
+    }
+    launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
+                                  K, stream);
+    OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,
+                                 stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+    for (auto& s : hns) {
+      if (s.done == 1) s.done = 0;  // refine in FP64
+      s.k_tol = k_tol;
+    }
+  } else {
+    launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
+    for (int i = 1; i < ninst; ++i)
+      OMX_HIP_CHECK(hipMemcpyAsync(b64.p(b64.X[0]) + i * b64.istride, b64.p(b64.X[0]), sizeof(c64) * b64.istride,
+                                   hipMemcpyDeviceToDevice, stream));
+    if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, ninst);
+    OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, stream));
+    for (auto& s : hns) s.k_tol = k_tol;
+  }
+  for (int step = 0; step < max_eigsolves; ++step) {
+    for (int i = 0; i < ninst; ++i) act[i] = hns[i].done ? 0 : 1;
+    if (std::none_of(act.begin(), act.end(), [](int a) { return a; })) break;
+    stats.it64 += b64.run(act, bands, nchk, eig_tol, maxit64, stats);
+    b64.newton(hns);
+    ++newton_total;
+  }
+  bool ok = true;
+  for (int i = 0; i < ninst; ++i) {
+    kmags[i] = hns[i].k;
+    if (hns[i].done != 1) ok = false;
+  }
+  // eigenvector of band i = column i of instance i; canonicalise phase (fields.jl:258-260)
+  last_evecs.ensure(sizeof(c64) * size_t(nev) * ld);
+  for (int i = 0; i < ninst; ++i)
+    OMX_HIP_CHECK(hipMemcpyAsync(last_evecs.as<c64>() + size_t(i) * ld, b64.p(b64.X[b64.cur]) + i * b64.istride + i * ld,
+                                 sizeof(c64) * K, hipMemcpyDeviceToDevice, stream));
+  last_nev = nev;
+  canonicalize_phase(b64.kd.as<double>(), last_evecs.as<c64>(), ld, ld, nev, 1);
+  OMX_HIP_CHECK(hipEventRecord(e1, stream));
+  if (evecs) get_evecs(evecs);
+  OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  float ms = 0;
+  OMX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (info) {
+    info->converged = ok ? 1 : 0;
+    info->n_newton = newton_total;
+    info->lobpcg_iters_fp32 = stats.it32;
+    info->lobpcg_iters_fp64 = stats.it64;
+    info->n_matvec = stats.matvec;
+    info->n_precon = stats.precon;
+    info->max_residual = 0;
+    info->device_ms = ms;
+  }
+  if (!ok) throw Error(OMX_ERR_NOT_CONVERGED, "solve_k: Newton iteration did not converge within max_eigsolves");
+}
+
+void Solver::get_evecs(double* evecs) {
+  if (!evecs) throw Error(OMX_ERR_INVALID_ARG, "evecs_out is NULL");
+  if (last_nev == 0) throw Error(OMX_ERR_INVALID_ARG, "no eigenvectors on this handle yet");
+  for (int j = 0; j < last_nev; ++j)
+    OMX_HIP_CHECK(hipMemcpyAsync(evecs + 2 * size_t(j) * K, last_evecs.as<c64>() + size_t(j) * ld, sizeof(c64) * K,
+                                 hipMemcpyDeviceToHost, stream));
+  OMX_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+}  // namespace omx

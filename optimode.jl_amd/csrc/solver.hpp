@@ -1,0 +1,118 @@
+// Host-side engine behind the omx C ABI: device buffers, operator plans, the batched
+// LOBPCG driver and the Newton solve_k.
+#pragma once
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "omx.h"
+#include "omx_common.hpp"
+#include "omx_errors.hpp"
+#include "operator.hpp"
+#include "block.hpp"
+#include "dense.hpp"
+
+namespace omx {
+
+// Owning device allocation.
+class DevMem {
+ public:
+  DevMem() = default;
+  explicit DevMem(size_t bytes) { alloc(bytes); }
+  ~DevMem() { release(); }
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  DevMem(DevMem&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevMem& operator=(DevMem&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) OMX_HIP_CHECK(hipMalloc(&p_, bytes));
+    n_ = bytes;
+  }
+  void ensure(size_t bytes) {
+    if (bytes > n_) alloc(bytes);
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename U> U* as() const { return static_cast<U*>(p_); }
+  size_t size() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+template <typename T> struct PrecisionData {
+  OpPlan<T> plan{};
+  DevMem eps6, twx, twy;
+};
+
+struct SolveStats {
+  int newton = 0;
+  int it32 = 0;
+  int it64 = 0;
+  long matvec = 0;
+  long precon = 0;
+  double max_residual = 0;
+  double device_ms = 0;
+};
+
+struct Options {
+  double fp32_tol_rel = 2e-6;   // FP32 LOBPCG locking threshold, relative to omega^2
+  double fp32_switch = 1e-6;    // |Δω²|/ω² below which Newton moves to the FP64 phase
+  int maxiter_fp64 = 200;       // LOBPCG iteration cap in the FP64 phase
+  int guard = 0;                // extra block columns beyond nev (unused by default)
+  uint64_t seed = 0x5eed;
+  int fp64_only = 0;            // skip the FP32 phase
+};
+
+class Solver {
+ public:
+  Solver(int device, int nx, int ny, double Dx, double Dy, const double* epsinv);
+  ~Solver();
+
+  void set_epsinv(const double* epsinv);
+  void apply(OpKind op, int prec, double k, const double* Hin, double* Hout, int m, double* dots);
+  void solve_omega2(double k, int nev, double tol, int maxiter, double* H_io, double* w2_out, omx_info* info);
+  void solve_k(double omega, int nev, double k_tol, double eig_tol, int max_eigsolves, int maxiter,
+               const double* kguess, const double* Hguess, int nguess, double* kmags, double* evecs, omx_info* info);
+  void get_evecs(double* evecs);
+  void adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
+               const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar);
+  void set_option(const std::string& key, double value);
+
+  std::mutex mu;
+  std::string last_error;
+
+  // shared state, public for the templated engine
+  int device;
+  hipStream_t stream = nullptr;
+  int nx, ny;
+  double Dx, Dy;
+  long N, K, ld;
+  PrecisionData<float> p32;
+  PrecisionData<double> p64;
+  Options opt;
+  std::vector<double> epsinv_host;  // (3,3,Nx,Ny) column-major copy
+  DevMem last_evecs;                // FP64, nev columns of ld complex
+  int last_nev = 0;
+  DevMem scratch_a, scratch_b, scratch_c;
+
+  template <typename T> PrecisionData<T>& pd();
+  void canonicalize_phase(const double* d_k, c64* vecs, long ldv, long istride, int nvec, int mper);
+};
+
+}  // namespace omx

@@ -1,0 +1,271 @@
+"""omx: MI355X (gfx950) plane-wave Maxwell eigensolver, Python host interface.
+
+This mirrors the MaxwellEigenmodes interface of OptiMode.jl for the HIP backend,
+with the same names, argument meanings, layouts and error behaviour:
+
+* ``Grid(Dx, Dy, Nx, Ny)``                 -- lib/DielectricSmoothing/src/grid.jl:46-71
+* ``HIPSolver(device=0, ...)``             -- a new ``AbstractEigensolver`` beside
+  ``GPUSolver`` (lib/MaxwellEigenmodes/src/solvers/gpu.jl:37-47)
+* ``solve_omega2(k, epsinv, grid, solver; nev, maxiter, tol)``  -- src/solve.jl:53-63
+* ``solve_k(omega, epsinv, grid, solver; nev, max_eigsolves, maxiter, k_tol, eig_tol,
+  kguess, Hguess)``                        -- src/solve.jl:182-196
+* ``rrule_solve_k(...)`` -> ``((kmags, evecs), pullback)`` -- src/solve.jl:213-289
+* ``HelmholtzMap(k, epsinv, grid, solver)`` with ``*``/``precondition``/``HMkH``
+                                           -- src/maxwell.jl:834-949, :256-275
+
+``epsinv`` is a ``(3, 3, Nx, Ny)`` float64 array indexed ``[a, b, ix, iy]`` (the Julia
+index order); eigenvectors are complex128 vectors of length ``2*Nx*Ny`` in the
+reference's flat order ``c + 2*(ix + Nx*iy)``.  Every computation runs in libomx.so on
+the GPU; nothing here falls back to the CPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import OmxError, OmxInfo, OMX_PREC_FP32, OMX_PREC_FP64  # noqa: F401
+
+__all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "rrule_solve_k", "k_guess",
+           "OmxError", "library_path", "exported_symbols"]
+
+
+def library_path():
+    return _lib.LIB_PATH
+
+
+def exported_symbols():
+    return [name for name, _, _ in _lib.SIGNATURES]
+
+
+class Grid:
+    """2D cell ``Dx x Dy`` (um) on ``Nx x Ny`` pixels (``Grid{2}``)."""
+
+    def __init__(self, Dx, Dy, Nx, Ny):
+        self.Dx, self.Dy, self.Nx, self.Ny = float(Dx), float(Dy), int(Nx), int(Ny)
+
+    @property
+    def N(self):
+        return self.Nx * self.Ny
+
+    def __repr__(self):
+        return f"Grid({self.Dx}, {self.Dy}, {self.Nx}, {self.Ny})"
+
+
+def k_guess(omega, epsinv):
+    """``k_guess`` (src/MaxwellEigenmodes.jl:63)."""
+    return float(omega) * np.sqrt(1.0 / min(np.min(epsinv[a, a]) for a in range(3)))
+
+
+def _epsinv_c(epsinv, grid):
+    e = np.asarray(epsinv, dtype=np.float64)
+    if e.shape != (3, 3, grid.Nx, grid.Ny):
+        raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"epsinv must have shape (3,3,{grid.Nx},{grid.Ny}), got {e.shape}")
+    # column-major (3,3,Nx,Ny) flat order: a + 3b + 9(ix + Nx iy)
+    return np.ascontiguousarray(e.transpose(3, 2, 1, 0))
+
+
+class _Handle:
+    """One ``omx_solver`` handle: a grid and an eps^-1 field resident on the GPU."""
+
+    def __init__(self, device, grid, epsinv_c):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        st = L.omx_create(ctypes.byref(h), int(device), grid.Nx, grid.Ny, grid.Dx, grid.Dy, _lib.dptr(epsinv_c))
+        if st != _lib.OMX_OK:
+            raise OmxError(st, L.omx_last_error(None).decode() or L.omx_status_string(st).decode())
+        self.h = h
+        self.grid = grid
+        self.eps_key = epsinv_c.tobytes()
+
+    def set_epsinv(self, epsinv_c):
+        _lib.check(_lib.lib().omx_set_epsinv(self.h, _lib.dptr(epsinv_c)), self.h)
+        self.eps_key = epsinv_c.tobytes()
+
+    def close(self):
+        if self.h:
+            _lib.lib().omx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HIPSolver:
+    """HIP/gfx950 eigensolver backend.
+
+    ``precision`` is the policy of the iterative phase: ``"mixed"`` (default: FP32
+    operator applications, then an FP64 refinement phase) or ``"fp64"``.  Tolerance
+    knobs of the FP32 phase are relative to ω²: ``fp32_tol`` (LOBPCG locking threshold)
+    and ``fp32_switch`` (|Δω²|/ω² at which Newton moves to FP64).
+    """
+
+    def __init__(self, device=0, precision="mixed", fp32_tol=None, fp32_switch=None, maxiter_fp64=None, seed=None):
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        self.device = int(device)
+        self.precision = precision
+        self.options = {}
+        if fp32_tol is not None:
+            self.options["fp32_tol"] = float(fp32_tol)
+        if fp32_switch is not None:
+            self.options["fp32_switch"] = float(fp32_switch)
+        if maxiter_fp64 is not None:
+            self.options["maxiter_fp64"] = float(maxiter_fp64)
+        if seed is not None:
+            self.options["seed"] = float(seed)
+        self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
+        self._h = None
+
+    def handle(self, grid, epsinv):
+        ec = _epsinv_c(epsinv, grid)
+        h = self._h
+        if h is None or (h.grid.Nx, h.grid.Ny, h.grid.Dx, h.grid.Dy) != (grid.Nx, grid.Ny, grid.Dx, grid.Dy):
+            if h is not None:
+                h.close()
+            h = self._h = _Handle(self.device, grid, ec)
+        elif h.eps_key != ec.tobytes():
+            h.set_epsinv(ec)
+        L = _lib.lib()
+        for k, v in self.options.items():
+            _lib.check(L.omx_set_option(h.h, k.encode(), v), h.h)
+        return h
+
+    def __repr__(self):
+        return f"HIPSolver(device={self.device}, precision={self.precision!r})"
+
+
+def _cols(H, n2):
+    H = np.asarray(H)
+    if H.ndim == 1:
+        H = H[:, None]
+    if H.shape[0] != n2:
+        raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"vectors must have length {n2}")
+    # contiguous per vector: (m, 2N)
+    return np.ascontiguousarray(H.T, dtype=np.complex128)
+
+
+class HelmholtzMap:
+    """Matrix-free ``M(k)`` on the GPU (``HelmholtzMap``, src/maxwell.jl:834-949)."""
+
+    def __init__(self, k, epsinv, grid, solver=None, precision=OMX_PREC_FP64):
+        self.k = float(k)
+        self.grid = grid
+        self.solver = solver if solver is not None else HIPSolver()
+        self.h = self.solver.handle(grid, epsinv)
+        self.precision = precision
+
+    def _run(self, fn, H):
+        n2 = 2 * self.grid.N
+        X = _cols(H, n2)
+        Y = np.empty_like(X)
+        _lib.check(fn(self.h.h, self.k, _lib.dptr(X), _lib.dptr(Y), X.shape[0], self.precision), self.h.h)
+        return Y.T if np.ndim(H) == 2 else Y[0]
+
+    def __mul__(self, H):
+        return self._run(_lib.lib().omx_apply_helmholtz, H)
+
+    def precondition(self, H):
+        return self._run(_lib.lib().omx_apply_preconditioner, H)
+
+    def HMkH(self, H):
+        n2 = 2 * self.grid.N
+        X = _cols(H, n2)
+        out = np.empty(X.shape[0])
+        _lib.check(_lib.lib().omx_hmkh(self.h.h, self.k, _lib.dptr(X), X.shape[0], _lib.dptr(out), self.precision),
+                   self.h.h)
+        return out if np.ndim(H) == 2 else float(out[0])
+
+
+def solve_omega2(k, epsinv, grid, solver, nev=1, maxiter=100, tol=1e-8, evecs_guess=None, return_info=False):
+    """``solve_ω²(k, ε⁻¹, grid, solver; nev, maxiter, tol, evecs_guess)`` -> ``(ω², evecs)``."""
+    h = solver.handle(grid, epsinv)
+    n2 = 2 * grid.N
+    H = np.zeros((nev, n2), dtype=np.complex128)
+    use_guess = evecs_guess is not None
+    if use_guess:
+        G = np.asarray(evecs_guess)
+        G = G[:, None] if G.ndim == 1 else G
+        if isinstance(evecs_guess, (list, tuple)):
+            G = np.stack([np.asarray(v) for v in evecs_guess], 1)
+        H[:G.shape[1]] = G.T[:nev]
+    w2 = np.empty(nev)
+    info = OmxInfo()
+    st = _lib.lib().omx_solve_omega2(h.h, float(k), int(nev), float(tol), int(maxiter),
+                                     _lib.dptr(H) if use_guess else None, _lib.dptr(w2), ctypes.byref(info))
+    _lib.check(st, h.h)
+    if not use_guess:
+        _lib.check(_lib.lib().omx_get_evecs(h.h, _lib.dptr(H)), h.h)
+    evecs = [H[j].copy() for j in range(nev)]
+    if return_info:
+        return w2, evecs, info.as_dict()
+    return w2, evecs
+
+
+def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k_tol=1e-8, eig_tol=1e-8,
+            kguess=None, Hguess=None, return_info=False, fetch_evecs=True):
+    """``solve_k(ω, ε⁻¹, grid, solver; nev, max_eigsolves, maxiter, k_tol, eig_tol, kguess,
+    Hguess)`` -> ``(kmags, evecs)``: unit-norm, phase-canonicalised eigenvectors."""
+    h = solver.handle(grid, epsinv)
+    n2 = 2 * grid.N
+    km = np.empty(nev)
+    kg = None
+    if kguess is not None:
+        kg = np.array([float(np.ravel([kguess])[0])])
+    Hg = None
+    ng = 0
+    if Hguess is not None:
+        cols = [np.asarray(Hguess)] if np.ndim(Hguess) == 1 else (
+            [np.asarray(v) for v in Hguess] if isinstance(Hguess, (list, tuple)) else
+            [np.asarray(Hguess)[:, j] for j in range(np.shape(Hguess)[1])])
+        cols = [c for c in cols if len(c) == n2][:nev]
+        if cols:
+            Hg = np.ascontiguousarray(np.stack(cols), dtype=np.complex128)
+            ng = len(cols)
+    ev = np.empty((nev, n2), dtype=np.complex128) if fetch_evecs else None
+    info = OmxInfo()
+    st = _lib.lib().omx_solve_k(h.h, float(omega), int(nev), float(k_tol), float(eig_tol), int(max_eigsolves),
+                                int(maxiter), _lib.dptr(kg), _lib.dptr(Hg), ng, _lib.dptr(km), _lib.dptr(ev),
+                                ctypes.byref(info))
+    _lib.check(st, h.h)
+    evecs = [ev[j].copy() for j in range(nev)] if fetch_evecs else None
+    if return_info:
+        return km, evecs, info.as_dict()
+    return km, evecs
+
+
+def rrule_solve_k(omega, epsinv, grid, solver, nev=1, **kw):
+    """``rrule(solve_k, ω, ε⁻¹, grid, solver; ...)`` -> ``((kmags, evecs), pullback)``.
+
+    ``pullback((kbar, evbar))`` returns ``(None, ω̄, ε̄⁻¹, None, None)``; ``kbar`` and
+    ``evbar`` may be ``None`` or contain ``None`` entries (zero cotangents)."""
+    kmags, evecs = solve_k(omega, epsinv, grid, solver, nev=nev, **kw)
+    h = solver.handle(grid, epsinv)
+    n2 = 2 * grid.N
+
+    def pullback(cot):
+        kbar, evbar = cot
+        kb = np.zeros(nev)
+        if kbar is not None:
+            for j, v in enumerate(kbar):
+                kb[j] = 0.0 if v is None else float(v)
+        eb = np.zeros((nev, n2), dtype=np.complex128)
+        mask = np.zeros(nev, dtype=np.uint8)
+        if evbar is not None:
+            for j, v in enumerate(evbar):
+                if v is not None:
+                    eb[j] = np.asarray(v, dtype=np.complex128)
+                    mask[j] = 1
+        E = np.ascontiguousarray(np.stack(evecs), dtype=np.complex128)
+        wbar = np.zeros(1)
+        eibar_c = np.zeros((grid.Ny, grid.Nx, 3, 3))
+        st = _lib.lib().omx_solve_k_adjoint(h.h, float(omega), int(nev), _lib.dptr(np.ascontiguousarray(kmags)),
+                                            _lib.dptr(E), _lib.dptr(kb), _lib.dptr(eb),
+                                            mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _lib.dptr(wbar),
+                                            _lib.dptr(eibar_c))
+        _lib.check(st, h.h)
+        return (None, float(wbar[0]), np.ascontiguousarray(eibar_c.transpose(3, 2, 1, 0)), None, None)
+
+    return (kmags, evecs), pullback

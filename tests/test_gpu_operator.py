@@ -1,0 +1,80 @@
+"""GPU parity of the fused operator kernels against the FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+import omx
+
+pytestmark = pytest.mark.gpu
+
+P_WG = [4.2, 2.1, 1.0, 0.6]
+
+
+def _rand(rng, n, m):
+    return rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+
+
+def _aniso(grid, seed=3):
+    xs, ys = grid.x(), grid.y()
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    bump = np.exp(-((X - 0.3 * grid.Dx / 6) ** 2 / 1.1 + (Y + 0.1 * grid.Dy) ** 2 / 0.5))
+    eps = np.zeros((3, 3, grid.Nx, grid.Ny))
+    for a, base in enumerate((2.0, 2.3, 2.1)):
+        eps[a, a] = base + 1.8 * bump * (1 + 0.1 * a)
+    eps[0, 1] = eps[1, 0] = 0.3 * bump * np.sign(X + 1e-3)
+    eps[0, 2] = eps[2, 0] = 0.1 * bump
+    eps[1, 2] = eps[2, 1] = -0.05 * bump * (Y > 0)
+    return O.sliceinv_3x3(eps)
+
+
+def _relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+SIZES = [(6.0, 4.0, 24, 16), (4.0, 3.0, 12, 10), (6.0, 4.0, 128, 128), (6.0, 4.0, 192, 192),
+         (8.0, 4.0, 256, 256), (8.0, 4.0, 384, 384), (8.0, 4.0, 512, 256), (5.0, 5.0, 60, 30)]
+
+
+@pytest.mark.parametrize("size", SIZES, ids=lambda s: f"{s[2]}x{s[3]}")
+def test_operator_family_matches_oracle(size):
+    print(f"operator parity {size}", flush=True)
+    Dx, Dy, nx, ny = size
+    og = O.Grid(Dx, Dy, nx, ny)
+    g = omx.Grid(Dx, Dy, nx, ny)
+    ei = _aniso(og)
+    k = 1.37
+    rng = np.random.default_rng(nx * 1000 + ny)
+    H = _rand(rng, 2 * og.N, 3)
+    ref = O.HelmholtzOracle(k, ei, og)
+    M_ref, P_ref = ref.M(H), ref.P(H)
+    hk_ref = np.array([ref.HMkH(H[:, j]) for j in range(3)])
+    solver = omx.HIPSolver()
+    for prec, tol in ((omx.OMX_PREC_FP64, 1e-12), (omx.OMX_PREC_FP32, 2e-6)):
+        Mg = omx.HelmholtzMap(k, ei, g, solver, precision=prec)
+        assert _relerr(Mg * H, M_ref) < tol
+        assert _relerr(Mg.precondition(H), P_ref) < tol
+        assert np.allclose(Mg.HMkH(H), hk_ref, rtol=10 * tol, atol=0)
+
+
+def test_fp64_hermitian_and_homogeneous():
+    og = O.Grid(6.0, 4.0, 24, 16)
+    g = omx.Grid(6.0, 4.0, 24, 16)
+    ei = O.gaussian_wg_epsi(P_WG, og)
+    Mg = omx.HelmholtzMap(1.2, ei, g, omx.HIPSolver())
+    rng = np.random.default_rng(0)
+    v, w = _rand(rng, 2 * og.N, 1)[:, 0], _rand(rng, 2 * og.N, 1)[:, 0]
+    assert np.vdot(w, Mg * v) == pytest.approx(np.conj(np.vdot(v, Mg * w)), rel=1e-12)
+    # homogeneous medium: plane waves are eigenvectors with |k - G|^2 / eps
+    eps = 2.25
+    eh = np.zeros((3, 3) + og.shape)
+    for a in range(3):
+        eh[a, a] = 1 / eps
+    Mh = omx.HelmholtzMap(0.9, eh, g, omx.HIPSolver())
+    gx, gy = O.gvec(og)
+    for (ix, iy, c) in [(0, 0, 0), (3, 5, 1), (23, 15, 0), (12, 8, 1)]:
+        h = np.zeros((2,) + og.shape, complex)
+        h[c, ix, iy] = 1.0
+        vv = O.to_flat(h)
+        lam = (gx[ix] ** 2 + gy[iy] ** 2 + 0.81) / eps
+        assert np.allclose(Mh * vv, lam * vv, atol=1e-13)
+        assert np.allclose(Mh.precondition(Mh * vv), vv, atol=1e-13)
