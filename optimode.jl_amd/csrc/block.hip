@@ -46,7 +46,9 @@ __global__ __launch_bounds__(BLK) void k_gram(ColList<T> A, ColList<T> B, long K
   }
   __syncthreads();
   const int npa = npa_s, npb = npb_s;
+  if (npa == 0 || npb == 0) return;
   const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  const bool busy = ty * TILE < npa && tx * TILE < npb;
   cplx<T> acc[TILE][TILE];
 #pragma unroll
   for (int i = 0; i < TILE; ++i)
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(BLK) void k_gram(ColList<T> A, ColList<T> B, long K
     }
     __syncthreads();
 #pragma unroll 4
-    for (int r = 0; r < SR; ++r) {
+    for (int r = 0; r < (busy ? SR : 0); ++r) {
       cplx<T> av[TILE], bv[TILE];
 #pragma unroll
       for (int i = 0; i < TILE; ++i) av[i] = As[r][ty * TILE + i];
@@ -258,13 +260,115 @@ __global__ __launch_bounds__(BLK) void k_copy_cols(const cplx<T>* src, cplx<T>* 
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Gram on the matrix cores (FP32): G = A^H B with A, B complex.  Over the interleaved
+// (re, im) rows of one complex row r, v_mfma_f32_32x32x2f32 with k = (re, im) gives
+//   Re G += [ar ai] . [br bi]^T      (B operand = (br, bi))
+//   Im G += [ar ai] . [bi -br]^T     (B operand = (bi, -br))
+// so each complex row costs two MFMAs per 32x32 output tile.  Lane layout (checked on
+// gfx950): A operand lane l -> (row l%32, k l/32); B operand lane l -> (k l/32, col l%32);
+// accumulator register q of lane l -> (row 4(l/32) + q%4 + 8(q/4), col l%32).
+// The 4 waves of a block split the (TA x TB) output tiles and, when fewer than 4 tiles,
+// the staged rows; partial tiles are summed through LDS and written once per block.
+// ---------------------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int TA, int TB>
+__global__ __launch_bounds__(BLK) void k_gram_mfma(ColList<float> A, ColList<float> B, long K, long rows_per_chunk,
+                                                   c64* partial, int pm) {
+  constexpr int NT = TA * TB;          // output tiles
+  constexpr int G = 4 / NT;            // row groups per tile (1, 2 or 4)
+  constexpr int SR = 64;               // complex rows staged per step
+  constexpr int WA = TA * 32, WB = TB * 32;
+  __shared__ const cplx<float>* pa_ptr[WA];
+  __shared__ const cplx<float>* pb_ptr[WB];
+  __shared__ int npa_s, npb_s;
+  __shared__ cplx<float> As[SR][WA + 1];
+  __shared__ cplx<float> Bs[SR][WB + 1];
+  __shared__ float red[4][32][64];
+  const int inst = blockIdx.y, chunk = blockIdx.x;
+  if (threadIdx.x == 0) {
+    npa_s = resolve_cols(A, inst, pa_ptr, WA);
+    npb_s = resolve_cols(B, inst, pb_ptr, WB);
+  }
+  __syncthreads();
+  const int npa = npa_s, npb = npb_s;
+  if (npa == 0 || npb == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = wave % NT, grp = wave / NT;
+  const int ta = tile / TB, tb = tile % TB;
+  const int hi = lane >> 5, col = lane & 31;
+  floatx16 accr = {0}, acci = {0};
+  const long r0 = long(chunk) * rows_per_chunk;
+  const long r1 = (r0 + rows_per_chunk < K) ? r0 + rows_per_chunk : K;
+  for (long rs = r0; rs < r1; rs += SR) {
+    for (int idx = threadIdx.x; idx < SR * WA; idx += BLK) {
+      const int a = idx / SR, r = idx - (idx / SR) * SR;
+      const long row = rs + r;
+      As[r][a] = (a < npa && row < r1) ? pa_ptr[a][row] : cplx<float>{0.f, 0.f};
+    }
+    for (int idx = threadIdx.x; idx < SR * WB; idx += BLK) {
+      const int b = idx / SR, r = idx - (idx / SR) * SR;
+      const long row = rs + r;
+      Bs[r][b] = (b < npb && row < r1) ? pb_ptr[b][row] : cplx<float>{0.f, 0.f};
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = grp; r < SR; r += G) {
+      const cplx<float> av = As[r][ta * 32 + col];
+      const cplx<float> bv = Bs[r][tb * 32 + col];
+      const float a = hi ? av.y : av.x;
+      const float b = hi ? bv.y : bv.x;
+      const float b2 = hi ? -bv.x : bv.y;
+      accr = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, accr, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b2, acci, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // sum the G row groups of each tile through LDS, then write FP64 partials
+  for (int q = 0; q < 16; ++q) {
+    red[wave][q][lane] = accr[q];
+    red[wave][16 + q][lane] = acci[q];
+  }
+  __syncthreads();
+  if (grp == 0) {
+    c64* out = partial + (long(inst) * gridDim.x + chunk) * pm * pm;
+    for (int q = 0; q < 16; ++q) {
+      float sr = 0.f, si = 0.f;
+      for (int g = 0; g < G; ++g) {
+        sr += red[tile + g * NT][q][lane];
+        si += red[tile + g * NT][16 + q][lane];
+      }
+      const int a = ta * 32 + 4 * hi + (q & 3) + 8 * (q >> 2);
+      const int b = tb * 32 + col;
+      if (a < pm && b < pm) out[a * pm + b] = c64{double(sr), double(si)};
+    }
+  }
+}
+
 }  // namespace
 
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
-                 hipStream_t stream) {
+                 hipStream_t stream, int ca, int cb) {
   const long rpc = (K + nchunk - 1) / nchunk;
   dim3 grid(nchunk, ninst);
+  if (sizeof(T) == 4 && ca > 0 && cb > 0 && ca <= 64 && cb <= 64) {
+    // matrix-core path: ca, cb bound the column counts of A and B
+    const ColList<float>& Af = reinterpret_cast<const ColList<float>&>(A);
+    const ColList<float>& Bf = reinterpret_cast<const ColList<float>&>(B);
+    const int ta = ca > 32 ? 2 : 1, tb = cb > 32 ? 2 : 1;
+    if (ta == 1 && tb == 1)
+      hipLaunchKernelGGL((k_gram_mfma<1, 1>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
+    else if (ta == 2 && tb == 1)
+      hipLaunchKernelGGL((k_gram_mfma<2, 1>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
+    else if (ta == 1 && tb == 2)
+      hipLaunchKernelGGL((k_gram_mfma<1, 2>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
+    else
+      hipLaunchKernelGGL((k_gram_mfma<2, 2>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
+    OMX_HIP_CHECK(hipGetLastError());
+    return;
+  }
   switch (pm) {
     case 16: hipLaunchKernelGGL((k_gram<T, 16>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial); break;
     case 32: hipLaunchKernelGGL((k_gram<T, 32>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial); break;
@@ -332,8 +436,8 @@ void launch_copy_cols(const cplx<T>* src, cplx<T>* dst, long ld, long istride, i
 
 template void launch_copy_cols<float>(const cplx<float>*, cplx<float>*, long, long, int, const int*, const int*, long, hipStream_t);
 template void launch_copy_cols<double>(const cplx<double>*, cplx<double>*, long, long, int, const int*, const int*, long, hipStream_t);
-template void launch_gram<float>(const ColList<float>&, const ColList<float>&, int, long, int, int, c64*, hipStream_t);
-template void launch_gram<double>(const ColList<double>&, const ColList<double>&, int, long, int, int, c64*, hipStream_t);
+template void launch_gram<float>(const ColList<float>&, const ColList<float>&, int, long, int, int, c64*, hipStream_t, int, int);
+template void launch_gram<double>(const ColList<double>&, const ColList<double>&, int, long, int, int, c64*, hipStream_t, int, int);
 template void launch_combine<float>(CombineMode, const ColList<float>&, const ColList<float>&, const CombineCoef&, int,
                                     OutView<float>, OutView<float>, OutView<float>, int, long, int, double*, hipStream_t);
 template void launch_combine<double>(CombineMode, const ColList<double>&, const ColList<double>&, const CombineCoef&, int,

@@ -39,10 +39,11 @@ template <typename T> struct OutView {
 };
 
 // G_i = A_i^H B_i, written as per-chunk partial sums (FP64) to
-// partial[(i*nchunk + chunk)*pm*pm + a*pm + b].  pm in {16, 32, 48}.
+// partial[(i*nchunk + chunk)*pm*pm + a*pm + b].  pm in {16, 32, 48}.  ca / cb are upper
+// bounds of the column counts of A and B (FP32 uses the matrix cores when both are <= 64).
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
-                 hipStream_t stream);
+                 hipStream_t stream, int ca = 0, int cb = 0);
 
 // out_i[e] = sum_chunk partial[(i*nchunk + chunk)*esz + e], fixed order (deterministic).
 void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream);

@@ -137,4 +137,21 @@ int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmag
   });
 }
 
+int omx_get_kernel_stats(omx_solver* h, const char* family, double* total_ms, int64_t* launches, double* bytes) {
+  if (!family || !total_ms || !launches || !bytes) return OMX_ERR_INVALID_ARG;
+  return guarded(h, [&](omx::Solver& s) {
+    s.prof.flush();
+    const omx::KernelStat* st = nullptr;
+    if (std::strcmp(family, "op_fp32") == 0)
+      st = &s.prof.op32;
+    else if (std::strcmp(family, "op_fp64") == 0)
+      st = &s.prof.op64;
+    else
+      throw omx::Error(OMX_ERR_INVALID_ARG, std::string("unknown kernel family '") + family + "'");
+    *total_ms = st->ms;
+    *launches = st->launches;
+    *bytes = st->bytes;
+  });
+}
+
 }  // extern "C"

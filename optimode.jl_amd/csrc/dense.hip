@@ -578,6 +578,93 @@ __global__ __launch_bounds__(DBLK) void k_dense_chol(DenseLaunch dl, LState* st,
   }
 }
 
+// One pass of R <- (R - Z B) U^{-1}: from the Gram partials of [Z R]^H R (Z = the M
+// X columns plus F_NPN P columns, R = F_NNEW columns) form B = Z^H R, S = R^H R - B^H B
+// (the Gram of the projected R for orthonormal Z), the upper Cholesky S = U^H U with
+// DFTK's shifting on failure (dftk.jl:163-192), and the coefficients
+// C = [-B U^{-1}; U^{-1}] of the update R <- [Z R] C.
+__global__ __launch_bounds__(DBLK) void k_dense_orth(DenseLaunch dl, LState* st, const int* shape, const c64* gp,
+                                                     int pmg, double* C) {
+  __shared__ c64 G[DMAX * SLD];
+  __shared__ c64 S[16 * SLD];
+  __shared__ c64 S0[16 * SLD];
+  __shared__ c64 W[16 * SLD];
+  const int i = blockIdx.x, ninst = dl.ninst, pm = dl.pm;
+  const int nr = shp(shape, F_NNEW, ninst, i);
+  if (nr == 0) return;
+  const int nz = shp(shape, F_M, ninst, i) + shp(shape, F_NPN, ninst, i);
+  const int pa = nz + nr;
+  const long esz = long(pmg) * pmg;
+  for (int e = threadIdx.x; e < pa * nr; e += blockDim.x) {
+    const int a = e / nr, b = e - (e / nr) * nr;
+    double sx = 0, sy = 0;
+    const c64* p = gp + long(i) * dl.nchunk * esz + a * pmg + b;
+    for (int c = 0; c < dl.nchunk; ++c) {
+      sx += p[c * esz].x;
+      sy += p[c * esz].y;
+    }
+    G[a * SLD + b] = c64{sx, sy};
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nr * nr; e += blockDim.x) {
+    const int a = e / nr, b = e - (e / nr) * nr;
+    c64 v = G[(nz + a) * SLD + b];
+    for (int l = 0; l < nz; ++l) v = csub(v, cmul(ccj(G[l * SLD + a]), G[l * SLD + b]));
+    S0[a * SLD + b] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nr * nr; e += blockDim.x) {
+    const int a = e / nr, b = e - (e / nr) * nr;
+    const c64 u = S0[a * SLD + b], v = S0[b * SLD + a];
+    S[a * SLD + b] = c64{0.5 * (u.x + v.x), 0.5 * (u.y - v.y)};
+  }
+  __syncthreads();
+  double tr = 0;
+  for (int a = 0; a < nr; ++a) tr += S[a * SLD + a].x;
+  for (int e = threadIdx.x; e < nr * nr; e += blockDim.x) S0[(e / nr) * SLD + e % nr] = S[(e / nr) * SLD + e % nr];
+  __syncthreads();
+  const double eps = st[i].eps;
+  bool ok = d_chol(S, SLD, nr);
+  double alpha = 100.0;
+  int nbad = 0;
+  while (!ok && nbad <= 10) {
+    for (int e = threadIdx.x; e < nr * nr; e += blockDim.x) {
+      const int a = e / nr, b = e - (e / nr) * nr;
+      S[a * SLD + b] = S0[a * SLD + b];
+      if (a == b) S[a * SLD + b].x += alpha * eps * tr;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr; e += blockDim.x) S0[e * SLD + e].x += alpha * eps * tr;
+    __syncthreads();
+    alpha *= 10;
+    ok = d_chol(S, SLD, nr);
+    ++nbad;
+  }
+  double* Ci = C + 2 * long(i) * pm * pm;
+  if (!ok) {
+    if (threadIdx.x == 0) st[i].status = OMX_ERR_CHOLESKY;
+    for (int e = threadIdx.x; e < pa * nr; e += blockDim.x) {
+      const int a = e / nr, b = e - (e / nr) * nr;
+      Ci[2 * (a * pm + b)] = (a - nz == b) ? 1.0 : 0.0;
+      Ci[2 * (a * pm + b) + 1] = 0.0;
+    }
+    return;
+  }
+  d_triinv(S, SLD, W, SLD, nr);
+  for (int e = threadIdx.x; e < pa * nr; e += blockDim.x) {
+    const int a = e / nr, q = e - (e / nr) * nr;
+    c64 v;
+    if (a < nz) {
+      v = c64{0, 0};
+      for (int l = 0; l < nr; ++l) v = csub(v, cmul(G[a * SLD + l], W[l * SLD + q]));
+    } else {
+      v = W[(a - nz) * SLD + q];
+    }
+    Ci[2 * (a * pm + q)] = v.x;
+    Ci[2 * (a * pm + q) + 1] = v.y;
+  }
+}
+
 __global__ void k_advance(DenseLaunch dl, LState* st, int* shape) {
   const int ninst = dl.ninst;
   for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
@@ -631,7 +718,7 @@ __global__ void k_lobpcg_init(DenseLaunch dl, LState* st, int* shape, const int*
   }
 }
 
-__global__ void k_newton(int ninst, NewtonState* ns, LState* st, const c64* rq, int rq_nchunk, int rq_pm,
+__global__ void k_newton(int ninst, int mper, NewtonState* ns, LState* st, const c64* rq, int rq_nchunk, int rq_pm,
                          const double* hp, int hnblk, double* kinst) {
   for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
     NewtonState& n = ns[i];
@@ -640,7 +727,7 @@ __global__ void k_newton(int ninst, NewtonState* ns, LState* st, const c64* rq, 
     const long esz = long(rq_pm) * rq_pm;
     for (int c = 0; c < rq_nchunk; ++c) lam += rq[(long(i) * rq_nchunk + c) * esz].x;
     double h = 0;
-    const long v = long(i) * st[i].M + n.band;
+    const long v = long(i) * mper + n.band;
     for (int b = 0; b < hnblk; ++b) h += hp[v * hnblk + b];
     const double f = lam - n.omega2;
     const double slope = 2.0 * h;
@@ -689,6 +776,12 @@ void launch_dense_chol(const DenseLaunch& dl, LState* st, const int* shape, int 
   OMX_HIP_CHECK(hipGetLastError());
 }
 
+void launch_dense_orth(const DenseLaunch& dl, LState* st, const int* shape, const c64* gram_partial, int pmg,
+                       double* C, hipStream_t stream) {
+  hipLaunchKernelGGL(k_dense_orth, dim3(dl.ninst), dim3(DBLK), 0, stream, dl, st, shape, gram_partial, pmg, C);
+  OMX_HIP_CHECK(hipGetLastError());
+}
+
 void launch_advance(const DenseLaunch& dl, LState* st, int* shape, hipStream_t stream) {
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, dl, st, shape);
   OMX_HIP_CHECK(hipGetLastError());
@@ -700,9 +793,9 @@ void launch_lobpcg_init(const DenseLaunch& dl, LState* st, int* shape, const int
   OMX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_newton(int ninst, NewtonState* ns, LState* st, const c64* rq_partial, int rq_nchunk, int rq_pm,
+void launch_newton(int ninst, int mper, NewtonState* ns, LState* st, const c64* rq_partial, int rq_nchunk, int rq_pm,
                    const double* hmkh_partial, int hmkh_nblk, double* kinst, hipStream_t stream) {
-  hipLaunchKernelGGL(k_newton, dim3(1), dim3(64), 0, stream, ninst, ns, st, rq_partial, rq_nchunk, rq_pm,
+  hipLaunchKernelGGL(k_newton, dim3(1), dim3(64), 0, stream, ninst, mper, ns, st, rq_partial, rq_nchunk, rq_pm,
                      hmkh_partial, hmkh_nblk, kinst);
   OMX_HIP_CHECK(hipGetLastError());
 }

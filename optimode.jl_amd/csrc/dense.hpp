@@ -77,6 +77,11 @@ void launch_dense_lock(const DenseLaunch& dl, LState* st, int* shape, const doub
 void launch_dense_chol(const DenseLaunch& dl, LState* st, const int* shape, int cnt_field, const c64* gram_partial,
                        double* C, hipStream_t stream);
 
+// R <- (R - Z Z^H R) U^{-1} coefficients from the Gram partials of [Z R]^H R, with
+// Z = X (F_M columns) + P (F_NPN) and R = F_NNEW columns; C is (nz + nr) x nr.
+void launch_dense_orth(const DenseLaunch& dl, LState* st, const int* shape, const c64* gram_partial, int pmg,
+                       double* C, hipStream_t stream);
+
 // End of iteration: roll next-iteration shape fields into place.
 void launch_advance(const DenseLaunch& dl, LState* st, int* shape, hipStream_t stream);
 
@@ -86,7 +91,7 @@ void launch_lobpcg_init(const DenseLaunch& dl, LState* st, int* shape, const int
 
 // Newton step: f = lam - omega^2 from the band's Rayleigh quotient (gram partials of
 // x^H A x), slope = 2 HMkH (dot partials), k update, done flags.
-void launch_newton(int ninst, NewtonState* ns, LState* st, const c64* rq_partial, int rq_nchunk, int rq_pm,
+void launch_newton(int ninst, int mper, NewtonState* ns, LState* st, const c64* rq_partial, int rq_nchunk, int rq_pm,
                    const double* hmkh_partial, int hmkh_nblk, double* kinst, hipStream_t stream);
 
 }  // namespace omx

@@ -7,8 +7,54 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace omx {
+
+hipEvent_t Profiler::take() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  OMX_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+hipEvent_t Profiler::begin(hipStream_t s) {
+  hipEvent_t a = take();
+  OMX_HIP_CHECK(hipEventRecord(a, s));
+  return a;
+}
+
+void Profiler::end(hipStream_t s, hipEvent_t a, KernelStat* st, double bytes) {
+  hipEvent_t b = take();
+  OMX_HIP_CHECK(hipEventRecord(b, s));
+  pending_.push_back(Pending{a, b, st, bytes});
+}
+
+void Profiler::flush() {
+  for (auto& p : pending_) {
+    OMX_HIP_CHECK(hipEventSynchronize(p.b));
+    float ms = 0;
+    OMX_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    p.st->ms += ms;
+    p.st->launches += 1;
+    p.st->bytes += p.bytes;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+Profiler::~Profiler() {
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
+}
 
 template <> PrecisionData<float>& Solver::pd<float>() { return p32; }
 template <> PrecisionData<double>& Solver::pd<double>() { return p64; }
@@ -109,6 +155,7 @@ template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
 // ---------------------------------------------------------------------------------------
 template <typename T> class Batch {
  public:
+  using value_type = T;
   Solver& S;
   int ninst, M;
   long K, ld, istride;
@@ -184,45 +231,50 @@ template <typename T> class Batch {
     BView<T> Xv{p(in), ld, istride}, Yv{p(outm), ld, istride};
     launch_operator<T>(op, S.pd<T>().plan, Xv, Yv, p(D), sel, kd.as<double>(), dots, dps, S.stream);
   }
-  void gram(const ColList<T>& A, const ColList<T>& B, int pmv = -1) {
-    launch_gram<T>(A, B, ninst, K, nchunk, pmv < 0 ? pm : pmv, gramp.as<c64>(), S.stream);
+  // ca, cb: upper bounds of the column counts of A and B
+  void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1) {
+    launch_gram<T>(A, B, ninst, K, nchunk, pmv < 0 ? pm : pmv, gramp.as<c64>(), S.stream, ca, cb);
   }
   // R <- R U^-1 twice (CholeskyQR2), R = list of one part with offset/count fields
   void cholqr(DevMem& Rm, const int* off, int cnt_field) {
     ColList<T> Rl = list({part(Rm, off, f(cnt_field))});
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
-      gram(Rl, Rl);
+      gram(Rl, Rl, M, M);
       launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, gramp.as<c64>(), Cw.as<double>(), S.stream);
       launch_combine<T>(CMB_PLAIN, Rl, none, coef(Cw, f(cnt_field)), qmax, out(Rm, off), out(Rm, off), out(Rm, off),
                         ninst, K, nblk, nullptr, S.stream);
     }
   }
-  // R <- R - Z (Z^H R), twice (classical Gram-Schmidt with reorthogonalisation)
-  void project(DevMem& Rm, const int* off, int cnt_field, const ColList<T>& Z) {
-    ColList<T> Rl = list({part(Rm, off, f(cnt_field))});
+  // ortho!(R, [X P]) (dftk.jl:487-495) as two passes of R <- (R - Z Z^H R) U^{-1}, each
+  // one Gram of [Z R]^H R, one small dense step and one in-place update; Z = X_n (M
+  // columns) and P_n (F_NPN columns), R = F_NNEW columns at offset F_NEWLY.
+  void ortho_r(DevMem& Xn, DevMem& Pn, DevMem& Rm) {
+    ColList<T> Rl = list({part(Rm, f(F_NEWLY), f(F_NNEW))});
+    ColList<T> ZR = list({part(Xn, nullptr, f(F_M)), part(Pn, nullptr, f(F_NPN)), part(Rm, f(F_NEWLY), f(F_NNEW))});
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
-      gram(Z, Rl);
-      launch_reduce(gramp.as<c64>(), ninst, nchunk, pm * pm, red.as<c64>(), S.stream);
-      launch_combine<T>(CMB_SUBTRACT, Z, none, coef(red, f(cnt_field)), qmax, out(Rm, off), out(Rm, off),
-                        out(Rm, off), ninst, K, nblk, nullptr, S.stream);
+      gram(ZR, Rl, 3 * M, M);
+      launch_dense_orth(dl, st.as<LState>(), shape.as<int>(), gramp.as<c64>(), pm, Cw.as<double>(), S.stream);
+      launch_combine<T>(CMB_PLAIN, ZR, none, coef(Cw, f(F_NNEW)), qmax, out(Rm, f(F_NEWLY)), out(Rm, f(F_NEWLY)),
+                        out(Rm, f(F_NEWLY)), ninst, K, nblk, nullptr, S.stream);
     }
   }
 
   // Run LOBPCG on every active instance until all finished.  Returns iterations summed
   // over instances; per-instance status in hst.
-  int run(const std::vector<int>& act, const std::vector<int>& bands, const std::vector<int>& ncheck, double tol,
-          int maxiter, SolveStats& stats) {
+  // msize[i]: block size of instance i (<= M); tols[i]: its locking threshold.
+  int run(const std::vector<int>& act, const std::vector<int>& bands, const std::vector<int>& ncheck,
+          const std::vector<int>& msize, const std::vector<double>& tols, int maxiter, SolveStats& stats) {
     if (maxiter > maxiter_alloc) maxiter = maxiter_alloc;
     dl.maxiter = maxiter;
     std::vector<LState> h(ninst);
     for (int i = 0; i < ninst; ++i) {
       LState& s = h[i];
       std::memset(&s, 0, sizeof(s));
-      s.M = M;
+      s.M = msize[i];
       s.ncheck = ncheck[i];
-      s.tol = tol;
+      s.tol = tols[i];
       s.eps = eps_of<T>();
       s.ortho_tol = 2 * eps_of<T>();
     }
@@ -231,18 +283,32 @@ template <typename T> class Batch {
     OMX_HIP_CHECK(hipMemcpyAsync(active.as<void>(), act.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
     OMX_HIP_CHECK(hipMemcpyAsync(band.as<void>(), bands.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
     launch_lobpcg_init(dl, st.as<LState>(), shape.as<int>(), active.as<int>(), band.as<int>(), sm);
-    int nact_inst = 0;
-    for (int i = 0; i < ninst; ++i) nact_inst += act[i] ? 1 : 0;
+    long ncols_active = 0;
+    for (int i = 0; i < ninst; ++i) ncols_active += act[i] ? msize[i] : 0;
     // X = ortho!(X); AX = A X   (dftk.jl:317, :326)
     cholqr(X[cur], nullptr, F_M);
     apply(OP_M, X[cur], AX[cur], f(F_XOFF), f(F_M));
-    stats.matvec += long(nact_inst) * M;
+    stats.matvec += ncols_active;
     ColList<T> none{};
-    std::vector<int> prev_nact(ninst, M);
+    std::vector<int> prev_nact(msize);
     int iters = 0;
     for (int it = 0;; ++it) {
       const int c = cur, n = cur ^ 1;
-      if (it > 0) apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
+      if (it > 0) {
+        if (S.prof.enabled) {
+          long m_act = 0;
+          for (int i = 0; i < ninst; ++i)
+            if (act[i]) m_act += prev_nact[i];
+          // algorithmic bytes of one application: read + write the vectors, read eps^-1 once
+          const double w = double(sizeof(T)) / 4.0;
+          const double bytes = w * (32.0 * S.N * m_act + 24.0 * S.N);
+          hipEvent_t a = S.prof.begin(sm);
+          apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
+          S.prof.end(sm, a, sizeof(T) == 4 ? &S.prof.op32 : &S.prof.op64, bytes);
+        } else {
+          apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
+        }
+      }
       ColList<T> Y, AY;
       if (it == 0) {
         Y = list({part(X[c], f(F_XOFF), f(F_NACT))});
@@ -251,7 +317,7 @@ template <typename T> class Batch {
         Y = list({part(X[c], f(F_XOFF), f(F_NACT)), part(R[c], f(F_ROFF), f(F_NACT)), part(P[c], nullptr, f(F_NP))});
         AY = list({part(AX[c], f(F_XOFF), f(F_NACT)), part(AR, f(F_ROFF), f(F_NACT)), part(AP[c], nullptr, f(F_NP))});
       }
-      gram(Y, AY);
+      gram(Y, AY, it == 0 ? M : 3 * M, it == 0 ? M : 3 * M);
       launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), gramp.as<c64>(), Cx.as<double>(), lam.as<double>(),
                       it == 0 ? 1 : 0, sm);
       launch_combine<T>(CMB_RESID, Y, AY, coef(Cx, f(F_NACT), lam.as<double>()), qmax, out(X[n], f(F_XOFF)),
@@ -264,9 +330,7 @@ template <typename T> class Batch {
                           out(AP[n], nullptr, 0), ninst, K, nblk, nullptr, sm);
       launch_copy_cols<T>(p(X[n]), p(X[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm);
       launch_copy_cols<T>(p(AX[n]), p(AX[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm);
-      ColList<T> Z = list({part(X[n], nullptr, f(F_M)), part(P[n], nullptr, f(F_NPN))});
-      project(R[n], f(F_NEWLY), F_NNEW, Z);
-      cholqr(R[n], f(F_NEWLY), F_NNEW);
+      ortho_r(X[n], P[n], R[n]);
       launch_advance(dl, st.as<LState>(), shape.as<int>(), sm);
       cur = n;
       OMX_HIP_CHECK(hipMemcpyAsync(hst.data(), st.as<void>(), sizeof(LState) * ninst, hipMemcpyDeviceToHost, sm));
@@ -299,19 +363,39 @@ template <typename T> class Batch {
     OMX_HIP_CHECK(hipMemcpyAsync(ns.as<void>(), hns.data(), sizeof(NewtonState) * ninst, hipMemcpyHostToDevice, sm));
     ColList<T> a = list({part(X[cur], f(F_EIG), f(F_ONE))});
     ColList<T> b = list({part(AX[cur], f(F_EIG), f(F_ONE))});
-    gram(a, b, 16);
+    gram(a, b, 1, 1, 16);
     apply(OP_MKH, X[cur], X[cur], f(F_EIG), f(F_ONE), dotp.as<double>());
-    launch_newton(ninst, ns.as<NewtonState>(), st.as<LState>(), gramp.as<c64>(), nchunk, 16, dotp.as<double>(), dps,
+    launch_newton(ninst, M, ns.as<NewtonState>(), st.as<LState>(), gramp.as<c64>(), nchunk, 16, dotp.as<double>(), dps,
                   kd.as<double>(), sm);
     OMX_HIP_CHECK(hipMemcpyAsync(hns.data(), ns.as<void>(), sizeof(NewtonState) * ninst, hipMemcpyDeviceToHost, sm));
     OMX_HIP_CHECK(hipStreamSynchronize(sm));
+  }
+
+  // HMkH of the first ncol columns of instance `inst` (others idle), summed on the host.
+  std::vector<double> hmkh_cols(int inst, int ncol) {
+    std::vector<int> c0(ninst, 0), nc(ninst, 0);
+    nc[inst] = ncol;
+    DevMem dc0(sizeof(int) * ninst), dnc(sizeof(int) * ninst);
+    OMX_HIP_CHECK(hipMemcpyAsync(dc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(dnc.as<void>(), nc.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
+    apply(OP_MKH, X[cur], X[cur], dc0.as<int>(), dnc.as<int>(), dotp.as<double>());
+    std::vector<double> h(size_t(ninst) * M * dps);
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, S.stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+    std::vector<double> out(ncol);
+    for (int j = 0; j < ncol; ++j) {
+      double acc = 0;
+      for (int b = 0; b < dps; ++b) acc += h[(size_t(inst) * M + j) * dps + b];
+      out[j] = acc;
+    }
+    return out;
   }
 
   // Rayleigh quotients diag(X^H A X) of all M columns of every instance (final_retval).
   std::vector<double> rayleigh() {
     ColList<T> a = list({part(X[cur], nullptr, nullptr, 0, M)});
     ColList<T> b = list({part(AX[cur], nullptr, nullptr, 0, M)});
-    gram(a, b);
+    gram(a, b, M, M);
     launch_reduce(gramp.as<c64>(), ninst, nchunk, pm * pm, red.as<c64>(), S.stream);
     std::vector<c64> h(size_t(ninst) * pm * pm);
     OMX_HIP_CHECK(hipMemcpyAsync(h.data(), red.as<void>(), sizeof(c64) * h.size(), hipMemcpyDeviceToHost, S.stream));
@@ -394,6 +478,10 @@ void Solver::set_option(const std::string& key, double v) {
     opt.seed = uint64_t(v);
   else if (key == "fp64_only")
     opt.fp64_only = int(v);
+  else if (key == "profile") {
+    prof.reset();
+    prof.enabled = v != 0;
+  }
   else
     throw Error(OMX_ERR_INVALID_ARG, "unknown option '" + key + "'");
 }
@@ -481,7 +569,7 @@ void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, nev, K, opt.seed, stream);
     // FP32 phase: locking threshold at the FP32 accuracy floor (relative to the Ritz values)
     const double tol32 = std::max(tol, opt.fp32_tol_rel * std::max(1e-300, std::fabs(k * k)));
-    stats.it32 += b32.run(act, bands, nchk, tol32, maxiter, stats);
+    stats.it32 += b32.run(act, bands, nchk, {nev}, {tol32}, maxiter, stats);
     launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, 1, nev, K,
                                   stream);
   } else {
@@ -490,7 +578,7 @@ void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_
     else
       launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, nev, K, opt.seed, stream);
   }
-  stats.it64 += b64.run(act, bands, nchk, tol, std::max(maxiter, opt.maxiter_fp64), stats);
+  stats.it64 += b64.run(act, bands, nchk, {nev}, {tol}, std::max(maxiter, opt.maxiter_fp64), stats);
   const int status = b64.hst[0].status;
   auto lamv = b64.rayleigh();
   for (int j = 0; j < nev; ++j) w2_out[j] = lamv[j];
@@ -521,6 +609,37 @@ void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_
   }
   if (status == OMX_ERR_NOT_CONVERGED) last_error = "LOBPCG reached maxiter before converging";
 }
+
+namespace {
+
+// Newton driver over a batch of band instances (instance i follows band i).
+template <typename T>
+int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
+                 const std::vector<int>& nchk, const std::vector<int>& msize, double tol_floor, int maxiter,
+                 int max_steps, SolveStats& stats, int& newton_total) {
+  const int ninst = b.ninst;
+  std::vector<int> act(ninst);
+  std::vector<double> tols(ninst);
+  int iters = 0;
+  for (int step = 0; step < max_steps; ++step) {
+    bool any = false;
+    for (int i = 0; i < ninst; ++i) {
+      act[i] = hns[i].done ? 0 : 1;
+      any = any || act[i];
+      // inexact Newton: the eigenvalue error ~ resid^2 / gap only has to stay well below
+      // the current |Δω²|, so the locking threshold follows it down to the floor
+      const double f = std::fabs(hns[i].f);
+      tols[i] = (hns[i].nsolves > 0 && std::isfinite(f)) ? std::max(tol_floor, 0.1 * f) : tol_floor;
+    }
+    if (!any) break;
+    iters += b.run(act, bands, nchk, msize, tols, maxiter, stats);
+    b.newton(hns);
+    ++newton_total;
+  }
+  return iters;
+}
+
+}  // namespace
 
 void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int max_eigsolves, int maxiter,
                      const double* kguess, const double* Hguess, int nguess, double* kmags, double* evecs,
@@ -559,33 +678,64 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     s.band = i;
     s.max_solves = max_eigsolves;
   }
-  std::vector<int> bands(ninst), nchk(ninst), act(ninst, 1);
+  // band i is instance i: it needs the i+1 lowest pairs locked (DFTK n_conv_check); its
+  // block carries `guard` further columns (all nev columns when guard < 0)
+  std::vector<int> bands(ninst), nchk(ninst), msize(ninst);
   for (int i = 0; i < ninst; ++i) {
     bands[i] = i;
-    nchk[i] = i + 1;  // band i needs the i+1 lowest pairs locked; the rest are guard vectors
+    nchk[i] = i + 1;
+    msize[i] = opt.guard < 0 ? M : std::min(M, i + 1 + opt.guard);
   }
-  std::vector<double> kh(ninst, k0);
   const int maxit64 = std::max(maxiter, opt.maxiter_fp64);
   Batch<double> b64(*this, ninst, M, maxit64);
   int newton_total = 0;
-  int nguess_used = (Hguess && nguess > 0) ? std::min(nguess, M) : 0;
+  const int nguess_used = (Hguess && nguess > 0) ? std::min(nguess, M) : 0;
+  const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
+
+  // first eigensolve, shared by every band: the lowest nev pairs at k0 (instance 0);
+  // each band then starts Newton from its own first-order estimate of k with this block
+  auto seed_bands = [&](auto& b, double tol0) {
+    using TT = typename std::remove_reference<decltype(b)>::type::value_type;
+    std::vector<int> act(ninst, 0), n0(ninst, M), m0(ninst, M);
+    act[0] = 1;
+    std::vector<double> t0(ninst, tol0);
+    std::vector<double> kh(ninst, k0);
+    OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice,
+                                 stream));
+    const int it = b.run(act, bands, n0, m0, t0, sizeof(TT) == 4 ? maxiter : maxit64, stats);
+    auto lam = b.rayleigh();
+    auto hk = b.hmkh_cols(0, M);
+    for (int i = 0; i < ninst; ++i) {
+      const double f = lam[i] - w2, slope = 2.0 * hk[i];
+      hns[i].f = f;
+      hns[i].lam = lam[i];
+      hns[i].slope = slope;
+      hns[i].nsolves = 1;
+      if (std::fabs(f) <= hns[i].k_tol) {
+        hns[i].done = 1;
+      } else {
+        const double kn = k0 - f / slope;
+        hns[i].k = std::isfinite(kn) && kn > 0 ? kn : k0;
+      }
+      kh[i] = hns[i].k;
+    }
+    for (int i = 1; i < ninst; ++i)
+      OMX_HIP_CHECK(hipMemcpyAsync(b.p(b.X[b.cur]) + i * b.istride, b.p(b.X[b.cur]), sizeof(cplx<TT>) * b.istride,
+                                   hipMemcpyDeviceToDevice, stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice,
+                                 stream));
+    ++newton_total;
+    return it;
+  };
+
   if (!opt.fp64_only) {
     Batch<float> b32(*this, ninst, M, maxiter);
     launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
-    for (int i = 1; i < ninst; ++i)
-      OMX_HIP_CHECK(hipMemcpyAsync(b32.p(b32.X[0]) + i * b32.istride, b32.p(b32.X[0]), sizeof(c32) * b32.istride,
-                                   hipMemcpyDeviceToDevice, stream));
-    if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, ninst);
-    OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, stream));
-    const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
+    if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
-    for (int step = 0; step < max_eigsolves; ++step) {
-      for (int i = 0; i < ninst; ++i) act[i] = hns[i].done ? 0 : 1;
-      if (std::none_of(act.begin(), act.end(), [](int a) { return a; })) break;
-      stats.it32 += b32.run(act, bands, nchk, tol32, maxiter, stats);
-      b32.newton(hns);
-      ++newton_total;
-    }
+    stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
+    stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
+                               newton_total);
     launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
                                   K, stream);
     OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,
@@ -597,20 +747,12 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     }
   } else {
     launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
-    for (int i = 1; i < ninst; ++i)
-      OMX_HIP_CHECK(hipMemcpyAsync(b64.p(b64.X[0]) + i * b64.istride, b64.p(b64.X[0]), sizeof(c64) * b64.istride,
-                                   hipMemcpyDeviceToDevice, stream));
-    if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, ninst);
-    OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, stream));
+    if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, 1);
     for (auto& s : hns) s.k_tol = k_tol;
+    stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
   }
-  for (int step = 0; step < max_eigsolves; ++step) {
-    for (int i = 0; i < ninst; ++i) act[i] = hns[i].done ? 0 : 1;
-    if (std::none_of(act.begin(), act.end(), [](int a) { return a; })) break;
-    stats.it64 += b64.run(act, bands, nchk, eig_tol, maxit64, stats);
-    b64.newton(hns);
-    ++newton_total;
-  }
+  const int left = std::max(1, max_eigsolves - newton_total);
+  stats.it64 += newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total);
   bool ok = true;
   for (int i = 0; i < ninst; ++i) {
     kmags[i] = hns[i].k;

@@ -70,11 +70,47 @@ struct SolveStats {
   double device_ms = 0;
 };
 
+// Live timing of one kernel family on the solver's stream (HIP events), with the
+// algorithmic byte count of each launch.
+struct KernelStat {
+  double ms = 0;
+  long launches = 0;
+  double bytes = 0;
+};
+
+class Profiler {
+ public:
+  bool enabled = false;
+  KernelStat op32, op64;
+  void record(hipStream_t s, KernelStat* st, double bytes, hipEvent_t* out_pair);
+  void flush();
+  void reset() {
+    flush();
+    op32 = KernelStat{};
+    op64 = KernelStat{};
+  }
+  ~Profiler();
+
+ private:
+  struct Pending {
+    hipEvent_t a, b;
+    KernelStat* st;
+    double bytes;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  hipEvent_t take();
+
+ public:
+  hipEvent_t begin(hipStream_t s);
+  void end(hipStream_t s, hipEvent_t a, KernelStat* st, double bytes);
+};
+
 struct Options {
-  double fp32_tol_rel = 2e-6;   // FP32 LOBPCG locking threshold, relative to omega^2
+  double fp32_tol_rel = 1e-5;   // FP32 LOBPCG locking threshold, relative to omega^2
   double fp32_switch = 1e-6;    // |Δω²|/ω² below which Newton moves to the FP64 phase
   int maxiter_fp64 = 200;       // LOBPCG iteration cap in the FP64 phase
-  int guard = 0;                // extra block columns beyond nev (unused by default)
+  int guard = 2;                // block columns of band i: min(nev, i + 1 + guard); < 0: nev
   uint64_t seed = 0x5eed;
   int fp64_only = 0;            // skip the FP32 phase
 };
@@ -110,6 +146,7 @@ class Solver {
   DevMem last_evecs;                // FP64, nev columns of ld complex
   int last_nev = 0;
   DevMem scratch_a, scratch_b, scratch_c;
+  Profiler prof;
 
   template <typename T> PrecisionData<T>& pd();
   void canonicalize_phase(const double* d_k, c64* vecs, long ldv, long istride, int nvec, int mper);

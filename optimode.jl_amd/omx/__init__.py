@@ -26,7 +26,7 @@ from . import _lib
 from ._lib import OmxError, OmxInfo, OMX_PREC_FP32, OMX_PREC_FP64  # noqa: F401
 
 __all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "rrule_solve_k", "k_guess",
-           "OmxError", "library_path", "exported_symbols"]
+           "OmxError", "library_path", "exported_symbols", "kernel_stats"]
 
 
 def library_path():
@@ -102,7 +102,8 @@ class HIPSolver:
     and ``fp32_switch`` (|Δω²|/ω² at which Newton moves to FP64).
     """
 
-    def __init__(self, device=0, precision="mixed", fp32_tol=None, fp32_switch=None, maxiter_fp64=None, seed=None):
+    def __init__(self, device=0, precision="mixed", fp32_tol=None, fp32_switch=None, maxiter_fp64=None, seed=None,
+                 profile=False):
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         self.device = int(device)
@@ -117,6 +118,7 @@ class HIPSolver:
         if seed is not None:
             self.options["seed"] = float(seed)
         self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
+        self.profile = bool(profile)
         self._h = None
 
     def handle(self, grid, epsinv):
@@ -131,10 +133,25 @@ class HIPSolver:
         L = _lib.lib()
         for k, v in self.options.items():
             _lib.check(L.omx_set_option(h.h, k.encode(), v), h.h)
+        if self.profile and not getattr(h, "profiling", False):
+            _lib.check(L.omx_set_option(h.h, b"profile", 1.0), h.h)
+            h.profiling = True
         return h
 
     def __repr__(self):
         return f"HIPSolver(device={self.device}, precision={self.precision!r})"
+
+
+def kernel_stats(solver, family):
+    """``(total_ms, launches, algorithmic_bytes)`` of a kernel family timed live on the
+    handle's stream (requires ``profile=True`` on the solver)."""
+    h = solver._h
+    if h is None:
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "solver has no handle yet")
+    ms, n, b = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+    _lib.check(_lib.lib().omx_get_kernel_stats(h.h, family.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                               ctypes.byref(b)), h.h)
+    return ms.value, n.value, b.value
 
 
 def _cols(H, n2):

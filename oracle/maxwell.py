@@ -12,7 +12,12 @@ The forward transform is NumPy's ``fft2`` (FFTW ``fft``: exponent ``-2*pi*i``), 
 backward one the unnormalised inverse (FFTW ``bfft``) -- the same pair the reference
 plans at ``lib/MaxwellEigenmodes/src/maxwell.jl:940-941``.
 """
+import os
+
 import numpy as np
+import scipy.fft as _sfft
+
+_WORKERS = int(os.environ.get("ORACLE_FFT_WORKERS", str(os.cpu_count() or 1)))
 
 __all__ = [
     "Grid", "my_fftfreq", "gvec", "k_guess", "mag_mn", "kx_tc", "kx_ct", "zx_tc",
@@ -114,17 +119,16 @@ def to_flat(Hg):
 
 
 def fft2(a):
-    return np.fft.fft2(a, axes=(-2, -1))
+    return _sfft.fft2(a, axes=(-2, -1), workers=_WORKERS)
 
 
 def bfft2(a):
     """Unnormalised inverse transform (FFTW ``bfft``)."""
-    n = a.shape[-1] * a.shape[-2]
-    return np.fft.ifft2(a, axes=(-2, -1)) * n
+    return _sfft.ifft2(a, axes=(-2, -1), norm="forward", workers=_WORKERS)
 
 
 def ifft2(a):
-    return np.fft.ifft2(a, axes=(-2, -1))
+    return _sfft.ifft2(a, axes=(-2, -1), workers=_WORKERS)
 
 
 def kx_tc(H, m, n, mag):
@@ -158,13 +162,32 @@ def epsinv_ave(epsinv):
     return 3.0 / (epsinv[0, 0] + epsinv[1, 1] + epsinv[2, 2])
 
 
+def _block_grid(Hflat, grid):
+    """(2N, m) block -> (m, 2, Nx, Ny)."""
+    m = Hflat.shape[1]
+    return np.asarray(Hflat).T.reshape((m, grid.Ny, grid.Nx, 2)).transpose(0, 3, 2, 1)
+
+
+def _block_flat(Hg):
+    """(m, 2, Nx, Ny) -> (2N, m)."""
+    m = Hg.shape[0]
+    return np.ascontiguousarray(Hg.transpose(0, 3, 2, 1)).reshape(m, -1).T
+
+
 def apply_M(Hflat, k, epsinv, grid, basis=None):
     """Helmholtz operator ``_M!`` (``maxwell.jl:471-481``): ``kx_tc -> fft -> eid ->
     bfft -> kx_ct * (1/N)``.  ``Hflat`` may be a vector (2N) or a matrix (2N, m)."""
     Hflat = np.asarray(Hflat)
-    if Hflat.ndim == 2:
-        return np.stack([apply_M(Hflat[:, j], k, epsinv, grid, basis) for j in range(Hflat.shape[1])], 1)
     mag, m, n = basis if basis is not None else mag_mn(k, grid)
+    if Hflat.ndim == 2:
+        H = _block_grid(Hflat, grid)
+        d = (H[:, 1, None] * m[None] - H[:, 0, None] * n[None]) * mag[None, None]
+        d = fft2(d)
+        e = bfft2(np.einsum("baxy,jbxy->jaxy", epsinv, d))
+        en = (e * n[None]).sum(1)
+        em = (e * m[None]).sum(1)
+        out = np.stack([-en * mag[None], em * mag[None]], 1) / grid.N
+        return _block_flat(out)
     H = to_grid(Hflat, grid)
     d = fft2(kx_tc(H, m, n, mag))
     e = bfft2(eid(epsinv, d))
@@ -177,10 +200,15 @@ def apply_P(Hflat, k, epsinv, grid, basis=None):
     ``:448-455``) -> ``fft`` -> ``H1 = (d.n)/(mag N)``, ``H2 = -(d.m)/(mag N)``
     (``kxinv_ct!`` ``:439-446``)."""
     Hflat = np.asarray(Hflat)
-    if Hflat.ndim == 2:
-        return np.stack([apply_P(Hflat[:, j], k, epsinv, grid, basis) for j in range(Hflat.shape[1])], 1)
     mag, m, n = basis if basis is not None else mag_mn(k, grid)
     inv_mag = 1.0 / mag
+    if Hflat.ndim == 2:
+        H = _block_grid(Hflat, grid)
+        e = (H[:, 0, None] * n[None] - H[:, 1, None] * m[None]) * inv_mag[None, None]
+        d = fft2(epsinv_ave(epsinv)[None, None] * bfft2(e))
+        scale = inv_mag / grid.N
+        out = np.stack([(d * n[None]).sum(1) * scale[None], -(d * m[None]).sum(1) * scale[None]], 1)
+        return _block_flat(out)
     H = to_grid(Hflat, grid)
     e = (H[0][None] * n - H[1][None] * m) * inv_mag[None]
     e = bfft2(e)

@@ -19,7 +19,7 @@ def solve_omega2(k, epsinv, grid, H0, tol=1e-8, maxiter=100, n_conv_check=None, 
 
 
 def solve_k_single(omega, epsinv, grid, state, eigind, nev, k_tol=1e-8, eig_tol=1e-8,
-                   max_eigsolves=60, maxiter=100, atol_mode="roots"):
+                   max_eigsolves=60, maxiter=100, n_conv_check=None):
     """One band of ``solve_k``: ``Roots.find_zero(_solve_Δω², k0, Newton(); atol=k_tol,
     maxevals=max_eigsolves)`` (``solve.jl:77-101``).
 
@@ -34,7 +34,7 @@ def solve_k_single(omega, epsinv, grid, state, eigind, nev, k_tol=1e-8, eig_tol=
     iters = []
     for nevals in range(1, max_eigsolves + 1):
         lam, X, info = solve_omega2(x, epsinv, grid, state["H"], tol=eig_tol, maxiter=maxiter,
-                                    seed=nevals)
+                                    n_conv_check=n_conv_check, seed=nevals)
         iters.append(info["niter"])
         state["H"] = X
         state["k"] = x
@@ -51,12 +51,16 @@ def solve_k_single(omega, epsinv, grid, state, eigind, nev, k_tol=1e-8, eig_tol=
 
 
 def solve_k(omega, epsinv, grid, nev=1, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=60, maxiter=100,
-            kguess=None, Hguess=None, seed=0, return_info=False):
+            kguess=None, Hguess=None, seed=0, return_info=False, lock_band_only=False):
     """``solve_k(ω, ε⁻¹, grid, DFTK_LOBPCG(); nev, ...)`` (``solve.jl:182-196`` ->
     ``:104-115``): start at ``k_guess`` (or ``kguess``) with a random ``(2N, nev)`` block
     (``ModeSolver``, ``maxwell.jl:956-972``; columns replaced by ``Hguess``,
     ``solve.jl:201-210``), run one Newton solve per band, each warm-started from the
-    previous band's ``k`` and block, and canonicalise every eigenvector's phase."""
+    previous band's ``k`` and block, and canonicalise every eigenvector's phase.
+
+    ``lock_band_only=True`` stops each eigensolve once bands ``1..eigind`` are locked
+    (DFTK's ``n_conv_check``) instead of all ``nev``; the converged pairs are the same,
+    only the work differs."""
     rng = np.random.default_rng(seed)
     n2 = 2 * grid.N
     H = rng.standard_normal((n2, nev)) + 1j * rng.standard_normal((n2, nev))
@@ -72,7 +76,8 @@ def solve_k(omega, epsinv, grid, nev=1, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=
     kmags, evecs, infos = [], [], []
     for eigind in range(nev):
         k, ev, nsolves, iters = solve_k_single(omega, epsinv, grid, state, eigind, nev, k_tol, eig_tol,
-                                               max_eigsolves, maxiter)
+                                               max_eigsolves, maxiter,
+                                               n_conv_check=(eigind + 1) if lock_band_only else None)
         kmags.append(k)
         evecs.append(canonicalize_phase(ev, k, epsinv, grid))
         infos.append(dict(n_eigsolves=nsolves, lobpcg_iters=iters))
