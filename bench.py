@@ -27,6 +27,10 @@ HBM_PEAK_GBS = 8000.0
 OMEGA = 1 / 1.55
 NEV = 8
 REL_TOL = 1e-8
+# HBM bytes per algorithmic byte of the FP32 operator application on this workload, from
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of k_pass1/2/3 (FETCH scaled by 2.0, WRITE by 1.0: the
+# factors a 256 MiB copy kernel reads back on the same box).  Update when the kernels change.
+OP_TRAFFIC_RATIO = 4.88
 
 
 def cpu_baseline(seconds_cap=60.0):
@@ -105,7 +109,8 @@ def main():
     if op_n > 0 and op_ms > 0:
         achieved = op_bytes / (op_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": round(OP_TRAFFIC_RATIO * op_bytes / op_n),
                     "kernel": "FP32 Helmholtz operator application (3 fused passes)",
                     "launches": op_n, "avg_us": round(1e3 * op_ms / op_n, 2),
                     "avg_alg_bytes": round(op_bytes / op_n)}

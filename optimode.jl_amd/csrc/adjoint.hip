@@ -1,11 +1,494 @@
-// Adjoint (pull-back) of solve_k: placeholder until the GPU adjoint lands.
+// Pull-back of rrule(solve_k) on the GPU, FP64 throughout (lib/MaxwellEigenmodes/src/
+// solve.jl:213-289; device twin solvers/gpu.jl:292-365).  Per band i with primal (k_i, ev_i)
+// and cotangents (kbar_i, evbar_i):
+//   slope = 2 HMkH(ev)
+//   if evbar != 0:
+//     lambda: (M - w^2) lambda = evbar - ev <ev, evbar>, lambda ⟂ ev  (eig_adjt,
+//             grads/solve.jl:96-113) -- here preconditioned MINRES on the complement of ev
+//             (M - w^2 is Hermitian, singular along ev and indefinite above band 1)
+//     d = fft(kx_tc(ev))/N, ld = fft(kx_tc(lambda)); epsbar += epsinv_bar(d, ld)
+//     le = bfft(eps^-1 ld / N), e = bfft(eps^-1 d); khbar = -grad_k(mag, m, n)(...)
+//   s = (kbar + khbar) / slope; epsbar += epsinv_bar(d, fft(kx_tc(s ev))); wbar += 2 w s
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "solver.hpp"
 
 namespace omx {
 
-void Solver::adjoint(double, int, const double*, const double*, const double*, const double*, const uint8_t*, double*,
-                     double*) {
-  throw Error(OMX_ERR_UNSUPPORTED, "solve_k adjoint is not implemented yet");
+namespace {
+
+constexpr int AB = 256;
+
+// out[(i*nb + b)] = Re sum_rows conj(a_i) b_i over block b's rows (vector i at base + i*ld)
+__global__ __launch_bounds__(AB) void k_vdot(const c64* A, const c64* B, long ld, long K, const int* act,
+                                             double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double s = 0;
+  if (act[i]) {
+    const c64* a = A + i * ld;
+    const c64* b = B + i * ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB)
+      s += a[r].x * b[r].x + a[r].y * b[r].y;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// complex dot <a_i, b_i> = sum conj(a) b, partials (re, im)
+__global__ __launch_bounds__(AB) void k_cdot(const c64* A, const c64* B, long ld, long K, const int* act,
+                                             c64* part) {
+  const int i = blockIdx.y;
+  __shared__ double rr[AB], ri[AB];
+  double sr = 0, si = 0;
+  if (act[i]) {
+    const c64* a = A + i * ld;
+    const c64* b = B + i * ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      sr += a[r].x * b[r].x + a[r].y * b[r].y;
+      si += a[r].x * b[r].y - a[r].y * b[r].x;
+    }
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) {
+      rr[threadIdx.x] += rr[threadIdx.x + t];
+      ri[threadIdx.x] += ri[threadIdx.x + t];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = c64{rr[0], ri[0]};
+}
+
+// out_i = sum_j c[i*4 + j] * in_j_i for up to 4 inputs (complex coefficients); out may
+// alias an input (each element is read before it is written by the same thread)
+struct Lin {
+  const c64* in[4];
+  int nin;
+  c64* out;
+  const c64* coef;  // per instance, 4 complex
+  long ld;
+};
+
+__global__ __launch_bounds__(AB) void k_lincomb(Lin L, long K, const int* act) {
+  const int i = blockIdx.y;
+  if (!act[i]) return;
+  const c64* c = L.coef + 4 * i;
+  const c64 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+  for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+    const long o = i * L.ld + r;
+    c64 s{0, 0};
+    const c64 cs[4] = {c0, c1, c2, c3};
+    for (int j = 0; j < L.nin; ++j) {
+      const c64 v = L.in[j][o];
+      s.x += cs[j].x * v.x - cs[j].y * v.y;
+      s.y += cs[j].x * v.y + cs[j].y * v.x;
+    }
+    L.out[o] = s;
+  }
+}
+
+struct MinresState {
+  double beta1, beta, oldb, dbar, epsln, phibar, cs, sn, alfa, phi, delta, oldeps, denom, rtol, shift;
+  int itn, done, active, pad;
+};
+
+double sum_part(const double* p, int nb) {
+  double s = 0;
+  for (int b = 0; b < nb; ++b) s += p[b];
+  return s;
+}
+
+// Scalar recurrences of MINRES (Paige & Saunders 1975) per instance; `stage` selects the
+// step of the iteration they follow.  Coefficients for the next lincomb go to coef.
+__global__ void k_minres_scalars(MinresState* ms, const double* part, int nb, c64* coef, int stage, int ninst,
+                                 const int* act) {
+  for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
+    MinresState& m = ms[i];
+    c64* c = coef + 4 * i;
+    for (int j = 0; j < 4; ++j) c[j] = c64{0, 0};
+    if (!act[i]) continue;
+    double d = 0;
+    for (int b = 0; b < nb; ++b) d += part[long(i) * nb + b];
+    if (stage == 0) {
+      // init: beta1 = sqrt(<r1, M r1>)
+      m.beta1 = sqrt(fmax(d, 0.0));
+      m.beta = m.beta1;
+      m.oldb = 0;
+      m.dbar = 0;
+      m.epsln = 0;
+      m.phibar = m.beta1;
+      m.cs = -1;
+      m.sn = 0;
+      m.itn = 0;
+      m.done = m.beta1 == 0 ? 1 : 0;
+      // next: v = y / beta
+      c[0] = c64{m.done ? 0.0 : 1.0 / m.beta, 0};
+    } else if (stage == 1) {
+      // after y = A v: y <- y - shift v - (beta/oldb) r1 ; inputs (y, v, r1)
+      c[0] = c64{1, 0};
+      c[1] = c64{-m.shift, 0};
+      c[2] = c64{m.itn >= 1 ? -m.beta / m.oldb : 0.0, 0};
+    } else if (stage == 2) {
+      // alfa = <v, y>; y <- y - (alfa/beta) r2 ; inputs (y, r2)
+      m.alfa = d;
+      c[0] = c64{1, 0};
+      c[1] = c64{-m.alfa / m.beta, 0};
+    } else if (stage == 3) {
+      // beta = sqrt(<r2, M r2>); rotations; w and x updates
+      m.itn += 1;
+      m.oldb = m.beta;
+      m.beta = sqrt(fmax(d, 0.0));
+      m.oldeps = m.epsln;
+      m.delta = m.cs * m.dbar + m.sn * m.alfa;
+      const double gbar = m.sn * m.dbar - m.cs * m.alfa;
+      m.epsln = m.sn * m.beta;
+      m.dbar = -m.cs * m.beta;
+      double gamma = sqrt(gbar * gbar + m.beta * m.beta);
+      gamma = fmax(gamma, 2.220446049250313e-16);
+      m.cs = gbar / gamma;
+      m.sn = m.beta / gamma;
+      m.phi = m.cs * m.phibar;
+      m.phibar = m.sn * m.phibar;
+      m.denom = 1.0 / gamma;
+      // w_new = (v - oldeps w1 - delta w2) denom ; inputs (v, w1, w2)
+      c[0] = c64{m.denom, 0};
+      c[1] = c64{-m.oldeps * m.denom, 0};
+      c[2] = c64{-m.delta * m.denom, 0};
+      if (m.phibar <= m.rtol * m.beta1 || m.beta == 0) m.done = 1;
+    } else if (stage == 4) {
+      // x += phi w ; inputs (x, w)
+      c[0] = c64{1, 0};
+      c[1] = c64{m.phi, 0};
+    } else if (stage == 5) {
+      // next v = y / beta
+      c[0] = c64{m.beta > 0 ? 1.0 / m.beta : 0.0, 0};
+    }
+  }
+}
+
+// epsbar (6 per pixel: xx, yy, zz, xy, xz, yz) += sum over instances of epsinv_bar(d, c_i * ld)
+// with the 2D mirrored convention (grads/solve.jl:490-515).  self_mode: ld = d.
+__global__ __launch_bounds__(AB) void k_epsbar(const c64* Dd, const c64* Dl, long n3, long N, int ninst, const int* act,
+                                               const double* cfac, int self_mode, double* eb6) {
+  for (long p = long(blockIdx.x) * AB + threadIdx.x; p < N; p += long(gridDim.x) * AB) {
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < ninst; ++i) {
+      if (!act[i]) continue;
+      const c64* d = Dd + i * n3 + 3 * p;
+      const c64* l = (self_mode ? Dd : Dl) + i * n3 + 3 * p;
+      const double c = cfac ? cfac[i] : 1.0;
+      // -Re(l_a conj d_a)
+      for (int q = 0; q < 3; ++q) a[q] -= c * (l[q].x * d[q].x + l[q].y * d[q].y);
+      // (i,j): -Re(l_j conj d_i + l_i conj d_j)
+      const int ii[3] = {0, 0, 1}, jj[3] = {1, 2, 2};
+      for (int q = 0; q < 3; ++q) {
+        const c64 li = l[ii[q]], lj = l[jj[q]], di = d[ii[q]], dj = d[jj[q]];
+        a[3 + q] -= c * ((lj.x * di.x + lj.y * di.y) + (li.x * dj.x + li.y * dj.y));
+      }
+    }
+    for (int q = 0; q < 6; ++q) eb6[6 * p + q] += a[q];
+  }
+}
+
+// k̄ from the basis sensitivities (maxwell.jl:560-566 ∇ₖmag_m_n with dk = z, as used at
+// solve.jl:258-270): per G, kx̄_m = Re(le conj(ev2) + e conj(l2)), kx̄_n = -Re(le conj(ev1)
+// + e conj(l1)), māg = n.kx̄_n + m.kx̄_m; partial sums of -(k̄_mag + k̄_m + k̄_n).
+__global__ __launch_bounds__(AB) void k_khbar(const c64* LE, const c64* EE, const c64* EV, const c64* LA, long n3,
+                                              long ld, int nx, int ny, double invdx, double invdy,
+                                              const double* kinst, const int* act, double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double acc = 0;
+  if (act[i]) {
+    const double k = kinst[i];
+    const long N = long(nx) * ny;
+    for (long p = long(blockIdx.x) * AB + threadIdx.x; p < N; p += long(gridDim.x) * AB) {
+      const int ix = int(p % nx), iy = int(p / nx);
+      const double gx = double(fft_index(ix, nx)) * invdx, gy = double(fft_index(iy, ny)) * invdy;
+      const double wx = -gx, wy = -gy, wz = k;
+      const double mag = sqrt(wx * wx + wy * wy + wz * wz);
+      double n0, n1;
+      const double rho2 = wx * wx + wy * wy;
+      if (rho2 > 0) {
+        const double ir = 1.0 / sqrt(rho2);
+        n0 = -wy * ir;
+        n1 = wx * ir;
+      } else {
+        n0 = 0;
+        n1 = 1;
+      }
+      const double n2 = 0;
+      double m0 = n1 * wz - n2 * wy, m1 = n2 * wx - n0 * wz, m2 = n0 * wy - n1 * wx;
+      const double im = 1.0 / sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+      m0 *= im;
+      m1 *= im;
+      m2 *= im;
+      const c64* le = LE + i * n3 + 3 * p;
+      const c64* ee = EE + i * n3 + 3 * p;
+      const c64 e1 = EV[i * ld + 2 * p], e2 = EV[i * ld + 2 * p + 1];
+      const c64 l1 = LA[i * ld + 2 * p], l2 = LA[i * ld + 2 * p + 1];
+      double km[3], kn[3];
+      for (int a = 0; a < 3; ++a) {
+        // Re(x conj(y)) = x.re y.re + x.im y.im
+        km[a] = (le[a].x * e2.x + le[a].y * e2.y) + (ee[a].x * l2.x + ee[a].y * l2.y);
+        kn[a] = -((le[a].x * e1.x + le[a].y * e1.y) + (ee[a].x * l1.x + ee[a].y * l1.y));
+      }
+      const double magbar = n0 * kn[0] + n1 * kn[1] + n2 * kn[2] + m0 * km[0] + m1 * km[1] + m2 * km[2];
+      // kpg_over_mag = (m x n) / mag
+      const double c0 = (m1 * n2 - m2 * n1) / mag, c1 = (m2 * n0 - m0 * n2) / mag, c2 = (m0 * n1 - m1 * n0) / mag;
+      double t = magbar * c2 * mag;
+      // q = kpg_over_mag x z = (c1, -c0, 0)
+      const double q0 = c1, q1 = -c0, q2 = 0;
+      const double mq0 = m1 * q2 - m2 * q1, mq1 = m2 * q0 - m0 * q2, mq2 = m0 * q1 - m1 * q0;
+      const double nq0 = n1 * q2 - n2 * q1, nq1 = n2 * q0 - n0 * q2, nq2 = n0 * q1 - n1 * q0;
+      t -= mag * (km[0] * mq0 + km[1] * mq1 + km[2] * mq2);
+      t -= mag * (kn[0] * nq0 + kn[1] * nq1 + kn[2] * nq2);
+      acc -= t;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = AB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+void Solver::adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
+                     const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar) {
+  if (nev < 1 || nev > 64) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..64");
+  if (!kmags || !evecs || !omegabar || !epsinvbar) throw Error(OMX_ERR_INVALID_ARG, "NULL argument");
+  if (!(omega > 0) || !std::isfinite(omega)) throw Error(OMX_ERR_INVALID_ARG, "omega must be positive and finite");
+  for (int i = 0; i < nev; ++i)
+    if (!std::isfinite(kmags[i])) throw Error(OMX_ERR_INVALID_ARG, "kmags must be finite");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  hipStream_t s = stream;
+  const int nb = nev;
+  const long n3 = 3 * N;
+  const int nblk = int(std::min<long>(256, (K + AB - 1) / AB));
+  DevMem dk(sizeof(double) * nb), EV(sizeof(c64) * nb * ld), EB(sizeof(c64) * nb * ld), LA(sizeof(c64) * nb * ld),
+      Dd(sizeof(c64) * nb * n3), Dl(sizeof(c64) * nb * n3), eb6(sizeof(double) * 6 * N), actd(sizeof(int) * nb),
+      part(sizeof(c64) * nb * nblk), coef(sizeof(c64) * 4 * nb), cfac(sizeof(double) * nb);
+  OMX_HIP_CHECK(hipMemsetAsync(eb6.as<void>(), 0, sizeof(double) * 6 * N, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(dk.as<void>(), kmags, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  for (int i = 0; i < nb; ++i)
+    OMX_HIP_CHECK(hipMemcpyAsync(EV.as<c64>() + i * ld, evecs + 2 * size_t(i) * K, sizeof(c64) * K,
+                                 hipMemcpyHostToDevice, s));
+  std::vector<int> all(nb, 1), hasv(nb, 0);
+  bool any_ev = false;
+  if (evbar) {
+    for (int i = 0; i < nb; ++i) {
+      hasv[i] = evbar_mask ? (evbar_mask[i] != 0) : 1;
+      any_ev = any_ev || hasv[i];
+    }
+  }
+  const int dps = ny / p64.plan.rows_per_block;
+  // slope = 2 HMkH(ev) for every band
+  DevMem hdots(sizeof(double) * nb * dps);
+  VecSel sel1{nb, 1, nullptr, nullptr};
+  BView<double> EVv{EV.as<c64>(), ld, ld};
+  launch_operator<double>(OP_MKH, p64.plan, EVv, EVv, Dd.as<c64>(), sel1, dk.as<double>(), hdots.as<double>(), dps, s);
+  std::vector<double> hh(size_t(nb) * dps);
+  OMX_HIP_CHECK(hipMemcpyAsync(hh.data(), hdots.as<void>(), sizeof(double) * hh.size(), hipMemcpyDeviceToHost, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<double> slope(nb), khb(nb, 0.0);
+  for (int i = 0; i < nb; ++i) slope[i] = 2.0 * sum_part(hh.data() + size_t(i) * dps, dps);
+
+  auto vdot = [&](const c64* A, const c64* B, const int* act) {
+    hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
+  };
+  auto lincomb = [&](std::initializer_list<const c64*> ins, c64* outp, const int* act) {
+    Lin L{};
+    int j = 0;
+    for (auto p : ins) L.in[j++] = p;
+    L.nin = j;
+    L.out = outp;
+    L.coef = coef.as<c64>();
+    L.ld = ld;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblk, nb), dim3(AB), 0, s, L, K, act);
+  };
+  // complex projection out = x - ev <ev, x> for active instances (coef set on host per call)
+  auto project = [&](c64* x, const int* act) {
+    hipLaunchKernelGGL(k_cdot, dim3(nblk, nb), dim3(AB), 0, s, EV.as<c64>(), x, ld, K, act, part.as<c64>());
+    std::vector<c64> hp(size_t(nb) * nblk);
+    OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), part.as<void>(), sizeof(c64) * hp.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<c64> hc(4 * size_t(nb), c64{0, 0});
+    for (int i = 0; i < nb; ++i) {
+      c64 d{0, 0};
+      for (int b = 0; b < nblk; ++b) {
+        d.x += hp[size_t(i) * nblk + b].x;
+        d.y += hp[size_t(i) * nblk + b].y;
+      }
+      hc[4 * i] = c64{1, 0};
+      hc[4 * i + 1] = c64{-d.x, -d.y};
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(coef.as<void>(), hc.data(), sizeof(c64) * hc.size(), hipMemcpyHostToDevice, s));
+    lincomb({x, EV.as<c64>()}, x, act);
+  };
+
+  if (any_ev) {
+    OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), hasv.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+    const int* act = actd.as<int>();
+    for (int i = 0; i < nb; ++i)
+      if (hasv[i])
+        OMX_HIP_CHECK(hipMemcpyAsync(EB.as<c64>() + i * ld, evbar + 2 * size_t(i) * K, sizeof(c64) * K,
+                                     hipMemcpyHostToDevice, s));
+    // ---- MINRES on (M - w^2) restricted to the complement of ev ----
+    DevMem X_(sizeof(c64) * nb * ld), R1(sizeof(c64) * nb * ld), R2(sizeof(c64) * nb * ld), Y(sizeof(c64) * nb * ld),
+        V(sizeof(c64) * nb * ld), W(sizeof(c64) * nb * ld), W1(sizeof(c64) * nb * ld), W2(sizeof(c64) * nb * ld),
+        AV(sizeof(c64) * nb * ld), Dm(sizeof(c64) * nb * n3), msd(sizeof(MinresState) * nb);
+    c64 *x = X_.as<c64>(), *r1 = R1.as<c64>(), *r2 = R2.as<c64>(), *y = Y.as<c64>(), *v = V.as<c64>(),
+        *w = W.as<c64>(), *w1 = W1.as<c64>(), *w2 = W2.as<c64>();
+    OMX_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nb * ld, s));
+    OMX_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(c64) * nb * ld, s));
+    OMX_HIP_CHECK(hipMemsetAsync(w2, 0, sizeof(c64) * nb * ld, s));
+    OMX_HIP_CHECK(hipMemsetAsync(w1, 0, sizeof(c64) * nb * ld, s));
+    std::vector<MinresState> hm(nb);
+    for (int i = 0; i < nb; ++i) {
+      std::memset(&hm[i], 0, sizeof(MinresState));
+      hm[i].rtol = 1e-10;
+      hm[i].shift = omega * omega;
+      hm[i].active = hasv[i];
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(msd.as<void>(), hm.data(), sizeof(MinresState) * nb, hipMemcpyHostToDevice, s));
+    MinresState* ms = msd.as<MinresState>();
+    // r1 = evbar - ev <ev, evbar>
+    OMX_HIP_CHECK(hipMemcpyAsync(r1, EB.as<c64>(), sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+    project(r1, act);
+    BView<double> Vv{v, ld, ld}, AVv{AV.as<c64>(), ld, ld}, R2v{r2, ld, ld}, Yv{y, ld, ld};
+    VecSel selA{nb, 1, nullptr, nullptr};
+    // y = P r1 (projected)
+    BView<double> R1v{r1, ld, ld};
+    launch_operator<double>(OP_P, p64.plan, R1v, Yv, Dm.as<c64>(), selA, dk.as<double>(), nullptr, 0, s);
+    project(y, act);
+    vdot(r1, y, act);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 0, nb,
+                       act);
+    OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+    std::vector<int> live(hasv);
+    DevMem lived(sizeof(int) * nb);
+    const int maxit = 1000;
+    int it = 0;
+    for (; it < maxit; ++it) {
+      const int* la = lived.as<int>();
+      if (it % 4 == 0) {
+        OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+        OMX_HIP_CHECK(hipStreamSynchronize(s));
+        bool anyl = false;
+        for (int i = 0; i < nb; ++i) {
+          live[i] = hasv[i] && !hm[i].done;
+          anyl = anyl || live[i];
+        }
+        if (!anyl) break;
+        OMX_HIP_CHECK(hipMemcpyAsync(lived.as<void>(), live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+      }
+      // v = y / beta (coefficients left by the previous scalar step)
+      lincomb({y}, v, la);
+      BView<double> Vb{v, ld, ld};
+      launch_operator<double>(OP_M, p64.plan, Vb, AVv, Dm.as<c64>(), VecSel{nb, 1, nullptr, nullptr}, dk.as<double>(),
+                              nullptr, 0, s);
+      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 1, nb,
+                         la);
+      lincomb({AV.as<c64>(), v, r1}, y, la);
+      vdot(v, y, la);
+      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 2, nb,
+                         la);
+      lincomb({y, r2}, y, la);
+      std::swap(r1, r2);  // r1 <- r2
+      std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
+      BView<double> R2b{r2, ld, ld}, Yb{y, ld, ld};
+      launch_operator<double>(OP_P, p64.plan, R2b, Yb, Dm.as<c64>(), VecSel{nb, 1, nullptr, nullptr}, dk.as<double>(),
+                              nullptr, 0, s);
+      project(y, la);
+      vdot(r2, y, la);
+      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 3, nb,
+                         la);
+      // w_new = (v - oldeps w1 - delta w2) denom, rotating w1 <- w2 <- w <- new
+      // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
+      // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
+      lincomb({v, w2, w}, w1, la);
+      {
+        c64* wn = w1;
+        w1 = w2;
+        w2 = w;
+        w = wn;
+      }
+      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 4, nb,
+                         la);
+      lincomb({x, w}, x, la);
+      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 5, nb,
+                         la);
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < nb; ++i)
+      if (hasv[i] && !hm[i].done)
+        throw Error(OMX_ERR_NOT_CONVERGED, "adjoint MINRES did not converge within 1000 iterations");
+    // lambda = x, orthogonalised against ev (twice, as eig_adjt and the rrule both do)
+    OMX_HIP_CHECK(hipMemcpyAsync(LA.as<c64>(), x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+    project(LA.as<c64>(), act);
+    project(LA.as<c64>(), act);
+    // d = fft(kx_tc(ev))/N, ld = fft(kx_tc(lambda)); epsbar += epsinv_bar(d, ld)
+    BView<double> LAv{LA.as<c64>(), ld, ld};
+    launch_fields<double>(FIELD_D, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);
+    launch_fields<double>(FIELD_D, p64.plan, LAv, Dl.as<c64>(), sel1, dk.as<double>(), 1.0, s);
+    hipLaunchKernelGGL(k_epsbar, dim3(256), dim3(AB), 0, s, Dd.as<c64>(), Dl.as<c64>(), n3, N, nb, act, nullptr, 0,
+                       eb6.as<double>());
+    // le = bfft(eps^-1 ld / N), e = bfft(eps^-1 d)  (in place)
+    launch_fields<double>(FIELD_EINV, p64.plan, LAv, Dl.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);
+    launch_fields<double>(FIELD_EINV, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0, s);
+    hipLaunchKernelGGL(k_khbar, dim3(nblk, nb), dim3(AB), 0, s, Dl.as<c64>(), Dd.as<c64>(), EV.as<c64>(), LA.as<c64>(),
+                       n3, ld, nx, ny, 1.0 / Dx, 1.0 / Dy, dk.as<double>(), act, part.as<double>());
+    std::vector<double> hp(size_t(nb) * nblk);
+    OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), part.as<void>(), sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < nb; ++i)
+      if (hasv[i]) khb[i] = sum_part(hp.data() + size_t(i) * nblk, nblk);
+  }
+  // eigenvalue channel: lambda = ((kbar + khbar)/slope) ev -> epsbar += epsinv_bar(d, s N d)
+  std::vector<double> cf(nb);
+  double wb = 0;
+  for (int i = 0; i < nb; ++i) {
+    const double kb = kbar ? kbar[i] : 0.0;
+    const double sc = (kb + khb[i]) / slope[i];
+    cf[i] = sc * double(N);
+    wb += 2.0 * omega * sc;
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(cfac.as<void>(), cf.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), all.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+  launch_fields<double>(FIELD_D, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);
+  hipLaunchKernelGGL(k_epsbar, dim3(256), dim3(AB), 0, s, Dd.as<c64>(), Dd.as<c64>(), n3, N, nb, actd.as<int>(),
+                     cfac.as<double>(), 1, eb6.as<double>());
+  OMX_HIP_CHECK(hipGetLastError());
+  std::vector<double> h6(6 * size_t(N));
+  OMX_HIP_CHECK(hipMemcpyAsync(h6.data(), eb6.as<void>(), sizeof(double) * h6.size(), hipMemcpyDeviceToHost, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  // (3,3,Nx,Ny) column-major, symmetric
+  for (long p = 0; p < N; ++p) {
+    const double* a = h6.data() + 6 * p;
+    double* o = epsinvbar + 9 * p;
+    o[0] = a[0];
+    o[4] = a[1];
+    o[8] = a[2];
+    o[3] = o[1] = a[3];
+    o[6] = o[2] = a[4];
+    o[7] = o[5] = a[5];
+  }
+  *omegabar = wb;
 }
 
 }  // namespace omx

@@ -82,7 +82,7 @@ void Solver::canonicalize_phase(const double* d_k, c64* vecs, long ldv, long ist
   DevMem Dm(sizeof(c64) * size_t(nvec) * n3);
   VecSel sel{nvec / mper, mper, nullptr, nullptr};
   BView<double> X{vecs, ldv, istride};
-  launch_efield<double>(p64.plan, X, Dm.as<c64>(), sel, d_k, stream);
+  launch_fields<double>(FIELD_E, p64.plan, X, Dm.as<c64>(), sel, d_k, 1.0, stream);
   const int nb = 64;
   DevMem pv(sizeof(double) * nvec * nb), pi(sizeof(long) * nvec * nb);
   hipLaunchKernelGGL(k_argmax_partial, dim3(nb, nvec), dim3(FB), 0, stream, Dm.as<c64>(), n3, pv.as<double>(),

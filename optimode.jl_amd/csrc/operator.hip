@@ -237,10 +237,11 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass3(OpPlan<T> plan, BView<T> Hin
   }
 }
 
-// E field of the mode: pass 1 (kx_tc + FFT along x) then FFT along y and eps^-1 . d,
-// left in D as E[v][iy][ix][c] -- i.e. fields.jl:197 ``ε⁻¹_dot(fft(kx_tc(H)))``.
-template <typename T, int VPT>
-__global__ __launch_bounds__(OP_BLOCK) void k_pass2_efield(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+// Forward half-pass along y for the field helpers: FFT along y, optionally eps^-1 . d,
+// times `scale`, left in D as [v][iy][ix][c].  With pass 1 in front this is
+// fields.jl:197 ``ε⁻¹_dot(fft(kx_tc(H)))`` (EPS) or ``_H2d!`` (fields.jl:81-86, no EPS).
+template <typename T, int VPT, bool EPS>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass2_fwd(OpPlan<T> plan, cplx<T>* D, VecSel sel, T scale) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   const int nx = plan.nx, ny = plan.ny, TX = plan.cols_per_block;
@@ -263,27 +264,108 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass2_efield(OpPlan<T> plan, cplx<
   for (int e = threadIdx.x; e < ny * TX; e += blockDim.x) {
     const int iy = e / TX;
     const int t = e - iy * TX;
-    const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
-    const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
     const cplx<T> d0 = buf[(t * 3 + 0) * ny + iy], d1 = buf[(t * 3 + 1) * ny + iy], d2 = buf[(t * 3 + 2) * ny + iy];
     const long o = (long(iy) * nx + ix0 + t) * 3;
-    Dv[o + 0] = d0 * exx + d1 * exy + d2 * exz;
-    Dv[o + 1] = d0 * exy + d1 * eyy + d2 * eyz;
-    Dv[o + 2] = d0 * exz + d1 * eyz + d2 * ezz;
+    if (EPS) {
+      const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+      const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
+      Dv[o + 0] = (d0 * exx + d1 * exy + d2 * exz) * scale;
+      Dv[o + 1] = (d0 * exy + d1 * eyy + d2 * eyz) * scale;
+      Dv[o + 2] = (d0 * exz + d1 * eyz + d2 * ezz) * scale;
+    } else {
+      Dv[o + 0] = d0 * scale;
+      Dv[o + 1] = d1 * scale;
+      Dv[o + 2] = d2 * scale;
+    }
+  }
+}
+
+// Inverse path of _d2ẽ! (fields.jl:88-93): e = bfft2(eps^-1 . d) * scale, in place in D.
+// Column pass: eps^-1 . d and inverse FFT along y.
+template <typename T, int VPT>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass2_inv(OpPlan<T> plan, cplx<T>* D, VecSel sel, T scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, TX = plan.cols_per_block;
+  cplx<T>* buf = tw + ny;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int ix0 = blockIdx.x * TX;
+  for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = plan.twy[t];
+  cplx<T>* Dv = D + long(v) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < ny * TX; e += blockDim.x) {
+    const int iy = e / TX;
+    const int t = e - iy * TX;
+    const long o = (long(iy) * nx + ix0 + t) * 3;
+    const cplx<T> d0 = Dv[o], d1 = Dv[o + 1], d2 = Dv[o + 2];
+    const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+    const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
+    buf[(t * 3 + 0) * ny + iy] = (d0 * exx + d1 * exy + d2 * exz) * scale;
+    buf[(t * 3 + 1) * ny + iy] = (d0 * exy + d1 * eyy + d2 * eyz) * scale;
+    buf[(t * 3 + 2) * ny + iy] = (d0 * exz + d1 * eyz + d2 * ezz) * scale;
+  }
+  __syncthreads();
+  fft_lds<T, +1, VPT>(buf, 3 * TX, ny, 1, plan.fy, tw);
+  for (int e = threadIdx.x; e < ny * TX * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int iy = q / TX;
+    const int t = q - iy * TX;
+    Dv[(long(iy) * nx + ix0 + t) * 3 + c] = buf[(t * 3 + c) * ny + iy];
+  }
+}
+
+// Row pass of the inverse path: inverse FFT along x, in place.
+template <typename T, int VPT>
+__global__ __launch_bounds__(OP_BLOCK) void k_pass3_inv(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, RB = plan.rows_per_block;
+  cplx<T>* buf = tw + nx;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int iy0 = blockIdx.x * RB;
+  for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = plan.twx[t];
+  cplx<T>* Dv = D + long(v) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < RB * nx * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int r = q / nx;
+    const int ix = q - r * nx;
+    buf[(r * 3 + c) * nx + ix] = Dv[(long(iy0 + r) * nx + ix) * 3 + c];
+  }
+  __syncthreads();
+  fft_lds<T, +1, VPT>(buf, 3 * RB, nx, 1, plan.fx, tw);
+  for (int e = threadIdx.x; e < RB * nx * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int r = q / nx;
+    const int ix = q - r * nx;
+    Dv[(long(iy0 + r) * nx + ix) * 3 + c] = buf[(r * 3 + c) * nx + ix];
   }
 }
 
 template <typename T, int VPT>
-void launch_efield_vpt(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst,
-                       hipStream_t stream) {
+void launch_fields_vpt(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
+                       const double* kinst, T scale, hipStream_t stream) {
   const int nvec = sel.ninst * sel.mper;
   const size_t cs = sizeof(cplx<T>);
   dim3 g1(plan.ny / plan.rows_per_block, nvec);
   dim3 g2(plan.nx / plan.cols_per_block, nvec);
   const size_t lds13 = cs * (size_t(plan.nx) + size_t(plan.rows_per_block) * 3 * plan.nx);
   const size_t lds2 = cs * (size_t(plan.ny) + size_t(plan.cols_per_block) * 3 * plan.ny);
-  hipLaunchKernelGGL((k_pass1<T, OP_M, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
-  hipLaunchKernelGGL((k_pass2_efield<T, VPT>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel);
+  if (fop == FIELD_E || fop == FIELD_D) {
+    hipLaunchKernelGGL((k_pass1<T, OP_M, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
+    if (fop == FIELD_E)
+      hipLaunchKernelGGL((k_pass2_fwd<T, VPT, true>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
+    else
+      hipLaunchKernelGGL((k_pass2_fwd<T, VPT, false>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
+  } else {
+    hipLaunchKernelGGL((k_pass2_inv<T, VPT>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
+    hipLaunchKernelGGL((k_pass3_inv<T, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, D, sel);
+  }
 }
 
 template <typename T, int OP, int VPT>
@@ -326,19 +408,19 @@ void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, c
 }
 
 template <typename T>
-void launch_efield(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst,
-                   hipStream_t stream) {
+void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
+                   const double* kinst, double scale, hipStream_t stream) {
   switch (plan.vpt) {
-    case 8: launch_efield_vpt<T, 8>(plan, X, D, sel, kinst, stream); break;
-    case 16: launch_efield_vpt<T, 16>(plan, X, D, sel, kinst, stream); break;
-    case 32: launch_efield_vpt<T, 32>(plan, X, D, sel, kinst, stream); break;
+    case 8: launch_fields_vpt<T, 8>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
+    case 16: launch_fields_vpt<T, 16>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
+    case 32: launch_fields_vpt<T, 32>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
     default: throw Error(OMX_ERR_INTERNAL, "operator plan has an unsupported values-per-thread bound");
   }
   OMX_HIP_CHECK(hipGetLastError());
 }
 
-template void launch_efield<double>(const OpPlan<double>&, BView<double>, cplx<double>*, const VecSel&, const double*,
-                                    hipStream_t);
+template void launch_fields<double>(FieldOp, const OpPlan<double>&, BView<double>, cplx<double>*, const VecSel&,
+                                    const double*, double, hipStream_t);
 template void launch_operator<float>(OpKind, const OpPlan<float>&, BView<float>, BView<float>, cplx<float>*,
                                      const VecSel&, const double*, double*, int, hipStream_t);
 template void launch_operator<double>(OpKind, const OpPlan<double>&, BView<double>, BView<double>,

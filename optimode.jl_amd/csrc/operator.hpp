@@ -57,10 +57,14 @@ void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, c
                      const VecSel& sel, const double* kinst, double* dot_partial, int dot_partial_stride,
                      hipStream_t stream);
 
-// E = eps^-1 . fft2(kx_tc(H)) per vector into D (layout [v][iy][ix][c], 3N per vector).
+// Field helpers on 3-component fields in D (layout [v][iy][ix][c], 3N per vector):
+//   FIELD_E: D = eps^-1 . fft2(kx_tc(H)) * scale   (E field, fields.jl:194-210)
+//   FIELD_D: D = fft2(kx_tc(H)) * scale             (_H2d!, fields.jl:81-86)
+//   FIELD_EINV: D <- bfft2(eps^-1 . D) * scale      (_d2ẽ!, fields.jl:88-93), X unused
+enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2 };
 template <typename T>
-void launch_efield(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst,
-                   hipStream_t stream);
+void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
+                   const double* kinst, double scale, hipStream_t stream);
 
 // Host helper: factor L into radices {8,4,3,5,2} (no lone factor 2 unless L is 2*odd).
 bool fft_factor(int L, FFTFactors& f);
