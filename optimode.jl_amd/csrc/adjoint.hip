@@ -442,6 +442,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
     OMX_HIP_CHECK(hipMemcpyAsync(LA.as<c64>(), x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
     project(LA.as<c64>(), act);
     project(LA.as<c64>(), act);
+    last_adjoint_iters = it;
     // d = fft(kx_tc(ev))/N, ld = fft(kx_tc(lambda)); epsbar += epsinv_bar(d, ld)
     BView<double> LAv{LA.as<c64>(), ld, ld};
     launch_fields<double>(FIELD_D, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);

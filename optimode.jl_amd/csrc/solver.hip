@@ -616,7 +616,7 @@ namespace {
 template <typename T>
 int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
                  const std::vector<int>& nchk, const std::vector<int>& msize, double tol_floor, int maxiter,
-                 int max_steps, SolveStats& stats, int& newton_total) {
+                 int max_steps, SolveStats& stats, int& newton_total, bool strict) {
   const int ninst = b.ninst;
   std::vector<int> act(ninst);
   std::vector<double> tols(ninst);
@@ -635,6 +635,13 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
     iters += b.run(act, bands, nchk, msize, tols, maxiter, stats);
     b.newton(hns);
     ++newton_total;
+    // a band may only finish on an eigenpair converged to the requested tolerance: when
+    // |Δω²| met k_tol on a looser (inexact-Newton) eigensolve, re-solve at the same k
+    for (int i = 0; i < ninst && strict; ++i)
+      if (act[i] && hns[i].done == 1 && tols[i] > tol_floor) {
+        hns[i].done = 0;
+        hns[i].f = 0.0;
+      }
   }
   return iters;
 }
@@ -735,7 +742,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
     stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
     stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
-                               newton_total);
+                               newton_total, false);
     launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
                                   K, stream);
     OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,
@@ -752,7 +759,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
   }
   const int left = std::max(1, max_eigsolves - newton_total);
-  stats.it64 += newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total);
+  stats.it64 +=
+      newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
   bool ok = true;
   for (int i = 0; i < ninst; ++i) {
     kmags[i] = hns[i].k;

@@ -145,6 +145,7 @@ class Solver {
   std::vector<double> epsinv_host;  // (3,3,Nx,Ny) column-major copy
   DevMem last_evecs;                // FP64, nev columns of ld complex
   int last_nev = 0;
+  int last_adjoint_iters = 0;
   DevMem scratch_a, scratch_b, scratch_c;
   Profiler prof;
 
