@@ -108,6 +108,13 @@ int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmag
                         const double* kbar, const double* evbar, const uint8_t* evbar_mask,
                         double* omegabar_out, double* epsinvbar_out);
 
+/* Live kernel statistics (no counterpart in the reference: instrumentation for
+   measurement).  With omx_set_option(h, "profile", 1) every operator application inside
+   a solve is bracketed by HIP events on the handle's stream; `family` is "op_fp32" or
+   "op_fp64".  Returns total milliseconds, launches and algorithmic bytes
+   (32 N m + 24 N per FP32 application of m vectors, twice that in FP64). */
+int omx_get_kernel_stats(omx_solver* h, const char* family, double* total_ms, int64_t* launches, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,159 @@
+# HIP (MI355X / gfx950) backend for MaxwellEigenmodes: a new AbstractEigensolver whose
+# solve_ω², solve_k and solve_k rrule call libomx through its C ABI (include/omx.h).
+#
+# Drop-in for the CPU path: same signatures, keyword arguments, layouts and return types as
+#   solve_ω²   lib/MaxwellEigenmodes/src/solve.jl:38-63, solvers/gpu.jl:201-211
+#   solve_k    lib/MaxwellEigenmodes/src/solve.jl:182-196, solvers/gpu.jl:213-242
+#   rrule      lib/MaxwellEigenmodes/src/solve.jl:213-289, solvers/gpu.jl:292-365
+# Host data stays Julia-owned Float64 / ComplexF64 column-major arrays; the library copies
+# them to the device.  Non-zero statuses become `error(...)` with the library's message.
+module MaxwellEigenmodesHIPExt
+
+using LinearAlgebra
+using Logging: AbstractLogger, NullLogger
+using ChainRulesCore
+using ChainRulesCore: NoTangent, AbstractZero
+using DielectricSmoothing: Grid, N
+import MaxwellEigenmodes: solve_ω², solve_k, AbstractEigensolver, ModeSolver
+
+export HIPSolver
+
+const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
+
+struct OmxInfo
+    converged::Int32
+    n_newton::Int32
+    lobpcg_iters_fp32::Int32
+    lobpcg_iters_fp64::Int32
+    n_matvec::Int64
+    n_precon::Int64
+    max_residual::Float64
+    device_ms::Float64
+end
+OmxInfo() = OmxInfo(0, 0, 0, 0, 0, 0, 0.0, 0.0)
+
+"""
+    HIPSolver(; device=0, precision=:mixed, logger=NullLogger())
+
+Eigensolver backend running every step of `solve_ω²` / `solve_k` and the `solve_k`
+adjoint on an AMD MI355X through libomx.  `precision=:mixed` runs the operator in FP32 and
+refines in FP64; `:fp64` runs FP64 throughout.
+"""
+mutable struct HIPSolver{L<:AbstractLogger} <: AbstractEigensolver{L}
+    logger::L
+    device::Int
+    precision::Symbol
+    handle::Ptr{Cvoid}
+    key::Tuple{Int,Int,Float64,Float64}
+end
+
+function HIPSolver(; device::Integer=0, precision::Symbol=:mixed, logger::L=NullLogger()) where {L}
+    precision in (:mixed, :fp64) || error("HIPSolver precision must be :mixed or :fp64")
+    s = HIPSolver{L}(logger, Int(device), precision, C_NULL, (0, 0, 0.0, 0.0))
+    finalizer(s) do x
+        x.handle == C_NULL || ccall((:omx_destroy, libomx), Cint, (Ptr{Cvoid},), x.handle)
+        x.handle = C_NULL
+    end
+    return s
+end
+
+_lasterr(h) = unsafe_string(ccall((:omx_last_error, libomx), Cstring, (Ptr{Cvoid},), h))
+_check(st, h) = st == 0 || error("HIPSolver (omx status $st): " * _lasterr(h))
+
+# one handle per (grid, device); ε⁻¹ is re-uploaded when it changes (update_ε⁻¹)
+function _handle!(s::HIPSolver, grid::Grid{2}, ε⁻¹::AbstractArray{<:Real,4})
+    ei = Array{Float64,4}(ε⁻¹)
+    key = (grid.Nx, grid.Ny, Float64(grid.Δx), Float64(grid.Δy))
+    if s.handle == C_NULL || s.key != key
+        s.handle == C_NULL || ccall((:omx_destroy, libomx), Cint, (Ptr{Cvoid},), s.handle)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:omx_create, libomx), Cint,
+                   (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Cdouble, Cdouble, Ptr{Cdouble}),
+                   h, s.device, grid.Nx, grid.Ny, grid.Δx, grid.Δy, ei)
+        _check(st, C_NULL)
+        s.handle, s.key = h[], key
+    else
+        _check(ccall((:omx_set_epsinv, libomx), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), s.handle, ei), s.handle)
+    end
+    _check(ccall((:omx_set_option, libomx), Cint, (Ptr{Cvoid}, Cstring, Cdouble),
+                 s.handle, "fp64_only", s.precision === :fp64 ? 1.0 : 0.0), s.handle)
+    return s.handle
+end
+
+function solve_ω²(ms::ModeSolver{2,T}, solver::HIPSolver; nev=1, maxiter=100, tol=1e-8, log=false,
+                  f_filter=nothing) where {T<:Real}
+    h = _handle!(solver, ms.grid, ms.M̂.ε⁻¹)
+    H = Matrix{ComplexF64}(ms.H⃗[:, 1:nev])
+    ω² = Vector{Float64}(undef, nev)
+    info = Ref(OmxInfo())
+    st = ccall((:omx_solve_omega2, libomx), Cint,
+               (Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cint, Ptr{ComplexF64}, Ptr{Cdouble}, Ref{OmxInfo}),
+               h, ms.M̂.k⃗[3], nev, tol, maxiter, H, ω², info)
+    _check(st, h)
+    ms.H⃗[:, 1:nev] .= H
+    ms.ω²[1:nev] .= complex.(ω²)
+    return ω², [H[:, j] for j in 1:nev]
+end
+
+function solve_k(ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64}, solver::HIPSolver; nev=1,
+                 max_eigsolves=60, maxiter=100, k_tol=1e-8, eig_tol=1e-8, log=false, kguess=nothing,
+                 Hguess=nothing, f_filter=nothing, overwrite=false)
+    h = _handle!(solver, grid, ε⁻¹)
+    n2 = 2 * N(grid)
+    kmags = Vector{Float64}(undef, nev)
+    evecs = Matrix{ComplexF64}(undef, n2, nev)
+    cols = Hguess === nothing ? Vector{ComplexF64}[] :
+           Hguess isa AbstractMatrix ? [Vector{ComplexF64}(c) for c in eachcol(Hguess)] :
+           eltype(Hguess) <: Number ? [Vector{ComplexF64}(Hguess)] : [Vector{ComplexF64}(c) for c in Hguess]
+    cols = [c for c in cols if length(c) == n2][1:min(end, nev)]
+    Hg = isempty(cols) ? Matrix{ComplexF64}(undef, n2, 0) : reduce(hcat, cols)
+    info = Ref(OmxInfo())
+    kgv = kguess === nothing ? nothing : Float64[first(kguess)]
+    GC.@preserve kgv Hg begin
+        st = ccall((:omx_solve_k, libomx), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble, Cint, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Cint,
+                    Ptr{Cdouble}, Ptr{ComplexF64}, Ref{OmxInfo}),
+                   h, ω, nev, k_tol, eig_tol, max_eigsolves, maxiter,
+                   kgv === nothing ? Ptr{Cdouble}(C_NULL) : kgv,
+                   isempty(cols) ? Ptr{ComplexF64}(C_NULL) : Hg, length(cols), kmags, evecs, info)
+    end
+    _check(st, h)
+    return kmags, [evecs[:, j] for j in 1:nev]
+end
+
+function ChainRulesCore.rrule(::typeof(solve_k), ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64},
+                              solver::HIPSolver; nev=1, kwargs...)
+    kmags, evecs = solve_k(ω, ε⁻¹, grid, solver; nev, kwargs...)
+    function hip_solve_k_pullback(ΔΩ)
+        k̄mags, ēvecs = ΔΩ
+        n2 = 2 * N(grid)
+        kb = zeros(Float64, nev)
+        if !(k̄mags isa AbstractZero)
+            for (j, v) in enumerate(k̄mags)
+                kb[j] = v isa AbstractZero ? 0.0 : Float64(v)
+            end
+        end
+        eb = zeros(ComplexF64, n2, nev)
+        mask = zeros(UInt8, nev)
+        if !(ēvecs isa AbstractZero)
+            for (j, v) in enumerate(ēvecs)
+                v isa AbstractZero && continue
+                eb[:, j] .= v
+                mask[j] = 0x01
+            end
+        end
+        E = reduce(hcat, evecs)
+        ω̄ = Ref(0.0)
+        ε̄⁻¹ = zeros(Float64, size(ε⁻¹))
+        h = _handle!(solver, grid, ε⁻¹)
+        st = ccall((:omx_solve_k_adjoint, libomx), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Ptr{Cdouble}, Ptr{ComplexF64},
+                    Ptr{UInt8}, Ref{Cdouble}, Ptr{Cdouble}),
+                   h, ω, nev, kmags, E, kb, eb, mask, ω̄, ε̄⁻¹)
+        _check(st, h)
+        return (NoTangent(), ω̄[], ε̄⁻¹, NoTangent(), NoTangent())
+    end
+    return (kmags, evecs), hip_solve_k_pullback
+end
+
+end # module
