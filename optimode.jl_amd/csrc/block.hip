@@ -25,85 +25,196 @@ __device__ int resolve_cols(const ColList<T>& L, int i, const cplx<T>** ptrs, in
 }
 
 // ---------------------------------------------------------------------------------------
-// Gram: partial[a][b] = sum_rows conj(A_a) B_b.  Thread (ty, tx) of a 16x16 layout owns a
-// TILE x TILE patch of the PM x PM output; rows are staged through LDS SR at a time.
+// Gram on the matrix cores: partial[a][b] = sum over a row chunk of conj(A_a) B_b, complex,
+// FP32 or FP64, with v_mfma_{f32,f64}_16x16x4.  Lane layout (checked on gfx950): A operand
+// lane l -> (i = l%16, k = l/16), B operand lane l -> (k = l/16, j = l%16) for both types;
+// accumulator register r of lane l -> (row 4(l/16) + r, col l%16) for f32 but
+// (row l/16 + 4r, col l%16) for f64.
+//
+// The four k slots of one MFMA are four rows of the chunk: lane (i, g) holds S consecutive
+// complex rows [rb, rb + S) of column i of its A tile and of column i of its B tile, with
+// rb = stage base + g S, loaded straight from HBM with 16-byte loads (no LDS staging).
+// Step s of the stage multiplies row rb + s:
+//   Re G += ar br + ai bi,   Im G += ar bi - ai br       (four real MFMAs per step).
+// Every 16 x 16 output tile of an instance that holds at least one column pair is computed
+// (runtime tile count <= TA x TB); the next stage's rows are loaded while the current one
+// is multiplied.  The 4 waves of a block take interleaved stages of the block's row chunk
+// and their tiles are summed through LDS before one FP64 write per chunk.
 // ---------------------------------------------------------------------------------------
-template <typename T, int PM>
-__global__ __launch_bounds__(BLK) void k_gram(ColList<T> A, ColList<T> B, long K, long rows_per_chunk,
-                                              c64* partial) {
-  constexpr int TILE = PM / 16;
-  constexpr int SR = 32;
-  __shared__ const cplx<T>* pa_ptr[PM];
-  __shared__ const cplx<T>* pb_ptr[PM];
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Acc4;
+template <> struct Acc4<float> { typedef floatx4 type; };
+template <> struct Acc4<double> { typedef doublex4 type; };
+
+__device__ __forceinline__ floatx4 mfma16(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// S consecutive complex rows of one column (16-byte loads), zero past `lim` or for a
+// missing column.
+template <int S>
+__device__ __forceinline__ void load_rows(const cplx<float>* col, long row, long lim, cplx<float> (&v)[S]) {
+  if (col != nullptr && row + S <= lim) {
+    const float4* q = reinterpret_cast<const float4*>(col + row);
+#pragma unroll
+    for (int j = 0; j < S / 2; ++j) {
+      const float4 w = q[j];
+      v[2 * j] = cplx<float>{w.x, w.y};
+      v[2 * j + 1] = cplx<float>{w.z, w.w};
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < S; ++j) v[j] = (col != nullptr && row + j < lim) ? col[row + j] : cplx<float>{0.f, 0.f};
+  }
+}
+template <int S>
+__device__ __forceinline__ void load_rows(const cplx<double>* col, long row, long lim, cplx<double> (&v)[S]) {
+  if (col != nullptr && row + S <= lim) {
+    const double2* q = reinterpret_cast<const double2*>(col + row);
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const double2 w = q[j];
+      v[j] = cplx<double>{w.x, w.y};
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < S; ++j) v[j] = (col != nullptr && row + j < lim) ? col[row + j] : cplx<double>{0.0, 0.0};
+  }
+}
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long K, long rows_per_chunk, c64* partial,
+                                                int pm) {
+  using acc_t = typename Acc4<T>::type;
+  // rows per lane and stage: 64 bytes of each tile column, halved for wide tile sets
+  constexpr int S = (sizeof(T) == 4 ? 8 : 4) / ((TA + TB) > 4 ? 2 : 1);
+  constexpr int WR = 4 * S;       // rows per wave and stage
+  constexpr int STRIDE = 4 * WR;  // rows per block and stage
+  __shared__ const cplx<T>* pa_ptr[16 * TA];
+  __shared__ const cplx<T>* pb_ptr[16 * TB];
   __shared__ int npa_s, npb_s;
-  __shared__ cplx<T> As[SR][PM + 1];
-  __shared__ cplx<T> Bs[SR][PM + 1];
-  const int inst = blockIdx.y;
-  const int chunk = blockIdx.x;
+  __shared__ T red[4][8][64];
+  const int inst = blockIdx.y, chunk = blockIdx.x;
   if (threadIdx.x == 0) {
-    npa_s = resolve_cols(A, inst, pa_ptr, PM);
-    npb_s = resolve_cols(B, inst, pb_ptr, PM);
+    npa_s = resolve_cols(A, inst, pa_ptr, 16 * TA);
+    npb_s = resolve_cols(B, inst, pb_ptr, 16 * TB);
   }
   __syncthreads();
   const int npa = npa_s, npb = npb_s;
   if (npa == 0 || npb == 0) return;
-  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
-  const bool busy = ty * TILE < npa && tx * TILE < npb;
-  cplx<T> acc[TILE][TILE];
+  const int nta = (npa + 15) >> 4, ntb = (npb + 15) >> 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, g = lane >> 4;
+  const cplx<T>* ap[TA];
+  const cplx<T>* bp[TB];
 #pragma unroll
-  for (int i = 0; i < TILE; ++i)
+  for (int t = 0; t < TA; ++t) ap[t] = (t * 16 + i16 < npa) ? pa_ptr[t * 16 + i16] : nullptr;
 #pragma unroll
-    for (int j = 0; j < TILE; ++j) acc[i][j] = cplx<T>{T(0), T(0)};
+  for (int t = 0; t < TB; ++t) bp[t] = (t * 16 + i16 < npb) ? pb_ptr[t * 16 + i16] : nullptr;
+  acc_t accr[TA][TB], acci[TA][TB];
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      accr[ta][tb] = acc_t{0, 0, 0, 0};
+      acci[ta][tb] = acc_t{0, 0, 0, 0};
+    }
   const long r0 = long(chunk) * rows_per_chunk;
   const long r1 = (r0 + rows_per_chunk < K) ? r0 + rows_per_chunk : K;
-  for (long rs = r0; rs < r1; rs += SR) {
-    for (int idx = threadIdx.x; idx < SR * PM; idx += BLK) {
-      const int a = idx / SR, r = idx - (idx / SR) * SR;
-      const long row = rs + r;
-      const bool ok = row < r1;
-      As[r][a] = (a < npa && ok) ? pa_ptr[a][row] : cplx<T>{T(0), T(0)};
-      Bs[r][a] = (a < npb && ok) ? pb_ptr[a][row] : cplx<T>{T(0), T(0)};
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int r = 0; r < (busy ? SR : 0); ++r) {
-      cplx<T> av[TILE], bv[TILE];
+  long rs = r0 + long(wave) * WR;
+  cplx<T> a[TA][S], b[TB][S];
+  if (rs < r1) {
 #pragma unroll
-      for (int i = 0; i < TILE; ++i) av[i] = As[r][ty * TILE + i];
+    for (int t = 0; t < TA; ++t)
+      if (t < nta) load_rows<S>(ap[t], rs + g * S, r1, a[t]);
 #pragma unroll
-      for (int j = 0; j < TILE; ++j) bv[j] = Bs[r][tx * TILE + j];
-#pragma unroll
-      for (int i = 0; i < TILE; ++i)
-#pragma unroll
-        for (int j = 0; j < TILE; ++j) {
-          // conj(a) * b
-          acc[i][j].x += av[i].x * bv[j].x + av[i].y * bv[j].y;
-          acc[i][j].y += av[i].x * bv[j].y - av[i].y * bv[j].x;
-        }
-    }
-    __syncthreads();
+    for (int t = 0; t < TB; ++t)
+      if (t < ntb) load_rows<S>(bp[t], rs + g * S, r1, b[t]);
   }
-  c64* out = partial + (long(inst) * gridDim.x + chunk) * PM * PM;
+  while (rs < r1) {
+    const long rn = rs + STRIDE;
+    cplx<T> na[TA][S], nb[TB][S];
+    if (rn < r1) {
 #pragma unroll
-  for (int i = 0; i < TILE; ++i)
+      for (int t = 0; t < TA; ++t)
+        if (t < nta) load_rows<S>(ap[t], rn + g * S, r1, na[t]);
 #pragma unroll
-    for (int j = 0; j < TILE; ++j) {
-      const int a = ty * TILE + i, b = tx * TILE + j;
-      out[a * PM + b] = c64{double(acc[i][j].x), double(acc[i][j].y)};
+      for (int t = 0; t < TB; ++t)
+        if (t < ntb) load_rows<S>(bp[t], rn + g * S, r1, nb[t]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          if (ta < nta && tb < ntb) {
+            const cplx<T> x = a[ta][s], y = b[tb][s];
+            accr[ta][tb] = mfma16(x.x, y.x, accr[ta][tb]);
+            acci[ta][tb] = mfma16(x.x, y.y, acci[ta][tb]);
+            accr[ta][tb] = mfma16(x.y, y.y, accr[ta][tb]);
+            acci[ta][tb] = mfma16(x.y, -y.x, acci[ta][tb]);
+          }
+#pragma unroll
+    for (int t = 0; t < TA; ++t)
+#pragma unroll
+      for (int s = 0; s < S; ++s) a[t][s] = na[t][s];
+#pragma unroll
+    for (int t = 0; t < TB; ++t)
+#pragma unroll
+      for (int s = 0; s < S; ++s) b[t][s] = nb[t][s];
+    rs = rn;
+  }
+  // sum the four waves tile by tile; wave w finishes accumulator register w
+  c64* out = partial + (long(inst) * gridDim.x + chunk) * pm * pm;
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      if (ta >= nta || tb >= ntb) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        red[wave][r][lane] = accr[ta][tb][r];
+        red[wave][4 + r][lane] = acci[ta][tb][r];
+      }
+      __syncthreads();
+      const int r = wave;
+      const double sr = (double(red[0][r][lane]) + double(red[1][r][lane])) +
+                        (double(red[2][r][lane]) + double(red[3][r][lane]));
+      const double si = (double(red[0][4 + r][lane]) + double(red[1][4 + r][lane])) +
+                        (double(red[2][4 + r][lane]) + double(red[3][4 + r][lane]));
+      const int row = ta * 16 + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
+      out[row * pm + tb * 16 + i16] = c64{sr, si};
+      __syncthreads();
     }
 }
 
+// out_i[e] = sum over chunks of partial_i[chunk][e]: 64 elements x 4 chunk groups per
+// block, the groups combined in a fixed order (deterministic).
 __global__ __launch_bounds__(BLK) void k_reduce(const c64* partial, int nchunk, int esz, c64* out) {
+  __shared__ double sx[4][64], sy[4][64];
   const int inst = blockIdx.y;
-  for (int e = blockIdx.x * BLK + threadIdx.x; e < esz; e += gridDim.x * BLK) {
-    double sx = 0.0, sy = 0.0;
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  double ax = 0.0, ay = 0.0;
+  if (e < esz) {
     const c64* p = partial + long(inst) * nchunk * esz + e;
-    for (int c = 0; c < nchunk; ++c) {
-      sx += p[long(c) * esz].x;
-      sy += p[long(c) * esz].y;
+    for (int c = grp; c < nchunk; c += 4) {
+      const c64 v = p[long(c) * esz];
+      ax += v.x;
+      ay += v.y;
     }
-    out[long(inst) * esz + e] = c64{sx, sy};
   }
+  sx[grp][lane] = ax;
+  sy[grp][lane] = ay;
+  __syncthreads();
+  if (grp == 0 && e < esz)
+    out[long(inst) * esz + e] = c64{(sx[0][lane] + sx[1][lane]) + (sx[2][lane] + sx[3][lane]),
+                                    (sy[0][lane] + sy[1][lane]) + (sy[2][lane] + sy[3][lane])};
 }
 
 // ---------------------------------------------------------------------------------------
@@ -260,126 +371,38 @@ __global__ __launch_bounds__(BLK) void k_copy_cols(const cplx<T>* src, cplx<T>* 
   }
 }
 
-// ---------------------------------------------------------------------------------------
-// Gram on the matrix cores (FP32): G = A^H B with A, B complex.  Over the interleaved
-// (re, im) rows of one complex row r, v_mfma_f32_32x32x2f32 with k = (re, im) gives
-//   Re G += [ar ai] . [br bi]^T      (B operand = (br, bi))
-//   Im G += [ar ai] . [bi -br]^T     (B operand = (bi, -br))
-// so each complex row costs two MFMAs per 32x32 output tile.  Lane layout (checked on
-// gfx950): A operand lane l -> (row l%32, k l/32); B operand lane l -> (k l/32, col l%32);
-// accumulator register q of lane l -> (row 4(l/32) + q%4 + 8(q/4), col l%32).
-// The 4 waves of a block split the (TA x TB) output tiles and, when fewer than 4 tiles,
-// the staged rows; partial tiles are summed through LDS and written once per block.
-// ---------------------------------------------------------------------------------------
-typedef float floatx16 __attribute__((ext_vector_type(16)));
+}  // namespace
 
-template <int TA, int TB>
-__global__ __launch_bounds__(BLK) void k_gram_mfma(ColList<float> A, ColList<float> B, long K, long rows_per_chunk,
-                                                   c64* partial, int pm) {
-  constexpr int NT = TA * TB;          // output tiles
-  constexpr int G = 4 / NT;            // row groups per tile (1, 2 or 4)
-  constexpr int SR = 64;               // complex rows staged per step
-  constexpr int WA = TA * 32, WB = TB * 32;
-  __shared__ const cplx<float>* pa_ptr[WA];
-  __shared__ const cplx<float>* pb_ptr[WB];
-  __shared__ int npa_s, npb_s;
-  __shared__ cplx<float> As[SR][WA + 1];
-  __shared__ cplx<float> Bs[SR][WB + 1];
-  __shared__ float red[4][32][64];
-  const int inst = blockIdx.y, chunk = blockIdx.x;
-  if (threadIdx.x == 0) {
-    npa_s = resolve_cols(A, inst, pa_ptr, WA);
-    npb_s = resolve_cols(B, inst, pb_ptr, WB);
-  }
-  __syncthreads();
-  const int npa = npa_s, npb = npb_s;
-  if (npa == 0 || npb == 0) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = wave % NT, grp = wave / NT;
-  const int ta = tile / TB, tb = tile % TB;
-  const int hi = lane >> 5, col = lane & 31;
-  floatx16 accr = {0}, acci = {0};
-  const long r0 = long(chunk) * rows_per_chunk;
-  const long r1 = (r0 + rows_per_chunk < K) ? r0 + rows_per_chunk : K;
-  for (long rs = r0; rs < r1; rs += SR) {
-    for (int idx = threadIdx.x; idx < SR * WA; idx += BLK) {
-      const int a = idx / SR, r = idx - (idx / SR) * SR;
-      const long row = rs + r;
-      As[r][a] = (a < npa && row < r1) ? pa_ptr[a][row] : cplx<float>{0.f, 0.f};
-    }
-    for (int idx = threadIdx.x; idx < SR * WB; idx += BLK) {
-      const int b = idx / SR, r = idx - (idx / SR) * SR;
-      const long row = rs + r;
-      Bs[r][b] = (b < npb && row < r1) ? pb_ptr[b][row] : cplx<float>{0.f, 0.f};
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int r = grp; r < SR; r += G) {
-      const cplx<float> av = As[r][ta * 32 + col];
-      const cplx<float> bv = Bs[r][tb * 32 + col];
-      const float a = hi ? av.y : av.x;
-      const float b = hi ? bv.y : bv.x;
-      const float b2 = hi ? -bv.x : bv.y;
-      accr = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, accr, 0, 0, 0);
-      acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b2, acci, 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  // sum the G row groups of each tile through LDS, then write FP64 partials
-  for (int q = 0; q < 16; ++q) {
-    red[wave][q][lane] = accr[q];
-    red[wave][16 + q][lane] = acci[q];
-  }
-  __syncthreads();
-  if (grp == 0) {
-    c64* out = partial + (long(inst) * gridDim.x + chunk) * pm * pm;
-    for (int q = 0; q < 16; ++q) {
-      float sr = 0.f, si = 0.f;
-      for (int g = 0; g < G; ++g) {
-        sr += red[tile + g * NT][q][lane];
-        si += red[tile + g * NT][16 + q][lane];
-      }
-      const int a = ta * 32 + 4 * hi + (q & 3) + 8 * (q >> 2);
-      const int b = tb * 32 + col;
-      if (a < pm && b < pm) out[a * pm + b] = c64{double(sr), double(si)};
-    }
+template <typename T, int TA>
+static void gram_tb(int tb, dim3 grid, hipStream_t stream, const ColList<T>& A, const ColList<T>& B, long K, long rpc,
+                    c64* partial, int pm) {
+  switch (tb) {
+    case 1: hipLaunchKernelGGL((k_gram16<T, TA, 1>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
+    case 2: hipLaunchKernelGGL((k_gram16<T, TA, 2>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
+    default: hipLaunchKernelGGL((k_gram16<T, TA, 3>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
   }
 }
-
-}  // namespace
 
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
                  hipStream_t stream, int ca, int cb) {
-  const long rpc = (K + nchunk - 1) / nchunk;
+  if (ca <= 0 || cb <= 0 || ca > 48 || cb > 48 || ca > pm || cb > pm || pm % 16 != 0)
+    throw Error(OMX_ERR_INTERNAL, "gram: column bounds out of range");
+  // chunks are whole block stages (128 rows), so only the last one can end mid-stage
+  long rpc = (K + nchunk - 1) / nchunk;
+  rpc = (rpc + 127) / 128 * 128;
   dim3 grid(nchunk, ninst);
-  if (sizeof(T) == 4 && ca > 0 && cb > 0 && ca <= 64 && cb <= 64) {
-    // matrix-core path: ca, cb bound the column counts of A and B
-    const ColList<float>& Af = reinterpret_cast<const ColList<float>&>(A);
-    const ColList<float>& Bf = reinterpret_cast<const ColList<float>&>(B);
-    const int ta = ca > 32 ? 2 : 1, tb = cb > 32 ? 2 : 1;
-    if (ta == 1 && tb == 1)
-      hipLaunchKernelGGL((k_gram_mfma<1, 1>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
-    else if (ta == 2 && tb == 1)
-      hipLaunchKernelGGL((k_gram_mfma<2, 1>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
-    else if (ta == 1 && tb == 2)
-      hipLaunchKernelGGL((k_gram_mfma<1, 2>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
-    else
-      hipLaunchKernelGGL((k_gram_mfma<2, 2>), grid, dim3(BLK), 0, stream, Af, Bf, K, rpc, partial, pm);
-    OMX_HIP_CHECK(hipGetLastError());
-    return;
-  }
-  switch (pm) {
-    case 16: hipLaunchKernelGGL((k_gram<T, 16>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial); break;
-    case 32: hipLaunchKernelGGL((k_gram<T, 32>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial); break;
-    case 48: hipLaunchKernelGGL((k_gram<T, 48>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial); break;
-    default: throw Error(OMX_ERR_INTERNAL, "gram: unsupported tile size");
+  const int ta = (ca + 15) / 16, tb = (cb + 15) / 16;
+  switch (ta) {
+    case 1: gram_tb<T, 1>(tb, grid, stream, A, B, K, rpc, partial, pm); break;
+    case 2: gram_tb<T, 2>(tb, grid, stream, A, B, K, rpc, partial, pm); break;
+    default: gram_tb<T, 3>(tb, grid, stream, A, B, K, rpc, partial, pm); break;
   }
   OMX_HIP_CHECK(hipGetLastError());
 }
 
 void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream) {
-  dim3 grid((esz + BLK - 1) / BLK, ninst);
+  dim3 grid((esz + 63) / 64, ninst);
   hipLaunchKernelGGL(k_reduce, grid, dim3(BLK), 0, stream, partial, nchunk, esz, out);
   OMX_HIP_CHECK(hipGetLastError());
 }

@@ -38,9 +38,10 @@ template <typename T> struct OutView {
   int fixed_off;
 };
 
-// G_i = A_i^H B_i, written as per-chunk partial sums (FP64) to
-// partial[(i*nchunk + chunk)*pm*pm + a*pm + b].  pm in {16, 32, 48}.  ca / cb are upper
-// bounds of the column counts of A and B (FP32 uses the matrix cores when both are <= 64).
+// G_i = A_i^H B_i on the matrix cores, written as per-chunk partial sums (FP64) to
+// partial[(i*nchunk + chunk)*pm*pm + a*pm + b] for a < 16 ceil(nA_i/16), b < 16 ceil(nB_i/16)
+// (other entries untouched).  pm in {16, 32, 48}; ca, cb (<= pm, <= 48) bound the column
+// counts of A and B.
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
                  hipStream_t stream, int ca = 0, int cb = 0);

@@ -279,83 +279,101 @@ __device__ int d_ortho_against(c64* X, int rows, int cx, const c64* Y, int cy, d
   return 0;
 }
 
-// Parallel cyclic Jacobi for Hermitian A (n x n, in LDS, destroyed): eigenvalues on the
-// diagonal, eigenvectors in the columns of V.  Round-robin pairing: n/2 disjoint
-// rotations per round, rows then columns.
-__device__ void d_jacobi(c64* A, c64* V, int n) {
-  __shared__ int rp[DMAX / 2 + 1], rq[DMAX / 2 + 1];
-  __shared__ double rc[DMAX / 2 + 1];
-  __shared__ c64 rs[DMAX / 2 + 1];
-  __shared__ int nrot;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e / n, j = e - (e / n) * n;
-    V[i * DLD + j] = c64{i == j ? 1.0 : 0.0, 0.0};
-  }
-  __syncthreads();
+// Parallel cyclic Jacobi for Hermitian A (n x n).  Round-robin pairing gives m/2
+// disjoint rotations per round (m = n rounded up to even; an odd n gets a zero row and
+// column, which never rotates).  The rotation of pair (p, q) zeroes A[p][q]:
+//   rows  A'[p] = c A[p] - s A[q],          A'[q] = conj(s) A[p] + c A[q]
+//   cols  A'[:,p] = c A[:,p] - conj(s) A[:,q],  A'[:,q] = s A[:,p] + c A[:,q]
+// so A' = L A L^H with L block diagonal.  One LDS pass per round: the items are the
+// (m/2)^2 2x2 blocks of A -- each recomputes the two rotations it needs from the previous
+// round's matrix (read from one buffer, written to the other) -- and the m x m/2 row
+// pairs of V (updated in place).  A sweep is m - 1 rounds; sweeps stop when one performs
+// no rotation.  Returns the buffer holding the final matrix (eigenvalues on its diagonal);
+// V holds the eigenvectors in its columns.
+__device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {
+  const int ia = (t == 0) ? 0 : 1 + (t - 1 + r) % (m - 1);
+  const int tb = m - 1 - t;
+  const int ib = 1 + (tb - 1 + r) % (m - 1);
+  p = ia < ib ? ia : ib;
+  q = ia < ib ? ib : ia;
+}
+
+__device__ __forceinline__ bool jac_rot(const c64* A, int p, int q, double& c, c64& s) {
+  const double app = A[p * DLD + p].x, aqq = A[q * DLD + q].x;
+  const c64 apq = A[p * DLD + q];
+  const double a = sqrt(apq.x * apq.x + apq.y * apq.y);
+  c = 1.0;
+  s = c64{0.0, 0.0};
+  if (!(a > 1e-300 && a > 1e-16 * sqrt(fabs(app * aqq)))) return false;
+  const c64 ph = c64{apq.x / a, apq.y / a};
+  const double zeta = (aqq - app) / (2.0 * a);
+  const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  c = 1.0 / sqrt(1.0 + tt * tt);
+  s = cscale(c * tt, ph);
+  return true;
+}
+
+__device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
+  __shared__ int rot[3];
   const int m = n + (n & 1);
   const int npair = m / 2;
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int i = e / m, j = e - (e / m) * m;
+    V[i * DLD + j] = c64{i == j ? 1.0 : 0.0, 0.0};
+    if (i == n || j == n) A0[i * DLD + j] = c64{0.0, 0.0};
+  }
+  if (threadIdx.x < 3) rot[threadIdx.x] = 0;
+  __syncthreads();
+  c64* Ac = A0;
+  c64* An = A1;
+  const int nblk = npair * npair, nitem = nblk + m * npair;
   for (int sweep = 0; sweep < 40; ++sweep) {
-    if (threadIdx.x == 0) nrot = 0;
-    __syncthreads();
+    const int sl = sweep % 3;
+    if (threadIdx.x == 0) rot[(sweep + 1) % 3] = 0;
     for (int r = 0; r < m - 1; ++r) {
-      if (threadIdx.x < npair) {
-        const int t = threadIdx.x;
-        const int ia = (t == 0) ? 0 : 1 + (t - 1 + r) % (m - 1);
-        const int tb = m - 1 - t;
-        const int ib = 1 + (tb - 1 + r) % (m - 1);
-        const int p = ia < ib ? ia : ib, q = ia < ib ? ib : ia;
-        double c = 1.0;
-        c64 s{0.0, 0.0};
-        if (q < n) {
-          const double app = A[p * DLD + p].x, aqq = A[q * DLD + q].x;
-          const c64 apq = A[p * DLD + q];
-          const double a = sqrt(apq.x * apq.x + apq.y * apq.y);
-          if (a > 1e-300 && a > 1e-16 * sqrt(fabs(app * aqq))) {
-            const c64 ph = c64{apq.x / a, apq.y / a};
-            const double zeta = (aqq - app) / (2.0 * a);
-            const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            c = 1.0 / sqrt(1.0 + tt * tt);
-            s = cscale(c * tt, ph);
-            atomicAdd(&nrot, 1);
+      for (int idx = threadIdx.x; idx < nitem; idx += blockDim.x) {
+        if (idx < nblk) {
+          const int t1 = idx / npair, t2 = idx - (idx / npair) * npair;
+          int p1, q1, p2, q2;
+          jac_pair(t1, r, m, p1, q1);
+          jac_pair(t2, r, m, p2, q2);
+          double c1, c2;
+          c64 s1, s2;
+          const bool on = jac_rot(Ac, p1, q1, c1, s1);
+          jac_rot(Ac, p2, q2, c2, s2);
+          if (on && t1 == t2) rot[sl] = 1;
+          const c64 x00 = Ac[p1 * DLD + p2], x01 = Ac[p1 * DLD + q2];
+          const c64 x10 = Ac[q1 * DLD + p2], x11 = Ac[q1 * DLD + q2];
+          // rows with rotation 1
+          const c64 y00 = csub(cscale(c1, x00), cmul(s1, x10)), y01 = csub(cscale(c1, x01), cmul(s1, x11));
+          const c64 y10 = cadd(cmul(ccj(s1), x00), cscale(c1, x10)), y11 = cadd(cmul(ccj(s1), x01), cscale(c1, x11));
+          // columns with rotation 2
+          An[p1 * DLD + p2] = csub(cscale(c2, y00), cmul(ccj(s2), y01));
+          An[p1 * DLD + q2] = cadd(cmul(s2, y00), cscale(c2, y01));
+          An[q1 * DLD + p2] = csub(cscale(c2, y10), cmul(ccj(s2), y11));
+          An[q1 * DLD + q2] = cadd(cmul(s2, y10), cscale(c2, y11));
+        } else {
+          const int e = idx - nblk;
+          const int j = e / npair, t = e - (e / npair) * npair;
+          int p, q;
+          jac_pair(t, r, m, p, q);
+          double c;
+          c64 sv;
+          if (jac_rot(Ac, p, q, c, sv)) {
+            const c64 vp = V[j * DLD + p], vq = V[j * DLD + q];
+            V[j * DLD + p] = csub(cscale(c, vp), cmul(ccj(sv), vq));
+            V[j * DLD + q] = cadd(cmul(sv, vp), cscale(c, vq));
           }
         }
-        rp[t] = p;
-        rq[t] = q < n ? q : p;
-        rc[t] = c;
-        rs[t] = s;
       }
       __syncthreads();
-      // rows: A[p][j] = c A[p][j] - s A[q][j];  A[q][j] = conj(s) A[p][j] + c A[q][j]
-      for (int e = threadIdx.x; e < npair * n; e += blockDim.x) {
-        const int t = e / n, j = e - (e / n) * n;
-        const int p = rp[t], q = rq[t];
-        if (p == q) continue;
-        const double c = rc[t];
-        const c64 s = rs[t];
-        const c64 ap = A[p * DLD + j], aq = A[q * DLD + j];
-        A[p * DLD + j] = csub(cscale(c, ap), cmul(s, aq));
-        A[q * DLD + j] = cadd(cmul(ccj(s), ap), cscale(c, aq));
-      }
-      __syncthreads();
-      // columns: A[j][p] = c A[j][p] - conj(s) A[j][q];  A[j][q] = s A[j][p] + c A[j][q]
-      for (int e = threadIdx.x; e < npair * n; e += blockDim.x) {
-        const int t = e / n, j = e - (e / n) * n;
-        const int p = rp[t], q = rq[t];
-        if (p == q) continue;
-        const double c = rc[t];
-        const c64 s = rs[t];
-        const c64 ap = A[j * DLD + p], aq = A[j * DLD + q];
-        A[j * DLD + p] = csub(cscale(c, ap), cmul(ccj(s), aq));
-        A[j * DLD + q] = cadd(cmul(s, ap), cscale(c, aq));
-        const c64 vp = V[j * DLD + p], vq = V[j * DLD + q];
-        V[j * DLD + p] = csub(cscale(c, vp), cmul(ccj(s), vq));
-        V[j * DLD + q] = cadd(cmul(s, vp), cscale(c, vq));
-      }
-      __syncthreads();
+      c64* tmp = Ac;
+      Ac = An;
+      An = tmp;
     }
-    if (nrot == 0) break;
-    __syncthreads();
+    if (rot[sl] == 0) break;
   }
+  return Ac;
 }
 
 __device__ __forceinline__ int shp(const int* shape, int f, int ninst, int i) { return shape[f * ninst + i]; }
@@ -364,9 +382,12 @@ __device__ __forceinline__ int shp(const int* shape, int f, int ninst, int i) { 
 // kernels
 // ----------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(DBLK) void k_dense_rr(DenseLaunch dl, LState* st, int* shape, const c64* gp,
+constexpr int RBLK = 512;  // Rayleigh-Ritz block: one thread per Jacobi item up to 24 x 24
+
+__global__ __launch_bounds__(RBLK) void k_dense_rr(DenseLaunch dl, LState* st, int* shape, const c64* gp,
                                                    double* Cx, double* lam, int diag_mode) {
   __shared__ c64 A[DMAX * DLD];
+  __shared__ c64 A2[DMAX * DLD];
   __shared__ c64 V[DMAX * DLD];
   __shared__ int rank[DMAX];
   const int i = blockIdx.x, ninst = dl.ninst, pm = dl.pm;
@@ -410,12 +431,12 @@ __global__ __launch_bounds__(DBLK) void k_dense_rr(DenseLaunch dl, LState* st, i
     for (int q = threadIdx.x; q < nact; q += blockDim.x) li[q] = A[q * DLD + q].x;
     return;
   }
-  d_jacobi(A, V, py);
+  const c64* Ad = d_jacobi(A, A2, V, py);
   for (int j = threadIdx.x; j < py; j += blockDim.x) {
-    const double wj = A[j * DLD + j].x;
+    const double wj = Ad[j * DLD + j].x;
     int r = 0;
     for (int l = 0; l < py; ++l) {
-      const double wl = A[l * DLD + l].x;
+      const double wl = Ad[l * DLD + l].x;
       if (wl < wj || (wl == wj && l < j)) ++r;
     }
     rank[j] = r;
@@ -430,7 +451,7 @@ __global__ __launch_bounds__(DBLK) void k_dense_rr(DenseLaunch dl, LState* st, i
     }
   }
   for (int j = threadIdx.x; j < py; j += blockDim.x)
-    if (rank[j] < nact) li[rank[j]] = A[j * DLD + j].x;
+    if (rank[j] < nact) li[rank[j]] = Ad[j * DLD + j].x;
 }
 
 __global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st, int* shape, const double* npart,
@@ -757,7 +778,7 @@ __global__ void k_newton(int ninst, int mper, NewtonState* ns, LState* st, const
 
 void launch_dense_rr(const DenseLaunch& dl, LState* st, int* shape, const c64* gram_partial, double* Cx,
                      double* lam, int diag_mode, hipStream_t stream) {
-  hipLaunchKernelGGL(k_dense_rr, dim3(dl.ninst), dim3(DBLK), 0, stream, dl, st, shape, gram_partial, Cx, lam,
+  hipLaunchKernelGGL(k_dense_rr, dim3(dl.ninst), dim3(RBLK), 0, stream, dl, st, shape, gram_partial, Cx, lam,
                      diag_mode);
   OMX_HIP_CHECK(hipGetLastError());
 }

@@ -183,7 +183,8 @@ template <typename T> class Batch {
     D.alloc(sizeof(cplx<T>) * size_t(ninst) * M * 3 * S.N);
     pm = (3 * M <= 16) ? 16 : (3 * M <= 32 ? 32 : 48);
     qmax = M <= 4 ? 4 : (M <= 8 ? 8 : 16);
-    nchunk = std::max(1, std::min<int>(int((K + 1023) / 1024), std::max(1, 512 / ninst)));
+    // about 512 Gram blocks (2 per CU), at least 512 rows each
+    nchunk = std::max(1, std::min<int>(int((K + 511) / 512), std::max(1, 512 / ninst)));
     nblk = std::max(1, std::min<int>(int((K + 255) / 256), std::max(1, 1024 / ninst)));
     dps = S.ny / S.pd<T>().plan.rows_per_block;
     st.alloc(sizeof(LState) * ninst);
@@ -204,7 +205,7 @@ template <typename T> class Batch {
     ns.alloc(sizeof(NewtonState) * ninst);
     dl.ninst = ninst;
     dl.pm = pm;
-    dl.nchunk = nchunk;
+    dl.nchunk = 1;  // dense kernels read the reduced Gram matrices
     dl.nblk = nblk;
     dl.qmax = qmax;
     hst.resize(ninst);
@@ -231,9 +232,12 @@ template <typename T> class Batch {
     BView<T> Xv{p(in), ld, istride}, Yv{p(outm), ld, istride};
     launch_operator<T>(op, S.pd<T>().plan, Xv, Yv, p(D), sel, kd.as<double>(), dots, dps, S.stream);
   }
-  // ca, cb: upper bounds of the column counts of A and B
+  // red_i = A_i^H B_i (pmv x pmv per instance, default pm); ca, cb: upper bounds of the
+  // column counts of A and B
   void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1) {
-    launch_gram<T>(A, B, ninst, K, nchunk, pmv < 0 ? pm : pmv, gramp.as<c64>(), S.stream, ca, cb);
+    const int pg = pmv < 0 ? pm : pmv;
+    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), S.stream, ca, cb);
+    launch_reduce(gramp.as<c64>(), ninst, nchunk, pg * pg, red.as<c64>(), S.stream);
   }
   // R <- R U^-1 twice (CholeskyQR2), R = list of one part with offset/count fields
   void cholqr(DevMem& Rm, const int* off, int cnt_field) {
@@ -241,7 +245,7 @@ template <typename T> class Batch {
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
       gram(Rl, Rl, M, M);
-      launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, gramp.as<c64>(), Cw.as<double>(), S.stream);
+      launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, red.as<c64>(), Cw.as<double>(), S.stream);
       launch_combine<T>(CMB_PLAIN, Rl, none, coef(Cw, f(cnt_field)), qmax, out(Rm, off), out(Rm, off), out(Rm, off),
                         ninst, K, nblk, nullptr, S.stream);
     }
@@ -255,7 +259,7 @@ template <typename T> class Batch {
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
       gram(ZR, Rl, 3 * M, M);
-      launch_dense_orth(dl, st.as<LState>(), shape.as<int>(), gramp.as<c64>(), pm, Cw.as<double>(), S.stream);
+      launch_dense_orth(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), pm, Cw.as<double>(), S.stream);
       launch_combine<T>(CMB_PLAIN, ZR, none, coef(Cw, f(F_NNEW)), qmax, out(Rm, f(F_NEWLY)), out(Rm, f(F_NEWLY)),
                         out(Rm, f(F_NEWLY)), ninst, K, nblk, nullptr, S.stream);
     }
@@ -318,7 +322,7 @@ template <typename T> class Batch {
         AY = list({part(AX[c], f(F_XOFF), f(F_NACT)), part(AR, f(F_ROFF), f(F_NACT)), part(AP[c], nullptr, f(F_NP))});
       }
       gram(Y, AY, it == 0 ? M : 3 * M, it == 0 ? M : 3 * M);
-      launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), gramp.as<c64>(), Cx.as<double>(), lam.as<double>(),
+      launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), Cx.as<double>(), lam.as<double>(),
                       it == 0 ? 1 : 0, sm);
       launch_combine<T>(CMB_RESID, Y, AY, coef(Cx, f(F_NACT), lam.as<double>()), qmax, out(X[n], f(F_XOFF)),
                         out(AX[n], f(F_XOFF)), out(R[n], nullptr, 0), ninst, K, nblk, normp.as<double>(), sm);
@@ -365,7 +369,7 @@ template <typename T> class Batch {
     ColList<T> b = list({part(AX[cur], f(F_EIG), f(F_ONE))});
     gram(a, b, 1, 1, 16);
     apply(OP_MKH, X[cur], X[cur], f(F_EIG), f(F_ONE), dotp.as<double>());
-    launch_newton(ninst, M, ns.as<NewtonState>(), st.as<LState>(), gramp.as<c64>(), nchunk, 16, dotp.as<double>(), dps,
+    launch_newton(ninst, M, ns.as<NewtonState>(), st.as<LState>(), red.as<c64>(), 1, 16, dotp.as<double>(), dps,
                   kd.as<double>(), sm);
     OMX_HIP_CHECK(hipMemcpyAsync(hns.data(), ns.as<void>(), sizeof(NewtonState) * ninst, hipMemcpyDeviceToHost, sm));
     OMX_HIP_CHECK(hipStreamSynchronize(sm));
@@ -396,7 +400,6 @@ template <typename T> class Batch {
     ColList<T> a = list({part(X[cur], nullptr, nullptr, 0, M)});
     ColList<T> b = list({part(AX[cur], nullptr, nullptr, 0, M)});
     gram(a, b, M, M);
-    launch_reduce(gramp.as<c64>(), ninst, nchunk, pm * pm, red.as<c64>(), S.stream);
     std::vector<c64> h(size_t(ninst) * pm * pm);
     OMX_HIP_CHECK(hipMemcpyAsync(h.data(), red.as<void>(), sizeof(c64) * h.size(), hipMemcpyDeviceToHost, S.stream));
     OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
