@@ -284,12 +284,11 @@ __device__ int d_ortho_against(c64* X, int rows, int cx, const c64* Y, int cy, d
 // column, which never rotates).  The rotation of pair (p, q) zeroes A[p][q]:
 //   rows  A'[p] = c A[p] - s A[q],          A'[q] = conj(s) A[p] + c A[q]
 //   cols  A'[:,p] = c A[:,p] - conj(s) A[:,q],  A'[:,q] = s A[:,p] + c A[:,q]
-// so A' = L A L^H with L block diagonal.  One LDS pass per round: the items are the
-// (m/2)^2 2x2 blocks of A -- each recomputes the two rotations it needs from the previous
-// round's matrix (read from one buffer, written to the other) -- and the m x m/2 row
-// pairs of V (updated in place).  A sweep is m - 1 rounds; sweeps stop when one performs
-// no rotation.  Returns the buffer holding the final matrix (eigenvalues on its diagonal);
-// V holds the eigenvectors in its columns.
+// so A' = L A L^H with L block diagonal.  A round is two LDS passes: the m/2 rotations
+// (one thread each), then the (m/2)^2 2x2 blocks of A (read from one buffer, written to
+// the other) and the m x m/2 row pairs of V (in place).  A sweep is m - 1 rounds; sweeps
+// stop when one performs no rotation.  Returns the buffer holding the final matrix
+// (eigenvalues on its diagonal); V holds the eigenvectors in its columns.
 __device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {
   const int ia = (t == 0) ? 0 : 1 + (t - 1 + r) % (m - 1);
   const int tb = m - 1 - t;
@@ -298,23 +297,38 @@ __device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {
   q = ia < ib ? ib : ia;
 }
 
+// c = cos(theta), s = sin(theta) e^{i arg A[p][q]} with tan(2 theta) = 2|A[p][q]| / (A[q][q] - A[p][p])
+// and |theta| <= pi/4: cos(2 theta) = |z| h, sin(2 theta) = h with z = (A[q][q] - A[p][p]) / (2|A[p][q]|)
+// and h = 1/sqrt(1 + z^2), so c = sqrt((1 + cos 2theta)/2) >= 1/sqrt(2) and |s| = sin(2theta)/(2c),
+// without cancellation.  Identity when |A[p][q]| is negligible.
 __device__ __forceinline__ bool jac_rot(const c64* A, int p, int q, double& c, c64& s) {
   const double app = A[p * DLD + p].x, aqq = A[q * DLD + q].x;
   const c64 apq = A[p * DLD + q];
-  const double a = sqrt(apq.x * apq.x + apq.y * apq.y);
+  const double a2 = apq.x * apq.x + apq.y * apq.y;
   c = 1.0;
   s = c64{0.0, 0.0};
-  if (!(a > 1e-300 && a > 1e-16 * sqrt(fabs(app * aqq)))) return false;
-  const c64 ph = c64{apq.x / a, apq.y / a};
-  const double zeta = (aqq - app) / (2.0 * a);
-  const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  c = 1.0 / sqrt(1.0 + tt * tt);
-  s = cscale(c * tt, ph);
+  if (!(a2 > 1e-300 && a2 > 1e-32 * fabs(app * aqq))) return false;
+  const double ia = rsqrt(a2);
+  const double z = 0.5 * (aqq - app) * ia;
+  const double az = fabs(z);
+  double sn;
+  if (az > 1e150) {
+    sn = 0.5 / az;
+  } else {
+    const double h = rsqrt(1.0 + z * z);
+    c = sqrt(0.5 * (1.0 + az * h));
+    sn = 0.5 * h / c;
+  }
+  const double f = (z >= 0 ? sn : -sn) * ia;
+  s = c64{f * apq.x, f * apq.y};
   return true;
 }
 
 __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
   __shared__ int rot[3];
+  __shared__ int rp[DMAX / 2], rq[DMAX / 2];
+  __shared__ double rc[DMAX / 2];
+  __shared__ c64 rs[DMAX / 2];
   const int m = n + (n & 1);
   const int npair = m / 2;
   for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
@@ -331,17 +345,25 @@ __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
     const int sl = sweep % 3;
     if (threadIdx.x == 0) rot[(sweep + 1) % 3] = 0;
     for (int r = 0; r < m - 1; ++r) {
+      if (threadIdx.x < npair) {
+        const int t = threadIdx.x;
+        int p, q;
+        jac_pair(t, r, m, p, q);
+        double c;
+        c64 sv;
+        if (jac_rot(Ac, p, q, c, sv)) rot[sl] = 1;
+        rp[t] = p;
+        rq[t] = q;
+        rc[t] = c;
+        rs[t] = sv;
+      }
+      __syncthreads();
       for (int idx = threadIdx.x; idx < nitem; idx += blockDim.x) {
         if (idx < nblk) {
           const int t1 = idx / npair, t2 = idx - (idx / npair) * npair;
-          int p1, q1, p2, q2;
-          jac_pair(t1, r, m, p1, q1);
-          jac_pair(t2, r, m, p2, q2);
-          double c1, c2;
-          c64 s1, s2;
-          const bool on = jac_rot(Ac, p1, q1, c1, s1);
-          jac_rot(Ac, p2, q2, c2, s2);
-          if (on && t1 == t2) rot[sl] = 1;
+          const int p1 = rp[t1], q1 = rq[t1], p2 = rp[t2], q2 = rq[t2];
+          const double c1 = rc[t1], c2 = rc[t2];
+          const c64 s1 = rs[t1], s2 = rs[t2];
           const c64 x00 = Ac[p1 * DLD + p2], x01 = Ac[p1 * DLD + q2];
           const c64 x10 = Ac[q1 * DLD + p2], x11 = Ac[q1 * DLD + q2];
           // rows with rotation 1
@@ -355,15 +377,12 @@ __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
         } else {
           const int e = idx - nblk;
           const int j = e / npair, t = e - (e / npair) * npair;
-          int p, q;
-          jac_pair(t, r, m, p, q);
-          double c;
-          c64 sv;
-          if (jac_rot(Ac, p, q, c, sv)) {
-            const c64 vp = V[j * DLD + p], vq = V[j * DLD + q];
-            V[j * DLD + p] = csub(cscale(c, vp), cmul(ccj(sv), vq));
-            V[j * DLD + q] = cadd(cmul(sv, vp), cscale(c, vq));
-          }
+          const int p = rp[t], q = rq[t];
+          const double c = rc[t];
+          const c64 sv = rs[t];
+          const c64 vp = V[j * DLD + p], vq = V[j * DLD + q];
+          V[j * DLD + p] = csub(cscale(c, vp), cmul(ccj(sv), vq));
+          V[j * DLD + q] = cadd(cmul(sv, vp), cscale(c, vq));
         }
       }
       __syncthreads();
