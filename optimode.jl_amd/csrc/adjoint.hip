@@ -295,7 +295,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
       any_ev = any_ev || hasv[i];
     }
   }
-  const int dps = ny / p64.plan.rows_per_block;
+  const int dps = p64.plan.dot_blocks;
   // slope = 2 HMkH(ev) for every band
   DevMem hdots(sizeof(double) * nb * dps);
   VecSel sel1{nb, 1, nullptr, nullptr};

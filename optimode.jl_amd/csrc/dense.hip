@@ -713,9 +713,11 @@ __global__ void k_advance(DenseLaunch dl, LState* st, int* shape) {
     const int nnew = shape[F_NNEW * ninst + i];
     const int niter_before = s.niter;
     if (nnew == 0) {
-      // finished this iteration: idle from now on
+      // finished this iteration (both X buffers already hold the final block): idle from
+      // now on, so that later iterations of the batch leave this instance untouched
       shape[F_NACT * ninst + i] = 0;
       shape[F_NP * ninst + i] = 0;
+      shape[F_NCOPY * ninst + i] = 0;
       continue;
     }
     s.prev_nlocked = s.nlocked;

@@ -44,10 +44,11 @@ __global__ __launch_bounds__(FB) void k_argmax_partial(const c64* D, long n3, do
   }
 }
 
-// one block per vector: finish the argmax, then multiply the vector by exp(-i arg E_max)
+// blocks (x, v): each finishes the argmax of vector v (the same fixed-order scan in every
+// block), then multiplies its slice of the vector by exp(-i arg E_max)
 __global__ __launch_bounds__(FB) void k_apply_phase(const c64* D, long n3, const double* pval, const long* pidx,
                                                     int nb, c64* vecs, long ldv, long K) {
-  const int v = blockIdx.x;
+  const int v = blockIdx.y;
   __shared__ c64 ph;
   if (threadIdx.x == 0) {
     double best = -1.0;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(FB) void k_apply_phase(const c64* D, long n3, const
   __syncthreads();
   const c64 p = ph;
   c64* x = vecs + long(v) * ldv;
-  for (long r = threadIdx.x; r < K; r += FB) {
+  for (long r = long(blockIdx.x) * FB + threadIdx.x; r < K; r += long(gridDim.x) * FB) {
     const c64 a = x[r];
     x[r] = c64{a.x * p.x - a.y * p.y, a.x * p.y + a.y * p.x};
   }
@@ -79,16 +80,17 @@ __global__ __launch_bounds__(FB) void k_apply_phase(const c64* D, long n3, const
 // as instances of mper vectors each.
 void Solver::canonicalize_phase(const double* d_k, c64* vecs, long ldv, long istride, int nvec, int mper) {
   const long n3 = 3 * N;
-  DevMem Dm(sizeof(c64) * size_t(nvec) * n3);
+  field_scratch.ensure(sizeof(c64) * size_t(nvec) * n3);
   VecSel sel{nvec / mper, mper, nullptr, nullptr};
   BView<double> X{vecs, ldv, istride};
-  launch_fields<double>(FIELD_E, p64.plan, X, Dm.as<c64>(), sel, d_k, 1.0, stream);
+  launch_fields<double>(FIELD_E, p64.plan, X, field_scratch.as<c64>(), sel, d_k, 1.0, stream);
   const int nb = 64;
-  DevMem pv(sizeof(double) * nvec * nb), pi(sizeof(long) * nvec * nb);
-  hipLaunchKernelGGL(k_argmax_partial, dim3(nb, nvec), dim3(FB), 0, stream, Dm.as<c64>(), n3, pv.as<double>(),
-                     pi.as<long>());
-  hipLaunchKernelGGL(k_apply_phase, dim3(nvec), dim3(FB), 0, stream, Dm.as<c64>(), n3, pv.as<double>(), pi.as<long>(),
-                     nb, vecs, ldv, K);
+  argmax_v.ensure(sizeof(double) * nvec * nb);
+  argmax_i.ensure(sizeof(long) * nvec * nb);
+  hipLaunchKernelGGL(k_argmax_partial, dim3(nb, nvec), dim3(FB), 0, stream, field_scratch.as<c64>(), n3,
+                     argmax_v.as<double>(), argmax_i.as<long>());
+  hipLaunchKernelGGL(k_apply_phase, dim3(64, nvec), dim3(FB), 0, stream, field_scratch.as<c64>(), n3,
+                     argmax_v.as<double>(), argmax_i.as<long>(), nb, vecs, ldv, K);
   OMX_HIP_CHECK(hipGetLastError());
   OMX_HIP_CHECK(hipStreamSynchronize(stream));
 }

@@ -1,5 +1,6 @@
 // Fused plane-wave Helmholtz operator kernels for gfx950 (see operator.hpp).
 #include "operator.hpp"
+#include <algorithm>
 #include "omx_errors.hpp"
 
 namespace omx {
@@ -347,6 +348,303 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass3_inv(OpPlan<T> plan, cplx<T>*
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Register-FFT path: power-of-two nx = GX VX, ny = GY VY; intermediate field [v][c][ix][iy].
+// ---------------------------------------------------------------------------------------
+
+// The same basis as basis_at, with one sqrt and one reciprocal square root:
+// |n x w| = |w| because n is a unit vector orthogonal to w.
+template <typename T>
+__device__ __forceinline__ Basis<T> basis_fast(int ix, int iy, int nx, int ny, double invdx, double invdy, T k) {
+  const T wx = -T(double(fft_index(ix, nx)) * invdx);
+  const T wy = -T(double(fft_index(iy, ny)) * invdy);
+  const T rho2 = wx * wx + wy * wy;
+  Basis<T> b;
+  b.mag = sqrt(rho2 + k * k);
+  if (rho2 > T(0)) {
+    const T ir = T(1) / sqrt(rho2);
+    b.n0 = -wy * ir;
+    b.n1 = wx * ir;
+  } else {
+    b.n0 = T(0);
+    b.n1 = T(1);
+  }
+  b.n2 = T(0);
+  const T im = T(1) / b.mag;
+  b.m0 = b.n1 * k * im;
+  b.m1 = -b.n0 * k * im;
+  b.m2 = (b.n0 * wy - b.n1 * wx) * im;
+  return b;
+}
+
+// pass 1: rows iy0 .. iy0 + RB - 1 of one vector; the LX/8 lanes of group r hold row
+// iy0 + r (lane g: ix = g + (LX/8) q, q < 8).  Curl in k-space, FFT along x, then the rows
+// are written column-major through an LDS tile (RB consecutive iy per ix: whole lines).
+template <typename T, int OP, int LX>
+__global__ __launch_bounds__((reg_row_threads<T, LX>())) void k_rpass1(OpPlan<T> plan, BView<T> X, cplx<T>* D,
+                                                                    VecSel sel, const double* kinst) {
+  constexpr int NT = reg_row_threads<T, LX>(), RB = reg_rows<T>(), GX = LX / 8;
+  constexpr int SG = reg_fft_scratch<LX>(), TS = RB + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* work = tw + LX;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int ny = plan.ny;
+  for (int t = threadIdx.x; t < LX; t += NT) tw[t] = plan.twx[t];
+  const int r = threadIdx.x / GX, g = threadIdx.x - r * GX;
+  const int iy0 = blockIdx.x * RB, iy = iy0 + r;
+  const T k = T(kinst[inst]);
+  const cplx<T>* Xv = X.base + inst * X.istride + col * X.ld;
+  cplx<T> d0[8], d1[8], d2[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int ix = g + GX * q;
+    const long p = ix + long(LX) * iy;
+    const cplx<T> h1 = Xv[2 * p], h2 = Xv[2 * p + 1];
+    const Basis<T> b = basis_fast<T>(ix, iy, LX, ny, plan.invdx, plan.invdy, k);
+    if (OP == OP_M) {
+      d0[q] = (h2 * b.m0 - h1 * b.n0) * b.mag;
+      d1[q] = (h2 * b.m1 - h1 * b.n1) * b.mag;
+      d2[q] = (h2 * b.m2) * b.mag;
+    } else if (OP == OP_P) {
+      const T im = T(1) / b.mag;
+      d0[q] = (h1 * b.n0 - h2 * b.m0) * im;
+      d1[q] = (h1 * b.n1 - h2 * b.m1) * im;
+      d2[q] = (h2 * b.m2) * (-im);
+    } else {
+      d0[q] = T(-1) * (h1 * b.m1 + h2 * b.n1);
+      d1[q] = h1 * b.m0 + h2 * b.n0;
+      d2[q] = cplx<T>{T(0), T(0)};
+    }
+  }
+  __syncthreads();  // twiddles
+  cplx<T>* xs = work + r * SG;
+  reg_fft<T, Dirs<OP>::x1, LX>(d0, g, xs, tw);
+  reg_fft<T, Dirs<OP>::x1, LX>(d1, g, xs, tw);
+  if (OP != OP_MKH) reg_fft<T, Dirs<OP>::x1, LX>(d2, g, xs, tw);
+  cplx<T>* Dv = D + long(v) * 3 * LX * ny;
+  auto store = [&](const cplx<T>(&dc)[8], int c) {
+    __syncthreads();  // FFT scratch and the previous tile are free
+#pragma unroll
+    for (int q = 0; q < 8; ++q) work[(g + GX * q) * TS + r] = dc[q];
+    __syncthreads();
+    cplx<T>* Dc = Dv + long(c) * LX * ny + iy0;
+#pragma unroll
+    for (int u = 0; u < LX * RB / NT; ++u) {
+      const int e = threadIdx.x + u * NT;
+      const int ix = e / RB, rr = e - (e / RB) * RB;
+      Dc[long(ix) * ny + rr] = work[ix * TS + rr];
+    }
+  };
+  store(d0, 0);
+  store(d1, 1);
+  store(d2, 2);
+}
+
+// pass 2: one column ix per group of LY/8 lanes (lane g holds iy = g + (LY/8) q): FFT along
+// y, eps^-1 (or eps_ave) per pixel, inverse FFT along y, in place.  No block barrier after
+// the twiddles are staged.
+template <typename T, int OP, int LY>
+__global__ __launch_bounds__(REG_COL_THREADS) void k_rpass2(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+  constexpr int NT = REG_COL_THREADS, GY = LY / 8, CB = NT / GY;
+  constexpr int SG = reg_fft_scratch<LY>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* work = tw + LY;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int nx = plan.nx;
+  for (int t = threadIdx.x; t < LY; t += NT) tw[t] = plan.twy[t];
+  const int cix = threadIdx.x / GY, g = threadIdx.x - cix * GY;
+  const int ix = blockIdx.x * CB + cix;
+  cplx<T>* D0 = D + long(v) * 3 * nx * LY + long(ix) * LY;
+  cplx<T>* D1 = D0 + long(nx) * LY;
+  cplx<T>* D2 = D1 + long(nx) * LY;
+  cplx<T> a0[8], a1[8], a2[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int iy = g + GY * q;
+    a0[q] = D0[iy];
+    a1[q] = D1[iy];
+    a2[q] = D2[iy];
+  }
+  __syncthreads();  // twiddles
+  cplx<T>* xs = work + cix * SG;
+  reg_fft<T, Dirs<OP>::y1, LY>(a0, g, xs, tw);
+  reg_fft<T, Dirs<OP>::y1, LY>(a1, g, xs, tw);
+  reg_fft<T, Dirs<OP>::y1, LY>(a2, g, xs, tw);
+  const T* ep = plan.eps6c + long(ix) * LY * 6;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const T* e = ep + (g + GY * q) * 6;
+    const cplx<T> x0 = a0[q], x1 = a1[q], x2 = a2[q];
+    if (OP == OP_P) {
+      const T ea = T(3) / (e[0] + e[1] + e[2]);
+      a0[q] = x0 * ea;
+      a1[q] = x1 * ea;
+      a2[q] = x2 * ea;
+    } else {
+      const T exx = e[0], eyy = e[1], ezz = e[2], exy = e[3], exz = e[4], eyz = e[5];
+      a0[q] = x0 * exx + x1 * exy + x2 * exz;
+      a1[q] = x0 * exy + x1 * eyy + x2 * eyz;
+      a2[q] = x0 * exz + x1 * eyz + x2 * ezz;
+    }
+  }
+  reg_fft<T, Dirs<OP>::y2, LY>(a0, g, xs, tw);
+  reg_fft<T, Dirs<OP>::y2, LY>(a1, g, xs, tw);
+  reg_fft<T, Dirs<OP>::y2, LY>(a2, g, xs, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int iy = g + GY * q;
+    D0[iy] = a0[q];
+    D1[iy] = a1[q];
+    D2[iy] = a2[q];
+  }
+}
+
+// pass 3: rows iy0 .. iy0 + RB - 1 read column-major through an LDS tile (the next
+// component's lines are fetched while the current one is unpacked), inverse FFT along x,
+// curl back onto (m, n) scaled by 1/N; for OP_MKH dotted with the input.
+template <typename T, int OP, int LX>
+__global__ __launch_bounds__((reg_row_threads<T, LX>())) void k_rpass3(OpPlan<T> plan, BView<T> Hin, BView<T> Y,
+                                                                    cplx<T>* D, VecSel sel, const double* kinst,
+                                                                    double* dot_partial, int dps) {
+  constexpr int NT = reg_row_threads<T, LX>(), RB = reg_rows<T>(), GX = LX / 8;
+  constexpr int SG = reg_fft_scratch<LX>(), TS = RB + 1, PER = LX * RB / NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* work = tw + LX;
+  const int v = blockIdx.y;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!vec_selected(sel, inst, col)) return;
+  const int ny = plan.ny;
+  for (int t = threadIdx.x; t < LX; t += NT) tw[t] = plan.twx[t];
+  const int r = threadIdx.x / GX, g = threadIdx.x - r * GX;
+  const int iy0 = blockIdx.x * RB, iy = iy0 + r;
+  const cplx<T>* Dv = D + long(v) * 3 * LX * ny + iy0;
+  cplx<T> e0[8], e1[8], e2[8];
+  cplx<T> pre[PER];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * NT;
+      const int ix = e / RB, rr = e - (e / RB) * RB;
+      pre[u] = Dv[long(c) * LX * ny + long(ix) * ny + rr];
+    }
+  };
+  auto unpack = [&](cplx<T>(&ec)[8], int cnext) {
+    __syncthreads();  // the tile is free
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * NT;
+      const int ix = e / RB, rr = e - (e / RB) * RB;
+      work[ix * TS + rr] = pre[u];
+    }
+    if (cnext < 3) fetch(cnext);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ec[q] = work[(g + GX * q) * TS + r];
+  };
+  fetch(0);
+  unpack(e0, 1);
+  unpack(e1, 2);
+  unpack(e2, 3);
+  __syncthreads();  // tile reads done: the FFT scratch overlays it
+  cplx<T>* xs = work + r * SG;
+  reg_fft<T, Dirs<OP>::x2, LX>(e0, g, xs, tw);
+  reg_fft<T, Dirs<OP>::x2, LX>(e1, g, xs, tw);
+  reg_fft<T, Dirs<OP>::x2, LX>(e2, g, xs, tw);
+  const T k = T(kinst[inst]);
+  const T scale = T(1.0 / (double(LX) * double(ny)));
+  double acc = 0.0;
+  const cplx<T>* Hv = Hin.base + inst * Hin.istride + col * Hin.ld;
+  cplx<T>* Yv = (OP == OP_MKH) ? nullptr : Y.base + inst * Y.istride + col * Y.ld;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int ix = g + GX * q;
+    const Basis<T> b = basis_fast<T>(ix, iy, LX, ny, plan.invdx, plan.invdy, k);
+    const cplx<T> en = e0[q] * b.n0 + e1[q] * b.n1;
+    const cplx<T> em = e0[q] * b.m0 + e1[q] * b.m1 + e2[q] * b.m2;
+    cplx<T> h1, h2;
+    if (OP == OP_P) {
+      const T sc = scale / b.mag;
+      h1 = en * sc;
+      h2 = em * (-sc);
+    } else {
+      const T sc = b.mag * scale;
+      h1 = en * (-sc);
+      h2 = em * sc;
+    }
+    const long p = ix + long(LX) * iy;
+    if (OP == OP_MKH) {
+      const cplx<T> a1 = Hv[2 * p], a2 = Hv[2 * p + 1];
+      acc += double(a1.x) * double(h1.x) + double(a1.y) * double(h1.y) + double(a2.x) * double(h2.x) +
+             double(a2.y) * double(h2.y);
+    } else {
+      Yv[2 * p] = h1;
+      Yv[2 * p + 1] = h2;
+    }
+  }
+  if (OP == OP_MKH) {
+    __shared__ double red[NT];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dot_partial[long(v) * dps + blockIdx.x] = -red[0];
+  }
+}
+
+template <typename T, int OP, int LX>
+void rop_rows(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+              const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  constexpr int NT = reg_row_threads<T, LX>(), RB = reg_rows<T>();
+  constexpr int SG = reg_fft_scratch<LX>(), TS = RB + 1;
+  const size_t lds = sizeof(cplx<T>) * (LX + std::max(RB * SG, LX * TS));
+  dim3 grid(plan.ny / RB, sel.ninst * sel.mper);
+  if (pass == 1)
+    hipLaunchKernelGGL((k_rpass1<T, OP, LX>), grid, dim3(NT), lds, stream, plan, X, D, sel, kinst);
+  else
+    hipLaunchKernelGGL((k_rpass3<T, OP, LX>), grid, dim3(NT), lds, stream, plan, X, Y, D, sel, kinst, dot_partial,
+                       dps);
+}
+
+template <typename T, int OP, int LY>
+void rop_cols(const OpPlan<T>& plan, cplx<T>* D, const VecSel& sel, hipStream_t stream) {
+  constexpr int NT = REG_COL_THREADS, CB = NT / (LY / 8);
+  const size_t lds = sizeof(cplx<T>) * (LY + CB * reg_fft_scratch<LY>());
+  dim3 grid(plan.nx / CB, sel.ninst * sel.mper);
+  hipLaunchKernelGGL((k_rpass2<T, OP, LY>), grid, dim3(NT), lds, stream, plan, D, sel);
+}
+
+template <typename T, int OP>
+void launch_rop(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel, const double* kinst,
+                double* dot_partial, int dps, hipStream_t stream) {
+  auto rows = [&](int pass) {
+    switch (plan.nx) {
+      case 64: rop_rows<T, OP, 64>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+      case 128: rop_rows<T, OP, 128>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+      case 256: rop_rows<T, OP, 256>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+      case 512: rop_rows<T, OP, 512>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+      default: throw Error(OMX_ERR_INTERNAL, "register FFT path: unsupported nx");
+    }
+  };
+  rows(1);
+  switch (plan.ny) {
+    case 64: rop_cols<T, OP, 64>(plan, D, sel, stream); break;
+    case 128: rop_cols<T, OP, 128>(plan, D, sel, stream); break;
+    case 256: rop_cols<T, OP, 256>(plan, D, sel, stream); break;
+    case 512: rop_cols<T, OP, 512>(plan, D, sel, stream); break;
+    default: throw Error(OMX_ERR_INTERNAL, "register FFT path: unsupported ny");
+  }
+  rows(3);
+}
+
 template <typename T, int VPT>
 void launch_fields_vpt(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                        const double* kinst, T scale, hipStream_t stream) {
@@ -386,6 +684,10 @@ void launch_op_vpt(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, co
 template <typename T, int OP>
 void launch_op(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
                const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  if (plan.reg) {
+    launch_rop<T, OP>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream);
+    return;
+  }
   switch (plan.vpt) {
     case 8: launch_op_vpt<T, OP, 8>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
     case 16: launch_op_vpt<T, OP, 16>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;

@@ -11,9 +11,17 @@
 // with the intermediate 3-component field in a scratch buffer (L2/MALL-resident for the
 // grid sizes of interest).  The basis (m, n, |k+G|) is recomputed on the fly from k and
 // the integer reciprocal-lattice indices, so no per-k basis array is ever stored.
+//
+// Two implementations share this interface.  Power-of-two nx, ny (64..512) use
+// register-resident FFTs (fft_reg.hpp: a group of L/8 lanes per 1D sequence, LDS
+// exchanges between radix passes) with the intermediate field column-major per component,
+// [v][c][ix][iy], so that the column pass reads and writes whole cache lines; eps^-1 is
+// kept column-major as well.  Other sizes use the LDS Stockham FFT (fft_lds.hpp) with the
+// intermediate field as [v][iy][ix][c].
 #pragma once
 #include "omx_common.hpp"
 #include "fft_lds.hpp"
+#include "fft_reg.hpp"
 
 namespace omx {
 
@@ -43,15 +51,26 @@ template <typename T> struct OpPlan {
   const cplx<T>* twx;             // device: exp(-2 pi i t/nx), t < nx
   const cplx<T>* twy;             // device: exp(-2 pi i t/ny)
   const T* eps6;                  // device: (ny, nx, 6) = xx, yy, zz, xy, xz, yz
-  int rows_per_block;             // pass 1/3
-  int cols_per_block;             // pass 2
-  int vpt;                        // values per thread bound (8, 16, 32)
+  const T* eps6c;                 // device: (nx, ny, 6), column-major copy (register path)
+  int rows_per_block;             // Stockham path: pass 1/3
+  int cols_per_block;             // Stockham path: pass 2
+  int vpt;                        // Stockham path: values per thread bound (8, 16, 32)
+  int reg;                        // 1: the operator uses the register-FFT path
+  int dot_blocks;                 // OP_MKH dot partials per vector (blocks of pass 3)
 };
+
+// Register path geometry: rows per block of passes 1 and 3 (the column-major intermediate
+// is written / read in runs of that many iy: whole 128-byte lines), threads per block of
+// pass 1 / 3 (a group of L/8 lanes per row) and of pass 2 (a group per column).
+template <typename T> constexpr int reg_rows() { return sizeof(T) == 4 ? 16 : 8; }
+template <typename T, int LX> constexpr int reg_row_threads() { return reg_rows<T>() * (LX / 8); }
+constexpr int REG_COL_THREADS = 256;
 
 // Launch one operator application on `stream`.
 //   X: input block, Y: output block (OP_M/OP_P), D: scratch of 3*N complex per vector
 //   (column v of the launch at D + v*3N), kinst: per-instance k (device, FP64).
-//   For OP_MKH, Y is unused and dot_out[v] receives -Re<H_v, MkH H_v> (FP64, device).
+//   For OP_MKH, Y is unused and dot_partial[v * dot_partial_stride + b], b < plan.dot_blocks,
+//   receive partial sums of -Re<H_v, MkH H_v> (FP64, device).
 template <typename T>
 void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D,
                      const VecSel& sel, const double* kinst, double* dot_partial, int dot_partial_stride,

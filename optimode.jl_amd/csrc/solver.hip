@@ -80,6 +80,8 @@ int choose_vpt(int nb, const FFTFactors& f) {
   return -1;
 }
 
+template <typename T> void select_operator(Solver& S, PrecisionData<T>& P);
+
 template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
   OpPlan<T>& pl = P.plan;
   pl.nx = S.nx;
@@ -127,6 +129,20 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
   const size_t lds13 = sizeof(cplx<T>) * (size_t(S.nx) + 3 * size_t(pl.rows_per_block) * S.nx);
   const size_t lds2 = sizeof(cplx<T>) * (size_t(S.ny) + 3 * size_t(pl.cols_per_block) * S.ny);
   if (lds13 > 160 * 1024 || lds2 > 160 * 1024) throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
+  // register-FFT operator for power-of-two sizes 64 .. 512
+  P.reg_ok = (reg_fft_ok(S.nx) && reg_fft_ok(S.ny) && S.ny % reg_rows<T>() == 0 &&
+              S.nx % (REG_COL_THREADS / (S.ny / 8)) == 0)
+                 ? 1
+                 : 0;
+  select_operator(S, P);
+}
+
+// operator implementation per the grid and the reg_ops option; the OP_MKH dot partial
+// count follows it
+template <typename T> void select_operator(Solver& S, PrecisionData<T>& P) {
+  OpPlan<T>& pl = P.plan;
+  pl.reg = (P.reg_ok && S.opt.reg_ops) ? 1 : 0;
+  pl.dot_blocks = S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block);
 }
 
 template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
@@ -148,6 +164,14 @@ template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
   P.eps6.ensure(sizeof(T) * 6 * N);
   OMX_HIP_CHECK(hipMemcpy(P.eps6.template as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
   P.plan.eps6 = P.eps6.template as<T>();
+  // column-major copy: pixel (ix, iy) at 6 (iy + ny ix)
+  std::vector<T> e6c(size_t(6) * N);
+  for (int ix = 0; ix < S.nx; ++ix)
+    for (int iy = 0; iy < S.ny; ++iy)
+      for (int a = 0; a < 6; ++a) e6c[6 * (iy + size_t(S.ny) * ix) + a] = e6[6 * (ix + size_t(S.nx) * iy) + a];
+  P.eps6c.ensure(sizeof(T) * 6 * N);
+  OMX_HIP_CHECK(hipMemcpy(P.eps6c.template as<void>(), e6c.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
+  P.plan.eps6c = P.eps6c.template as<T>();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -162,6 +186,8 @@ template <typename T> class Batch {
   DevMem X[2], AX[2], R[2], AR, P[2], AP[2], D;
   DevMem st, shape, gramp, red, normp, Cx, Cp, Cw, lam, rhist, dotp, active, band, kd, ns;
   DenseLaunch dl{};
+  PinnedMem snap[2];  // LState snapshots of consecutive iterations
+  hipEvent_t snap_ev[2] = {nullptr, nullptr};
   int pm, nchunk, nblk, qmax, dps;
   int cur = 0;
   int maxiter_alloc;
@@ -186,7 +212,7 @@ template <typename T> class Batch {
     // about 512 Gram blocks (2 per CU), at least 512 rows each
     nchunk = std::max(1, std::min<int>(int((K + 511) / 512), std::max(1, 512 / ninst)));
     nblk = std::max(1, std::min<int>(int((K + 255) / 256), std::max(1, 1024 / ninst)));
-    dps = S.ny / S.pd<T>().plan.rows_per_block;
+    dps = S.pd<T>().plan.dot_blocks;
     st.alloc(sizeof(LState) * ninst);
     shape.alloc(sizeof(int) * F_NFIELDS * ninst);
     OMX_HIP_CHECK(hipMemset(shape.as<void>(), 0, sizeof(int) * F_NFIELDS * ninst));
@@ -209,7 +235,17 @@ template <typename T> class Batch {
     dl.nblk = nblk;
     dl.qmax = qmax;
     hst.resize(ninst);
+    for (int b = 0; b < 2; ++b) {
+      snap[b].alloc(sizeof(LState) * ninst);
+      OMX_HIP_CHECK(hipEventCreateWithFlags(&snap_ev[b], hipEventDisableTiming));
+    }
   }
+  ~Batch() {
+    for (int b = 0; b < 2; ++b)
+      if (snap_ev[b]) (void)hipEventDestroy(snap_ev[b]);
+  }
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
 
   cplx<T>* p(DevMem& m) { return m.as<cplx<T>>(); }
   int* f(int field) { return shape.as<int>() + field * ninst; }
@@ -294,24 +330,24 @@ template <typename T> class Batch {
     apply(OP_M, X[cur], AX[cur], f(F_XOFF), f(F_M));
     stats.matvec += ncols_active;
     ColList<T> none{};
-    std::vector<int> prev_nact(msize);
-    int iters = 0;
-    for (int it = 0;; ++it) {
+    // The host reads the per-instance state one iteration late: iteration it is enqueued
+    // before the state after iteration it - 1 is examined, so the device never waits for
+    // the host.  An iteration enqueued after every instance has finished does nothing
+    // (every kernel skips instances with no active columns).
+    std::vector<int> nact(ninst);  // active columns entering the next examined iteration
+    for (int i = 0; i < ninst; ++i) nact[i] = act[i] ? msize[i] : 0;
+    struct OpMark {
+      hipEvent_t a, b;
+    };
+    std::vector<OpMark> marks;  // operator intervals of iterations 1.., bytes attributed on examination
+    const bool prof = S.prof.enabled;
+    int it = 0;
+    for (;; ++it) {
       const int c = cur, n = cur ^ 1;
       if (it > 0) {
-        if (S.prof.enabled) {
-          long m_act = 0;
-          for (int i = 0; i < ninst; ++i)
-            if (act[i]) m_act += prev_nact[i];
-          // algorithmic bytes of one application: read + write the vectors, read eps^-1 once
-          const double w = double(sizeof(T)) / 4.0;
-          const double bytes = w * (32.0 * S.N * m_act + 24.0 * S.N);
-          hipEvent_t a = S.prof.begin(sm);
-          apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
-          S.prof.end(sm, a, sizeof(T) == 4 ? &S.prof.op32 : &S.prof.op64, bytes);
-        } else {
-          apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
-        }
+        hipEvent_t a = prof ? S.prof.begin(sm) : nullptr;
+        apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
+        if (prof) marks.push_back(OpMark{a, S.prof.begin(sm)});
       }
       ColList<T> Y, AY;
       if (it == 0) {
@@ -337,27 +373,47 @@ template <typename T> class Batch {
       ortho_r(X[n], P[n], R[n]);
       launch_advance(dl, st.as<LState>(), shape.as<int>(), sm);
       cur = n;
-      OMX_HIP_CHECK(hipMemcpyAsync(hst.data(), st.as<void>(), sizeof(LState) * ninst, hipMemcpyDeviceToHost, sm));
-      OMX_HIP_CHECK(hipStreamSynchronize(sm));
+      OMX_HIP_CHECK(hipMemcpyAsync(snap[it & 1].as<void>(), st.as<void>(), sizeof(LState) * ninst,
+                                   hipMemcpyDeviceToHost, sm));
+      OMX_HIP_CHECK(hipEventRecord(snap_ev[it & 1], sm));
+      if (it == 0) continue;
+      // examine the state after iteration it - 1
+      OMX_HIP_CHECK(hipEventSynchronize(snap_ev[(it - 1) & 1]));
+      const LState* hs = snap[(it - 1) & 1].as<LState>();
       bool all = true;
+      long m_next = 0;
       for (int i = 0; i < ninst; ++i) {
         if (!act[i]) continue;
-        if (it > 0 && prev_nact[i] > 0) stats.matvec += prev_nact[i];
-        if (prev_nact[i] > 0) stats.precon += prev_nact[i];
-        if (!hst[i].finished) all = false;
-        prev_nact[i] = hst[i].finished ? 0 : hst[i].nact;
+        if (nact[i] > 0) {
+          stats.precon += nact[i];
+          if (it - 1 > 0) stats.matvec += nact[i];
+        }
+        if (!hs[i].finished) all = false;
+        nact[i] = hs[i].finished ? 0 : hs[i].nact;
+        m_next += nact[i];
       }
-      ++iters;
+      if (prof) {
+        // iteration it applied the operator to the columns active after iteration it - 1
+        const double w = double(sizeof(T)) / 4.0;
+        const OpMark& mk = marks.back();
+        if (m_next > 0)
+          S.prof.commit(mk.a, mk.b, sizeof(T) == 4 ? &S.prof.op32 : &S.prof.op64,
+                        w * (32.0 * S.N * m_next + 24.0 * S.N));
+        else
+          S.prof.recycle(mk.a, mk.b);
+      }
       if (all) break;
       if (it > maxiter + 2) throw Error(OMX_ERR_INTERNAL, "LOBPCG driver did not terminate");
     }
+    // iteration it changed nothing: the state after it equals the state after it - 1
+    OMX_HIP_CHECK(hipEventSynchronize(snap_ev[it & 1]));
+    std::memcpy(hst.data(), snap[it & 1].as<LState>(), sizeof(LState) * ninst);
     int tot = 0;
     for (int i = 0; i < ninst; ++i)
       if (act[i]) tot += hst[i].niter;
     for (int i = 0; i < ninst; ++i)
       if (act[i] && hst[i].status == OMX_ERR_CHOLESKY)
         throw Error(OMX_ERR_CHOLESKY, "Cholesky orthogonalisation failed even after shifting");
-    (void)iters;
     return tot;
   }
 
@@ -448,6 +504,8 @@ Solver::~Solver() {
 void Solver::set_epsinv(const double* epsinv) {
   if (!epsinv) throw Error(OMX_ERR_INVALID_ARG, "epsinv is NULL");
   const long n9 = 9 * N;
+  // unchanged field (a solve per frequency or per Newton sweep on one geometry): nothing to do
+  if (long(epsinv_host.size()) == n9 && std::memcmp(epsinv_host.data(), epsinv, sizeof(double) * n9) == 0) return;
   double amax = 0;
   for (long i = 0; i < n9; ++i) {
     if (!std::isfinite(epsinv[i])) throw Error(OMX_ERR_INVALID_ARG, "epsinv has non-finite entries");
@@ -479,7 +537,12 @@ void Solver::set_option(const std::string& key, double v) {
     opt.guard = int(v);
   else if (key == "seed")
     opt.seed = uint64_t(v);
-  else if (key == "fp64_only")
+  else if (key == "reg_ops") {
+    opt.reg_ops = v != 0 ? 1 : 0;
+    select_operator(*this, p32);
+    select_operator(*this, p64);
+    workspaces.clear();  // batches size their dot partials by the operator in use
+  } else if (key == "fp64_only")
     opt.fp64_only = int(v);
   else if (key == "profile") {
     prof.reset();
@@ -501,7 +564,7 @@ void Solver::apply(OpKind op, int prec, double k, const double* Hin, double* Hou
   DevMem kbuf(sizeof(double));
   OMX_HIP_CHECK(hipMemcpyAsync(kbuf.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
   OMX_HIP_CHECK(hipMemcpyAsync(scratch_a.as<void>(), Hin, nbytes64, hipMemcpyHostToDevice, stream));
-  const int dps32 = ny / p32.plan.rows_per_block, dps64 = ny / p64.plan.rows_per_block;
+  const int dps32 = p32.plan.dot_blocks, dps64 = p64.plan.dot_blocks;
   const int dps = prec == OMX_PREC_FP32 ? dps32 : dps64;
   DevMem dotbuf(sizeof(double) * size_t(m) * dps);
   VecSel sel{1, m, nullptr, nullptr};
@@ -536,6 +599,22 @@ void Solver::apply(OpKind op, int prec, double k, const double* Hin, double* Hou
 
 namespace {
 
+// A batch of the requested shape from the handle's workspace cache (created on first use;
+// at most four shapes are kept, the oldest is freed first).  Returned with cur = 0.
+template <typename T> Batch<T>& workspace(Solver& S, int ninst, int M, int maxiter) {
+  const int tb = int(sizeof(T));
+  for (auto& w : S.workspaces)
+    if (w.tbytes == tb && w.ninst == ninst && w.M == M && w.maxiter >= maxiter) {
+      Batch<T>& b = *static_cast<Batch<T>*>(w.batch.get());
+      b.cur = 0;
+      return b;
+    }
+  if (S.workspaces.size() >= 4) S.workspaces.erase(S.workspaces.begin());
+  std::shared_ptr<void> p(new Batch<T>(S, ninst, M, maxiter), [](void* q) { delete static_cast<Batch<T>*>(q); });
+  S.workspaces.push_back(Solver::Workspace{tb, ninst, M, maxiter, p});
+  return *static_cast<Batch<T>*>(p.get());
+}
+
 template <typename T>
 void upload_block(Solver& S, Batch<T>& b, const double* H, int ncols, int inst_from, int inst_to) {
   // H: ncols host vectors (K complex FP64 each) -> columns 0..ncols-1 of instances [from, to)
@@ -561,10 +640,10 @@ void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_
   OMX_HIP_CHECK(hipEventRecord(e0, stream));
   SolveStats stats;
   std::vector<int> act{1}, bands{0}, nchk{nev};
-  Batch<double> b64(*this, 1, nev, std::max(maxiter, opt.maxiter_fp64));
+  Batch<double>& b64 = workspace<double>(*this, 1, nev, std::max(maxiter, opt.maxiter_fp64));
   OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
   if (!opt.fp64_only) {
-    Batch<float> b32(*this, 1, nev, maxiter);
+    Batch<float>& b32 = workspace<float>(*this, 1, nev, maxiter);
     OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
     if (H_io)
       upload_block<float>(*this, b32, H_io, nev, 0, 1);
@@ -697,7 +776,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     msize[i] = opt.guard < 0 ? M : std::min(M, i + 1 + opt.guard);
   }
   const int maxit64 = std::max(maxiter, opt.maxiter_fp64);
-  Batch<double> b64(*this, ninst, M, maxit64);
+  Batch<double>& b64 = workspace<double>(*this, ninst, M, maxit64);
   int newton_total = 0;
   const int nguess_used = (Hguess && nguess > 0) ? std::min(nguess, M) : 0;
   const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
@@ -739,7 +818,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   };
 
   if (!opt.fp64_only) {
-    Batch<float> b32(*this, ninst, M, maxiter);
+    Batch<float>& b32 = workspace<float>(*this, ninst, M, maxiter);
     launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
     if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);

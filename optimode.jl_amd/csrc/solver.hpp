@@ -57,7 +57,8 @@ class DevMem {
 
 template <typename T> struct PrecisionData {
   OpPlan<T> plan{};
-  DevMem eps6, twx, twy;
+  DevMem eps6, eps6c, twx, twy;
+  int reg_ok = 0;  // the grid admits the register-FFT operator
 };
 
 struct SolveStats {
@@ -76,6 +77,26 @@ struct KernelStat {
   double ms = 0;
   long launches = 0;
   double bytes = 0;
+};
+
+// Page-locked host memory (hipHostMalloc) for small device-to-host snapshots.
+class PinnedMem {
+ public:
+  PinnedMem() = default;
+  ~PinnedMem() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  PinnedMem(const PinnedMem&) = delete;
+  PinnedMem& operator=(const PinnedMem&) = delete;
+  void alloc(size_t bytes) {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    if (bytes) OMX_HIP_CHECK(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
+  }
+  template <typename U> U* as() const { return static_cast<U*>(p_); }
+
+ private:
+  void* p_ = nullptr;
 };
 
 class Profiler {
@@ -104,6 +125,13 @@ class Profiler {
  public:
   hipEvent_t begin(hipStream_t s);
   void end(hipStream_t s, hipEvent_t a, KernelStat* st, double bytes);
+  // Two recorded events whose interval is attributed later, once its byte count is known.
+  void commit(hipEvent_t a, hipEvent_t b, KernelStat* st, double bytes) { pending_.push_back(Pending{a, b, st, bytes}); }
+  // Return two recorded events unused (an interval that did no work).
+  void recycle(hipEvent_t a, hipEvent_t b) {
+    pool_.push_back(a);
+    pool_.push_back(b);
+  }
 };
 
 struct Options {
@@ -113,6 +141,7 @@ struct Options {
   int guard = 2;                // block columns of band i: min(nev, i + 1 + guard); < 0: nev
   uint64_t seed = 0x5eed;
   int fp64_only = 0;            // skip the FP32 phase
+  int reg_ops = 1;              // register-FFT operator where the grid admits it
 };
 
 class Solver {
@@ -147,7 +176,16 @@ class Solver {
   int last_nev = 0;
   int last_adjoint_iters = 0;
   DevMem scratch_a, scratch_b, scratch_c;
+  DevMem field_scratch, argmax_v, argmax_i;  // canonicalize_phase
   Profiler prof;
+
+  // LOBPCG workspaces kept between solves (a solve allocates nothing once its shape has
+  // been seen): type-erased batches keyed by (precision bytes, instances, block, maxiter).
+  struct Workspace {
+    int tbytes, ninst, M, maxiter;
+    std::shared_ptr<void> batch;
+  };
+  std::vector<Workspace> workspaces;
 
   template <typename T> PrecisionData<T>& pd();
   void canonicalize_phase(const double* d_k, c64* vecs, long ldv, long istride, int nvec, int mper);

@@ -75,11 +75,10 @@ class _Handle:
             raise OmxError(st, L.omx_last_error(None).decode() or L.omx_status_string(st).decode())
         self.h = h
         self.grid = grid
-        self.eps_key = epsinv_c.tobytes()
+        self.eps_src = None  # the eps^-1 array (caller layout) now on the device
 
     def set_epsinv(self, epsinv_c):
         _lib.check(_lib.lib().omx_set_epsinv(self.h, _lib.dptr(epsinv_c)), self.h)
-        self.eps_key = epsinv_c.tobytes()
 
     def close(self):
         if self.h:
@@ -99,11 +98,13 @@ class HIPSolver:
     ``precision`` is the policy of the iterative phase: ``"mixed"`` (default: FP32
     operator applications, then an FP64 refinement phase) or ``"fp64"``.  Tolerance
     knobs of the FP32 phase are relative to ω²: ``fp32_tol`` (LOBPCG locking threshold)
-    and ``fp32_switch`` (|Δω²|/ω² at which Newton moves to FP64).
+    and ``fp32_switch`` (|Δω²|/ω² at which Newton moves to FP64).  ``reg_ops=False`` keeps
+    the FP32 operator on the LDS Stockham FFT even where the register-FFT kernels apply
+    (power-of-two grids up to 256).
     """
 
     def __init__(self, device=0, precision="mixed", fp32_tol=None, fp32_switch=None, maxiter_fp64=None, seed=None,
-                 profile=False):
+                 profile=False, reg_ops=None):
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         self.device = int(device)
@@ -117,19 +118,24 @@ class HIPSolver:
             self.options["maxiter_fp64"] = float(maxiter_fp64)
         if seed is not None:
             self.options["seed"] = float(seed)
+        if reg_ops is not None:
+            self.options["reg_ops"] = 1.0 if reg_ops else 0.0
         self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
         self.profile = bool(profile)
         self._h = None
 
     def handle(self, grid, epsinv):
-        ec = _epsinv_c(epsinv, grid)
         h = self._h
-        if h is None or (h.grid.Nx, h.grid.Ny, h.grid.Dx, h.grid.Dy) != (grid.Nx, grid.Ny, grid.Dx, grid.Dy):
+        same_grid = h is not None and (h.grid.Nx, h.grid.Ny, h.grid.Dx, h.grid.Dy) == (grid.Nx, grid.Ny, grid.Dx, grid.Dy)
+        if not same_grid:
+            ec = _epsinv_c(epsinv, grid)
             if h is not None:
                 h.close()
             h = self._h = _Handle(self.device, grid, ec)
-        elif h.eps_key != ec.tobytes():
-            h.set_epsinv(ec)
+            h.eps_src = np.array(epsinv, dtype=np.float64, copy=True)
+        elif not (np.shape(epsinv) == h.eps_src.shape and np.array_equal(epsinv, h.eps_src)):
+            h.set_epsinv(_epsinv_c(epsinv, grid))
+            h.eps_src = np.array(epsinv, dtype=np.float64, copy=True)
         L = _lib.lib()
         for k, v in self.options.items():
             _lib.check(L.omx_set_option(h.h, k.encode(), v), h.h)

@@ -32,7 +32,8 @@ def _relerr(a, b):
 
 
 SIZES = [(6.0, 4.0, 24, 16), (4.0, 3.0, 12, 10), (6.0, 4.0, 128, 128), (6.0, 4.0, 192, 192),
-         (8.0, 4.0, 256, 256), (8.0, 4.0, 384, 384), (8.0, 4.0, 512, 256), (5.0, 5.0, 60, 30)]
+         (8.0, 4.0, 256, 256), (8.0, 4.0, 384, 384), (8.0, 4.0, 512, 256), (5.0, 5.0, 60, 30),
+         (4.0, 4.0, 64, 64), (6.0, 4.0, 128, 256), (8.0, 3.0, 256, 64)]
 
 
 @pytest.mark.parametrize("size", SIZES, ids=lambda s: f"{s[2]}x{s[3]}")
@@ -78,3 +79,25 @@ def test_fp64_hermitian_and_homogeneous():
         lam = (gx[ix] ** 2 + gy[iy] ** 2 + 0.81) / eps
         assert np.allclose(Mh * vv, lam * vv, atol=1e-13)
         assert np.allclose(Mh.precondition(Mh * vv), vv, atol=1e-13)
+
+
+@pytest.mark.parametrize("size", [(4.0, 4.0, 64, 64), (6.0, 4.0, 128, 256), (8.0, 4.0, 256, 256)],
+                         ids=lambda s: f"{s[2]}x{s[3]}")
+def test_fp32_register_and_stockham_operators_agree(size):
+    """The two FP32 operator implementations (register FFT, LDS Stockham FFT) give the same
+    M, P and HMkH to FP32 rounding on power-of-two grids."""
+    print(f"register vs Stockham {size}", flush=True)
+    Dx, Dy, nx, ny = size
+    og = O.Grid(Dx, Dy, nx, ny)
+    g = omx.Grid(Dx, Dy, nx, ny)
+    ei = _aniso(og)
+    rng = np.random.default_rng(7)
+    H = _rand(rng, 2 * og.N, 4)
+    out = {}
+    for reg in (True, False):
+        solver = omx.HIPSolver(reg_ops=reg)
+        Mg = omx.HelmholtzMap(0.9, ei, g, solver, precision=omx.OMX_PREC_FP32)
+        out[reg] = (Mg * H, Mg.precondition(H), Mg.HMkH(H))
+    assert _relerr(out[True][0], out[False][0]) < 2e-6
+    assert _relerr(out[True][1], out[False][1]) < 2e-6
+    assert np.allclose(out[True][2], out[False][2], rtol=2e-5, atol=0)
