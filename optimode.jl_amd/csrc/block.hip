@@ -86,9 +86,12 @@ __device__ __forceinline__ void load_rows(const cplx<double>* col, long row, lon
   }
 }
 
-template <typename T, int TA, int TB>
+// BPRE: B is the leading npb (<= 16) columns of A, so the B tile is taken from A's first
+// tile instead of being loaded again.
+template <typename T, int TA, int TB, bool BPRE>
 __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long K, long rows_per_chunk, c64* partial,
                                                 int pm) {
+  static_assert(!BPRE || TB == 1, "a B prefix of A fits one tile");
   using acc_t = typename Acc4<T>::type;
   // rows per lane and stage: 64 bytes of each tile column, halved for wide tile sets
   constexpr int S = (sizeof(T) == 4 ? 8 : 4) / ((TA + TB) > 4 ? 2 : 1);
@@ -127,13 +130,16 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
   const long r1 = (r0 + rows_per_chunk < K) ? r0 + rows_per_chunk : K;
   long rs = r0 + long(wave) * WR;
   cplx<T> a[TA][S], b[TB][S];
+  const bool bown = i16 < npb;  // BPRE: this lane's B column is its A column
   if (rs < r1) {
 #pragma unroll
     for (int t = 0; t < TA; ++t)
       if (t < nta) load_rows<S>(ap[t], rs + g * S, r1, a[t]);
+    if constexpr (!BPRE) {
 #pragma unroll
-    for (int t = 0; t < TB; ++t)
-      if (t < ntb) load_rows<S>(bp[t], rs + g * S, r1, b[t]);
+      for (int t = 0; t < TB; ++t)
+        if (t < ntb) load_rows<S>(bp[t], rs + g * S, r1, b[t]);
+    }
   }
   while (rs < r1) {
     const long rn = rs + STRIDE;
@@ -142,9 +148,15 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
 #pragma unroll
       for (int t = 0; t < TA; ++t)
         if (t < nta) load_rows<S>(ap[t], rn + g * S, r1, na[t]);
+      if constexpr (!BPRE) {
 #pragma unroll
-      for (int t = 0; t < TB; ++t)
-        if (t < ntb) load_rows<S>(bp[t], rn + g * S, r1, nb[t]);
+        for (int t = 0; t < TB; ++t)
+          if (t < ntb) load_rows<S>(bp[t], rn + g * S, r1, nb[t]);
+      }
+    }
+    if constexpr (BPRE) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) b[0][s] = bown ? a[0][s] : cplx<T>{T(0), T(0)};
     }
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -163,10 +175,12 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
     for (int t = 0; t < TA; ++t)
 #pragma unroll
       for (int s = 0; s < S; ++s) a[t][s] = na[t][s];
+    if constexpr (!BPRE) {
 #pragma unroll
-    for (int t = 0; t < TB; ++t)
+      for (int t = 0; t < TB; ++t)
 #pragma unroll
-      for (int s = 0; s < S; ++s) b[t][s] = nb[t][s];
+        for (int s = 0; s < S; ++s) b[t][s] = nb[t][s];
+    }
     rs = rn;
   }
   // sum the four waves tile by tile; wave w finishes accumulator register w
@@ -374,19 +388,23 @@ __global__ __launch_bounds__(BLK) void k_copy_cols(const cplx<T>* src, cplx<T>* 
 }  // namespace
 
 template <typename T, int TA>
-static void gram_tb(int tb, dim3 grid, hipStream_t stream, const ColList<T>& A, const ColList<T>& B, long K, long rpc,
-                    c64* partial, int pm) {
+static void gram_tb(int tb, bool bpre, dim3 grid, hipStream_t stream, const ColList<T>& A, const ColList<T>& B, long K,
+                    long rpc, c64* partial, int pm) {
+  if (bpre) {
+    hipLaunchKernelGGL((k_gram16<T, TA, 1, true>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
+    return;
+  }
   switch (tb) {
-    case 1: hipLaunchKernelGGL((k_gram16<T, TA, 1>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
-    case 2: hipLaunchKernelGGL((k_gram16<T, TA, 2>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
-    default: hipLaunchKernelGGL((k_gram16<T, TA, 3>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
+    case 1: hipLaunchKernelGGL((k_gram16<T, TA, 1, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
+    case 2: hipLaunchKernelGGL((k_gram16<T, TA, 2, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
+    default: hipLaunchKernelGGL((k_gram16<T, TA, 3, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
   }
 }
 
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
-                 hipStream_t stream, int ca, int cb) {
-  if (ca <= 0 || cb <= 0 || ca > 48 || cb > 48 || ca > pm || cb > pm || pm % 16 != 0)
+                 hipStream_t stream, int ca, int cb, bool b_prefix_of_a) {
+  if (ca <= 0 || cb <= 0 || ca > 48 || cb > 48 || ca > pm || cb > pm || pm % 16 != 0 || (b_prefix_of_a && cb > 16))
     throw Error(OMX_ERR_INTERNAL, "gram: column bounds out of range");
   // chunks are whole block stages (128 rows), so only the last one can end mid-stage
   long rpc = (K + nchunk - 1) / nchunk;
@@ -394,9 +412,9 @@ void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, in
   dim3 grid(nchunk, ninst);
   const int ta = (ca + 15) / 16, tb = (cb + 15) / 16;
   switch (ta) {
-    case 1: gram_tb<T, 1>(tb, grid, stream, A, B, K, rpc, partial, pm); break;
-    case 2: gram_tb<T, 2>(tb, grid, stream, A, B, K, rpc, partial, pm); break;
-    default: gram_tb<T, 3>(tb, grid, stream, A, B, K, rpc, partial, pm); break;
+    case 1: gram_tb<T, 1>(tb, b_prefix_of_a, grid, stream, A, B, K, rpc, partial, pm); break;
+    case 2: gram_tb<T, 2>(tb, b_prefix_of_a, grid, stream, A, B, K, rpc, partial, pm); break;
+    default: gram_tb<T, 3>(tb, b_prefix_of_a, grid, stream, A, B, K, rpc, partial, pm); break;
   }
   OMX_HIP_CHECK(hipGetLastError());
 }
@@ -459,8 +477,10 @@ void launch_copy_cols(const cplx<T>* src, cplx<T>* dst, long ld, long istride, i
 
 template void launch_copy_cols<float>(const cplx<float>*, cplx<float>*, long, long, int, const int*, const int*, long, hipStream_t);
 template void launch_copy_cols<double>(const cplx<double>*, cplx<double>*, long, long, int, const int*, const int*, long, hipStream_t);
-template void launch_gram<float>(const ColList<float>&, const ColList<float>&, int, long, int, int, c64*, hipStream_t, int, int);
-template void launch_gram<double>(const ColList<double>&, const ColList<double>&, int, long, int, int, c64*, hipStream_t, int, int);
+template void launch_gram<float>(const ColList<float>&, const ColList<float>&, int, long, int, int, c64*, hipStream_t, int, int,
+                                 bool);
+template void launch_gram<double>(const ColList<double>&, const ColList<double>&, int, long, int, int, c64*, hipStream_t,
+                                  int, int, bool);
 template void launch_combine<float>(CombineMode, const ColList<float>&, const ColList<float>&, const CombineCoef&, int,
                                     OutView<float>, OutView<float>, OutView<float>, int, long, int, double*, hipStream_t);
 template void launch_combine<double>(CombineMode, const ColList<double>&, const ColList<double>&, const CombineCoef&, int,

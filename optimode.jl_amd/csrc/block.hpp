@@ -41,10 +41,11 @@ template <typename T> struct OutView {
 // G_i = A_i^H B_i on the matrix cores, written as per-chunk partial sums (FP64) to
 // partial[(i*nchunk + chunk)*pm*pm + a*pm + b] for a < 16 ceil(nA_i/16), b < 16 ceil(nB_i/16)
 // (other entries untouched).  pm in {16, 32, 48}; ca, cb (<= pm, <= 48) bound the column
-// counts of A and B.
+// counts of A and B.  b_prefix_of_a: B is the leading columns of A (cb <= 16), whose values
+// are then loaded once.
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
-                 hipStream_t stream, int ca = 0, int cb = 0);
+                 hipStream_t stream, int ca, int cb, bool b_prefix_of_a = false);
 
 // out_i[e] = sum_chunk partial[(i*nchunk + chunk)*esz + e], fixed order (deterministic).
 void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream);

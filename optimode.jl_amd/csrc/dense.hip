@@ -618,11 +618,11 @@ __global__ __launch_bounds__(DBLK) void k_dense_chol(DenseLaunch dl, LState* st,
   }
 }
 
-// One pass of R <- (R - Z B) U^{-1}: from the Gram partials of [Z R]^H R (Z = the M
-// X columns plus F_NPN P columns, R = F_NNEW columns) form B = Z^H R, S = R^H R - B^H B
-// (the Gram of the projected R for orthonormal Z), the upper Cholesky S = U^H U with
-// DFTK's shifting on failure (dftk.jl:163-192), and the coefficients
-// C = [-B U^{-1}; U^{-1}] of the update R <- [Z R] C.
+// One pass of R <- (R - Z B) U^{-1}: from the Gram of [R Z]^H R (R = F_NNEW columns, Z =
+// the M X columns plus F_NPN P columns) form B = Z^H R, S = R^H R - B^H B (the Gram of the
+// projected R for orthonormal Z), the upper Cholesky S = U^H U with DFTK's shifting on
+// failure (dftk.jl:163-192), and the coefficients C = [U^{-1}; -B U^{-1}] of the update
+// R <- [R Z] C.
 __global__ __launch_bounds__(DBLK) void k_dense_orth(DenseLaunch dl, LState* st, const int* shape, const c64* gp,
                                                      int pmg, double* C) {
   __shared__ c64 G[DMAX * SLD];
@@ -646,10 +646,11 @@ __global__ __launch_bounds__(DBLK) void k_dense_orth(DenseLaunch dl, LState* st,
     G[a * SLD + b] = c64{sx, sy};
   }
   __syncthreads();
+  // G rows: 0 .. nr-1 = R^H R, nr .. nr+nz-1 = Z^H R = B
   for (int e = threadIdx.x; e < nr * nr; e += blockDim.x) {
     const int a = e / nr, b = e - (e / nr) * nr;
-    c64 v = G[(nz + a) * SLD + b];
-    for (int l = 0; l < nz; ++l) v = csub(v, cmul(ccj(G[l * SLD + a]), G[l * SLD + b]));
+    c64 v = G[a * SLD + b];
+    for (int l = 0; l < nz; ++l) v = csub(v, cmul(ccj(G[(nr + l) * SLD + a]), G[(nr + l) * SLD + b]));
     S0[a * SLD + b] = v;
   }
   __syncthreads();
@@ -685,7 +686,7 @@ __global__ __launch_bounds__(DBLK) void k_dense_orth(DenseLaunch dl, LState* st,
     if (threadIdx.x == 0) st[i].status = OMX_ERR_CHOLESKY;
     for (int e = threadIdx.x; e < pa * nr; e += blockDim.x) {
       const int a = e / nr, b = e - (e / nr) * nr;
-      Ci[2 * (a * pm + b)] = (a - nz == b) ? 1.0 : 0.0;
+      Ci[2 * (a * pm + b)] = (a == b) ? 1.0 : 0.0;
       Ci[2 * (a * pm + b) + 1] = 0.0;
     }
     return;
@@ -694,11 +695,11 @@ __global__ __launch_bounds__(DBLK) void k_dense_orth(DenseLaunch dl, LState* st,
   for (int e = threadIdx.x; e < pa * nr; e += blockDim.x) {
     const int a = e / nr, q = e - (e / nr) * nr;
     c64 v;
-    if (a < nz) {
+    if (a < nr) {
+      v = W[a * SLD + q];
+    } else {
       v = c64{0, 0};
       for (int l = 0; l < nr; ++l) v = csub(v, cmul(G[a * SLD + l], W[l * SLD + q]));
-    } else {
-      v = W[(a - nz) * SLD + q];
     }
     Ci[2 * (a * pm + q)] = v.x;
     Ci[2 * (a * pm + q) + 1] = v.y;

@@ -77,8 +77,8 @@ void launch_dense_lock(const DenseLaunch& dl, LState* st, int* shape, const doub
 void launch_dense_chol(const DenseLaunch& dl, LState* st, const int* shape, int cnt_field, const c64* gram_partial,
                        double* C, hipStream_t stream);
 
-// R <- (R - Z Z^H R) U^{-1} coefficients from the Gram partials of [Z R]^H R, with
-// Z = X (F_M columns) + P (F_NPN) and R = F_NNEW columns; C is (nz + nr) x nr.
+// R <- (R - Z Z^H R) U^{-1} coefficients from the Gram of [R Z]^H R, with R = F_NNEW
+// columns and Z = X (F_M columns) + P (F_NPN); C is (nr + nz) x nr, rows in [R Z] order.
 void launch_dense_orth(const DenseLaunch& dl, LState* st, const int* shape, const c64* gram_partial, int pmg,
                        double* C, hipStream_t stream);
 

@@ -269,10 +269,10 @@ template <typename T> class Batch {
     launch_operator<T>(op, S.pd<T>().plan, Xv, Yv, p(D), sel, kd.as<double>(), dots, dps, S.stream);
   }
   // red_i = A_i^H B_i (pmv x pmv per instance, default pm); ca, cb: upper bounds of the
-  // column counts of A and B
-  void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1) {
+  // column counts of A and B; bpre: B is the leading columns of A
+  void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1, bool bpre = false) {
     const int pg = pmv < 0 ? pm : pmv;
-    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), S.stream, ca, cb);
+    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), S.stream, ca, cb, bpre);
     launch_reduce(gramp.as<c64>(), ninst, nchunk, pg * pg, red.as<c64>(), S.stream);
   }
   // R <- R U^-1 twice (CholeskyQR2), R = list of one part with offset/count fields
@@ -280,23 +280,24 @@ template <typename T> class Batch {
     ColList<T> Rl = list({part(Rm, off, f(cnt_field))});
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
-      gram(Rl, Rl, M, M);
+      gram(Rl, Rl, M, M, -1, true);
       launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, red.as<c64>(), Cw.as<double>(), S.stream);
       launch_combine<T>(CMB_PLAIN, Rl, none, coef(Cw, f(cnt_field)), qmax, out(Rm, off), out(Rm, off), out(Rm, off),
                         ninst, K, nblk, nullptr, S.stream);
     }
   }
   // ortho!(R, [X P]) (dftk.jl:487-495) as two passes of R <- (R - Z Z^H R) U^{-1}, each
-  // one Gram of [Z R]^H R, one small dense step and one in-place update; Z = X_n (M
-  // columns) and P_n (F_NPN columns), R = F_NNEW columns at offset F_NEWLY.
+  // one Gram of [R Z]^H R, one small dense step and one in-place update; R = F_NNEW
+  // columns at offset F_NEWLY, Z = X_n (M columns) and P_n (F_NPN columns).  R leads the
+  // list so that the Gram reads its columns once.
   void ortho_r(DevMem& Xn, DevMem& Pn, DevMem& Rm) {
     ColList<T> Rl = list({part(Rm, f(F_NEWLY), f(F_NNEW))});
-    ColList<T> ZR = list({part(Xn, nullptr, f(F_M)), part(Pn, nullptr, f(F_NPN)), part(Rm, f(F_NEWLY), f(F_NNEW))});
+    ColList<T> RZ = list({part(Rm, f(F_NEWLY), f(F_NNEW)), part(Xn, nullptr, f(F_M)), part(Pn, nullptr, f(F_NPN))});
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
-      gram(ZR, Rl, 3 * M, M);
+      gram(RZ, Rl, 3 * M, M, -1, M <= 16);
       launch_dense_orth(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), pm, Cw.as<double>(), S.stream);
-      launch_combine<T>(CMB_PLAIN, ZR, none, coef(Cw, f(F_NNEW)), qmax, out(Rm, f(F_NEWLY)), out(Rm, f(F_NEWLY)),
+      launch_combine<T>(CMB_PLAIN, RZ, none, coef(Cw, f(F_NNEW)), qmax, out(Rm, f(F_NEWLY)), out(Rm, f(F_NEWLY)),
                         out(Rm, f(F_NEWLY)), ninst, K, nblk, nullptr, S.stream);
     }
   }
