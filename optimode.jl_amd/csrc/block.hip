@@ -54,15 +54,20 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// S consecutive complex rows of one column (16-byte loads), zero past `lim` or for a
-// missing column.
+// S consecutive complex rows of one column: 16-byte global loads when the column exists
+// and all S rows lie below `lim` (every stage of a chunk but a final partial one), else
+// element-wise with zeros past `lim` or for a missing column.
+template <typename T> using gcptr = const __attribute__((address_space(1))) T*;
+typedef float gf4 __attribute__((ext_vector_type(4)));
+typedef double gd2 __attribute__((ext_vector_type(2)));
+
 template <int S>
 __device__ __forceinline__ void load_rows(const cplx<float>* col, long row, long lim, cplx<float> (&v)[S]) {
   if (col != nullptr && row + S <= lim) {
-    const float4* q = reinterpret_cast<const float4*>(col + row);
+    gcptr<gf4> q = (gcptr<gf4>)(col + row);
 #pragma unroll
     for (int j = 0; j < S / 2; ++j) {
-      const float4 w = q[j];
+      const gf4 w = q[j];
       v[2 * j] = cplx<float>{w.x, w.y};
       v[2 * j + 1] = cplx<float>{w.z, w.w};
     }
@@ -74,15 +79,65 @@ __device__ __forceinline__ void load_rows(const cplx<float>* col, long row, long
 template <int S>
 __device__ __forceinline__ void load_rows(const cplx<double>* col, long row, long lim, cplx<double> (&v)[S]) {
   if (col != nullptr && row + S <= lim) {
-    const double2* q = reinterpret_cast<const double2*>(col + row);
+    gcptr<gd2> q = (gcptr<gd2>)(col + row);
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      const double2 w = q[j];
+      const gd2 w = q[j];
       v[j] = cplx<double>{w.x, w.y};
     }
   } else {
 #pragma unroll
     for (int j = 0; j < S; ++j) v[j] = (col != nullptr && row + j < lim) ? col[row + j] : cplx<double>{0.0, 0.0};
+  }
+}
+
+// The body of k_gram16 for exactly NA x NB active output tiles (compile-time, so the MFMA
+// sequence has no predicates); S rows per lane and stage.  Returns the accumulators.
+template <typename T, int NA, int NB, int S, bool BPRE, typename Acc>
+__device__ __forceinline__ void gram_tiles(const cplx<T>* const (&ap)[3], const cplx<T>* const (&bp)[3], bool bown,
+                                           long rs, long r1, long stride, int g, Acc (&accr)[3][3],
+                                           Acc (&acci)[3][3]) {
+  cplx<T> a[NA][S], b[NB][S];
+  auto load = [&](long r, cplx<T>(&xa)[NA][S], cplx<T>(&xb)[NB][S]) {
+#pragma unroll
+    for (int t = 0; t < NA; ++t) load_rows<S>(ap[t], r + g * S, r1, xa[t]);
+    if constexpr (!BPRE) {
+#pragma unroll
+      for (int t = 0; t < NB; ++t) load_rows<S>(bp[t], r + g * S, r1, xb[t]);
+    }
+  };
+  if (rs < r1) load(rs, a, b);
+  while (rs < r1) {
+    const long rn = rs + stride;
+    cplx<T> na[NA][S], nb[NB][S];
+    if (rn < r1) load(rn, na, nb);
+    if constexpr (BPRE) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) b[0][s] = bown ? a[0][s] : cplx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int ta = 0; ta < NA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < NB; ++tb) {
+          const cplx<T> x = a[ta][s], y = b[tb][s];
+          accr[ta][tb] = mfma16(x.x, y.x, accr[ta][tb]);
+          acci[ta][tb] = mfma16(x.x, y.y, acci[ta][tb]);
+          accr[ta][tb] = mfma16(x.y, y.y, accr[ta][tb]);
+          acci[ta][tb] = mfma16(x.y, -y.x, acci[ta][tb]);
+        }
+#pragma unroll
+    for (int t = 0; t < NA; ++t)
+#pragma unroll
+      for (int s = 0; s < S; ++s) a[t][s] = na[t][s];
+    if constexpr (!BPRE) {
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+#pragma unroll
+        for (int s = 0; s < S; ++s) b[t][s] = nb[t][s];
+    }
+    rs = rn;
   }
 }
 
@@ -112,76 +167,42 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
   const int nta = (npa + 15) >> 4, ntb = (npb + 15) >> 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, g = lane >> 4;
-  const cplx<T>* ap[TA];
-  const cplx<T>* bp[TB];
+  const cplx<T>* ap[3] = {nullptr, nullptr, nullptr};
+  const cplx<T>* bp[3] = {nullptr, nullptr, nullptr};
 #pragma unroll
   for (int t = 0; t < TA; ++t) ap[t] = (t * 16 + i16 < npa) ? pa_ptr[t * 16 + i16] : nullptr;
 #pragma unroll
   for (int t = 0; t < TB; ++t) bp[t] = (t * 16 + i16 < npb) ? pb_ptr[t * 16 + i16] : nullptr;
-  acc_t accr[TA][TB], acci[TA][TB];
+  acc_t accr[3][3], acci[3][3];
 #pragma unroll
-  for (int ta = 0; ta < TA; ++ta)
+  for (int ta = 0; ta < 3; ++ta)
 #pragma unroll
-    for (int tb = 0; tb < TB; ++tb) {
+    for (int tb = 0; tb < 3; ++tb) {
       accr[ta][tb] = acc_t{0, 0, 0, 0};
       acci[ta][tb] = acc_t{0, 0, 0, 0};
     }
   const long r0 = long(chunk) * rows_per_chunk;
   const long r1 = (r0 + rows_per_chunk < K) ? r0 + rows_per_chunk : K;
-  long rs = r0 + long(wave) * WR;
-  cplx<T> a[TA][S], b[TB][S];
-  const bool bown = i16 < npb;  // BPRE: this lane's B column is its A column
-  if (rs < r1) {
-#pragma unroll
-    for (int t = 0; t < TA; ++t)
-      if (t < nta) load_rows<S>(ap[t], rs + g * S, r1, a[t]);
-    if constexpr (!BPRE) {
-#pragma unroll
-      for (int t = 0; t < TB; ++t)
-        if (t < ntb) load_rows<S>(bp[t], rs + g * S, r1, b[t]);
-    }
-  }
-  while (rs < r1) {
-    const long rn = rs + STRIDE;
-    cplx<T> na[TA][S], nb[TB][S];
-    if (rn < r1) {
-#pragma unroll
-      for (int t = 0; t < TA; ++t)
-        if (t < nta) load_rows<S>(ap[t], rn + g * S, r1, na[t]);
-      if constexpr (!BPRE) {
-#pragma unroll
-        for (int t = 0; t < TB; ++t)
-          if (t < ntb) load_rows<S>(bp[t], rn + g * S, r1, nb[t]);
-      }
-    }
-    if constexpr (BPRE) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) b[0][s] = bown ? a[0][s] : cplx<T>{T(0), T(0)};
-    }
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-      for (int ta = 0; ta < TA; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < TB; ++tb)
-          if (ta < nta && tb < ntb) {
-            const cplx<T> x = a[ta][s], y = b[tb][s];
-            accr[ta][tb] = mfma16(x.x, y.x, accr[ta][tb]);
-            acci[ta][tb] = mfma16(x.x, y.y, acci[ta][tb]);
-            accr[ta][tb] = mfma16(x.y, y.y, accr[ta][tb]);
-            acci[ta][tb] = mfma16(x.y, -y.x, acci[ta][tb]);
-          }
-#pragma unroll
-    for (int t = 0; t < TA; ++t)
-#pragma unroll
-      for (int s = 0; s < S; ++s) a[t][s] = na[t][s];
-    if constexpr (!BPRE) {
-#pragma unroll
-      for (int t = 0; t < TB; ++t)
-#pragma unroll
-        for (int s = 0; s < S; ++s) b[t][s] = nb[t][s];
-    }
-    rs = rn;
+  const long rs = r0 + long(wave) * WR;
+  const bool bown = i16 < npb;
+  // one fully unrolled body per active tile shape (block-uniform)
+  switch (nta * 4 + ntb) {
+#define OMX_GRAM_CASE(na, nb)                                                                                   \
+  case na * 4 + nb:                                                                                             \
+    if constexpr (na <= TA && nb <= TB && (!BPRE || nb == 1))                                                   \
+      gram_tiles<T, na, nb, S, BPRE>(ap, bp, bown, rs, r1, STRIDE, g, accr, acci);                              \
+    break;
+    OMX_GRAM_CASE(1, 1)
+    OMX_GRAM_CASE(1, 2)
+    OMX_GRAM_CASE(1, 3)
+    OMX_GRAM_CASE(2, 1)
+    OMX_GRAM_CASE(2, 2)
+    OMX_GRAM_CASE(2, 3)
+    OMX_GRAM_CASE(3, 1)
+    OMX_GRAM_CASE(3, 2)
+    OMX_GRAM_CASE(3, 3)
+#undef OMX_GRAM_CASE
+    default: break;
   }
   // sum the four waves tile by tile; wave w finishes accumulator register w
   c64* out = partial + (long(inst) * gridDim.x + chunk) * pm * pm;

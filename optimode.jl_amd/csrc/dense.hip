@@ -7,7 +7,9 @@ namespace omx {
 
 namespace {
 
-constexpr int DBLK = 256;
+// One wavefront per instance: the matrices here have at most 48 x 16 entries, so the
+// steps are latency-bound and a single-wave workgroup makes each barrier nearly free.
+constexpr int DBLK = 64;
 constexpr int DMAX = 48;       // max Y columns (3 x 16)
 constexpr int DLD = DMAX + 1;  // padded LDS row stride of Y-sized matrices
 constexpr int SLD = 17;        // LDS row stride of matrices with <= 16 columns
@@ -487,10 +489,25 @@ __global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st,
   LState& s = st[i];
   const int nact = shp(shape, F_NACT, ninst, i);
   if (nact == 0) return;
-  for (int q = threadIdx.x; q < nact; q += blockDim.x) {
-    double acc = 0;
-    for (int b = 0; b < dl.nblk; ++b) acc += npart[(long(i) * dl.nblk + b) * dl.qmax + q];
-    nrm[q] = sqrt(acc);
+  // residual norms from the per-block partials: lane l sums blocks l, l + 64, ... of every
+  // column (independent loads), then one wavefront reduction per column (fixed order)
+  {
+    double part[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) part[q] = 0.0;
+    for (int b = threadIdx.x; b < dl.nblk; b += DBLK) {
+      const double* np = npart + (long(i) * dl.nblk + b) * dl.qmax;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q < nact) part[q] += np[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      double v = part[q];
+#pragma unroll
+      for (int off = DBLK / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, DBLK);
+      if (threadIdx.x == 0 && q < nact) nrm[q] = sqrt(v);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
