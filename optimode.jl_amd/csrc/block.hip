@@ -93,7 +93,7 @@ __device__ __forceinline__ void load_rows(const cplx<double>* col, long row, lon
 
 // The body of k_gram16 for exactly NA x NB active output tiles (compile-time, so the MFMA
 // sequence has no predicates); S rows per lane and stage.  Returns the accumulators.
-template <typename T, int NA, int NB, int S, bool BPRE, typename Acc>
+template <typename T, int NA, int NB, int S, bool BPRE, bool HERM, typename Acc>
 __device__ __forceinline__ void gram_tiles(const cplx<T>* const (&ap)[3], const cplx<T>* const (&bp)[3], bool bown,
                                            long rs, long r1, long stride, int g, Acc (&accr)[3][3],
                                            Acc (&acci)[3][3]) {
@@ -121,6 +121,7 @@ __device__ __forceinline__ void gram_tiles(const cplx<T>* const (&ap)[3], const 
       for (int ta = 0; ta < NA; ++ta)
 #pragma unroll
         for (int tb = 0; tb < NB; ++tb) {
+          if (HERM && ta > tb) continue;
           const cplx<T> x = a[ta][s], y = b[tb][s];
           accr[ta][tb] = mfma16(x.x, y.x, accr[ta][tb]);
           acci[ta][tb] = mfma16(x.x, y.y, acci[ta][tb]);
@@ -142,14 +143,17 @@ __device__ __forceinline__ void gram_tiles(const cplx<T>* const (&ap)[3], const 
 }
 
 // BPRE: B is the leading npb (<= 16) columns of A, so the B tile is taken from A's first
-// tile instead of being loaded again.
-template <typename T, int TA, int TB, bool BPRE>
+// tile instead of being loaded again.  HERM: A^H B is Hermitian (B = M A for Hermitian M):
+// only the tiles on and above the tile diagonal are computed and written.
+template <typename T, int TA, int TB, bool BPRE, bool HERM>
 __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long K, long rows_per_chunk, c64* partial,
                                                 int pm) {
   static_assert(!BPRE || TB == 1, "a B prefix of A fits one tile");
+  static_assert(!HERM || TA == TB, "a Hermitian Gram is square");
   using acc_t = typename Acc4<T>::type;
-  // rows per lane and stage: 64 bytes of each tile column, halved for wide tile sets
-  constexpr int S = (sizeof(T) == 4 ? 8 : 4) / ((TA + TB) > 4 ? 2 : 1);
+  // rows per lane and stage: 64 bytes of each tile column (FP32) or 32 (FP64), halved for
+  // wide tile sets, to bound the registers of the prefetched stage
+  constexpr int S = (sizeof(T) == 4 ? (TA + TB > 4 ? 4 : 8) : (TA + TB >= 4 ? 2 : 4));
   constexpr int WR = 4 * S;       // rows per wave and stage
   constexpr int STRIDE = 4 * WR;  // rows per block and stage
   __shared__ const cplx<T>* pa_ptr[16 * TA];
@@ -189,8 +193,8 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
   switch (nta * 4 + ntb) {
 #define OMX_GRAM_CASE(na, nb)                                                                                   \
   case na * 4 + nb:                                                                                             \
-    if constexpr (na <= TA && nb <= TB && (!BPRE || nb == 1))                                                   \
-      gram_tiles<T, na, nb, S, BPRE>(ap, bp, bown, rs, r1, STRIDE, g, accr, acci);                              \
+    if constexpr (na <= TA && nb <= TB && (!BPRE || nb == 1) && (!HERM || na == nb))                            \
+      gram_tiles<T, na, nb, S, BPRE, HERM>(ap, bp, bown, rs, r1, STRIDE, g, accr, acci);                        \
     break;
     OMX_GRAM_CASE(1, 1)
     OMX_GRAM_CASE(1, 2)
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
   for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      if (ta >= nta || tb >= ntb) continue;
+      if (ta >= nta || tb >= ntb || (HERM && ta > tb)) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         red[wave][r][lane] = accr[ta][tb][r];
@@ -228,28 +232,36 @@ __global__ __launch_bounds__(BLK) void k_gram16(ColList<T> A, ColList<T> B, long
     }
 }
 
-// out_i[e] = sum over chunks of partial_i[chunk][e]: 64 elements x 4 chunk groups per
-// block, the groups combined in a fixed order (deterministic).
-__global__ __launch_bounds__(BLK) void k_reduce(const c64* partial, int nchunk, int esz, c64* out) {
-  __shared__ double sx[4][64], sy[4][64];
+// out_i[e] = sum over chunks of partial_i[chunk][e]: 16 elements x 16 chunk groups per
+// block (16 consecutive elements are 256 contiguous bytes of one chunk), the groups
+// combined in a fixed order (deterministic).  Instances with cnt[i] == 0 are skipped.
+__global__ __launch_bounds__(BLK) void k_reduce(const c64* partial, int nchunk, int esz, c64* out, const int* cnt) {
+  __shared__ double sx[16][17], sy[16][17];
   const int inst = blockIdx.y;
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int e = blockIdx.x * 64 + lane;
+  if (cnt != nullptr && cnt[inst] == 0) return;
+  const int el = threadIdx.x & 15, grp = threadIdx.x >> 4;
+  const int e = blockIdx.x * 16 + el;
   double ax = 0.0, ay = 0.0;
   if (e < esz) {
     const c64* p = partial + long(inst) * nchunk * esz + e;
-    for (int c = grp; c < nchunk; c += 4) {
+    for (int c = grp; c < nchunk; c += 16) {
       const c64 v = p[long(c) * esz];
       ax += v.x;
       ay += v.y;
     }
   }
-  sx[grp][lane] = ax;
-  sy[grp][lane] = ay;
+  sx[grp][el] = ax;
+  sy[grp][el] = ay;
   __syncthreads();
-  if (grp == 0 && e < esz)
-    out[long(inst) * esz + e] = c64{(sx[0][lane] + sx[1][lane]) + (sx[2][lane] + sx[3][lane]),
-                                    (sy[0][lane] + sy[1][lane]) + (sy[2][lane] + sy[3][lane])};
+  if (grp == 0 && e < esz) {
+    double tx = 0.0, ty = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      tx += sx[q][el];
+      ty += sy[q][el];
+    }
+    out[long(inst) * esz + e] = c64{tx, ty};
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -409,23 +421,34 @@ __global__ __launch_bounds__(BLK) void k_copy_cols(const cplx<T>* src, cplx<T>* 
 }  // namespace
 
 template <typename T, int TA>
-static void gram_tb(int tb, bool bpre, dim3 grid, hipStream_t stream, const ColList<T>& A, const ColList<T>& B, long K,
-                    long rpc, c64* partial, int pm) {
+static void gram_tb(int tb, bool bpre, bool herm, dim3 grid, hipStream_t stream, const ColList<T>& A,
+                    const ColList<T>& B, long K, long rpc, c64* partial, int pm) {
   if (bpre) {
-    hipLaunchKernelGGL((k_gram16<T, TA, 1, true>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
+    hipLaunchKernelGGL((k_gram16<T, TA, 1, true, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
+    return;
+  }
+  if (herm) {
+    hipLaunchKernelGGL((k_gram16<T, TA, TA, false, true>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
     return;
   }
   switch (tb) {
-    case 1: hipLaunchKernelGGL((k_gram16<T, TA, 1, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
-    case 2: hipLaunchKernelGGL((k_gram16<T, TA, 2, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
-    default: hipLaunchKernelGGL((k_gram16<T, TA, 3, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm); break;
+    case 1:
+      hipLaunchKernelGGL((k_gram16<T, TA, 1, false, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_gram16<T, TA, 2, false, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gram16<T, TA, 3, false, false>), grid, dim3(BLK), 0, stream, A, B, K, rpc, partial, pm);
+      break;
   }
 }
 
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
-                 hipStream_t stream, int ca, int cb, bool b_prefix_of_a) {
-  if (ca <= 0 || cb <= 0 || ca > 48 || cb > 48 || ca > pm || cb > pm || pm % 16 != 0 || (b_prefix_of_a && cb > 16))
+                 hipStream_t stream, int ca, int cb, bool b_prefix_of_a, bool hermitian) {
+  if (ca <= 0 || cb <= 0 || ca > 48 || cb > 48 || ca > pm || cb > pm || pm % 16 != 0 || (b_prefix_of_a && cb > 16) ||
+      (hermitian && (b_prefix_of_a || (ca + 15) / 16 != (cb + 15) / 16)))
     throw Error(OMX_ERR_INTERNAL, "gram: column bounds out of range");
   // chunks are whole block stages (128 rows), so only the last one can end mid-stage
   long rpc = (K + nchunk - 1) / nchunk;
@@ -433,16 +456,16 @@ void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, in
   dim3 grid(nchunk, ninst);
   const int ta = (ca + 15) / 16, tb = (cb + 15) / 16;
   switch (ta) {
-    case 1: gram_tb<T, 1>(tb, b_prefix_of_a, grid, stream, A, B, K, rpc, partial, pm); break;
-    case 2: gram_tb<T, 2>(tb, b_prefix_of_a, grid, stream, A, B, K, rpc, partial, pm); break;
-    default: gram_tb<T, 3>(tb, b_prefix_of_a, grid, stream, A, B, K, rpc, partial, pm); break;
+    case 1: gram_tb<T, 1>(tb, b_prefix_of_a, hermitian, grid, stream, A, B, K, rpc, partial, pm); break;
+    case 2: gram_tb<T, 2>(tb, b_prefix_of_a, hermitian, grid, stream, A, B, K, rpc, partial, pm); break;
+    default: gram_tb<T, 3>(tb, b_prefix_of_a, hermitian, grid, stream, A, B, K, rpc, partial, pm); break;
   }
   OMX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream) {
-  dim3 grid((esz + 63) / 64, ninst);
-  hipLaunchKernelGGL(k_reduce, grid, dim3(BLK), 0, stream, partial, nchunk, esz, out);
+void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream, const int* cnt) {
+  dim3 grid((esz + 15) / 16, ninst);
+  hipLaunchKernelGGL(k_reduce, grid, dim3(BLK), 0, stream, partial, nchunk, esz, out, cnt);
   OMX_HIP_CHECK(hipGetLastError());
 }
 
@@ -499,9 +522,9 @@ void launch_copy_cols(const cplx<T>* src, cplx<T>* dst, long ld, long istride, i
 template void launch_copy_cols<float>(const cplx<float>*, cplx<float>*, long, long, int, const int*, const int*, long, hipStream_t);
 template void launch_copy_cols<double>(const cplx<double>*, cplx<double>*, long, long, int, const int*, const int*, long, hipStream_t);
 template void launch_gram<float>(const ColList<float>&, const ColList<float>&, int, long, int, int, c64*, hipStream_t, int, int,
-                                 bool);
+                                 bool, bool);
 template void launch_gram<double>(const ColList<double>&, const ColList<double>&, int, long, int, int, c64*, hipStream_t,
-                                  int, int, bool);
+                                  int, int, bool, bool);
 template void launch_combine<float>(CombineMode, const ColList<float>&, const ColList<float>&, const CombineCoef&, int,
                                     OutView<float>, OutView<float>, OutView<float>, int, long, int, double*, hipStream_t);
 template void launch_combine<double>(CombineMode, const ColList<double>&, const ColList<double>&, const CombineCoef&, int,

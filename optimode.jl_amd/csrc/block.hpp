@@ -42,13 +42,16 @@ template <typename T> struct OutView {
 // partial[(i*nchunk + chunk)*pm*pm + a*pm + b] for a < 16 ceil(nA_i/16), b < 16 ceil(nB_i/16)
 // (other entries untouched).  pm in {16, 32, 48}; ca, cb (<= pm, <= 48) bound the column
 // counts of A and B.  b_prefix_of_a: B is the leading columns of A (cb <= 16), whose values
-// are then loaded once.
+// are then loaded once.  hermitian: A^H B is Hermitian (B = M A, M Hermitian); only the
+// 16 x 16 tiles on and above the tile diagonal are computed (the others stay untouched).
 template <typename T>
 void launch_gram(const ColList<T>& A, const ColList<T>& B, int ninst, long K, int nchunk, int pm, c64* partial,
-                 hipStream_t stream, int ca, int cb, bool b_prefix_of_a = false);
+                 hipStream_t stream, int ca, int cb, bool b_prefix_of_a = false, bool hermitian = false);
 
-// out_i[e] = sum_chunk partial[(i*nchunk + chunk)*esz + e], fixed order (deterministic).
-void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream);
+// out_i[e] = sum_chunk partial[(i*nchunk + chunk)*esz + e], fixed order (deterministic);
+// instances with cnt[i] == 0 are skipped (cnt: device, may be null).
+void launch_reduce(const c64* partial, int ninst, int nchunk, int esz, c64* out, hipStream_t stream,
+                   const int* cnt = nullptr);
 
 enum CombineMode : int {
   CMB_PLAIN = 0,     // O1 = Y C

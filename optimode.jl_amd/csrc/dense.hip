@@ -428,12 +428,17 @@ __global__ __launch_bounds__(RBLK) void k_dense_rr(DenseLaunch dl, LState* st, i
     A[a * DLD + b] = c64{sx, sy};
   }
   __syncthreads();
-  // Hermitian part
+  // Hermitian part.  The Gram holds the 16 x 16 tiles on and above the tile diagonal:
+  // entries in diagonal tiles are averaged with their mirror, the others are mirrored.
   for (int e = threadIdx.x; e < py * py; e += blockDim.x) {
     const int a = e / py, b = e - (e / py) * py;
     if (a < b) {
-      const c64 u = A[a * DLD + b], v = A[b * DLD + a];
-      const c64 h = c64{0.5 * (u.x + v.x), 0.5 * (u.y - v.y)};
+      const c64 u = A[a * DLD + b];
+      c64 h = u;
+      if ((a >> 4) == (b >> 4)) {
+        const c64 v = A[b * DLD + a];
+        h = c64{0.5 * (u.x + v.x), 0.5 * (u.y - v.y)};
+      }
       A[a * DLD + b] = h;
       A[b * DLD + a] = ccj(h);
     } else if (a == b) {

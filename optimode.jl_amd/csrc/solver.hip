@@ -189,6 +189,7 @@ template <typename T> class Batch {
   PinnedMem snap[2];  // LState snapshots of consecutive iterations
   hipEvent_t snap_ev[2] = {nullptr, nullptr};
   int pm, nchunk, nblk, qmax, dps;
+  int nchunk_max, nblk_max;  // for one active instance (buffer sizes)
   int cur = 0;
   int maxiter_alloc;
   std::vector<LState> hst;
@@ -210,15 +211,16 @@ template <typename T> class Batch {
     pm = (3 * M <= 16) ? 16 : (3 * M <= 32 ? 32 : 48);
     qmax = M <= 4 ? 4 : (M <= 8 ? 8 : 16);
     // about 512 Gram blocks (2 per CU), at least 512 rows each
-    nchunk = std::max(1, std::min<int>(int((K + 511) / 512), std::max(1, 512 / ninst)));
-    nblk = std::max(1, std::min<int>(int((K + 255) / 256), std::max(1, 1024 / ninst)));
+    nchunk_max = std::max(1, std::min<int>(int((K + 511) / 512), 512));
+    nblk_max = std::max(1, std::min<int>(int((K + 255) / 256), 1024));
+    set_active(ninst);
     dps = S.pd<T>().plan.dot_blocks;
     st.alloc(sizeof(LState) * ninst);
     shape.alloc(sizeof(int) * F_NFIELDS * ninst);
     OMX_HIP_CHECK(hipMemset(shape.as<void>(), 0, sizeof(int) * F_NFIELDS * ninst));
-    gramp.alloc(sizeof(c64) * size_t(ninst) * nchunk * pm * pm);
+    gramp.alloc(sizeof(c64) * size_t(ninst) * nchunk_max * pm * pm);
     red.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
-    normp.alloc(sizeof(double) * size_t(ninst) * nblk * qmax);
+    normp.alloc(sizeof(double) * size_t(ninst) * nblk_max * qmax);
     Cx.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
     Cp.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
     Cw.alloc(sizeof(c64) * size_t(ninst) * pm * pm);
@@ -232,7 +234,6 @@ template <typename T> class Batch {
     dl.ninst = ninst;
     dl.pm = pm;
     dl.nchunk = 1;  // dense kernels read the reduced Gram matrices
-    dl.nblk = nblk;
     dl.qmax = qmax;
     hst.resize(ninst);
     for (int b = 0; b < 2; ++b) {
@@ -246,6 +247,16 @@ template <typename T> class Batch {
   }
   Batch(const Batch&) = delete;
   Batch& operator=(const Batch&) = delete;
+
+  // Row partitioning of the Gram products and block updates for `na` instances with work
+  // (an upper bound is enough: the kernels skip instances without columns): about 512
+  // Gram blocks and 1024 update blocks in total, whatever the number of instances left.
+  void set_active(int na) {
+    na = std::max(1, std::min(na, ninst));
+    nchunk = std::max(1, std::min(nchunk_max, 512 / na));
+    nblk = std::max(1, std::min(nblk_max, 1024 / na));
+    dl.nblk = nblk;
+  }
 
   cplx<T>* p(DevMem& m) { return m.as<cplx<T>>(); }
   int* f(int field) { return shape.as<int>() + field * ninst; }
@@ -270,10 +281,12 @@ template <typename T> class Batch {
   }
   // red_i = A_i^H B_i (pmv x pmv per instance, default pm); ca, cb: upper bounds of the
   // column counts of A and B; bpre: B is the leading columns of A
-  void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1, bool bpre = false) {
+  void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1, bool bpre = false,
+            bool herm = false) {
     const int pg = pmv < 0 ? pm : pmv;
-    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), S.stream, ca, cb, bpre);
-    launch_reduce(gramp.as<c64>(), ninst, nchunk, pg * pg, red.as<c64>(), S.stream);
+    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), S.stream, ca, cb, bpre, herm);
+    // instances whose A has no columns were skipped by the Gram: skip their reduction too
+    launch_reduce(gramp.as<c64>(), ninst, nchunk, pg * pg, red.as<c64>(), S.stream, A.p[0].s.cnt);
   }
   // R <- R U^-1 twice (CholeskyQR2), R = list of one part with offset/count fields
   void cholqr(DevMem& Rm, const int* off, int cnt_field) {
@@ -342,6 +355,11 @@ template <typename T> class Batch {
     };
     std::vector<OpMark> marks;  // operator intervals of iterations 1.., bytes attributed on examination
     const bool prof = S.prof.enabled;
+    {
+      int na = 0;
+      for (int i = 0; i < ninst; ++i) na += act[i] ? 1 : 0;
+      set_active(na);
+    }
     int it = 0;
     for (;; ++it) {
       const int c = cur, n = cur ^ 1;
@@ -358,7 +376,8 @@ template <typename T> class Batch {
         Y = list({part(X[c], f(F_XOFF), f(F_NACT)), part(R[c], f(F_ROFF), f(F_NACT)), part(P[c], nullptr, f(F_NP))});
         AY = list({part(AX[c], f(F_XOFF), f(F_NACT)), part(AR, f(F_ROFF), f(F_NACT)), part(AP[c], nullptr, f(F_NP))});
       }
-      gram(Y, AY, it == 0 ? M : 3 * M, it == 0 ? M : 3 * M);
+      // Y^H A Y is Hermitian: upper tiles only (the Rayleigh-Ritz step mirrors them)
+      gram(Y, AY, it == 0 ? M : 3 * M, it == 0 ? M : 3 * M, -1, false, true);
       launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), Cx.as<double>(), lam.as<double>(),
                       it == 0 ? 1 : 0, sm);
       launch_combine<T>(CMB_RESID, Y, AY, coef(Cx, f(F_NACT), lam.as<double>()), qmax, out(X[n], f(F_XOFF)),
@@ -392,6 +411,11 @@ template <typename T> class Batch {
         if (!hs[i].finished) all = false;
         nact[i] = hs[i].finished ? 0 : hs[i].nact;
         m_next += nact[i];
+      }
+      {
+        int na = 0;
+        for (int i = 0; i < ninst; ++i) na += nact[i] > 0 ? 1 : 0;
+        set_active(na);
       }
       if (prof) {
         // iteration it applied the operator to the columns active after iteration it - 1
