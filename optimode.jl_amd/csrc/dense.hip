@@ -288,9 +288,10 @@ __device__ int d_ortho_against(c64* X, int rows, int cx, const c64* Y, int cy, d
 //   cols  A'[:,p] = c A[:,p] - conj(s) A[:,q],  A'[:,q] = s A[:,p] + c A[:,q]
 // so A' = L A L^H with L block diagonal.  A round is two LDS passes: the m/2 rotations
 // (one thread each), then the (m/2)^2 2x2 blocks of A (read from one buffer, written to
-// the other) and the m x m/2 row pairs of V (in place).  A sweep is m - 1 rounds; sweeps
-// stop when one performs no rotation.  Returns the buffer holding the final matrix
-// (eigenvalues on its diagonal); V holds the eigenvectors in its columns.
+// the other) and the m x m/2 row pairs of V (in place).  A sweep is m - 1 rounds.  Sweeps
+// stop after one whose rotations all have |sin| <= 1e-9 (the off-diagonal part left is
+// then below 1e-18 |A|) or that performs none.  Returns the buffer holding the final
+// matrix (eigenvalues on its diagonal); V holds the eigenvectors in its columns.
 __device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {
   const int ia = (t == 0) ? 0 : 1 + (t - 1 + r) % (m - 1);
   const int tb = m - 1 - t;
@@ -302,14 +303,15 @@ __device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {
 // c = cos(theta), s = sin(theta) e^{i arg A[p][q]} with tan(2 theta) = 2|A[p][q]| / (A[q][q] - A[p][p])
 // and |theta| <= pi/4: cos(2 theta) = |z| h, sin(2 theta) = h with z = (A[q][q] - A[p][p]) / (2|A[p][q]|)
 // and h = 1/sqrt(1 + z^2), so c = sqrt((1 + cos 2theta)/2) >= 1/sqrt(2) and |s| = sin(2theta)/(2c),
-// without cancellation.  Identity when |A[p][q]| is negligible.
-__device__ __forceinline__ bool jac_rot(const c64* A, int p, int q, double& c, c64& s) {
+// without cancellation.  Identity when |A[p][q]| is negligible.  Returns |s|, or -1 for
+// the identity.
+__device__ __forceinline__ double jac_rot(const c64* A, int p, int q, double& c, c64& s) {
   const double app = A[p * DLD + p].x, aqq = A[q * DLD + q].x;
   const c64 apq = A[p * DLD + q];
   const double a2 = apq.x * apq.x + apq.y * apq.y;
   c = 1.0;
   s = c64{0.0, 0.0};
-  if (!(a2 > 1e-300 && a2 > 1e-32 * fabs(app * aqq))) return false;
+  if (!(a2 > 1e-300 && a2 > 1e-32 * fabs(app * aqq))) return -1.0;
   const double ia = rsqrt(a2);
   const double z = 0.5 * (aqq - app) * ia;
   const double az = fabs(z);
@@ -318,16 +320,18 @@ __device__ __forceinline__ bool jac_rot(const c64* A, int p, int q, double& c, c
     sn = 0.5 / az;
   } else {
     const double h = rsqrt(1.0 + z * z);
-    c = sqrt(0.5 * (1.0 + az * h));
-    sn = 0.5 * h / c;
+    const double x = 0.5 * (1.0 + az * h);
+    const double rx = rsqrt(x);
+    c = x * rx;
+    sn = 0.5 * h * rx;
   }
   const double f = (z >= 0 ? sn : -sn) * ia;
   s = c64{f * apq.x, f * apq.y};
-  return true;
+  return sn;
 }
 
 __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
-  __shared__ int rot[3];
+  __shared__ int rot[3], big[3];
   __shared__ int rp[DMAX / 2], rq[DMAX / 2];
   __shared__ double rc[DMAX / 2];
   __shared__ c64 rs[DMAX / 2];
@@ -338,14 +342,20 @@ __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
     V[i * DLD + j] = c64{i == j ? 1.0 : 0.0, 0.0};
     if (i == n || j == n) A0[i * DLD + j] = c64{0.0, 0.0};
   }
-  if (threadIdx.x < 3) rot[threadIdx.x] = 0;
+  if (threadIdx.x < 3) {
+    rot[threadIdx.x] = 0;
+    big[threadIdx.x] = 0;
+  }
   __syncthreads();
   c64* Ac = A0;
   c64* An = A1;
   const int nblk = npair * npair, nitem = nblk + m * npair;
   for (int sweep = 0; sweep < 40; ++sweep) {
     const int sl = sweep % 3;
-    if (threadIdx.x == 0) rot[(sweep + 1) % 3] = 0;
+    if (threadIdx.x == 0) {
+      rot[(sweep + 1) % 3] = 0;
+      big[(sweep + 1) % 3] = 0;
+    }
     for (int r = 0; r < m - 1; ++r) {
       if (threadIdx.x < npair) {
         const int t = threadIdx.x;
@@ -353,7 +363,9 @@ __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
         jac_pair(t, r, m, p, q);
         double c;
         c64 sv;
-        if (jac_rot(Ac, p, q, c, sv)) rot[sl] = 1;
+        const double sn = jac_rot(Ac, p, q, c, sv);
+        if (sn >= 0.0) rot[sl] = 1;
+        if (sn > 1e-9) big[sl] = 1;
         rp[t] = p;
         rq[t] = q;
         rc[t] = c;
@@ -392,7 +404,7 @@ __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
       Ac = An;
       An = tmp;
     }
-    if (rot[sl] == 0) break;
+    if (rot[sl] == 0 || big[sl] == 0) break;
   }
   return Ac;
 }

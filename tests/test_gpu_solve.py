@@ -18,16 +18,18 @@ def _dense(k, ei, og):
     return (A + A.conj().T) / 2
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_solve_omega2_matches_dense(precision):
-    print(f"solve_omega2 dense {precision}", flush=True)
+@pytest.mark.parametrize("precision,nev", [("mixed", 4), ("fp64", 4), ("mixed", 16), ("fp64", 13)])
+def test_solve_omega2_matches_dense(precision, nev):
+    """nev = 16 / 13 exercise the widest blocks: 48 / 39 Rayleigh-Ritz columns (three Gram
+    tiles, several Jacobi items per thread, an odd Jacobi size)."""
+    print(f"solve_omega2 dense {precision} nev={nev}", flush=True)
     og = O.Grid(6.0, 4.0, 24, 16)
     g = omx.Grid(6.0, 4.0, 24, 16)
     ei = O.gaussian_wg_epsi(P_WG, og)
     k = 1.1
-    w = np.linalg.eigvalsh(_dense(k, ei, og))[:4]
+    w = np.linalg.eigvalsh(_dense(k, ei, og))[:nev]
     solver = omx.HIPSolver(precision=precision)
-    w2, evs, info = omx.solve_omega2(k, ei, g, solver, nev=4, tol=1e-10, maxiter=300, return_info=True)
+    w2, evs, info = omx.solve_omega2(k, ei, g, solver, nev=nev, tol=1e-10, maxiter=300, return_info=True)
     print(info, flush=True)
     assert np.allclose(w2, w, rtol=1e-11, atol=0)
     ref = O.HelmholtzOracle(k, ei, og)
