@@ -81,6 +81,18 @@ int omx_apply_preconditioner(omx_solver* h, double k, const double* H_in, double
 /* HMₖH (maxwell.jl:267-275) per vector: out[j] = -Re<H_j, kx_ct(ifft(eps^-1 fft(zx_tc H_j)))> */
 int omx_hmkh(omx_solver* h, double k, const double* H, int m, double* out, int precision);
 
+/* HMH (src/maxwell.jl:288-291) per vector, FP64: out[j] = Re<H_j, kx_ct(ifft(A fft(kx_tc H_j)))>
+   with normalised ifft, for a symmetric per-pixel field A (3,3,Nx,Ny) column-major, or
+   A = NULL for the handle's eps^-1 (then out[j] = <H_j, M(k) H_j>, = ω² for a unit eigenvector). */
+int omx_hmh(omx_solver* h, double k, const double* A, const double* H, int m, double* out);
+
+/* group_index(k, ev, ω, ε⁻¹, ∂ε_∂ω, grid) (lib/ModeAnalysis/src/group_index.jl:21-29), FP64:
+   ng_out[j] = (ω + HMH(ev_j, ε⁻¹ ∂ε_∂ω ε⁻¹)/2) / HMₖH(ev_j) at k = kmags[j], for nev modes
+   (evecs: nev vectors of 2N complex, as omx_solve_k returns them).  deps_domega: (3,3,Nx,Ny)
+   column-major, symmetric per pixel, or NULL for a non-dispersive medium. */
+int omx_group_index(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+                    const double* deps_domega, double* ng_out);
+
 /* solve_ω²(ms, solver; nev, tol, maxiter) (src/solve.jl:38-63, solvers/dftk.jl:32-37):
    lowest nev eigenpairs of M(k).  H_io: in = start block (2N x nev complex) or NULL for a
    seeded random start; out = eigenvectors (written when non-NULL). */

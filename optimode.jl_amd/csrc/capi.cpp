@@ -112,6 +112,15 @@ int omx_hmkh(omx_solver* h, double k, const double* H, int m, double* out, int p
   return guarded(h, [&](omx::Solver& s) { s.apply(omx::OP_MKH, precision, k, H, nullptr, m, out); });
 }
 
+int omx_hmh(omx_solver* h, double k, const double* A, const double* H, int m, double* out) {
+  return guarded(h, [&](omx::Solver& s) { s.hmh(k, A, H, m, out); });
+}
+
+int omx_group_index(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+                    const double* deps_domega, double* ng_out) {
+  return guarded(h, [&](omx::Solver& s) { s.group_index(omega, nev, kmags, evecs, deps_domega, ng_out); });
+}
+
 int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, double* H_io, double* omega2_out,
                      omx_info* info) {
   return guarded(h, [&](omx::Solver& s) { s.solve_omega2(k, nev, tol, maxiter, H_io, omega2_out, info); });

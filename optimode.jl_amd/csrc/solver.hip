@@ -145,10 +145,13 @@ template <typename T> void select_operator(Solver& S, PrecisionData<T>& P) {
   pl.dot_blocks = S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block);
 }
 
-template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
+// A symmetric per-pixel 3x3 field (column-major (3,3,Nx,Ny), FP64 host) in the two device
+// layouts the operator reads: 6 unique entries xx, yy, zz, xy, xz, yz per pixel, row-major
+// (iy, ix) and column-major (ix, iy).
+template <typename T>
+void upload_tensor6(const Solver& S, const double* E, DevMem& rowm, DevMem& colm) {
   const long N = S.N;
   std::vector<T> e6(size_t(6) * N);
-  const double* E = S.epsinv_host.data();
   for (int iy = 0; iy < S.ny; ++iy)
     for (int ix = 0; ix < S.nx; ++ix) {
       const long p = ix + long(S.nx) * iy;
@@ -161,17 +164,37 @@ template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
       o[4] = T(e[6]);  // (0,2)
       o[5] = T(e[7]);  // (1,2)
     }
-  P.eps6.ensure(sizeof(T) * 6 * N);
-  OMX_HIP_CHECK(hipMemcpy(P.eps6.template as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
-  P.plan.eps6 = P.eps6.template as<T>();
+  rowm.ensure(sizeof(T) * 6 * N);
+  OMX_HIP_CHECK(hipMemcpy(rowm.as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
   // column-major copy: pixel (ix, iy) at 6 (iy + ny ix)
   std::vector<T> e6c(size_t(6) * N);
   for (int ix = 0; ix < S.nx; ++ix)
     for (int iy = 0; iy < S.ny; ++iy)
       for (int a = 0; a < 6; ++a) e6c[6 * (iy + size_t(S.ny) * ix) + a] = e6[6 * (ix + size_t(S.nx) * iy) + a];
-  P.eps6c.ensure(sizeof(T) * 6 * N);
-  OMX_HIP_CHECK(hipMemcpy(P.eps6c.template as<void>(), e6c.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
+  colm.ensure(sizeof(T) * 6 * N);
+  OMX_HIP_CHECK(hipMemcpy(colm.as<void>(), e6c.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
+}
+
+template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
+  upload_tensor6<T>(S, S.epsinv_host.data(), P.eps6, P.eps6c);
+  P.plan.eps6 = P.eps6.template as<T>();
   P.plan.eps6c = P.eps6c.template as<T>();
+}
+
+// the symmetry test set_epsinv applies, for any per-pixel 3x3 field
+void check_symmetric(long N, const double* E, const char* what) {
+  double amax = 0;
+  for (long i = 0; i < 9 * N; ++i) {
+    if (!std::isfinite(E[i])) throw Error(OMX_ERR_INVALID_ARG, std::string(what) + " has non-finite entries");
+    amax = std::max(amax, std::fabs(E[i]));
+  }
+  for (long p = 0; p < N; ++p) {
+    const double* e = E + 9 * p;
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b)
+        if (std::fabs(e[a + 3 * b] - e[b + 3 * a]) > 1e-12 * amax)
+          throw Error(OMX_ERR_UNSUPPORTED, std::string(what) + " must be symmetric per pixel");
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -619,6 +642,88 @@ void Solver::apply(OpKind op, int prec, double k, const double* Hin, double* Hou
   } else {
     OMX_HIP_CHECK(hipMemcpyAsync(Hout, scratch_b.as<void>(), nbytes64, hipMemcpyDeviceToHost, stream));
     OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  }
+}
+
+// HMH (maxwell.jl:288-291): out[j] = Re<H_j, kx_ct(ifft(A . fft(kx_tc H_j)))>, i.e. the FP64
+// operator with the per-pixel field A in place of eps^-1 (A = NULL: eps^-1 itself, so that
+// out = <H, M H>), dotted with its input.
+void Solver::hmh(double k, const double* A, const double* H, int m, double* out) {
+  if (m <= 0) throw Error(OMX_ERR_INVALID_ARG, "m must be positive");
+  if (!H || !out) throw Error(OMX_ERR_INVALID_ARG, "NULL buffer");
+  if (!std::isfinite(k)) throw Error(OMX_ERR_INVALID_ARG, "k must be finite");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  OpPlan<double> plan = p64.plan;
+  DevMem a6, a6c;
+  if (A) {
+    check_symmetric(N, A, "A");
+    upload_tensor6<double>(*this, A, a6, a6c);
+    plan.eps6 = a6.as<double>();
+    plan.eps6c = a6c.as<double>();
+  }
+  const size_t nbytes = sizeof(c64) * size_t(m) * K;
+  scratch_a.ensure(nbytes);
+  scratch_b.ensure(nbytes);
+  DevMem kbuf(sizeof(double)), Dm(sizeof(c64) * size_t(m) * 3 * N);
+  OMX_HIP_CHECK(hipMemcpyAsync(kbuf.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
+  OMX_HIP_CHECK(hipMemcpyAsync(scratch_a.as<void>(), H, nbytes, hipMemcpyHostToDevice, stream));
+  BView<double> X{scratch_a.as<c64>(), K, 0}, Y{scratch_b.as<c64>(), K, 0};
+  launch_operator<double>(OP_M, plan, X, Y, Dm.as<c64>(), VecSel{1, m, nullptr, nullptr}, kbuf.as<double>(), nullptr,
+                          0, stream);
+  std::vector<double> y(size_t(2) * m * K);
+  OMX_HIP_CHECK(hipMemcpyAsync(y.data(), scratch_b.as<void>(), nbytes, hipMemcpyDeviceToHost, stream));
+  OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  for (int j = 0; j < m; ++j) {
+    const double* h = H + 2 * size_t(j) * K;
+    const double* v = y.data() + 2 * size_t(j) * K;
+    double acc = 0;
+    for (long i = 0; i < 2 * K; ++i) acc += h[i] * v[i];
+    out[j] = acc;
+  }
+}
+
+// group_index (lib/ModeAnalysis/src/group_index.jl:21-29) per mode j at its own k_j:
+// n_g = (omega + HMH(ev_j, eps^-1 deps/domega eps^-1) / 2) / HMkH(ev_j); deps_domega NULL
+// means a non-dispersive medium (the HMH term vanishes).
+void Solver::group_index(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
+                         double* ng_out) {
+  if (nev <= 0 || !kmags || !evecs || !ng_out) throw Error(OMX_ERR_INVALID_ARG, "bad group_index arguments");
+  if (!(omega > 0) || !std::isfinite(omega)) throw Error(OMX_ERR_INVALID_ARG, "omega must be positive and finite");
+  std::vector<double> A;
+  if (deps_domega) {
+    check_symmetric(N, deps_domega, "deps_domega");
+    A.assign(size_t(9) * N, 0.0);
+    for (long p = 0; p < N; ++p) {
+      const double* ei = epsinv_host.data() + 9 * p;
+      const double* dw = deps_domega + 9 * p;
+      double t[9];
+      // column-major 3x3: M[a + 3b]; t = ei * dw, A = t * ei
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          double acc = 0;
+          for (int c = 0; c < 3; ++c) acc += ei[a + 3 * c] * dw[c + 3 * b];
+          t[a + 3 * b] = acc;
+        }
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          double acc = 0;
+          for (int c = 0; c < 3; ++c) acc += t[a + 3 * c] * ei[c + 3 * b];
+          A[9 * p + a + 3 * b] = acc;
+        }
+      // exact symmetry of ei dw ei for symmetric ei, dw: average away rounding
+      for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b) {
+          const double s = 0.5 * (A[9 * p + a + 3 * b] + A[9 * p + b + 3 * a]);
+          A[9 * p + a + 3 * b] = A[9 * p + b + 3 * a] = s;
+        }
+    }
+  }
+  for (int j = 0; j < nev; ++j) {
+    const double* ev = evecs + 2 * size_t(j) * K;
+    double hk = 0, hh = 0;
+    apply(OP_MKH, OMX_PREC_FP64, kmags[j], ev, nullptr, 1, &hk);
+    if (deps_domega) hmh(kmags[j], A.data(), ev, 1, &hh);
+    ng_out[j] = (omega + 0.5 * hh) / hk;
   }
 }
 

@@ -155,6 +155,9 @@ class Solver {
   void solve_k(double omega, int nev, double k_tol, double eig_tol, int max_eigsolves, int maxiter,
                const double* kguess, const double* Hguess, int nguess, double* kmags, double* evecs, omx_info* info);
   void get_evecs(double* evecs);
+  void hmh(double k, const double* A, const double* H, int m, double* out);
+  void group_index(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
+                   double* ng_out);
   void adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
                const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar);
   void set_option(const std::string& key, double value);

@@ -193,6 +193,17 @@ class HelmholtzMap:
     def precondition(self, H):
         return self._run(_lib.lib().omx_apply_preconditioner, H)
 
+    def HMH(self, H, A=None):
+        """``HMH`` (src/maxwell.jl:288-291): ``Re <H, kx_ct(ifft(A . fft(kx_tc H)))>`` per column
+        in FP64; ``A`` a symmetric (3,3,Nx,Ny) field, default ε⁻¹ (then ``<H, M H>``)."""
+        n2 = 2 * self.grid.N
+        X = _cols(H, n2)
+        out = np.empty(X.shape[0])
+        Ac = None if A is None else _epsinv_c(A, self.grid)
+        _lib.check(_lib.lib().omx_hmh(self.h.h, self.k, _lib.dptr(Ac), _lib.dptr(X), X.shape[0], _lib.dptr(out)),
+                   self.h.h)
+        return out if np.ndim(H) == 2 else float(out[0])
+
     def HMkH(self, H):
         n2 = 2 * self.grid.N
         X = _cols(H, n2)
@@ -257,6 +268,24 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
     if return_info:
         return km, evecs, info.as_dict()
     return km, evecs
+
+
+def group_index(k, ev, omega, epsinv, deps_domega, grid, solver):
+    """``group_index(k, ev, ω, ε⁻¹, ∂ε_∂ω, grid)`` (lib/ModeAnalysis/src/group_index.jl:21-29):
+    ``n_g = (ω + HMH(ev, ε⁻¹ ∂ωε ε⁻¹)/2) / HMₖH(ev)`` for one mode or a list of modes (``k``
+    and ``ev`` matching); ``deps_domega=None`` for a non-dispersive medium."""
+    h = solver.handle(grid, epsinv)
+    single = np.ndim(k) == 0
+    ks = np.atleast_1d(np.asarray(k, dtype=np.float64))
+    evs = [ev] if single else list(ev)
+    if len(evs) != len(ks):
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "one eigenvector per k is needed")
+    E = _cols(np.stack(evs, 1), 2 * grid.N)
+    D = None if deps_domega is None else _epsinv_c(deps_domega, grid)
+    ng = np.empty(len(ks))
+    _lib.check(_lib.lib().omx_group_index(h.h, float(omega), len(ks), _lib.dptr(ks), _lib.dptr(E), _lib.dptr(D),
+                                          _lib.dptr(ng)), h.h)
+    return float(ng[0]) if single else ng
 
 
 def rrule_solve_k(omega, epsinv, grid, solver, nev=1, **kw):
