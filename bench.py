@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-algorithm comparison solve")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every solve here fits one MI355X and "
@@ -106,6 +107,23 @@ def main():
     value = converged / elapsed
 
     op_ms, op_n, op_bytes = omx.kernel_stats(solver, "op_fp32")
+
+    # the reference's algorithm on the same kernels (FP64, nev-column block per band, random
+    # start per band, exact eigensolves): one warm-up and one timed solve, outside the timed
+    # region above
+    plain = None
+    if not a.no_plain:
+        sp = omx.HIPSolver(precision="fp64", plain=True)
+        omx.solve_k(OMEGA, ei, grid, sp, **kw)
+        tp = time.perf_counter()
+        kmp, _, infop = omx.solve_k(OMEGA, ei, grid, sp, **kw)
+        dtp = time.perf_counter() - tp
+        relp = np.abs(slope_ref * (kmp - k_ref)) / OMEGA ** 2
+        nconv = int(np.sum(relp <= REL_TOL))
+        plain = {"modes_per_s": round(nconv / dtp, 3), "ms_per_step": round(1e3 * dtp, 2),
+                 "modes_converged": nconv, "lobpcg_iters_fp64": infop["lobpcg_iters_fp64"],
+                 "newton_steps": infop["n_newton"], "matvecs": infop["n_matvec"],
+                 "speedup": round(value / (nconv / dtp), 2) if nconv else None}
     roofline = None
     if op_n > 0 and op_ms > 0:
         achieved = op_bytes / (op_ms * 1e-3) / 1e9
@@ -144,6 +162,7 @@ def main():
             "matvecs_per_step": float(np.mean([i["n_matvec"] for i in infos])),
             "parallelism": "single GPU; bands batched as independent Newton instances",
         },
+        "plain": plain,
         "roofline": roofline,
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }

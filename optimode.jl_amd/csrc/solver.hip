@@ -585,6 +585,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.guard = int(v);
   else if (key == "seed")
     opt.seed = uint64_t(v);
+  else if (key == "plain")
+    opt.plain = v != 0 ? 1 : 0;
   else if (key == "reg_ops") {
     opt.reg_ops = v != 0 ? 1 : 0;
     select_operator(*this, p32);
@@ -828,7 +830,7 @@ namespace {
 template <typename T>
 int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
                  const std::vector<int>& nchk, const std::vector<int>& msize, double tol_floor, int maxiter,
-                 int max_steps, SolveStats& stats, int& newton_total, bool strict) {
+                 int max_steps, SolveStats& stats, int& newton_total, bool strict, bool inexact = true) {
   const int ninst = b.ninst;
   std::vector<int> act(ninst);
   std::vector<double> tols(ninst);
@@ -841,7 +843,7 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
       // inexact Newton: the eigenvalue error ~ resid^2 / gap only has to stay well below
       // the current |Δω²|, so the locking threshold follows it down to the floor
       const double f = std::fabs(hns[i].f);
-      tols[i] = (hns[i].nsolves > 0 && std::isfinite(f)) ? std::max(tol_floor, 0.1 * f) : tol_floor;
+      tols[i] = (inexact && hns[i].nsolves > 0 && std::isfinite(f)) ? std::max(tol_floor, 0.1 * f) : tol_floor;
     }
     if (!any) break;
     iters += b.run(act, bands, nchk, msize, tols, maxiter, stats);
@@ -903,7 +905,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   for (int i = 0; i < ninst; ++i) {
     bands[i] = i;
     nchk[i] = i + 1;
-    msize[i] = opt.guard < 0 ? M : std::min(M, i + 1 + opt.guard);
+    msize[i] = (opt.guard < 0 || opt.plain) ? M : std::min(M, i + 1 + opt.guard);
   }
   const int maxit64 = std::max(maxiter, opt.maxiter_fp64);
   Batch<double>& b64 = workspace<double>(*this, ninst, M, maxit64);
@@ -947,7 +949,29 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     return it;
   };
 
-  if (!opt.fp64_only) {
+  if (opt.plain) {
+    // the reference's algorithm (solve.jl:89-113 with the DFTK backend): each band runs
+    // Newton from k_guess with its own random nev-column start, every eigensolve to eig_tol
+    std::vector<double> kh(ninst, k0);
+    OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, stream));
+    for (auto& st : hns) st.k_tol = k_tol;
+    // one band at a time (a single solve per band, as the reference runs them); the band's
+    // start block goes into the batch's current buffer when its turn comes
+    for (int b = 0; b < ninst; ++b) {
+      launch_randn<double>(b64.p(b64.X[b64.cur]) + b * b64.istride, ld, b64.istride, 1, M, K, opt.seed + 7919u * b,
+                           stream);
+      if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, b, b + 1);
+      const std::vector<NewtonState> keep = hns;
+      for (int i = 0; i < ninst; ++i)
+        if (i != b) hns[i].done = 3;  // not this band's turn
+      int steps = 0;
+      stats.it64 +=
+          newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, max_eigsolves, stats, steps, false, false);
+      newton_total = std::max(newton_total, steps);
+      for (int i = 0; i < ninst; ++i)
+        if (i != b) hns[i] = keep[i];
+    }
+  } else if (!opt.fp64_only) {
     Batch<float>& b32 = workspace<float>(*this, ninst, M, maxiter);
     launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
     if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
@@ -970,9 +994,11 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     for (auto& s : hns) s.k_tol = k_tol;
     stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
   }
-  const int left = std::max(1, max_eigsolves - newton_total);
-  stats.it64 +=
-      newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
+  if (!opt.plain) {
+    const int left = std::max(1, max_eigsolves - newton_total);
+    stats.it64 +=
+        newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
+  }
   bool ok = true;
   for (int i = 0; i < ninst; ++i) {
     kmags[i] = hns[i].k;

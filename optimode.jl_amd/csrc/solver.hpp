@@ -142,6 +142,8 @@ struct Options {
   uint64_t seed = 0x5eed;
   int fp64_only = 0;            // skip the FP32 phase
   int reg_ops = 1;              // register-FFT operator where the grid admits it
+  int plain = 0;                // the reference's algorithm: FP64 only, nev-column block per band,
+                                // random start at k_guess per band, every eigensolve to eig_tol
 };
 
 class Solver {

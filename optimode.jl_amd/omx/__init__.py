@@ -100,11 +100,14 @@ class HIPSolver:
     knobs of the FP32 phase are relative to ω²: ``fp32_tol`` (LOBPCG locking threshold)
     and ``fp32_switch`` (|Δω²|/ω² at which Newton moves to FP64).  ``reg_ops=False`` keeps
     the FP32 operator on the LDS Stockham FFT even where the register-FFT kernels apply
-    (power-of-two grids up to 256).
+    (power-of-two grids 64 to 512).  ``plain=True`` runs the reference's algorithm on the same
+    kernels, as the baseline the optimised path is measured against: FP64 throughout, an
+    nev-column block for every band, a random start at ``k_guess`` per band and every
+    eigensolve to ``eig_tol`` (``solve.jl:89-113``).
     """
 
     def __init__(self, device=0, precision="mixed", fp32_tol=None, fp32_switch=None, maxiter_fp64=None, seed=None,
-                 profile=False, reg_ops=None):
+                 profile=False, reg_ops=None, plain=False):
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         self.device = int(device)
@@ -120,6 +123,7 @@ class HIPSolver:
             self.options["seed"] = float(seed)
         if reg_ops is not None:
             self.options["reg_ops"] = 1.0 if reg_ops else 0.0
+        self.options["plain"] = 1.0 if plain else 0.0
         self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
         self.profile = bool(profile)
         self._h = None

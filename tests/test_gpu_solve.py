@@ -38,14 +38,17 @@ def test_solve_omega2_matches_dense(precision, nev):
         assert np.linalg.norm(ref.M(v) - lam * v) < 1e-9
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "plain"])
 def test_solve_k_matches_oracle(precision):
+    """mixed / fp64: the optimised path; plain: the reference's algorithm on the same kernels
+    (the bench's comparison baseline)."""
     print(f"solve_k oracle {precision}", flush=True)
     og = O.Grid(6.0, 4.0, 24, 16)
     g = omx.Grid(6.0, 4.0, 24, 16)
     ei = O.gaussian_wg_epsi(P_WG, og)
     km_ref, ev_ref = O.solve_k(W0, ei, og, nev=2, k_tol=1e-12, eig_tol=1e-11)
-    solver = omx.HIPSolver(precision=precision)
+    solver = (omx.HIPSolver(precision="fp64", plain=True) if precision == "plain"
+              else omx.HIPSolver(precision=precision))
     km, evs, info = omx.solve_k(W0, ei, g, solver, nev=2, k_tol=1e-12, eig_tol=1e-11, return_info=True)
     print(km, km_ref, info, flush=True)
     assert np.allclose(km, km_ref, rtol=1e-10, atol=0)
