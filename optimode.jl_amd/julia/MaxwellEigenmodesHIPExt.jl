@@ -16,7 +16,7 @@ using ChainRulesCore: NoTangent, AbstractZero
 using DielectricSmoothing: Grid, N
 import MaxwellEigenmodes: solve_ω², solve_k, AbstractEigensolver, ModeSolver
 
-export HIPSolver
+export HIPSolver, hip_group_index
 
 const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
 
@@ -154,6 +154,28 @@ function ChainRulesCore.rrule(::typeof(solve_k), ω::Real, ε⁻¹::AbstractArra
         return (NoTangent(), ω̄[], ε̄⁻¹, NoTangent(), NoTangent())
     end
     return (kmags, evecs), hip_solve_k_pullback
+end
+
+"""
+    hip_group_index(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, grid)
+
+`group_index` (lib/ModeAnalysis/src/group_index.jl:21-29) for every mode on the GPU:
+n_g = (ω + HMH(ev, ε⁻¹ ∂ωε ε⁻¹)/2) / HMₖH(ev); `∂ε_∂ω = nothing` for a non-dispersive medium.
+"""
+function hip_group_index(solver::HIPSolver, kmags::AbstractVector{<:Real}, evecs, ω::Real,
+                         ε⁻¹::AbstractArray{Float64,4}, ∂ε_∂ω, grid::Grid{2,Float64})
+    h = _handle!(solver, grid, ε⁻¹)
+    km = Vector{Float64}(kmags)
+    E = reduce(hcat, [Vector{ComplexF64}(v) for v in evecs])
+    D = ∂ε_∂ω === nothing ? nothing : Array{Float64,4}(∂ε_∂ω)
+    ng = Vector{Float64}(undef, length(km))
+    GC.@preserve D begin
+        st = ccall((:omx_group_index, libomx), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Ptr{Cdouble}, Ptr{Cdouble}),
+                   h, ω, length(km), km, E, D === nothing ? Ptr{Cdouble}(C_NULL) : D, ng)
+    end
+    _check(st, h)
+    return ng
 end
 
 end # module
