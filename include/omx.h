@@ -120,6 +120,19 @@ int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmag
                         const double* kbar, const double* evbar, const uint8_t* evbar_mask,
                         double* omegabar_out, double* epsinvbar_out);
 
+/* Kottke sub-pixel smoothing on the GPU (lib/DielectricSmoothing/src/kottke.jl:44-95,
+   smooth.jl:115-170, smooth_eps_single and smooth_eps): eps^-1 (3,3,Nx,Ny) column-major
+   of nshapes convex polygons over a background, on the grid of cell Dx x Dy with pixel centres at -D/2 + i D/N.  Shape s has
+   nverts[s] counter-clockwise vertices, (x, y) pairs concatenated in verts, and the
+   symmetric tensor eps_shapes[9 s .. 9 s + 8] (column-major 3x3); earlier shapes win where
+   shapes overlap.  Pixels are supersampled nsub x nsub to find the fill fraction and the
+   interface normal (the reference uses GeometryPrimitives' surfpt_nearby / volfrac).  No
+   handle: errors are reported through omx_last_error(NULL).  Limits: nshapes <= 15,
+   1 <= nsub <= 64. */
+int omx_smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshapes, const int* nverts,
+                      const double* verts, const double* eps_shapes, const double* eps_background, int nsub,
+                      double* epsinv_out);
+
 /* Live kernel statistics (no counterpart in the reference: instrumentation for
    measurement).  With omx_set_option(h, "profile", 1) every operator application inside
    a solve is bracketed by HIP events on the handle's stream; `family` is "op_fp32" or

@@ -86,6 +86,21 @@ int omx_destroy(omx_solver* h) {
   return OMX_OK;
 }
 
+int omx_smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshapes, const int* nverts,
+                      const double* verts, const double* eps_shapes, const double* eps_background, int nsub,
+                      double* epsinv_out) {
+  try {
+    omx::smooth_epsinv(device, nx, ny, Dx, Dy, nshapes, nverts, verts, eps_shapes, eps_background, nsub, epsinv_out);
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  }
+}
+
 const char* omx_last_error(const omx_solver* h) {
   if (!h || !h->s) return g_error.c_str();
   return h->s->last_error.c_str();

@@ -1,6 +1,7 @@
 // Host-side engine behind the omx C ABI: device buffers, operator plans, the batched
 // LOBPCG driver and the Newton solve_k.
 #pragma once
+#include <cmath>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -195,5 +196,10 @@ class Solver {
   template <typename T> PrecisionData<T>& pd();
   void canonicalize_phase(const double* d_k, c64* vecs, long ldv, long istride, int nvec, int mper);
 };
+
+// Kottke-smoothed eps^-1 of convex polygons (smooth.hip); no handle needed.
+void smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshape, const int* nverts,
+                   const double* verts, const double* eps_shapes, const double* eps_background, int nsub,
+                   double* epsinv_out);
 
 }  // namespace omx

@@ -67,6 +67,9 @@ SIGNATURES = [
     ("omx_solve_k_adjoint", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
                                            ctypes.POINTER(ctypes.c_uint8), _dp, _dp]),
     ("omx_get_kernel_stats", ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
+    ("omx_smooth_epsinv", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp, ctypes.c_int,
+                                         _dp]),
 ]
 
 _lib = None

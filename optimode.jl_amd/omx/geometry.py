@@ -13,7 +13,7 @@ bulk pixels are exact.
 """
 import numpy as np
 
-__all__ = ["Polygon", "Rect", "smooth_epsinv", "rect_waveguide", "tfln_rib", "ridge_waveguide",
+__all__ = ["Polygon", "Rect", "kottke_average", "smooth_epsinv", "rect_waveguide", "tfln_rib", "ridge_waveguide",
            "coupler_slice", "diag_eps"]
 
 
@@ -77,10 +77,25 @@ def _tau_inv(t):
     return e
 
 
-def smooth_epsinv(shapes, eps_background, grid, nsub=8):
+def kottke_average(eps1, eps2, normal, f):
+    """Kottke average of ``eps1`` (fill fraction ``f``) and ``eps2`` across a plane interface
+    with in-plane unit normal ``normal`` = (n_x, n_y) (lib/DielectricSmoothing/src/kottke.jl:90-95, ``avg_param``):
+    ``S τ⁻¹(f τ(Sᵀε₁S) + (1-f) τ(Sᵀε₂S)) Sᵀ`` with ``S = [n, ẑ×n, ẑ]``."""
+    n = np.array([normal[0], normal[1], 0.0])
+    S = np.stack([n, np.array([-n[1], n[0], 0.0]), np.array([0.0, 0.0, 1.0])], 1)
+    tau = f * _tau(S.T @ np.asarray(eps1, float) @ S) + (1 - f) * _tau(S.T @ np.asarray(eps2, float) @ S)
+    return S @ _tau_inv(tau) @ S.T
+
+
+def smooth_epsinv(shapes, eps_background, grid, nsub=8, backend="host", device=0):
     """Kottke-smoothed ε⁻¹ of ``shapes`` (earlier shapes take precedence where they
     overlap) over ``eps_background`` on ``grid`` (any object with Dx, Dy, Nx, Ny).
-    Returns ``(3, 3, Nx, Ny)`` float64."""
+    Returns ``(3, 3, Nx, Ny)`` float64.  ``backend="gpu"`` runs the same algorithm in the
+    HIP library (``omx_smooth_epsinv``)."""
+    if backend == "gpu":
+        return _smooth_epsinv_gpu(shapes, eps_background, grid, nsub, device)
+    if backend != "host":
+        raise ValueError("backend must be 'host' or 'gpu'")
     Nx, Ny = grid.Nx, grid.Ny
     dx, dy = grid.Dx / Nx, grid.Dy / Ny
     xc = -grid.Dx / 2 + np.arange(Nx) * dx
@@ -108,31 +123,47 @@ def smooth_epsinv(shapes, eps_background, grid, nsub=8):
         eps[dom == m] = mats[m]
     # two-material pixels: Kottke average with the first-moment normal
     two = np.argwhere(npres >= 2)
-    ox = off[:, None] * np.ones((1, nsub))
-    oy = off[None, :] * np.ones((nsub, 1))
+    # sub-sample offsets in units of d / (2 nsub): odd integers summing to zero
+    wo = 2 * np.arange(nsub) + 1 - nsub
     for (ix, iy) in two:
         c = counts[ix, iy]
-        order = np.argsort(-c)
+        order = np.argsort(-c, kind="stable")
         m1, m2 = order[0], order[1]
-        ind = (mat[ix, iy] == m1).astype(float)
-        f = ind.mean()
-        # first moment of material-1 indicator -> direction from material 1 into material 2
-        gx = ((ind - f) * ox).sum() * dx
-        gy = ((ind - f) * oy).sum() * dy
-        nrm = np.hypot(gx, gy)
+        ind = mat[ix, iy] == m1
+        f = c[m1] / ns
+        # first moment of the material-1 indicator about its mean, in exact integers (the
+        # mean term drops out because the offsets sum to zero); -moment points from
+        # material 1 into material 2
+        gx = float((ind * wo[:, None]).sum()) * dx
+        gy = float((ind * wo[None, :]).sum()) * dy
+        nrm = np.sqrt(gx * gx + gy * gy)
         if nrm == 0 or npres[ix, iy] > 2:
             ei = f * np.linalg.inv(mats[m1]) + (1 - f) * np.linalg.inv(mats[m2])
             eps[ix, iy] = np.linalg.inv(ei)
             continue
-        n = -np.array([gx, gy, 0.0]) / nrm
-        t = np.array([-n[1], n[0], 0.0])
-        S = np.stack([n, t, np.array([0.0, 0.0, 1.0])], 1)
-        tau = f * _tau(S.T @ mats[m1] @ S) + (1 - f) * _tau(S.T @ mats[m2] @ S)
-        eps[ix, iy] = S @ _tau_inv(tau) @ S.T
+        eps[ix, iy] = kottke_average(mats[m1], mats[m2], (-gx / nrm, -gy / nrm), f)
     eps = 0.5 * (eps + np.swapaxes(eps, -1, -2))
     einv = np.linalg.inv(eps)
     einv = 0.5 * (einv + np.swapaxes(einv, -1, -2))
     return np.ascontiguousarray(np.moveaxis(einv, (-2, -1), (0, 1)))
+
+
+def _smooth_epsinv_gpu(shapes, eps_background, grid, nsub, device):
+    import ctypes
+    from . import _lib
+    bg = np.asarray(eps_background, float) if np.ndim(eps_background) == 2 else diag_eps(float(eps_background))
+    nverts = (ctypes.c_int * max(1, len(shapes)))(*[len(sh.v) for sh in shapes])
+    verts = np.ascontiguousarray(np.concatenate([sh.v.reshape(-1) for sh in shapes]) if shapes else np.zeros(2))
+    eps_s = np.ascontiguousarray(np.concatenate([sh.eps.reshape(-1, order="F") for sh in shapes])
+                                 if shapes else np.zeros(9))
+    bgf = np.ascontiguousarray(bg.reshape(-1, order="F"))
+    out = np.empty(9 * grid.Nx * grid.Ny)
+    L = _lib.lib()
+    st = L.omx_smooth_epsinv(int(device), grid.Nx, grid.Ny, float(grid.Dx), float(grid.Dy), len(shapes), nverts,
+                             _lib.dptr(verts), _lib.dptr(eps_s), _lib.dptr(bgf), int(nsub), _lib.dptr(out))
+    _lib.check(st, None)
+    # column-major (3, 3, Nx, Ny) -> numpy (3, 3, Nx, Ny)
+    return np.ascontiguousarray(out.reshape(grid.Ny, grid.Nx, 3, 3).transpose(3, 2, 1, 0))
 
 
 # ---------------------------------------------------------------------------------------
@@ -144,9 +175,9 @@ EPS_SI3N4 = 3.96
 EPS_LN = (4.89, 4.57, 4.89)
 
 
-def rect_waveguide(grid, w=1.6, h=0.7, eps_core=EPS_SI3N4, eps_clad=EPS_SIO2, nsub=8):
+def rect_waveguide(grid, w=1.6, h=0.7, eps_core=EPS_SI3N4, eps_clad=EPS_SIO2, nsub=8, backend="host"):
     """Centred rectangular core ``w x h`` in a uniform cladding (Si₃N₄ in SiO₂)."""
-    return smooth_epsinv([Rect(0.0, 0.0, w, h, eps_core)], eps_clad, grid, nsub)
+    return smooth_epsinv([Rect(0.0, 0.0, w, h, eps_core)], eps_clad, grid, nsub, backend)
 
 
 def _rib_shapes(top_w, film, etch, sidewall_deg, eps_film, eps_sub, y0=0.0, x0=0.0):
@@ -158,27 +189,28 @@ def _rib_shapes(top_w, film, etch, sidewall_deg, eps_film, eps_sub, y0=0.0, x0=0
 
 
 def tfln_rib(grid, top_w=1.2, film=0.6, etch=0.3, sidewall_deg=60.0, eps_ln=EPS_LN, eps_sub=EPS_SIO2,
-             eps_clad=1.0, nsub=8):
+             eps_clad=1.0, nsub=8, backend="host"):
     """x-cut thin-film LiNbO₃ rib (diag ε), SiO₂ below, air above; film bottom at y = 0."""
     big = 10 * (grid.Dx + grid.Dy)
     rib, slab = _rib_shapes(top_w, film, etch, sidewall_deg, eps_ln, eps_sub)
     slabp = Polygon([(-big, 0.0), (big, 0.0), (big, slab), (-big, slab)], diag_eps(*eps_ln))
     sub = Polygon([(-big, -big), (big, -big), (big, 0.0), (-big, 0.0)], eps_sub)
-    return smooth_epsinv([rib, slabp, sub], eps_clad, grid, nsub)
+    return smooth_epsinv([rib, slabp, sub], eps_clad, grid, nsub, backend)
 
 
-def ridge_waveguide(grid, w, h, sidewall_deg, eps_core, eps_sub=EPS_SIO2, eps_clad=1.0, nsub=8):
+def ridge_waveguide(grid, w, h, sidewall_deg, eps_core, eps_sub=EPS_SIO2, eps_clad=1.0, nsub=8, backend="host"):
     """Fully etched trapezoidal ridge of top width ``w`` and height ``h`` on SiO₂."""
     big = 10 * (grid.Dx + grid.Dy)
     b = h / np.tan(np.radians(sidewall_deg))
     core = Polygon([(-w / 2 - b, 0.0), (w / 2 + b, 0.0), (w / 2, h), (-w / 2, h)],
                    eps_core if np.ndim(eps_core) == 2 else diag_eps(float(eps_core)))
     sub = Polygon([(-big, -big), (big, -big), (big, 0.0), (-big, 0.0)], eps_sub)
-    return smooth_epsinv([core, sub], eps_clad, grid, nsub)
+    return smooth_epsinv([core, sub], eps_clad, grid, nsub, backend)
 
 
-def coupler_slice(grid, gap, w1, w2, h=0.8, eps_core=EPS_SI3N4, eps_clad=EPS_SIO2, nsub=8):
+def coupler_slice(grid, gap, w1, w2, h=0.8, eps_core=EPS_SI3N4, eps_clad=EPS_SIO2, nsub=8, backend="host"):
     """Two rectangular Si₃N₄ cores of widths ``w1``, ``w2`` separated by ``gap``."""
     x1 = -(gap / 2 + w1 / 2)
     x2 = gap / 2 + w2 / 2
-    return smooth_epsinv([Rect(x1, 0.0, w1, h, eps_core), Rect(x2, 0.0, w2, h, eps_core)], eps_clad, grid, nsub)
+    return smooth_epsinv([Rect(x1, 0.0, w1, h, eps_core), Rect(x2, 0.0, w2, h, eps_core)], eps_clad, grid, nsub,
+                         backend)
