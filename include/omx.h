@@ -93,6 +93,16 @@ int omx_hmh(omx_solver* h, double k, const double* A, const double* H, int m, do
 int omx_group_index(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
                     const double* deps_domega, double* ng_out);
 
+/* ng_gvd(ω, k, ev, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid) and ng_gvd_E (lib/ModeAnalysis/src/
+   group_index.jl:126-162, :55-95), FP64, for nev modes at once: the group index ng_out[j]
+   and the group-velocity dispersion gvd_out[j] = d n_g / d ω (one linear solve with
+   M - ω² per mode).  deps_domega, d2eps_domega2: (3,3,Nx,Ny) column-major, symmetric per
+   pixel (not NULL).  E_out: NULL, or nev fields E = ε⁻¹ fft(kx_tc(ev_j)) of 3N complex each
+   ((3,Nx,Ny) column-major, unnormalised), as ng_gvd_E returns them. */
+int omx_ng_gvd(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+               const double* deps_domega, const double* d2eps_domega2, double* ng_out, double* gvd_out,
+               double* E_out);
+
 /* solve_ω²(ms, solver; nev, tol, maxiter) (src/solve.jl:38-63, solvers/dftk.jl:32-37):
    lowest nev eigenpairs of M(k).  H_io: in = start block (2N x nev complex) or NULL for a
    seeded random start; out = eigenvectors (written when non-NULL). */

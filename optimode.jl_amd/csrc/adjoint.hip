@@ -265,6 +265,207 @@ __global__ __launch_bounds__(AB) void k_khbar(const c64* LE, const c64* EE, cons
   if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
 }
 
+// out[i*nb + b] partials of Re sum_p X_p^H A_p Y_p over the pixels (3-component fields in
+// the [v][iy][ix][c] layout, A symmetric as 6 entries xx yy zz xy xz yz per pixel)
+__global__ __launch_bounds__(AB) void k_qform(const c64* X, const c64* Y, const double* A6, long n3, long N,
+                                              double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double acc = 0;
+  const c64* x = X + i * n3;
+  const c64* y = Y + i * n3;
+  for (long p = long(blockIdx.x) * AB + threadIdx.x; p < N; p += long(gridDim.x) * AB) {
+    const double* a = A6 + 6 * p;
+    const c64 y0 = y[3 * p], y1 = y[3 * p + 1], y2 = y[3 * p + 2];
+    const c64 x0 = x[3 * p], x1 = x[3 * p + 1], x2 = x[3 * p + 2];
+    const double z0r = a[0] * y0.x + a[3] * y1.x + a[4] * y2.x, z0i = a[0] * y0.y + a[3] * y1.y + a[4] * y2.y;
+    const double z1r = a[3] * y0.x + a[1] * y1.x + a[5] * y2.x, z1i = a[3] * y0.y + a[1] * y1.y + a[5] * y2.y;
+    const double z2r = a[4] * y0.x + a[5] * y1.x + a[2] * y2.x, z2i = a[4] * y0.y + a[5] * y1.y + a[2] * y2.y;
+    acc += (x0.x * z0r + x0.y * z0i) + (x1.x * z1r + x1.y * z1i) + (x2.x * z2r + x2.y * z2i);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out_i = c0 kx_ct(F1_i) + c1 zx_ct(F2_i) + c2 Y_i + c3 H_i with per-instance coefficients
+// c = cf[4 i ..]; F1, F2 are 3-component k-space fields ([v][iy][ix][c]), kx_ct(e) =
+// (-(e.n) mag, (e.m) mag) and zx_ct(e) = (-e_x m_y + e_y m_x, -e_x n_y + e_y n_x)
+// (maxwell.jl:209-216, :231-240).
+__global__ __launch_bounds__(AB) void k_ct_combine(const c64* F1, const c64* F2, const c64* Y, const c64* H,
+                                                   long n3, long ld, int nx, int ny, double invdx, double invdy,
+                                                   const double* kinst, const double* cf, c64* out) {
+  const int i = blockIdx.y;
+  const double c0 = cf[4 * i], c1 = cf[4 * i + 1], c2 = cf[4 * i + 2], c3 = cf[4 * i + 3];
+  const double k = kinst[i];
+  const long N = long(nx) * ny;
+  for (long p = long(blockIdx.x) * AB + threadIdx.x; p < N; p += long(gridDim.x) * AB) {
+    const int ix = int(p % nx), iy = int(p / nx);
+    const Basis<double> b = basis_at<double>(ix, iy, nx, ny, invdx, invdy, k);
+    const c64* e = F1 + i * n3 + 3 * p;
+    const c64* f = F2 + i * n3 + 3 * p;
+    const c64 en{e[0].x * b.n0 + e[1].x * b.n1, e[0].y * b.n0 + e[1].y * b.n1};
+    const c64 em{e[0].x * b.m0 + e[1].x * b.m1 + e[2].x * b.m2, e[0].y * b.m0 + e[1].y * b.m1 + e[2].y * b.m2};
+    const c64 z1{-f[0].x * b.m1 + f[1].x * b.m0, -f[0].y * b.m1 + f[1].y * b.m0};
+    const c64 z2{-f[0].x * b.n1 + f[1].x * b.n0, -f[0].y * b.n1 + f[1].y * b.n0};
+    const long o = i * ld + 2 * p;
+    const c64 y1 = Y[o], y2 = Y[o + 1], h1 = H[o], h2 = H[o + 1];
+    out[o] = c64{-c0 * b.mag * en.x + c1 * z1.x + c2 * y1.x + c3 * h1.x,
+                 -c0 * b.mag * en.y + c1 * z1.y + c2 * y1.y + c3 * h1.y};
+    out[o + 1] = c64{c0 * b.mag * em.x + c1 * z2.x + c2 * y2.x + c3 * h2.x,
+                     c0 * b.mag * em.y + c1 * z2.y + c2 * y2.y + c3 * h2.y};
+  }
+}
+
+// Batched vector helpers over nb instances of K complex entries at stride ld, against
+// the unit vectors EV (one per instance).
+struct VecOps {
+  hipStream_t s;
+  int nb;
+  long ld, K;
+  int nblk;
+  const c64* EV;
+  DevMem part, coef;
+  VecOps(hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_)
+      : s(s_), nb(nb_), ld(ld_), K(K_), nblk(int(std::min<long>(256, (K_ + AB - 1) / AB))), EV(EV_),
+        part(sizeof(c64) * size_t(nb_) * 256), coef(sizeof(c64) * 4 * nb_) {}
+  void vdot(const c64* A, const c64* B, const int* act) {
+    hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
+  }
+  // out = sum_j coef[i][j] in_j (coefficients already on the device)
+  void lincomb(std::initializer_list<const c64*> ins, c64* outp, const int* act) {
+    Lin L{};
+    int j = 0;
+    for (auto p : ins) L.in[j++] = p;
+    L.nin = j;
+    L.out = outp;
+    L.coef = coef.as<c64>();
+    L.ld = ld;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblk, nb), dim3(AB), 0, s, L, K, act);
+  }
+  // x <- x - ev <ev, x> for active instances
+  void project(c64* x, const int* act) {
+    hipLaunchKernelGGL(k_cdot, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.as<c64>());
+    std::vector<c64> hp(size_t(nb) * nblk);
+    OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), part.as<void>(), sizeof(c64) * hp.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<c64> hc(4 * size_t(nb), c64{0, 0});
+    for (int i = 0; i < nb; ++i) {
+      c64 d{0, 0};
+      for (int b = 0; b < nblk; ++b) {
+        d.x += hp[size_t(i) * nblk + b].x;
+        d.y += hp[size_t(i) * nblk + b].y;
+      }
+      hc[4 * i] = c64{1, 0};
+      hc[4 * i + 1] = c64{-d.x, -d.y};
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(coef.as<void>(), hc.data(), sizeof(c64) * hc.size(), hipMemcpyHostToDevice, s));
+    lincomb({x, const_cast<c64*>(EV)}, x, act);
+  }
+};
+
+// Solves (M(k_i) - w^2) x_i = r_i on the complement of ev_i for the instances with want[i]
+// (MINRES, Paige & Saunders 1975, preconditioned by P projected onto the complement; M - w^2
+// is Hermitian, singular along ev and indefinite above band 1).  r is projected on entry and
+// overwritten; x receives the solution, orthogonalised against ev twice.  Returns the
+// iteration count; throws OMX_ERR_NOT_CONVERGED after 1000 iterations.
+int minres_complement(Solver& S, VecOps& V, double omega, const double* dk, c64* r, c64* xout,
+                      const std::vector<int>& want, double rtol) {
+  hipStream_t s = S.stream;
+  const int nb = V.nb;
+  const long ld = S.ld, n3 = 3 * S.N;
+  DevMem actd(sizeof(int) * nb), lived(sizeof(int) * nb);
+  OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), want.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+  const int* act = actd.as<int>();
+  DevMem X_(sizeof(c64) * nb * ld), R2(sizeof(c64) * nb * ld), Y(sizeof(c64) * nb * ld), Vb(sizeof(c64) * nb * ld),
+      W(sizeof(c64) * nb * ld), W1(sizeof(c64) * nb * ld), W2(sizeof(c64) * nb * ld), AV(sizeof(c64) * nb * ld),
+      Dm(sizeof(c64) * nb * n3), msd(sizeof(MinresState) * nb);
+  c64 *x = X_.as<c64>(), *r1 = r, *r2 = R2.as<c64>(), *y = Y.as<c64>(), *v = Vb.as<c64>(), *w = W.as<c64>(),
+      *w1 = W1.as<c64>(), *w2 = W2.as<c64>();
+  OMX_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w2, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w1, 0, sizeof(c64) * nb * ld, s));
+  std::vector<MinresState> hm(nb);
+  for (int i = 0; i < nb; ++i) {
+    std::memset(&hm[i], 0, sizeof(MinresState));
+    hm[i].rtol = rtol;
+    hm[i].shift = omega * omega;
+    hm[i].active = want[i];
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(msd.as<void>(), hm.data(), sizeof(MinresState) * nb, hipMemcpyHostToDevice, s));
+  MinresState* ms = msd.as<MinresState>();
+  const int nblk = V.nblk;
+  double* part = V.part.as<double>();
+  c64* coef = V.coef.as<c64>();
+  V.project(r1, act);
+  BView<double> AVv{AV.as<c64>(), ld, ld}, Yv{y, ld, ld}, R1v{r1, ld, ld};
+  const VecSel sel{nb, 1, nullptr, nullptr};
+  // y = P r1 (projected)
+  launch_operator<double>(OP_P, S.p64.plan, R1v, Yv, Dm.as<c64>(), sel, dk, nullptr, 0, s);
+  V.project(y, act);
+  V.vdot(r1, y, act);
+  hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 0, nb, act);
+  OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+  std::vector<int> live(want);
+  const int maxit = 1000;
+  int it = 0;
+  for (; it < maxit; ++it) {
+    const int* la = lived.as<int>();
+    if (it % 4 == 0) {
+      OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+      OMX_HIP_CHECK(hipStreamSynchronize(s));
+      bool anyl = false;
+      for (int i = 0; i < nb; ++i) {
+        live[i] = want[i] && !hm[i].done;
+        anyl = anyl || live[i];
+      }
+      if (!anyl) break;
+      OMX_HIP_CHECK(hipMemcpyAsync(lived.as<void>(), live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+    }
+    // v = y / beta (coefficients left by the previous scalar step)
+    V.lincomb({y}, v, la);
+    BView<double> Vv{v, ld, ld};
+    launch_operator<double>(OP_M, S.p64.plan, Vv, AVv, Dm.as<c64>(), sel, dk, nullptr, 0, s);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 1, nb, la);
+    V.lincomb({AV.as<c64>(), v, r1}, y, la);
+    V.vdot(v, y, la);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 2, nb, la);
+    V.lincomb({y, r2}, y, la);
+    std::swap(r1, r2);  // r1 <- r2
+    std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
+    BView<double> R2b{r2, ld, ld}, Yb{y, ld, ld};
+    launch_operator<double>(OP_P, S.p64.plan, R2b, Yb, Dm.as<c64>(), sel, dk, nullptr, 0, s);
+    V.project(y, la);
+    V.vdot(r2, y, la);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 3, nb, la);
+    // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
+    // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
+    V.lincomb({v, w2, w}, w1, la);
+    {
+      c64* wn = w1;
+      w1 = w2;
+      w2 = w;
+      w = wn;
+    }
+    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 4, nb, la);
+    V.lincomb({x, w}, x, la);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 5, nb, la);
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < nb; ++i)
+    if (want[i] && !hm[i].done) throw Error(OMX_ERR_NOT_CONVERGED, "MINRES did not converge within 1000 iterations");
+  OMX_HIP_CHECK(hipMemcpyAsync(xout, x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+  V.project(xout, act);
+  V.project(xout, act);
+  return it;
+}
+
 }  // namespace
 
 void Solver::adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
@@ -281,7 +482,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
   const int nblk = int(std::min<long>(256, (K + AB - 1) / AB));
   DevMem dk(sizeof(double) * nb), EV(sizeof(c64) * nb * ld), EB(sizeof(c64) * nb * ld), LA(sizeof(c64) * nb * ld),
       Dd(sizeof(c64) * nb * n3), Dl(sizeof(c64) * nb * n3), eb6(sizeof(double) * 6 * N), actd(sizeof(int) * nb),
-      part(sizeof(c64) * nb * nblk), coef(sizeof(c64) * 4 * nb), cfac(sizeof(double) * nb);
+      part(sizeof(c64) * nb * nblk), cfac(sizeof(double) * nb);
   OMX_HIP_CHECK(hipMemsetAsync(eb6.as<void>(), 0, sizeof(double) * 6 * N, s));
   OMX_HIP_CHECK(hipMemcpyAsync(dk.as<void>(), kmags, sizeof(double) * nb, hipMemcpyHostToDevice, s));
   for (int i = 0; i < nb; ++i)
@@ -307,39 +508,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
   std::vector<double> slope(nb), khb(nb, 0.0);
   for (int i = 0; i < nb; ++i) slope[i] = 2.0 * sum_part(hh.data() + size_t(i) * dps, dps);
 
-  auto vdot = [&](const c64* A, const c64* B, const int* act) {
-    hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
-  };
-  auto lincomb = [&](std::initializer_list<const c64*> ins, c64* outp, const int* act) {
-    Lin L{};
-    int j = 0;
-    for (auto p : ins) L.in[j++] = p;
-    L.nin = j;
-    L.out = outp;
-    L.coef = coef.as<c64>();
-    L.ld = ld;
-    hipLaunchKernelGGL(k_lincomb, dim3(nblk, nb), dim3(AB), 0, s, L, K, act);
-  };
-  // complex projection out = x - ev <ev, x> for active instances (coef set on host per call)
-  auto project = [&](c64* x, const int* act) {
-    hipLaunchKernelGGL(k_cdot, dim3(nblk, nb), dim3(AB), 0, s, EV.as<c64>(), x, ld, K, act, part.as<c64>());
-    std::vector<c64> hp(size_t(nb) * nblk);
-    OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), part.as<void>(), sizeof(c64) * hp.size(), hipMemcpyDeviceToHost, s));
-    OMX_HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<c64> hc(4 * size_t(nb), c64{0, 0});
-    for (int i = 0; i < nb; ++i) {
-      c64 d{0, 0};
-      for (int b = 0; b < nblk; ++b) {
-        d.x += hp[size_t(i) * nblk + b].x;
-        d.y += hp[size_t(i) * nblk + b].y;
-      }
-      hc[4 * i] = c64{1, 0};
-      hc[4 * i + 1] = c64{-d.x, -d.y};
-    }
-    OMX_HIP_CHECK(hipMemcpyAsync(coef.as<void>(), hc.data(), sizeof(c64) * hc.size(), hipMemcpyHostToDevice, s));
-    lincomb({x, EV.as<c64>()}, x, act);
-  };
-
+  VecOps V(s, nb, ld, K, EV.as<c64>());
   if (any_ev) {
     OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), hasv.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
     const int* act = actd.as<int>();
@@ -347,102 +516,8 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
       if (hasv[i])
         OMX_HIP_CHECK(hipMemcpyAsync(EB.as<c64>() + i * ld, evbar + 2 * size_t(i) * K, sizeof(c64) * K,
                                      hipMemcpyHostToDevice, s));
-    // ---- MINRES on (M - w^2) restricted to the complement of ev ----
-    DevMem X_(sizeof(c64) * nb * ld), R1(sizeof(c64) * nb * ld), R2(sizeof(c64) * nb * ld), Y(sizeof(c64) * nb * ld),
-        V(sizeof(c64) * nb * ld), W(sizeof(c64) * nb * ld), W1(sizeof(c64) * nb * ld), W2(sizeof(c64) * nb * ld),
-        AV(sizeof(c64) * nb * ld), Dm(sizeof(c64) * nb * n3), msd(sizeof(MinresState) * nb);
-    c64 *x = X_.as<c64>(), *r1 = R1.as<c64>(), *r2 = R2.as<c64>(), *y = Y.as<c64>(), *v = V.as<c64>(),
-        *w = W.as<c64>(), *w1 = W1.as<c64>(), *w2 = W2.as<c64>();
-    OMX_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nb * ld, s));
-    OMX_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(c64) * nb * ld, s));
-    OMX_HIP_CHECK(hipMemsetAsync(w2, 0, sizeof(c64) * nb * ld, s));
-    OMX_HIP_CHECK(hipMemsetAsync(w1, 0, sizeof(c64) * nb * ld, s));
-    std::vector<MinresState> hm(nb);
-    for (int i = 0; i < nb; ++i) {
-      std::memset(&hm[i], 0, sizeof(MinresState));
-      hm[i].rtol = 1e-10;
-      hm[i].shift = omega * omega;
-      hm[i].active = hasv[i];
-    }
-    OMX_HIP_CHECK(hipMemcpyAsync(msd.as<void>(), hm.data(), sizeof(MinresState) * nb, hipMemcpyHostToDevice, s));
-    MinresState* ms = msd.as<MinresState>();
-    // r1 = evbar - ev <ev, evbar>
-    OMX_HIP_CHECK(hipMemcpyAsync(r1, EB.as<c64>(), sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
-    project(r1, act);
-    BView<double> Vv{v, ld, ld}, AVv{AV.as<c64>(), ld, ld}, R2v{r2, ld, ld}, Yv{y, ld, ld};
-    VecSel selA{nb, 1, nullptr, nullptr};
-    // y = P r1 (projected)
-    BView<double> R1v{r1, ld, ld};
-    launch_operator<double>(OP_P, p64.plan, R1v, Yv, Dm.as<c64>(), selA, dk.as<double>(), nullptr, 0, s);
-    project(y, act);
-    vdot(r1, y, act);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 0, nb,
-                       act);
-    OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
-    std::vector<int> live(hasv);
-    DevMem lived(sizeof(int) * nb);
-    const int maxit = 1000;
-    int it = 0;
-    for (; it < maxit; ++it) {
-      const int* la = lived.as<int>();
-      if (it % 4 == 0) {
-        OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
-        OMX_HIP_CHECK(hipStreamSynchronize(s));
-        bool anyl = false;
-        for (int i = 0; i < nb; ++i) {
-          live[i] = hasv[i] && !hm[i].done;
-          anyl = anyl || live[i];
-        }
-        if (!anyl) break;
-        OMX_HIP_CHECK(hipMemcpyAsync(lived.as<void>(), live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
-      }
-      // v = y / beta (coefficients left by the previous scalar step)
-      lincomb({y}, v, la);
-      BView<double> Vb{v, ld, ld};
-      launch_operator<double>(OP_M, p64.plan, Vb, AVv, Dm.as<c64>(), VecSel{nb, 1, nullptr, nullptr}, dk.as<double>(),
-                              nullptr, 0, s);
-      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 1, nb,
-                         la);
-      lincomb({AV.as<c64>(), v, r1}, y, la);
-      vdot(v, y, la);
-      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 2, nb,
-                         la);
-      lincomb({y, r2}, y, la);
-      std::swap(r1, r2);  // r1 <- r2
-      std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
-      BView<double> R2b{r2, ld, ld}, Yb{y, ld, ld};
-      launch_operator<double>(OP_P, p64.plan, R2b, Yb, Dm.as<c64>(), VecSel{nb, 1, nullptr, nullptr}, dk.as<double>(),
-                              nullptr, 0, s);
-      project(y, la);
-      vdot(r2, y, la);
-      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 3, nb,
-                         la);
-      // w_new = (v - oldeps w1 - delta w2) denom, rotating w1 <- w2 <- w <- new
-      // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
-      // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
-      lincomb({v, w2, w}, w1, la);
-      {
-        c64* wn = w1;
-        w1 = w2;
-        w2 = w;
-        w = wn;
-      }
-      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 4, nb,
-                         la);
-      lincomb({x, w}, x, la);
-      hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part.as<double>(), nblk, coef.as<c64>(), 5, nb,
-                         la);
-    }
-    OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
-    OMX_HIP_CHECK(hipStreamSynchronize(s));
-    for (int i = 0; i < nb; ++i)
-      if (hasv[i] && !hm[i].done)
-        throw Error(OMX_ERR_NOT_CONVERGED, "adjoint MINRES did not converge within 1000 iterations");
-    // lambda = x, orthogonalised against ev (twice, as eig_adjt and the rrule both do)
-    OMX_HIP_CHECK(hipMemcpyAsync(LA.as<c64>(), x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
-    project(LA.as<c64>(), act);
-    project(LA.as<c64>(), act);
-    last_adjoint_iters = it;
+    // lambda: (M - w^2) lambda = evbar - ev <ev, evbar> on the complement of ev
+    last_adjoint_iters = minres_complement(*this, V, omega, dk.as<double>(), EB.as<c64>(), LA.as<c64>(), hasv, 1e-10);
     // d = fft(kx_tc(ev))/N, ld = fft(kx_tc(lambda)); epsbar += epsinv_bar(d, ld)
     BView<double> LAv{LA.as<c64>(), ld, ld};
     launch_fields<double>(FIELD_D, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);
@@ -490,6 +565,136 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
     o[7] = o[5] = a[5];
   }
   *omegabar = wb;
+}
+
+// ng_gvd / ng_gvd_E (lib/ModeAnalysis/src/group_index.jl:126-162, :55-95) per mode, FP64:
+// the group index n_g = dk/dw from the mode field and GVD = dn_g/dw from one linear solve.
+// The reference propagates the sensitivity of n_g backwards (eig_adjt with xbar = dn_g/dev,
+// then the basis k-map); here the same derivative is taken forwards, which needs the same
+// single solve with M - w^2 and no tensor-valued cotangents.  With Ei = eps^-1, De = deps/dw,
+// DDe = d2eps/dw2, B = Ei De Ei, D = fft(kx_tc ev), D_k = fft(d kx_tc/dk ev),
+// D_kk = fft(d2 kx_tc/dk2 ev) and Q(X, A, Y) = Re sum_p X_p^H A_p Y_p / N:
+//   n_g   = (w + Q(D,B,D)/2) / Q(D,Ei,D_k)                      (= group_index.jl:27)
+//   ev'   : (M - w^2) ev' = -n_g dM/dk ev + M_B ev + 2 w ev on the complement of ev, with
+//           dM/dk ev = (kx_ct bfft(Ei D_k) - zx_ct bfft(Ei D)) / N
+//   num'  = 1 + Q(D,B',D)/2 + Q(D',B,D) + n_g Q(D_k,B,D),  B' = Ei DDe Ei - B De Ei - Ei De B
+//   den'  = -Q(D,B,D_k) + Q(D',Ei,D_k) + Q(D,Ei,D'_k) + n_g (Q(D_k,Ei,D_k) + Q(D,Ei,D_kk))
+//   GVD   = (num' - n_g den') / Q(D,Ei,D_k)
+// E_out (optional): E = Ei . fft(kx_tc(ev)) per mode, (3,Nx,Ny) complex column-major.
+void Solver::ng_gvd(double omega, int nev, const double* kmags, const double* evecs, const double* deps,
+                    const double* ddeps, double* ng_out, double* gvd_out, double* E_out) {
+  if (nev < 1 || nev > 64) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..64");
+  if (!kmags || !evecs || !deps || !ddeps || !ng_out || !gvd_out) throw Error(OMX_ERR_INVALID_ARG, "NULL argument");
+  if (!(omega > 0) || !std::isfinite(omega)) throw Error(OMX_ERR_INVALID_ARG, "omega must be positive and finite");
+  for (int i = 0; i < nev; ++i)
+    if (!(kmags[i] > 0) || !std::isfinite(kmags[i])) throw Error(OMX_ERR_INVALID_ARG, "kmags must be positive");
+  check_symmetric(N, deps, "deps_domega");
+  check_symmetric(N, ddeps, "d2eps_domega2");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  hipStream_t s = stream;
+  const int nb = nev;
+  const long n3 = 3 * N;
+  const double* Ei = epsinv_host.data();
+  // B = Ei De Ei and B' = Ei DDe Ei - B De Ei - Ei De B (host, per pixel)
+  std::vector<double> B(9 * size_t(N)), Bp(9 * size_t(N)), t1(9 * size_t(N)), t2(9 * size_t(N));
+  pixel_product3(N, Ei, deps, Ei, B.data());
+  pixel_product3(N, Ei, ddeps, Ei, Bp.data());
+  // B De Ei and Ei De B are transposes of each other (all factors symmetric): their sum is
+  // twice the symmetric part of B De Ei
+  pixel_product3(N, B.data(), deps, Ei, t1.data());
+  for (size_t q = 0; q < Bp.size(); ++q) Bp[q] -= 2.0 * t1[q];
+  DevMem b6, b6c, bp6, bp6c;
+  upload_tensor6<double>(*this, B.data(), b6, b6c);
+  upload_tensor6<double>(*this, Bp.data(), bp6, bp6c);
+  const double* ei6 = p64.plan.eps6;
+  OpPlan<double> planB = p64.plan;
+  planB.eps6 = b6.as<double>();
+  planB.eps6c = b6c.as<double>();
+
+  const int nblk = int(std::min<long>(256, (N + AB - 1) / AB));
+  DevMem dk(sizeof(double) * nb), EV(sizeof(c64) * nb * ld), Hp(sizeof(c64) * nb * ld), R(sizeof(c64) * nb * ld),
+      Y(sizeof(c64) * nb * ld), D(sizeof(c64) * nb * n3), Dk(sizeof(c64) * nb * n3), F1(sizeof(c64) * nb * n3),
+      F2(sizeof(c64) * nb * n3), qp(sizeof(double) * 9 * nb * nblk), cf(sizeof(double) * 4 * nb);
+  OMX_HIP_CHECK(hipMemcpyAsync(dk.as<void>(), kmags, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemsetAsync(EV.as<void>(), 0, sizeof(c64) * nb * ld, s));
+  for (int i = 0; i < nb; ++i)
+    OMX_HIP_CHECK(hipMemcpyAsync(EV.as<c64>() + i * ld, evecs + 2 * size_t(i) * K, sizeof(c64) * K,
+                                 hipMemcpyHostToDevice, s));
+  const VecSel sel{nb, 1, nullptr, nullptr};
+  BView<double> EVv{EV.as<c64>(), ld, ld}, Hpv{Hp.as<c64>(), ld, ld}, Yv{Y.as<c64>(), ld, ld};
+  auto qform = [&](int slot, const DevMem& X, const DevMem& Yf, const double* A6) {
+    hipLaunchKernelGGL(k_qform, dim3(nblk, nb), dim3(AB), 0, s, X.as<c64>(), Yf.as<c64>(), A6, n3, N,
+                       qp.as<double>() + size_t(slot) * nb * nblk);
+  };
+  auto fetch_q = [&](int slots) {
+    std::vector<double> h(size_t(slots) * nb * nblk);
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), qp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> q(size_t(slots) * nb);
+    for (int j = 0; j < slots; ++j)
+      for (int i = 0; i < nb; ++i) q[size_t(j) * nb + i] = sum_part(h.data() + (size_t(j) * nb + i) * nblk, nblk) / double(N);
+    return q;
+  };
+  // fields of ev: D, D_k, and D_kk (in F1)
+  launch_fields<double>(FIELD_D, p64.plan, EVv, D.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  launch_fields<double>(FIELD_DK, p64.plan, EVv, Dk.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  launch_fields<double>(FIELD_DKK, p64.plan, EVv, F1.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  qform(0, D, D, b6.as<double>());    // Q(D,B,D)
+  qform(1, D, Dk, ei6);               // Q(D,Ei,D_k)
+  qform(2, D, D, bp6.as<double>());   // Q(D,B',D)
+  qform(3, Dk, D, b6.as<double>());   // Q(D_k,B,D)
+  qform(4, Dk, Dk, ei6);              // Q(D_k,Ei,D_k)
+  qform(5, D, F1, ei6);               // Q(D,Ei,D_kk)
+  const std::vector<double> q0 = fetch_q(6);
+  std::vector<double> ng(nb);
+  std::vector<double> hcf(4 * size_t(nb));
+  for (int i = 0; i < nb; ++i) {
+    ng[i] = (omega + 0.5 * q0[i]) / q0[size_t(nb) + i];
+    hcf[4 * i] = -ng[i] / double(N);
+    hcf[4 * i + 1] = ng[i] / double(N);
+    hcf[4 * i + 2] = 1.0;
+    hcf[4 * i + 3] = 2.0 * omega;
+  }
+  // right-hand side r = -n_g dM/dk ev + M_B ev + 2 w ev
+  OMX_HIP_CHECK(hipMemcpyAsync(F1.as<void>(), Dk.as<void>(), sizeof(c64) * nb * n3, hipMemcpyDeviceToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(F2.as<void>(), D.as<void>(), sizeof(c64) * nb * n3, hipMemcpyDeviceToDevice, s));
+  launch_fields<double>(FIELD_EINV, p64.plan, EVv, F1.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  launch_fields<double>(FIELD_EINV, p64.plan, EVv, F2.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  {
+    DevMem Dm(sizeof(c64) * nb * n3);
+    launch_operator<double>(OP_M, planB, EVv, Yv, Dm.as<c64>(), sel, dk.as<double>(), nullptr, 0, s);
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(cf.as<void>(), hcf.data(), sizeof(double) * hcf.size(), hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemsetAsync(R.as<void>(), 0, sizeof(c64) * nb * ld, s));
+  hipLaunchKernelGGL(k_ct_combine, dim3(nblk, nb), dim3(AB), 0, s, F1.as<c64>(), F2.as<c64>(), Y.as<c64>(),
+                     EV.as<c64>(), n3, ld, nx, ny, 1.0 / Dx, 1.0 / Dy, dk.as<double>(), cf.as<double>(), R.as<c64>());
+  OMX_HIP_CHECK(hipGetLastError());
+  // ev' on the complement of ev
+  VecOps V(s, nb, ld, K, EV.as<c64>());
+  const std::vector<int> all(nb, 1);
+  last_adjoint_iters = minres_complement(*this, V, omega, dk.as<double>(), R.as<c64>(), Hp.as<c64>(), all, 1e-12);
+  // fields of ev': D' (F1) and D'_k (F2)
+  launch_fields<double>(FIELD_D, p64.plan, Hpv, F1.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  launch_fields<double>(FIELD_DK, p64.plan, Hpv, F2.as<c64>(), sel, dk.as<double>(), 1.0, s);
+  qform(0, F1, D, b6.as<double>());   // Q(D',B,D)
+  qform(1, F1, Dk, ei6);              // Q(D',Ei,D_k)
+  qform(2, D, F2, ei6);               // Q(D,Ei,D'_k)
+  const std::vector<double> q1 = fetch_q(3);
+  for (int i = 0; i < nb; ++i) {
+    const double QDEDk = q0[nb + i], QDBpD = q0[2 * nb + i], QDkBD = q0[3 * nb + i], QDkEDk = q0[4 * nb + i],
+                 QDEDkk = q0[5 * nb + i];
+    const double QDpBD = q1[i], QDpEDk = q1[nb + i], QDEDpk = q1[2 * nb + i];
+    const double g = ng[i];
+    const double nump = 1.0 + 0.5 * QDBpD + QDpBD + g * QDkBD;
+    const double denp = -QDkBD + QDpEDk + QDEDpk + g * (QDkEDk + QDEDkk);
+    ng_out[i] = g;
+    gvd_out[i] = (nump - g * denp) / QDEDk;
+  }
+  if (E_out) {
+    launch_fields<double>(FIELD_E, p64.plan, EVv, F1.as<c64>(), sel, dk.as<double>(), 1.0, s);
+    OMX_HIP_CHECK(hipMemcpyAsync(E_out, F1.as<void>(), sizeof(c64) * nb * n3, hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+  }
 }
 
 }  // namespace omx

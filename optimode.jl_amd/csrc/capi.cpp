@@ -136,6 +136,14 @@ int omx_group_index(omx_solver* h, double omega, int nev, const double* kmags, c
   return guarded(h, [&](omx::Solver& s) { s.group_index(omega, nev, kmags, evecs, deps_domega, ng_out); });
 }
 
+int omx_ng_gvd(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+               const double* deps_domega, const double* d2eps_domega2, double* ng_out, double* gvd_out,
+               double* E_out) {
+  return guarded(h, [&](omx::Solver& s) {
+    s.ng_gvd(omega, nev, kmags, evecs, deps_domega, d2eps_domega2, ng_out, gvd_out, E_out);
+  });
+}
+
 int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, double* H_io, double* omega2_out,
                      omx_info* info) {
   return guarded(h, [&](omx::Solver& s) { s.solve_omega2(k, nev, tol, maxiter, H_io, omega2_out, info); });

@@ -9,39 +9,6 @@ namespace {
 
 constexpr int OP_BLOCK = 256;
 
-template <typename T> struct Basis {
-  T mag, m0, m1, m2, n0, n1, n2;
-};
-
-// Transverse basis at w = (0,0,k) - G: mag = |w|, n = normalize(z x w) (y at the gimbal
-// point), m = normalize(n x w)   (maxwell.jl:646-660).
-template <typename T>
-__device__ __forceinline__ Basis<T> basis_at(int ix, int iy, int nx, int ny, double invdx, double invdy, T k) {
-  const T gx = T(double(fft_index(ix, nx)) * invdx);
-  const T gy = T(double(fft_index(iy, ny)) * invdy);
-  const T wx = -gx, wy = -gy, wz = k;
-  Basis<T> b;
-  b.mag = sqrt(wx * wx + wy * wy + wz * wz);
-  const T rho2 = wx * wx + wy * wy;
-  if (rho2 > T(0)) {
-    const T ir = T(1) / sqrt(rho2);
-    b.n0 = -wy * ir;
-    b.n1 = wx * ir;
-  } else {
-    b.n0 = T(0);
-    b.n1 = T(1);
-  }
-  b.n2 = T(0);
-  const T cx = b.n1 * wz - b.n2 * wy;
-  const T cy = b.n2 * wx - b.n0 * wz;
-  const T cz = b.n0 * wy - b.n1 * wx;
-  const T ic = T(1) / sqrt(cx * cx + cy * cy + cz * cz);
-  b.m0 = cx * ic;
-  b.m1 = cy * ic;
-  b.m2 = cz * ic;
-  return b;
-}
-
 __device__ __forceinline__ bool vec_selected(const VecSel& sel, int inst, int col) {
   if (sel.col0 == nullptr) return true;
   const int c0 = sel.col0[inst];
@@ -52,9 +19,12 @@ template <int OP> struct Dirs;
 template <> struct Dirs<OP_M> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
 template <> struct Dirs<OP_MKH> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
 template <> struct Dirs<OP_P> { static constexpr int x1 = +1, y1 = +1, y2 = -1, x2 = -1; };
+// pass-1 curl of the field helper FIELD_DKK (second k-derivative of kx_tc); forward only
+constexpr int CURL_DKK = 3;
+template <> struct Dirs<CURL_DKK> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
 
 // ---------------------------------------------------------------------------------------
-// pass 1: k-space curl (kx_tc / kxinv_tc / zx_tc) + FFT along x
+// pass 1: k-space curl (kx_tc / kxinv_tc / zx_tc / d2 kx_tc / dk2) + FFT along x
 // ---------------------------------------------------------------------------------------
 template <typename T, int OP, int VPT>
 __global__ __launch_bounds__(OP_BLOCK) void k_pass1(OpPlan<T> plan, BView<T> X, cplx<T>* D, VecSel sel,
@@ -87,9 +57,16 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass1(OpPlan<T> plan, BView<T> X, 
       d0 = (h1 * b.n0 - h2 * b.m0) * im;
       d1 = (h1 * b.n1 - h2 * b.m1) * im;
       d2 = (h1 * b.n2 - h2 * b.m2) * im;
-    } else {
+    } else if (OP == OP_MKH) {
       d0 = T(-1) * (h1 * b.m1 + h2 * b.n1);
       d1 = h1 * b.m0 + h2 * b.n0;
+      d2 = cplx<T>{T(0), T(0)};
+    } else {
+      // d^2/dk^2 of kx_tc at fixed (h1, h2): -h1 n rho^2 / |k+G|^3, rho = |G|
+      const T gx = T(double(fft_index(ix, nx)) * plan.invdx), gy = T(double(fft_index(iy, ny)) * plan.invdy);
+      const T f = -(gx * gx + gy * gy) / (b.mag * b.mag * b.mag);
+      d0 = h1 * (b.n0 * f);
+      d1 = h1 * (b.n1 * f);
       d2 = cplx<T>{T(0), T(0)};
     }
     buf[(r * 3 + 0) * nx + ix] = d0;
@@ -654,7 +631,15 @@ void launch_fields_vpt(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* 
   dim3 g2(plan.nx / plan.cols_per_block, nvec);
   const size_t lds13 = cs * (size_t(plan.nx) + size_t(plan.rows_per_block) * 3 * plan.nx);
   const size_t lds2 = cs * (size_t(plan.ny) + size_t(plan.cols_per_block) * 3 * plan.ny);
-  if (fop == FIELD_E || fop == FIELD_D) {
+  if (fop == FIELD_DK || fop == FIELD_DKK) {
+    if (fop == FIELD_DK) {
+      hipLaunchKernelGGL((k_pass1<T, OP_MKH, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
+      scale = -scale;  // d kx_tc / dk = -zx_tc
+    } else {
+      hipLaunchKernelGGL((k_pass1<T, CURL_DKK, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
+    }
+    hipLaunchKernelGGL((k_pass2_fwd<T, VPT, false>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
+  } else if (fop == FIELD_E || fop == FIELD_D) {
     hipLaunchKernelGGL((k_pass1<T, OP_M, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
     if (fop == FIELD_E)
       hipLaunchKernelGGL((k_pass2_fwd<T, VPT, true>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);

@@ -27,6 +27,39 @@ namespace omx {
 
 enum OpKind : int { OP_M = 0, OP_P = 1, OP_MKH = 2 };
 
+template <typename T> struct Basis {
+  T mag, m0, m1, m2, n0, n1, n2;
+};
+
+// Transverse basis at w = (0,0,k) - G: mag = |w|, n = normalize(z x w) (y at the gimbal
+// point), m = normalize(n x w)   (maxwell.jl:646-660).
+template <typename T>
+__device__ __forceinline__ Basis<T> basis_at(int ix, int iy, int nx, int ny, double invdx, double invdy, T k) {
+  const T gx = T(double(fft_index(ix, nx)) * invdx);
+  const T gy = T(double(fft_index(iy, ny)) * invdy);
+  const T wx = -gx, wy = -gy, wz = k;
+  Basis<T> b;
+  b.mag = sqrt(wx * wx + wy * wy + wz * wz);
+  const T rho2 = wx * wx + wy * wy;
+  if (rho2 > T(0)) {
+    const T ir = T(1) / sqrt(rho2);
+    b.n0 = -wy * ir;
+    b.n1 = wx * ir;
+  } else {
+    b.n0 = T(0);
+    b.n1 = T(1);
+  }
+  b.n2 = T(0);
+  const T cx = b.n1 * wz - b.n2 * wy;
+  const T cy = b.n2 * wx - b.n0 * wz;
+  const T cz = b.n0 * wy - b.n1 * wx;
+  const T ic = T(1) / sqrt(cx * cx + cy * cy + cz * cz);
+  b.m0 = cx * ic;
+  b.m1 = cy * ic;
+  b.m2 = cz * ic;
+  return b;
+}
+
 // Batched block view: column c of instance i starts at base + i*istride + c*ld.
 template <typename T> struct BView {
   cplx<T>* base;
@@ -80,7 +113,11 @@ void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, c
 //   FIELD_E: D = eps^-1 . fft2(kx_tc(H)) * scale   (E field, fields.jl:194-210)
 //   FIELD_D: D = fft2(kx_tc(H)) * scale             (_H2d!, fields.jl:81-86)
 //   FIELD_EINV: D <- bfft2(eps^-1 . D) * scale      (_d2ẽ!, fields.jl:88-93), X unused
-enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2 };
+//   FIELD_DK: D = fft2(d kx_tc/dk (H)) * scale = -fft2(zx_tc(H)) * scale
+//   FIELD_DKK: D = fft2(d^2 kx_tc/dk^2 (H)) * scale = fft2(-H1 n |G|^2 / |k+G|^3) * scale
+// (the k-derivatives are taken at fixed transverse coefficients H, with the basis
+// m(k), n, |k+G|(k) of basis_at)
+enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2, FIELD_DK = 3, FIELD_DKK = 4 };
 template <typename T>
 void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                    const double* kinst, double scale, hipStream_t stream);

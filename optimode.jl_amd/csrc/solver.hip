@@ -59,6 +59,70 @@ Profiler::~Profiler() {
 template <> PrecisionData<float>& Solver::pd<float>() { return p32; }
 template <> PrecisionData<double>& Solver::pd<double>() { return p64; }
 
+// A symmetric per-pixel 3x3 field (column-major (3,3,Nx,Ny), FP64 host) in the two device
+// layouts the operator reads: 6 unique entries xx, yy, zz, xy, xz, yz per pixel, row-major
+// (iy, ix) and column-major (ix, iy).
+template <typename T>
+void upload_tensor6(const Solver& S, const double* E, DevMem& rowm, DevMem& colm) {
+  const long N = S.N;
+  std::vector<T> e6(size_t(6) * N);
+  for (int iy = 0; iy < S.ny; ++iy)
+    for (int ix = 0; ix < S.nx; ++ix) {
+      const long p = ix + long(S.nx) * iy;
+      const double* e = E + 9 * p;
+      T* o = e6.data() + 6 * p;
+      o[0] = T(e[0]);  // xx
+      o[1] = T(e[4]);  // yy
+      o[2] = T(e[8]);  // zz
+      o[3] = T(e[3]);  // (0,1)
+      o[4] = T(e[6]);  // (0,2)
+      o[5] = T(e[7]);  // (1,2)
+    }
+  rowm.ensure(sizeof(T) * 6 * N);
+  OMX_HIP_CHECK(hipMemcpy(rowm.as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
+  // column-major copy: pixel (ix, iy) at 6 (iy + ny ix)
+  std::vector<T> e6c(size_t(6) * N);
+  for (int ix = 0; ix < S.nx; ++ix)
+    for (int iy = 0; iy < S.ny; ++iy)
+      for (int a = 0; a < 6; ++a) e6c[6 * (iy + size_t(S.ny) * ix) + a] = e6[6 * (ix + size_t(S.nx) * iy) + a];
+  colm.ensure(sizeof(T) * 6 * N);
+  OMX_HIP_CHECK(hipMemcpy(colm.as<void>(), e6c.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
+}
+
+template void upload_tensor6<float>(const Solver&, const double*, DevMem&, DevMem&);
+template void upload_tensor6<double>(const Solver&, const double*, DevMem&, DevMem&);
+
+// the symmetry test set_epsinv applies, for any per-pixel 3x3 field
+void check_symmetric(long N, const double* E, const char* what) {
+  double amax = 0;
+  for (long i = 0; i < 9 * N; ++i) {
+    if (!std::isfinite(E[i])) throw Error(OMX_ERR_INVALID_ARG, std::string(what) + " has non-finite entries");
+    amax = std::max(amax, std::fabs(E[i]));
+  }
+  for (long p = 0; p < N; ++p) {
+    const double* e = E + 9 * p;
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b)
+        if (std::fabs(e[a + 3 * b] - e[b + 3 * a]) > 1e-12 * amax)
+          throw Error(OMX_ERR_UNSUPPORTED, std::string(what) + " must be symmetric per pixel");
+  }
+}
+
+// per pixel, out = sym(L M R) for column-major 3x3 fields (3,3,Nx,Ny)
+void pixel_product3(long N, const double* L, const double* M, const double* R, double* out) {
+  for (long p = 0; p < N; ++p) {
+    const double *l = L + 9 * p, *m = M + 9 * p, *r = R + 9 * p;
+    double t[9];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) t[a + 3 * b] = l[a] * m[3 * b] + l[a + 3] * m[1 + 3 * b] + l[a + 6] * m[2 + 3 * b];
+    double* o = out + 9 * p;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) o[a + 3 * b] = t[a] * r[3 * b] + t[a + 3] * r[1 + 3 * b] + t[a + 6] * r[2 + 3 * b];
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b) o[a + 3 * b] = o[b + 3 * a] = 0.5 * (o[a + 3 * b] + o[b + 3 * a]);
+  }
+}
+
 namespace {
 
 constexpr double EPS32 = 1.1920928955078125e-07;
@@ -145,56 +209,10 @@ template <typename T> void select_operator(Solver& S, PrecisionData<T>& P) {
   pl.dot_blocks = S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block);
 }
 
-// A symmetric per-pixel 3x3 field (column-major (3,3,Nx,Ny), FP64 host) in the two device
-// layouts the operator reads: 6 unique entries xx, yy, zz, xy, xz, yz per pixel, row-major
-// (iy, ix) and column-major (ix, iy).
-template <typename T>
-void upload_tensor6(const Solver& S, const double* E, DevMem& rowm, DevMem& colm) {
-  const long N = S.N;
-  std::vector<T> e6(size_t(6) * N);
-  for (int iy = 0; iy < S.ny; ++iy)
-    for (int ix = 0; ix < S.nx; ++ix) {
-      const long p = ix + long(S.nx) * iy;
-      const double* e = E + 9 * p;
-      T* o = e6.data() + 6 * p;
-      o[0] = T(e[0]);  // xx
-      o[1] = T(e[4]);  // yy
-      o[2] = T(e[8]);  // zz
-      o[3] = T(e[3]);  // (0,1)
-      o[4] = T(e[6]);  // (0,2)
-      o[5] = T(e[7]);  // (1,2)
-    }
-  rowm.ensure(sizeof(T) * 6 * N);
-  OMX_HIP_CHECK(hipMemcpy(rowm.as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
-  // column-major copy: pixel (ix, iy) at 6 (iy + ny ix)
-  std::vector<T> e6c(size_t(6) * N);
-  for (int ix = 0; ix < S.nx; ++ix)
-    for (int iy = 0; iy < S.ny; ++iy)
-      for (int a = 0; a < 6; ++a) e6c[6 * (iy + size_t(S.ny) * ix) + a] = e6[6 * (ix + size_t(S.nx) * iy) + a];
-  colm.ensure(sizeof(T) * 6 * N);
-  OMX_HIP_CHECK(hipMemcpy(colm.as<void>(), e6c.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
-}
-
 template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
   upload_tensor6<T>(S, S.epsinv_host.data(), P.eps6, P.eps6c);
   P.plan.eps6 = P.eps6.template as<T>();
   P.plan.eps6c = P.eps6c.template as<T>();
-}
-
-// the symmetry test set_epsinv applies, for any per-pixel 3x3 field
-void check_symmetric(long N, const double* E, const char* what) {
-  double amax = 0;
-  for (long i = 0; i < 9 * N; ++i) {
-    if (!std::isfinite(E[i])) throw Error(OMX_ERR_INVALID_ARG, std::string(what) + " has non-finite entries");
-    amax = std::max(amax, std::fabs(E[i]));
-  }
-  for (long p = 0; p < N; ++p) {
-    const double* e = E + 9 * p;
-    for (int a = 0; a < 3; ++a)
-      for (int b = a + 1; b < 3; ++b)
-        if (std::fabs(e[a + 3 * b] - e[b + 3 * a]) > 1e-12 * amax)
-          throw Error(OMX_ERR_UNSUPPORTED, std::string(what) + " must be symmetric per pixel");
-  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -695,30 +713,7 @@ void Solver::group_index(double omega, int nev, const double* kmags, const doubl
   if (deps_domega) {
     check_symmetric(N, deps_domega, "deps_domega");
     A.assign(size_t(9) * N, 0.0);
-    for (long p = 0; p < N; ++p) {
-      const double* ei = epsinv_host.data() + 9 * p;
-      const double* dw = deps_domega + 9 * p;
-      double t[9];
-      // column-major 3x3: M[a + 3b]; t = ei * dw, A = t * ei
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          double acc = 0;
-          for (int c = 0; c < 3; ++c) acc += ei[a + 3 * c] * dw[c + 3 * b];
-          t[a + 3 * b] = acc;
-        }
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          double acc = 0;
-          for (int c = 0; c < 3; ++c) acc += t[a + 3 * c] * ei[c + 3 * b];
-          A[9 * p + a + 3 * b] = acc;
-        }
-      // exact symmetry of ei dw ei for symmetric ei, dw: average away rounding
-      for (int a = 0; a < 3; ++a)
-        for (int b = a + 1; b < 3; ++b) {
-          const double s = 0.5 * (A[9 * p + a + 3 * b] + A[9 * p + b + 3 * a]);
-          A[9 * p + a + 3 * b] = A[9 * p + b + 3 * a] = s;
-        }
-    }
+    pixel_product3(N, epsinv_host.data(), deps_domega, epsinv_host.data(), A.data());
   }
   for (int j = 0; j < nev; ++j) {
     const double* ev = evecs + 2 * size_t(j) * K;

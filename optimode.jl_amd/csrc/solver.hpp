@@ -163,6 +163,8 @@ class Solver {
                    double* ng_out);
   void adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
                const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar);
+  void ng_gvd(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
+              const double* d2eps_domega2, double* ng_out, double* gvd_out, double* E_out);
   void set_option(const std::string& key, double value);
 
   std::mutex mu;
@@ -196,6 +198,14 @@ class Solver {
   template <typename T> PrecisionData<T>& pd();
   void canonicalize_phase(const double* d_k, c64* vecs, long ldv, long istride, int nvec, int mper);
 };
+
+// A symmetric per-pixel 3x3 field (column-major (3,3,Nx,Ny), FP64 host) uploaded in the two
+// device layouts the operator reads (row-major and column-major pixel order, 6 entries each).
+template <typename T> void upload_tensor6(const Solver& S, const double* E, DevMem& rowm, DevMem& colm);
+// throws unless every pixel of E is finite and symmetric
+void check_symmetric(long N, const double* E, const char* what);
+// per pixel, out = sym(L M R) for column-major 3x3 fields
+void pixel_product3(long N, const double* L, const double* M, const double* R, double* out);
 
 // Kottke-smoothed eps^-1 of convex polygons (smooth.hip); no handle needed.
 void smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshape, const int* nverts,

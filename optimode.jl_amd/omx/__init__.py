@@ -25,8 +25,8 @@ import numpy as np
 from . import _lib
 from ._lib import OmxError, OmxInfo, OMX_PREC_FP32, OMX_PREC_FP64  # noqa: F401
 
-__all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "rrule_solve_k", "k_guess",
-           "OmxError", "library_path", "exported_symbols", "kernel_stats"]
+__all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "rrule_solve_k", "group_index",
+           "ng_gvd", "k_guess", "OmxError", "library_path", "exported_symbols", "kernel_stats"]
 
 
 def library_path():
@@ -290,6 +290,31 @@ def group_index(k, ev, omega, epsinv, deps_domega, grid, solver):
     _lib.check(_lib.lib().omx_group_index(h.h, float(omega), len(ks), _lib.dptr(ks), _lib.dptr(E), _lib.dptr(D),
                                           _lib.dptr(ng)), h.h)
     return float(ng[0]) if single else ng
+
+
+def ng_gvd(omega, k, ev, epsinv, deps_domega, d2eps_domega2, grid, solver, return_E=False):
+    """``ng_gvd(ω, k, ev, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid)`` (lib/ModeAnalysis/src/group_index.jl:126-162):
+    ``(n_g, GVD)`` of one mode, or arrays of both for a list of modes (``k`` and ``ev``
+    matching).  ``return_E=True`` adds the real-space ``E = ε⁻¹ fft(kx_tc(ev))`` of each mode,
+    shape ``(3, Nx, Ny)``, as ``ng_gvd_E`` (``:55-95``) returns it."""
+    h = solver.handle(grid, epsinv)
+    single = np.ndim(k) == 0
+    ks = np.atleast_1d(np.asarray(k, dtype=np.float64))
+    evs = [ev] if single else list(ev)
+    if len(evs) != len(ks):
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "one eigenvector per k is needed")
+    E = _cols(np.stack(evs, 1), 2 * grid.N)
+    De = _epsinv_c(deps_domega, grid)
+    DDe = _epsinv_c(d2eps_domega2, grid)
+    ng = np.empty(len(ks))
+    gvd = np.empty(len(ks))
+    Ef = np.empty(3 * grid.N * len(ks), dtype=np.complex128) if return_E else None
+    _lib.check(_lib.lib().omx_ng_gvd(h.h, float(omega), len(ks), _lib.dptr(ks), _lib.dptr(E), _lib.dptr(De),
+                                     _lib.dptr(DDe), _lib.dptr(ng), _lib.dptr(gvd), _lib.dptr(Ef)), h.h)
+    if return_E:
+        Es = [Ef[3 * grid.N * j:3 * grid.N * (j + 1)].reshape((3, grid.Nx, grid.Ny), order="F") for j in range(len(ks))]
+        return (float(ng[0]), float(gvd[0]), Es[0]) if single else (ng, gvd, Es)
+    return (float(ng[0]), float(gvd[0])) if single else (ng, gvd)
 
 
 def rrule_solve_k(omega, epsinv, grid, solver, nev=1, **kw):

@@ -7,11 +7,13 @@ leg of ``bench.py`` may import it.  The product path (``optimode.jl_amd``) never
 imports, calls or falls back to anything in here.
 
 Every function cites the reference ``file:line`` it restates.  How each one is pinned
-(closed forms, independent dense assembly, brute force, finite differences) is listed
-in ``oracle/PINNING.md``; a function without such a check says "parity unpinned".
+(closed forms, independent dense assembly, brute force, finite differences) is stated
+by its tests in ``tests/test_oracle.py`` and ``tests/test_oracle_analysis.py``; a function
+without such a check says "parity unpinned" in its docstring.
 """
 from .maxwell import *  # noqa: F401,F403
 from .lobpcg import lobpcg, ortho, ortho_against  # noqa: F401
 from .solve import solve_omega2, solve_k, solve_k_single  # noqa: F401
 from .adjoint import (eig_adjt, epsinv_bar, grad_k_mag_m_n, solve_k_pullback,  # noqa: F401
                       sliceinv_3x3, sliceinv_3x3_pullback, group_index)
+from .analysis import ng_gvd, gaussian_wg_eps_dispersive  # noqa: F401
