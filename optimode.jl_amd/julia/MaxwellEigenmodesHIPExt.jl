@@ -16,7 +16,7 @@ using ChainRulesCore: NoTangent, AbstractZero
 using DielectricSmoothing: Grid, N
 import MaxwellEigenmodes: solve_ω², solve_k, AbstractEigensolver, ModeSolver
 
-export HIPSolver, hip_group_index
+export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E
 
 const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
 
@@ -176,6 +176,48 @@ function hip_group_index(solver::HIPSolver, kmags::AbstractVector{<:Real}, evecs
     end
     _check(st, h)
     return ng
+end
+
+function _ng_gvd(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid, want_E::Bool)
+    h = _handle!(solver, grid, ε⁻¹)
+    km = Vector{Float64}(kmags)
+    E = reduce(hcat, [Vector{ComplexF64}(v) for v in evecs])
+    D1 = Array{Float64,4}(∂ε_∂ω)
+    D2 = Array{Float64,4}(∂²ε_∂ω²)
+    ng = Vector{Float64}(undef, length(km))
+    gvd = Vector{Float64}(undef, length(km))
+    Ef = want_E ? Array{ComplexF64,4}(undef, 3, size(grid)..., length(km)) : nothing
+    GC.@preserve Ef begin
+        st = ccall((:omx_ng_gvd, libomx), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Ptr{Cdouble}, Ptr{Cdouble},
+                    Ptr{Cdouble}, Ptr{Cdouble}, Ptr{ComplexF64}),
+                   h, ω, length(km), km, E, D1, D2, ng, gvd, Ef === nothing ? Ptr{ComplexF64}(C_NULL) : Ef)
+    end
+    _check(st, h)
+    return ng, gvd, Ef
+end
+
+"""
+    hip_ng_gvd(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid)
+
+`ng_gvd` (lib/ModeAnalysis/src/group_index.jl:126-162) for every mode on the GPU: returns
+the vectors `(ng, gvd)`, `gvd = ∂n_g/∂ω` from one linear solve per mode.
+"""
+function hip_ng_gvd(solver::HIPSolver, kmags::AbstractVector{<:Real}, evecs, ω::Real,
+                    ε⁻¹::AbstractArray{Float64,4}, ∂ε_∂ω, ∂²ε_∂ω², grid::Grid{2,Float64})
+    ng, gvd, _ = _ng_gvd(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid, false)
+    return ng, gvd
+end
+
+"""
+    hip_ng_gvd_E(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid)
+
+`ng_gvd_E` (group_index.jl:55-95): `(ng, gvd, E)` with `E[:, :, :, j]` the real-space
+field ε⁻¹ fft(kx_tc(ev_j)) of mode j, size (3, Nx, Ny).
+"""
+function hip_ng_gvd_E(solver::HIPSolver, kmags::AbstractVector{<:Real}, evecs, ω::Real,
+                      ε⁻¹::AbstractArray{Float64,4}, ∂ε_∂ω, ∂²ε_∂ω², grid::Grid{2,Float64})
+    return _ng_gvd(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid, true)
 end
 
 end # module
