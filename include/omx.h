@@ -103,6 +103,24 @@ int omx_ng_gvd(omx_solver* h, double omega, int nev, const double* kmags, const 
                const double* deps_domega, const double* d2eps_domega2, double* ng_out, double* gvd_out,
                double* E_out);
 
+/* EME per-slice mode set (lib/EigenmodeExpansion/src/modes.jl:105-122, mode_fields and
+   build_mode): E = i ε⁻¹ fft(kx_tc(ev_j)) and H = fft(tc(ev_j)) of nev <= 16 modes at
+   k = kmags[j], kept on the device in mode set `slot` (0..3) of the handle.  evecs: nev host
+   vectors (2N complex each), or NULL for the eigenvectors of the handle's last omx_solve_k.
+   normalize != 0 divides both fields of mode j by nrm_j = sqrt(inner_product(m_j, m_j;
+   conjugate)) (principal square root); norms_out: NULL or nev complex nrm_j. */
+int omx_build_modes(omx_solver* h, int slot, int nev, const double* kmags, const double* evecs, int conjugate,
+                    int normalize, double* norms_out);
+
+/* overlap_matrix(modes_a, modes_b; conjugate) (modes.jl:135-170, MEOW inner product):
+   out[i + na j] = 1/2 Σ (E_a,i,x H_b,j,y − E_a,i,y H_b,j,x) δA over the pixels, E_a
+   conjugated when conjugate != 0; na x nb complex, column-major. */
+int omx_overlap_matrix(omx_solver* h, int slot_a, int slot_b, int conjugate, double* out);
+
+/* The fields of mode set `slot`: E_out, H_out (either may be NULL) receive nev fields of
+   3N complex each, (3,Nx,Ny) column-major, as Mode.E / Mode.H hold them. */
+int omx_get_mode_fields(omx_solver* h, int slot, double* E_out, double* H_out);
+
 /* solve_ω²(ms, solver; nev, tol, maxiter) (src/solve.jl:38-63, solvers/dftk.jl:32-37):
    lowest nev eigenpairs of M(k).  H_io: in = start block (2N x nev complex) or NULL for a
    seeded random start; out = eigenvectors (written when non-NULL). */
@@ -114,7 +132,10 @@ int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, 
    ω²(k) - ω² = 0 with slope 2 HMₖH; eigenvectors unit-norm and phase-canonicalised
    (fields.jl:258-260).  kguess may be NULL (k_guess, MaxwellEigenmodes.jl:63); Hguess may
    be NULL or hold nguess start vectors.  evecs_out may be NULL: the eigenvectors then stay
-   on the device (fetch them with omx_get_evecs). */
+   on the device (fetch them with omx_get_evecs).  With option "warm_start" set and both
+   kguess and Hguess NULL, every band starts at the previous solve's k of that band with
+   the previous eigenvectors (on the device) as its block -- the EME slice chain
+   (lib/EigenmodeExpansion/src/eme.jl:58-76). */
 int omx_solve_k(omx_solver* h, double omega, int nev, double k_tol, double eig_tol, int max_eigsolves,
                 int maxiter, const double* kguess, const double* Hguess, int nguess, double* kmags_out,
                 double* evecs_out, omx_info* info);

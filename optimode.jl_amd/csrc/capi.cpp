@@ -144,6 +144,19 @@ int omx_ng_gvd(omx_solver* h, double omega, int nev, const double* kmags, const 
   });
 }
 
+int omx_build_modes(omx_solver* h, int slot, int nev, const double* kmags, const double* evecs, int conjugate,
+                    int normalize, double* norms_out) {
+  return guarded(h, [&](omx::Solver& s) { s.build_modes(slot, nev, kmags, evecs, conjugate, normalize, norms_out); });
+}
+
+int omx_overlap_matrix(omx_solver* h, int slot_a, int slot_b, int conjugate, double* out) {
+  return guarded(h, [&](omx::Solver& s) { s.overlap(slot_a, slot_b, conjugate, out); });
+}
+
+int omx_get_mode_fields(omx_solver* h, int slot, double* E_out, double* H_out) {
+  return guarded(h, [&](omx::Solver& s) { s.get_mode_fields(slot, E_out, H_out); });
+}
+
 int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, double* H_io, double* omega2_out,
                      omx_info* info) {
   return guarded(h, [&](omx::Solver& s) { s.solve_omega2(k, nev, tol, maxiter, H_io, omega2_out, info); });

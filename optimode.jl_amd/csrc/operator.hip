@@ -19,12 +19,15 @@ template <int OP> struct Dirs;
 template <> struct Dirs<OP_M> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
 template <> struct Dirs<OP_MKH> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
 template <> struct Dirs<OP_P> { static constexpr int x1 = +1, y1 = +1, y2 = -1, x2 = -1; };
-// pass-1 curl of the field helper FIELD_DKK (second k-derivative of kx_tc); forward only
+// pass-1 maps of the field helpers FIELD_DKK (second k-derivative of kx_tc) and FIELD_H
+// (tc: transverse to Cartesian); forward only
 constexpr int CURL_DKK = 3;
+constexpr int MAP_TC = 4;
 template <> struct Dirs<CURL_DKK> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
+template <> struct Dirs<MAP_TC> { static constexpr int x1 = -1, y1 = -1, y2 = +1, x2 = +1; };
 
 // ---------------------------------------------------------------------------------------
-// pass 1: k-space curl (kx_tc / kxinv_tc / zx_tc / d2 kx_tc / dk2) + FFT along x
+// pass 1: k-space map (kx_tc / kxinv_tc / zx_tc / d2 kx_tc / dk2 / tc) + FFT along x
 // ---------------------------------------------------------------------------------------
 template <typename T, int OP, int VPT>
 __global__ __launch_bounds__(OP_BLOCK) void k_pass1(OpPlan<T> plan, BView<T> X, cplx<T>* D, VecSel sel,
@@ -61,6 +64,10 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass1(OpPlan<T> plan, BView<T> X, 
       d0 = T(-1) * (h1 * b.m1 + h2 * b.n1);
       d1 = h1 * b.m0 + h2 * b.n0;
       d2 = cplx<T>{T(0), T(0)};
+    } else if (OP == MAP_TC) {
+      d0 = h1 * b.m0 + h2 * b.n0;
+      d1 = h1 * b.m1 + h2 * b.n1;
+      d2 = h1 * b.m2 + h2 * b.n2;
     } else {
       // d^2/dk^2 of kx_tc at fixed (h1, h2): -h1 n rho^2 / |k+G|^3, rho = |G|
       const T gx = T(double(fft_index(ix, nx)) * plan.invdx), gy = T(double(fft_index(iy, ny)) * plan.invdy);
@@ -631,10 +638,12 @@ void launch_fields_vpt(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* 
   dim3 g2(plan.nx / plan.cols_per_block, nvec);
   const size_t lds13 = cs * (size_t(plan.nx) + size_t(plan.rows_per_block) * 3 * plan.nx);
   const size_t lds2 = cs * (size_t(plan.ny) + size_t(plan.cols_per_block) * 3 * plan.ny);
-  if (fop == FIELD_DK || fop == FIELD_DKK) {
+  if (fop == FIELD_DK || fop == FIELD_DKK || fop == FIELD_H) {
     if (fop == FIELD_DK) {
       hipLaunchKernelGGL((k_pass1<T, OP_MKH, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
       scale = -scale;  // d kx_tc / dk = -zx_tc
+    } else if (fop == FIELD_H) {
+      hipLaunchKernelGGL((k_pass1<T, MAP_TC, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
     } else {
       hipLaunchKernelGGL((k_pass1<T, CURL_DKK, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, X, D, sel, kinst);
     }

@@ -605,6 +605,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.seed = uint64_t(v);
   else if (key == "plain")
     opt.plain = v != 0 ? 1 : 0;
+  else if (key == "warm_start")
+    opt.warm_start = v != 0 ? 1 : 0;
   else if (key == "reg_ops") {
     opt.reg_ops = v != 0 ? 1 : 0;
     select_operator(*this, p32);
@@ -944,6 +946,28 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     return it;
   };
 
+  // previous-solve warm start (the EME slice chain, eme.jl:58-76): every band starts at the
+  // previous solve's k of that band with the previous eigenvectors as its block (kept on
+  // the device), and the shared first eigensolve is skipped
+  const bool warm = opt.warm_start && !opt.plain && !kguess && !Hguess && last_nev > 0 &&
+                    int(last_kmags.size()) >= last_nev;
+  auto warm_block = [&](auto& b) {
+    using TT = typename std::remove_reference<decltype(b)>::type::value_type;
+    const int nw = std::min(last_nev, M);
+    if (nw < M) launch_randn<TT>(b.p(b.X[b.cur]), ld, b.istride, ninst, M, K, opt.seed, stream);
+    for (int i = 0; i < ninst; ++i)
+      launch_convert<double, TT>(last_evecs.as<c64>(), ld, 0, b.p(b.X[b.cur]) + i * b.istride, b.ld, 0, 1, nw, K,
+                                 stream);
+    std::vector<double> kh(ninst);
+    for (int i = 0; i < ninst; ++i) {
+      kh[i] = last_kmags[std::min(i, nw - 1)];
+      hns[i].k = kh[i];
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice,
+                                 stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  };
+
   if (opt.plain) {
     // the reference's algorithm (solve.jl:89-113 with the DFTK backend): each band runs
     // Newton from k_guess with its own random nev-column start, every eigensolve to eig_tol
@@ -968,10 +992,14 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     }
   } else if (!opt.fp64_only) {
     Batch<float>& b32 = workspace<float>(*this, ninst, M, maxiter);
-    launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
-    if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
-    stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
+    if (warm) {
+      warm_block(b32);
+    } else {
+      launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
+      if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
+      stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
+    }
     stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
                                newton_total, false);
     launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
@@ -984,10 +1012,14 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       s.k_tol = k_tol;
     }
   } else {
-    launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
-    if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, 1);
     for (auto& s : hns) s.k_tol = k_tol;
-    stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
+    if (warm) {
+      warm_block(b64);
+    } else {
+      launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
+      if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, 1);
+      stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
+    }
   }
   if (!opt.plain) {
     const int left = std::max(1, max_eigsolves - newton_total);
@@ -1005,6 +1037,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     OMX_HIP_CHECK(hipMemcpyAsync(last_evecs.as<c64>() + size_t(i) * ld, b64.p(b64.X[b64.cur]) + i * b64.istride + i * ld,
                                  sizeof(c64) * K, hipMemcpyDeviceToDevice, stream));
   last_nev = nev;
+  last_kmags.assign(kmags, kmags + nev);
   canonicalize_phase(b64.kd.as<double>(), last_evecs.as<c64>(), ld, ld, nev, 1);
   OMX_HIP_CHECK(hipEventRecord(e1, stream));
   if (evecs) get_evecs(evecs);

@@ -145,6 +145,16 @@ struct Options {
   int reg_ops = 1;              // register-FFT operator where the grid admits it
   int plain = 0;                // the reference's algorithm: FP64 only, nev-column block per band,
                                 // random start at k_guess per band, every eigensolve to eig_tol
+  int warm_start = 0;           // solve_k without kguess/Hguess starts every band from the previous
+                                // solve's k and eigenvectors (kept on the device)
+};
+
+// Device-resident EME mode set: E and H of nev modes, planar per mode and component
+// ((v * 3 + c) * N + p, p = ix + Nx iy), complex FP64.
+constexpr int MODE_SLOTS = 4;
+struct ModeSlot {
+  int nev = 0;
+  DevMem E, H;
 };
 
 class Solver {
@@ -165,6 +175,10 @@ class Solver {
                const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar);
   void ng_gvd(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
               const double* d2eps_domega2, double* ng_out, double* gvd_out, double* E_out);
+  void build_modes(int slot, int nev, const double* kmags, const double* evecs, int conjugate, int normalize,
+                   double* norms_out);
+  void overlap(int slot_a, int slot_b, int conjugate, double* out);
+  void get_mode_fields(int slot, double* E_out, double* H_out);
   void set_option(const std::string& key, double value);
 
   std::mutex mu;
@@ -182,7 +196,9 @@ class Solver {
   std::vector<double> epsinv_host;  // (3,3,Nx,Ny) column-major copy
   DevMem last_evecs;                // FP64, nev columns of ld complex
   int last_nev = 0;
+  std::vector<double> last_kmags;   // k of the eigenvectors in last_evecs
   int last_adjoint_iters = 0;
+  ModeSlot modes[MODE_SLOTS];
   DevMem scratch_a, scratch_b, scratch_c;
   DevMem field_scratch, argmax_v, argmax_i;  // canonicalize_phase
   Profiler prof;

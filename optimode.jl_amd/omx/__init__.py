@@ -103,11 +103,13 @@ class HIPSolver:
     (power-of-two grids 64 to 512).  ``plain=True`` runs the reference's algorithm on the same
     kernels, as the baseline the optimised path is measured against: FP64 throughout, an
     nev-column block for every band, a random start at ``k_guess`` per band and every
-    eigensolve to ``eig_tol`` (``solve.jl:89-113``).
+    eigensolve to ``eig_tol`` (``solve.jl:89-113``).  ``warm_start=True`` makes ``solve_k``
+    calls without ``kguess``/``Hguess`` start every band from the previous solve's k and
+    eigenvectors, which stay on the device (the EME slice chain, ``eme.jl:58-76``).
     """
 
     def __init__(self, device=0, precision="mixed", fp32_tol=None, fp32_switch=None, maxiter_fp64=None, seed=None,
-                 profile=False, reg_ops=None, plain=False):
+                 profile=False, reg_ops=None, plain=False, warm_start=False):
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         self.device = int(device)
@@ -124,6 +126,7 @@ class HIPSolver:
         if reg_ops is not None:
             self.options["reg_ops"] = 1.0 if reg_ops else 0.0
         self.options["plain"] = 1.0 if plain else 0.0
+        self.options["warm_start"] = 1.0 if warm_start else 0.0
         self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
         self.profile = bool(profile)
         self._h = None

@@ -60,6 +60,9 @@ SIGNATURES = [
     ("omx_hmh", ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int, _dp]),
     ("omx_group_index", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]),
     ("omx_ng_gvd", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("omx_build_modes", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]),
+    ("omx_overlap_matrix", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    ("omx_get_mode_fields", ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     ("omx_solve_omega2", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
                                         ctypes.POINTER(OmxInfo)]),
     ("omx_solve_k", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,

@@ -88,7 +88,9 @@ def test_ng_gvd_matches_oracle(profile, shape):
     solver, km, evs, ei, de, dde = _gpu_mode(profile, W0, g, og, 2)
     ng, gvd, Es = omx.ng_gvd(W0, km, evs, ei, de, dde, g, solver, return_E=True)
     for j in range(2):
-        rng_, rgvd, rE = O.ng_gvd(W0, km[j], evs[j], ei, de, dde, og, return_E=True)
+        # GMRES on the oracle's undeflated system stalls near 1e-13 with an eigenvector from
+        # another solver (residual ~1e-11); 1e-10 moves GVD by ~1e-11 relative
+        rng_, rgvd, rE = O.ng_gvd(W0, km[j], evs[j], ei, de, dde, og, adj_rtol=1e-10, return_E=True)
         print(f"  mode {j}: ng {ng[j]:.15f} ({rng_:.15f}) gvd {gvd[j]:.12e} ({rgvd:.12e})", flush=True)
         assert ng[j] == pytest.approx(rng_, rel=1e-12)
         assert gvd[j] == pytest.approx(rgvd, rel=1e-9)
