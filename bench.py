@@ -56,6 +56,82 @@ def cpu_baseline(seconds_cap=60.0):
                       f"k_tol 1e-10: {dt:.1f} s for 1 mode (k={km[0]:.8f})"}
 
 
+EME_SLICES = 200      # cross-sections of the whole coupler
+EME_LENGTH = 200.0    # um
+
+
+def eme_cell(grid, i, backend="gpu"):
+    """Cross-section i of the adiabatic Si3N4 coupler: gap 0.3 -> 1.5 um and widths
+    0.8 <-> 1.2 um linear along the device, 0.8 um thick, SiO2 cladding."""
+    from omx import geometry as G
+    s = i / (EME_SLICES - 1)
+    return G.coupler_slice(grid, 0.3 + 1.2 * s, 0.8 + 0.4 * s, 1.2 - 0.4 * s, h=0.8, backend=backend)
+
+
+def eme_workload(a):
+    """EME slice chain at 512 x 256 with 16 modes per slice: per step (one slice) GPU
+    smoothing, solve_k warm-started from the previous slice, the power-normalised mode set
+    and the overlaps with the previous slice on the GPU, and the host interface/propagation
+    star products.  Prints one JSON line (slices/s and modes/s)."""
+    import numpy as np
+    import omx
+    from omx import eme as X
+
+    grid = omx.Grid(8.0, 4.0, 512, 256)
+    nev = 16
+    L = EME_LENGTH / EME_SLICES
+    kw = dict(k_tol=1e-10, eig_tol=1e-8)
+    nw, ns = a.warmup, a.steps
+    cells = [(lambda i=i: eme_cell(grid, i)) for i in range(nw + ns)]
+    solver = omx.HIPSolver(precision=a.precision)
+    # warm-up slices (also the chain's start), then the timed slices continue the same chain
+    X.eme(cells[:nw], [L] * nw, OMEGA, grid, solver, nev=nev, warmstart=True, **kw)
+    t0 = time.perf_counter()
+    res = X.eme(cells[nw:], [L] * ns, OMEGA, grid, solver, nev=nev, warmstart=True, continue_chain=True, **kw)
+    elapsed = time.perf_counter() - t0
+    conv = sum(int(st["converged"]) for st in res.stats)
+    # the last timed slice against a cold FP64 solve at a tighter tolerance
+    ref = omx.HIPSolver(precision="fp64")
+    ei_last = eme_cell(grid, nw + ns - 1)
+    kr, _ = omx.solve_k(OMEGA, ei_last, grid, ref, nev=nev, k_tol=1e-12, eig_tol=1e-10, fetch_evecs=False)
+    rel = float(np.max(np.abs(res.kmags[-1] ** 2 - kr ** 2) / kr ** 2))
+    # a cold chain over the first few timed slices for the warm-start gain
+    cold_n = min(ns, 4)
+    sc = omx.HIPSolver(precision=a.precision)
+    tc = time.perf_counter()
+    rc = X.eme(cells[nw:nw + cold_n], [L] * cold_n, OMEGA, grid, sc, nev=nev, warmstart=False, **kw)
+    cold_ms = 1e3 * (time.perf_counter() - tc) / cold_n
+    out = {
+        "metric": "EME slices/s at 512x256, 16 modes per slice (previous-slice warm start)",
+        "value": round(ns / elapsed, 3),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": ns,
+        "warmup": nw,
+        "ms_per_step": round(1e3 * elapsed / ns, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if a.precision == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "EME chain: slices of an adiabatic Si3N4 coupler (gap 0.3->1.5 um, widths 0.8<->1.2 um, "
+                        "0.8 um thick, SiO2), 8x4 um, 512x256, omega=1/1.55, 16 modes/slice, k_tol 1e-10, "
+                        "eig_tol 1e-8; per slice: GPU smoothing, warm solve_k, mode set, overlaps, star products",
+            "modes_per_s": round(nev * conv / elapsed, 2),
+            "slices_converged": conv,
+            "newton_per_slice": float(np.mean([st["n_newton"] for st in res.stats])),
+            "lobpcg_iters_fp32_per_slice": float(np.mean([st["lobpcg_iters_fp32"] for st in res.stats])),
+            "lobpcg_iters_fp64_per_slice": float(np.mean([st["lobpcg_iters_fp64"] for st in res.stats])),
+            "last_slice_max_rel_err_k2_vs_cold_fp64": rel,
+            "cold_chain_ms_per_slice": round(cold_ms, 2),
+            "cold_newton_per_slice": float(np.mean([st["n_newton"] for st in rc.stats])),
+            "power_coupling_1_1": X.power_coupling(res),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,11 +140,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-algorithm comparison solve")
+    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme"],
+                    help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every solve here fits one MI355X and "
               "multi-GPU sharding is by independent instances, which this build does not launch.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "eme":
+        eme_workload(a)
+        return
 
     import numpy as np
     import omx

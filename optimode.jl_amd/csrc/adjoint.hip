@@ -13,168 +13,13 @@
 #include <cmath>
 #include <cstring>
 
-#include "solver.hpp"
+#include "krylov.hpp"
 
 namespace omx {
 
 namespace {
 
 constexpr int AB = 256;
-
-// out[(i*nb + b)] = Re sum_rows conj(a_i) b_i over block b's rows (vector i at base + i*ld)
-__global__ __launch_bounds__(AB) void k_vdot(const c64* A, const c64* B, long ld, long K, const int* act,
-                                             double* part) {
-  const int i = blockIdx.y;
-  __shared__ double red[AB];
-  double s = 0;
-  if (act[i]) {
-    const c64* a = A + i * ld;
-    const c64* b = B + i * ld;
-    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB)
-      s += a[r].x * b[r].x + a[r].y * b[r].y;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int t = AB / 2; t > 0; t >>= 1) {
-    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
-}
-
-// complex dot <a_i, b_i> = sum conj(a) b, partials (re, im)
-__global__ __launch_bounds__(AB) void k_cdot(const c64* A, const c64* B, long ld, long K, const int* act,
-                                             c64* part) {
-  const int i = blockIdx.y;
-  __shared__ double rr[AB], ri[AB];
-  double sr = 0, si = 0;
-  if (act[i]) {
-    const c64* a = A + i * ld;
-    const c64* b = B + i * ld;
-    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-      sr += a[r].x * b[r].x + a[r].y * b[r].y;
-      si += a[r].x * b[r].y - a[r].y * b[r].x;
-    }
-  }
-  rr[threadIdx.x] = sr;
-  ri[threadIdx.x] = si;
-  __syncthreads();
-  for (int t = AB / 2; t > 0; t >>= 1) {
-    if (threadIdx.x < t) {
-      rr[threadIdx.x] += rr[threadIdx.x + t];
-      ri[threadIdx.x] += ri[threadIdx.x + t];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = c64{rr[0], ri[0]};
-}
-
-// out_i = sum_j c[i*4 + j] * in_j_i for up to 4 inputs (complex coefficients); out may
-// alias an input (each element is read before it is written by the same thread)
-struct Lin {
-  const c64* in[4];
-  int nin;
-  c64* out;
-  const c64* coef;  // per instance, 4 complex
-  long ld;
-};
-
-__global__ __launch_bounds__(AB) void k_lincomb(Lin L, long K, const int* act) {
-  const int i = blockIdx.y;
-  if (!act[i]) return;
-  const c64* c = L.coef + 4 * i;
-  const c64 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-  for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-    const long o = i * L.ld + r;
-    c64 s{0, 0};
-    const c64 cs[4] = {c0, c1, c2, c3};
-    for (int j = 0; j < L.nin; ++j) {
-      const c64 v = L.in[j][o];
-      s.x += cs[j].x * v.x - cs[j].y * v.y;
-      s.y += cs[j].x * v.y + cs[j].y * v.x;
-    }
-    L.out[o] = s;
-  }
-}
-
-struct MinresState {
-  double beta1, beta, oldb, dbar, epsln, phibar, cs, sn, alfa, phi, delta, oldeps, denom, rtol, shift;
-  int itn, done, active, pad;
-};
-
-double sum_part(const double* p, int nb) {
-  double s = 0;
-  for (int b = 0; b < nb; ++b) s += p[b];
-  return s;
-}
-
-// Scalar recurrences of MINRES (Paige & Saunders 1975) per instance; `stage` selects the
-// step of the iteration they follow.  Coefficients for the next lincomb go to coef.
-__global__ void k_minres_scalars(MinresState* ms, const double* part, int nb, c64* coef, int stage, int ninst,
-                                 const int* act) {
-  for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
-    MinresState& m = ms[i];
-    c64* c = coef + 4 * i;
-    for (int j = 0; j < 4; ++j) c[j] = c64{0, 0};
-    if (!act[i]) continue;
-    double d = 0;
-    for (int b = 0; b < nb; ++b) d += part[long(i) * nb + b];
-    if (stage == 0) {
-      // init: beta1 = sqrt(<r1, M r1>)
-      m.beta1 = sqrt(fmax(d, 0.0));
-      m.beta = m.beta1;
-      m.oldb = 0;
-      m.dbar = 0;
-      m.epsln = 0;
-      m.phibar = m.beta1;
-      m.cs = -1;
-      m.sn = 0;
-      m.itn = 0;
-      m.done = m.beta1 == 0 ? 1 : 0;
-      // next: v = y / beta
-      c[0] = c64{m.done ? 0.0 : 1.0 / m.beta, 0};
-    } else if (stage == 1) {
-      // after y = A v: y <- y - shift v - (beta/oldb) r1 ; inputs (y, v, r1)
-      c[0] = c64{1, 0};
-      c[1] = c64{-m.shift, 0};
-      c[2] = c64{m.itn >= 1 ? -m.beta / m.oldb : 0.0, 0};
-    } else if (stage == 2) {
-      // alfa = <v, y>; y <- y - (alfa/beta) r2 ; inputs (y, r2)
-      m.alfa = d;
-      c[0] = c64{1, 0};
-      c[1] = c64{-m.alfa / m.beta, 0};
-    } else if (stage == 3) {
-      // beta = sqrt(<r2, M r2>); rotations; w and x updates
-      m.itn += 1;
-      m.oldb = m.beta;
-      m.beta = sqrt(fmax(d, 0.0));
-      m.oldeps = m.epsln;
-      m.delta = m.cs * m.dbar + m.sn * m.alfa;
-      const double gbar = m.sn * m.dbar - m.cs * m.alfa;
-      m.epsln = m.sn * m.beta;
-      m.dbar = -m.cs * m.beta;
-      double gamma = sqrt(gbar * gbar + m.beta * m.beta);
-      gamma = fmax(gamma, 2.220446049250313e-16);
-      m.cs = gbar / gamma;
-      m.sn = m.beta / gamma;
-      m.phi = m.cs * m.phibar;
-      m.phibar = m.sn * m.phibar;
-      m.denom = 1.0 / gamma;
-      // w_new = (v - oldeps w1 - delta w2) denom ; inputs (v, w1, w2)
-      c[0] = c64{m.denom, 0};
-      c[1] = c64{-m.oldeps * m.denom, 0};
-      c[2] = c64{-m.delta * m.denom, 0};
-      if (m.phibar <= m.rtol * m.beta1 || m.beta == 0) m.done = 1;
-    } else if (stage == 4) {
-      // x += phi w ; inputs (x, w)
-      c[0] = c64{1, 0};
-      c[1] = c64{m.phi, 0};
-    } else if (stage == 5) {
-      // next v = y / beta
-      c[0] = c64{m.beta > 0 ? 1.0 / m.beta : 0.0, 0};
-    }
-  }
-}
 
 // epsbar (6 per pixel: xx, yy, zz, xy, xz, yz) += sum over instances of epsinv_bar(d, c_i * ld)
 // with the 2D mirrored convention (grads/solve.jl:490-515).  self_mode: ld = d.
@@ -321,151 +166,6 @@ __global__ __launch_bounds__(AB) void k_ct_combine(const c64* F1, const c64* F2,
   }
 }
 
-// Batched vector helpers over nb instances of K complex entries at stride ld, against
-// the unit vectors EV (one per instance).
-struct VecOps {
-  hipStream_t s;
-  int nb;
-  long ld, K;
-  int nblk;
-  const c64* EV;
-  DevMem part, coef;
-  VecOps(hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_)
-      : s(s_), nb(nb_), ld(ld_), K(K_), nblk(int(std::min<long>(256, (K_ + AB - 1) / AB))), EV(EV_),
-        part(sizeof(c64) * size_t(nb_) * 256), coef(sizeof(c64) * 4 * nb_) {}
-  void vdot(const c64* A, const c64* B, const int* act) {
-    hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
-  }
-  // out = sum_j coef[i][j] in_j (coefficients already on the device)
-  void lincomb(std::initializer_list<const c64*> ins, c64* outp, const int* act) {
-    Lin L{};
-    int j = 0;
-    for (auto p : ins) L.in[j++] = p;
-    L.nin = j;
-    L.out = outp;
-    L.coef = coef.as<c64>();
-    L.ld = ld;
-    hipLaunchKernelGGL(k_lincomb, dim3(nblk, nb), dim3(AB), 0, s, L, K, act);
-  }
-  // x <- x - ev <ev, x> for active instances
-  void project(c64* x, const int* act) {
-    hipLaunchKernelGGL(k_cdot, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.as<c64>());
-    std::vector<c64> hp(size_t(nb) * nblk);
-    OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), part.as<void>(), sizeof(c64) * hp.size(), hipMemcpyDeviceToHost, s));
-    OMX_HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<c64> hc(4 * size_t(nb), c64{0, 0});
-    for (int i = 0; i < nb; ++i) {
-      c64 d{0, 0};
-      for (int b = 0; b < nblk; ++b) {
-        d.x += hp[size_t(i) * nblk + b].x;
-        d.y += hp[size_t(i) * nblk + b].y;
-      }
-      hc[4 * i] = c64{1, 0};
-      hc[4 * i + 1] = c64{-d.x, -d.y};
-    }
-    OMX_HIP_CHECK(hipMemcpyAsync(coef.as<void>(), hc.data(), sizeof(c64) * hc.size(), hipMemcpyHostToDevice, s));
-    lincomb({x, const_cast<c64*>(EV)}, x, act);
-  }
-};
-
-// Solves (M(k_i) - w^2) x_i = r_i on the complement of ev_i for the instances with want[i]
-// (MINRES, Paige & Saunders 1975, preconditioned by P projected onto the complement; M - w^2
-// is Hermitian, singular along ev and indefinite above band 1).  r is projected on entry and
-// overwritten; x receives the solution, orthogonalised against ev twice.  Returns the
-// iteration count; throws OMX_ERR_NOT_CONVERGED after 1000 iterations.
-int minres_complement(Solver& S, VecOps& V, double omega, const double* dk, c64* r, c64* xout,
-                      const std::vector<int>& want, double rtol) {
-  hipStream_t s = S.stream;
-  const int nb = V.nb;
-  const long ld = S.ld, n3 = 3 * S.N;
-  DevMem actd(sizeof(int) * nb), lived(sizeof(int) * nb);
-  OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), want.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
-  const int* act = actd.as<int>();
-  DevMem X_(sizeof(c64) * nb * ld), R2(sizeof(c64) * nb * ld), Y(sizeof(c64) * nb * ld), Vb(sizeof(c64) * nb * ld),
-      W(sizeof(c64) * nb * ld), W1(sizeof(c64) * nb * ld), W2(sizeof(c64) * nb * ld), AV(sizeof(c64) * nb * ld),
-      Dm(sizeof(c64) * nb * n3), msd(sizeof(MinresState) * nb);
-  c64 *x = X_.as<c64>(), *r1 = r, *r2 = R2.as<c64>(), *y = Y.as<c64>(), *v = Vb.as<c64>(), *w = W.as<c64>(),
-      *w1 = W1.as<c64>(), *w2 = W2.as<c64>();
-  OMX_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nb * ld, s));
-  OMX_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(c64) * nb * ld, s));
-  OMX_HIP_CHECK(hipMemsetAsync(w2, 0, sizeof(c64) * nb * ld, s));
-  OMX_HIP_CHECK(hipMemsetAsync(w1, 0, sizeof(c64) * nb * ld, s));
-  std::vector<MinresState> hm(nb);
-  for (int i = 0; i < nb; ++i) {
-    std::memset(&hm[i], 0, sizeof(MinresState));
-    hm[i].rtol = rtol;
-    hm[i].shift = omega * omega;
-    hm[i].active = want[i];
-  }
-  OMX_HIP_CHECK(hipMemcpyAsync(msd.as<void>(), hm.data(), sizeof(MinresState) * nb, hipMemcpyHostToDevice, s));
-  MinresState* ms = msd.as<MinresState>();
-  const int nblk = V.nblk;
-  double* part = V.part.as<double>();
-  c64* coef = V.coef.as<c64>();
-  V.project(r1, act);
-  BView<double> AVv{AV.as<c64>(), ld, ld}, Yv{y, ld, ld}, R1v{r1, ld, ld};
-  const VecSel sel{nb, 1, nullptr, nullptr};
-  // y = P r1 (projected)
-  launch_operator<double>(OP_P, S.p64.plan, R1v, Yv, Dm.as<c64>(), sel, dk, nullptr, 0, s);
-  V.project(y, act);
-  V.vdot(r1, y, act);
-  hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 0, nb, act);
-  OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
-  std::vector<int> live(want);
-  const int maxit = 1000;
-  int it = 0;
-  for (; it < maxit; ++it) {
-    const int* la = lived.as<int>();
-    if (it % 4 == 0) {
-      OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
-      OMX_HIP_CHECK(hipStreamSynchronize(s));
-      bool anyl = false;
-      for (int i = 0; i < nb; ++i) {
-        live[i] = want[i] && !hm[i].done;
-        anyl = anyl || live[i];
-      }
-      if (!anyl) break;
-      OMX_HIP_CHECK(hipMemcpyAsync(lived.as<void>(), live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
-    }
-    // v = y / beta (coefficients left by the previous scalar step)
-    V.lincomb({y}, v, la);
-    BView<double> Vv{v, ld, ld};
-    launch_operator<double>(OP_M, S.p64.plan, Vv, AVv, Dm.as<c64>(), sel, dk, nullptr, 0, s);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 1, nb, la);
-    V.lincomb({AV.as<c64>(), v, r1}, y, la);
-    V.vdot(v, y, la);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 2, nb, la);
-    V.lincomb({y, r2}, y, la);
-    std::swap(r1, r2);  // r1 <- r2
-    std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
-    BView<double> R2b{r2, ld, ld}, Yb{y, ld, ld};
-    launch_operator<double>(OP_P, S.p64.plan, R2b, Yb, Dm.as<c64>(), sel, dk, nullptr, 0, s);
-    V.project(y, la);
-    V.vdot(r2, y, la);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 3, nb, la);
-    // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
-    // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
-    V.lincomb({v, w2, w}, w1, la);
-    {
-      c64* wn = w1;
-      w1 = w2;
-      w2 = w;
-      w = wn;
-    }
-    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 4, nb, la);
-    V.lincomb({x, w}, x, la);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(64), 0, s, ms, part, nblk, coef, 5, nb, la);
-  }
-  OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), msd.as<void>(), sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
-  OMX_HIP_CHECK(hipStreamSynchronize(s));
-  for (int i = 0; i < nb; ++i)
-    if (want[i] && !hm[i].done) throw Error(OMX_ERR_NOT_CONVERGED, "MINRES did not converge within 1000 iterations");
-  OMX_HIP_CHECK(hipMemcpyAsync(xout, x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
-  V.project(xout, act);
-  V.project(xout, act);
-  return it;
-}
-
 }  // namespace
 
 void Solver::adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
@@ -517,7 +217,8 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
         OMX_HIP_CHECK(hipMemcpyAsync(EB.as<c64>() + i * ld, evbar + 2 * size_t(i) * K, sizeof(c64) * K,
                                      hipMemcpyHostToDevice, s));
     // lambda: (M - w^2) lambda = evbar - ev <ev, evbar> on the complement of ev
-    last_adjoint_iters = minres_complement(*this, V, omega, dk.as<double>(), EB.as<c64>(), LA.as<c64>(), hasv, 1e-10);
+    last_adjoint_iters = minres_complement(*this, V, std::vector<double>(nb, omega * omega), dk.as<double>(),
+                                           EB.as<c64>(), LA.as<c64>(), hasv, 1e-10);
     // d = fft(kx_tc(ev))/N, ld = fft(kx_tc(lambda)); epsbar += epsinv_bar(d, ld)
     BView<double> LAv{LA.as<c64>(), ld, ld};
     launch_fields<double>(FIELD_D, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);
@@ -672,7 +373,8 @@ void Solver::ng_gvd(double omega, int nev, const double* kmags, const double* ev
   // ev' on the complement of ev
   VecOps V(s, nb, ld, K, EV.as<c64>());
   const std::vector<int> all(nb, 1);
-  last_adjoint_iters = minres_complement(*this, V, omega, dk.as<double>(), R.as<c64>(), Hp.as<c64>(), all, 1e-12);
+  last_adjoint_iters = minres_complement(*this, V, std::vector<double>(nb, omega * omega), dk.as<double>(),
+                                         R.as<c64>(), Hp.as<c64>(), all, 1e-12);
   // fields of ev': D' (F1) and D'_k (F2)
   launch_fields<double>(FIELD_D, p64.plan, Hpv, F1.as<c64>(), sel, dk.as<double>(), 1.0, s);
   launch_fields<double>(FIELD_DK, p64.plan, Hpv, F2.as<c64>(), sel, dk.as<double>(), 1.0, s);

@@ -1,0 +1,352 @@
+// Batched FP64 vector helpers and preconditioned MINRES (see krylov.hpp).
+#include "krylov.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace omx {
+
+namespace {
+
+constexpr int AB = 256;
+
+// out[(i*nb + b)] = Re sum_rows conj(a_i) b_i over block b's rows (vector i at base + i*ld)
+__global__ __launch_bounds__(AB) void k_vdot(const c64* A, const c64* B, long ld, long K, const int* act,
+                                             double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double s = 0;
+  if (act[i]) {
+    const c64* a = A + i * ld;
+    const c64* b = B + i * ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB)
+      s += a[r].x * b[r].x + a[r].y * b[r].y;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// complex dot <a_i, b_i> = sum conj(a) b, partials (re, im)
+__global__ __launch_bounds__(AB) void k_cdot(const c64* A, const c64* B, long ld, long K, const int* act,
+                                             c64* part) {
+  const int i = blockIdx.y;
+  __shared__ double rr[AB], ri[AB];
+  double sr = 0, si = 0;
+  if (act[i]) {
+    const c64* a = A + i * ld;
+    const c64* b = B + i * ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      sr += a[r].x * b[r].x + a[r].y * b[r].y;
+      si += a[r].x * b[r].y - a[r].y * b[r].x;
+    }
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) {
+      rr[threadIdx.x] += rr[threadIdx.x + t];
+      ri[threadIdx.x] += ri[threadIdx.x + t];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = c64{rr[0], ri[0]};
+}
+
+// out_i = sum_j c[i*4 + j] * in_j_i for up to 4 inputs (complex coefficients); out may
+// alias an input (each element is read before it is written by the same thread)
+struct Lin {
+  const c64* in[4];
+  int nin;
+  c64* out;
+  const c64* coef;  // per instance, 4 complex
+  long ld;
+};
+
+template <int NIN>
+__global__ __launch_bounds__(AB) void k_lincomb(Lin L, long K, const int* act) {
+  const int i = blockIdx.y;
+  if (!act[i]) return;
+  c64 cs[NIN];
+  const c64* in[NIN];
+#pragma unroll
+  for (int j = 0; j < NIN; ++j) {
+    cs[j] = L.coef[4 * i + j];
+    in[j] = L.in[j] + i * L.ld;
+  }
+  c64* out = L.out + i * L.ld;
+  for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+    c64 v[NIN];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) v[j] = in[j][r];
+    double sx = 0, sy = 0;
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) {
+      sx += cs[j].x * v[j].x - cs[j].y * v[j].y;
+      sy += cs[j].x * v[j].y + cs[j].y * v[j].x;
+    }
+    out[r] = c64{sx, sy};
+  }
+}
+
+struct MinresState {
+  double beta1, beta, oldb, dbar, epsln, phibar, cs, sn, alfa, phi, delta, oldeps, denom, rtol, shift;
+  int itn, done, active, pad;
+};
+
+
+// Scalar recurrences of MINRES (Paige & Saunders 1975) per instance; `stage` selects the
+// step of the iteration they follow.  Coefficients for the next lincomb go to coef.
+// one block per instance: the block sums the instance's partials (tree), then its first
+// thread runs the recurrence
+__global__ __launch_bounds__(AB) void k_minres_scalars(MinresState* ms, const double* part, int nb, c64* coef,
+                                                       int stage, int ninst, const int* act) {
+  const int i = blockIdx.x;
+  __shared__ double red[AB];
+  double acc = 0;
+  for (int b = threadIdx.x; b < nb; b += AB) acc += part[long(i) * nb + b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    MinresState& m = ms[i];
+    c64* c = coef + 4 * i;
+    for (int j = 0; j < 4; ++j) c[j] = c64{0, 0};
+    if (!act[i]) return;
+    const double d = red[0];
+    if (stage == 0) {
+      // init: beta1 = sqrt(<r1, M r1>)
+      m.beta1 = sqrt(fmax(d, 0.0));
+      m.beta = m.beta1;
+      m.oldb = 0;
+      m.dbar = 0;
+      m.epsln = 0;
+      m.phibar = m.beta1;
+      m.cs = -1;
+      m.sn = 0;
+      m.itn = 0;
+      m.done = m.beta1 == 0 ? 1 : 0;
+      // next: v = y / beta
+      c[0] = c64{m.done ? 0.0 : 1.0 / m.beta, 0};
+    } else if (stage == 1) {
+      // after y = A v: y <- y - shift v - (beta/oldb) r1 ; inputs (y, v, r1)
+      c[0] = c64{1, 0};
+      c[1] = c64{-m.shift, 0};
+      c[2] = c64{m.itn >= 1 ? -m.beta / m.oldb : 0.0, 0};
+    } else if (stage == 2) {
+      // alfa = <v, y>; y <- y - (alfa/beta) r2 ; inputs (y, r2)
+      m.alfa = d;
+      c[0] = c64{1, 0};
+      c[1] = c64{-m.alfa / m.beta, 0};
+    } else if (stage == 3) {
+      // beta = sqrt(<r2, M r2>); rotations; w and x updates
+      m.itn += 1;
+      m.oldb = m.beta;
+      m.beta = sqrt(fmax(d, 0.0));
+      m.oldeps = m.epsln;
+      m.delta = m.cs * m.dbar + m.sn * m.alfa;
+      const double gbar = m.sn * m.dbar - m.cs * m.alfa;
+      m.epsln = m.sn * m.beta;
+      m.dbar = -m.cs * m.beta;
+      double gamma = sqrt(gbar * gbar + m.beta * m.beta);
+      gamma = fmax(gamma, 2.220446049250313e-16);
+      m.cs = gbar / gamma;
+      m.sn = m.beta / gamma;
+      m.phi = m.cs * m.phibar;
+      m.phibar = m.sn * m.phibar;
+      m.denom = 1.0 / gamma;
+      // w_new = (v - oldeps w1 - delta w2) denom ; inputs (v, w1, w2)
+      c[0] = c64{m.denom, 0};
+      c[1] = c64{-m.oldeps * m.denom, 0};
+      c[2] = c64{-m.delta * m.denom, 0};
+      if (m.phibar <= m.rtol * m.beta1 || m.beta == 0) m.done = 1;
+    } else if (stage == 4) {
+      // x += phi w ; inputs (x, w)
+      c[0] = c64{1, 0};
+      c[1] = c64{m.phi, 0};
+    } else if (stage == 5) {
+      // next v = y / beta
+      c[0] = c64{m.beta > 0 ? 1.0 / m.beta : 0.0, 0};
+    }
+  }
+}
+
+// coef[4 i] = 1, coef[4 i + 1] = -sum of the complex partials of instance i (x - ev <ev, x>);
+// one block per instance
+__global__ __launch_bounds__(AB) void k_proj_coef(const c64* part, int nblk, c64* coef) {
+  const int i = blockIdx.x;
+  __shared__ double rr[AB], ri[AB];
+  double sr = 0, si = 0;
+  for (int b = threadIdx.x; b < nblk; b += AB) {
+    sr += part[long(i) * nblk + b].x;
+    si += part[long(i) * nblk + b].y;
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) {
+      rr[threadIdx.x] += rr[threadIdx.x + t];
+      ri[threadIdx.x] += ri[threadIdx.x + t];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    coef[4 * i] = c64{1, 0};
+    coef[4 * i + 1] = c64{-rr[0], -ri[0]};
+    coef[4 * i + 2] = c64{0, 0};
+    coef[4 * i + 3] = c64{0, 0};
+  }
+}
+
+}  // namespace
+
+double sum_part(const double* p, int nb) {
+  double s = 0;
+  for (int b = 0; b < nb; ++b) s += p[b];
+  return s;
+}
+
+VecOps::VecOps(hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_)
+    : s(s_), nb(nb_), ld(ld_), K(K_), nblk(int(std::min<long>(256, (K_ + AB - 1) / AB))), EV(EV_),
+      part(sizeof(c64) * size_t(nb_) * 256), coef(sizeof(c64) * 4 * nb_) {}
+
+void VecOps::vdot(const c64* A, const c64* B, const int* act) {
+  hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
+}
+
+void VecOps::lincomb(std::initializer_list<const c64*> ins, c64* outp, const int* act) {
+  Lin L{};
+  int j = 0;
+  for (auto p : ins) L.in[j++] = p;
+  L.nin = j;
+  L.out = outp;
+  L.coef = coef.as<c64>();
+  L.ld = ld;
+  const dim3 g(nblk, nb);
+  switch (L.nin) {
+    case 1: hipLaunchKernelGGL(k_lincomb<1>, g, dim3(AB), 0, s, L, K, act); break;
+    case 2: hipLaunchKernelGGL(k_lincomb<2>, g, dim3(AB), 0, s, L, K, act); break;
+    case 3: hipLaunchKernelGGL(k_lincomb<3>, g, dim3(AB), 0, s, L, K, act); break;
+    default: hipLaunchKernelGGL(k_lincomb<4>, g, dim3(AB), 0, s, L, K, act); break;
+  }
+}
+
+void VecOps::project(c64* x, const int* act) {
+  hipLaunchKernelGGL(k_cdot, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.as<c64>());
+  hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(AB), 0, s, part.as<c64>(), nblk, coef.as<c64>());
+  lincomb({x, EV}, x, act);
+}
+
+int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, const double* dk, c64* r, c64* xout,
+                      const std::vector<int>& want, double rtol, int maxit, bool throw_on_cap) {
+  hipStream_t s = S.stream;
+  const int nb = V.nb;
+  const long ld = S.ld, n3 = 3 * S.N;
+  // workspace kept on the handle (grown on demand): 8 vectors and the operator scratch per
+  // instance, the MINRES states and two flag arrays
+  const size_t vb = sizeof(c64) * nb * ld;
+  const int iblk = ((nb + 63) / 64) * 64;  // ints per flag array
+  const size_t need = 8 * vb + sizeof(c64) * nb * n3 + sizeof(MinresState) * nb + 2 * sizeof(int) * iblk;
+  S.krylov_ws.ensure(need);
+  char* wp = S.krylov_ws.as<char>();
+  c64 *x = reinterpret_cast<c64*>(wp), *r1 = r, *r2 = reinterpret_cast<c64*>(wp + vb),
+      *y = reinterpret_cast<c64*>(wp + 2 * vb), *v = reinterpret_cast<c64*>(wp + 3 * vb),
+      *w = reinterpret_cast<c64*>(wp + 4 * vb), *w1 = reinterpret_cast<c64*>(wp + 5 * vb),
+      *w2 = reinterpret_cast<c64*>(wp + 6 * vb), *avp = reinterpret_cast<c64*>(wp + 7 * vb);
+  c64* dm = reinterpret_cast<c64*>(wp + 8 * vb);
+  MinresState* ms = reinterpret_cast<MinresState*>(wp + 8 * vb + sizeof(c64) * nb * n3);
+  int* actp = reinterpret_cast<int*>(reinterpret_cast<char*>(ms) + sizeof(MinresState) * nb);
+  int* livep = actp + iblk;
+  OMX_HIP_CHECK(hipMemcpyAsync(actp, want.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+  const int* act = actp;
+  OMX_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w2, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w1, 0, sizeof(c64) * nb * ld, s));
+  std::vector<MinresState> hm(nb);
+  for (int i = 0; i < nb; ++i) {
+    std::memset(&hm[i], 0, sizeof(MinresState));
+    hm[i].rtol = rtol;
+    hm[i].shift = shifts[i];
+    hm[i].active = want[i];
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(ms, hm.data(), sizeof(MinresState) * nb, hipMemcpyHostToDevice, s));
+  const int nblk = V.nblk;
+  double* part = V.part.as<double>();
+  c64* coef = V.coef.as<c64>();
+  V.project(r1, act);
+  BView<double> AVv{avp, ld, ld}, Yv{y, ld, ld}, R1v{r1, ld, ld};
+  const VecSel sel{nb, 1, nullptr, nullptr};
+  // y = P r1 (projected)
+  launch_operator<double>(OP_P, S.p64.plan, R1v, Yv, dm, sel, dk, nullptr, 0, s);
+  V.project(y, act);
+  V.vdot(r1, y, act);
+  hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 0, nb, act);
+  OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+  std::vector<int> live(want);
+  int it = 0;
+  for (; it < maxit; ++it) {
+    const int* la = livep;
+    if (it % 4 == 0) {
+      OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), ms, sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+      OMX_HIP_CHECK(hipStreamSynchronize(s));
+      bool anyl = false;
+      for (int i = 0; i < nb; ++i) {
+        live[i] = want[i] && !hm[i].done;
+        anyl = anyl || live[i];
+      }
+      if (!anyl) break;
+      OMX_HIP_CHECK(hipMemcpyAsync(livep, live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+    }
+    // v = y / beta (coefficients left by the previous scalar step)
+    V.lincomb({y}, v, la);
+    BView<double> Vv{v, ld, ld};
+    launch_operator<double>(OP_M, S.p64.plan, Vv, AVv, dm, sel, dk, nullptr, 0, s);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 1, nb, la);
+    V.lincomb({avp, v, r1}, y, la);
+    V.vdot(v, y, la);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 2, nb, la);
+    V.lincomb({y, r2}, y, la);
+    std::swap(r1, r2);  // r1 <- r2
+    std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
+    BView<double> R2b{r2, ld, ld}, Yb{y, ld, ld};
+    launch_operator<double>(OP_P, S.p64.plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
+    V.project(y, la);
+    V.vdot(r2, y, la);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 3, nb, la);
+    // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
+    // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
+    V.lincomb({v, w2, w}, w1, la);
+    {
+      c64* wn = w1;
+      w1 = w2;
+      w2 = w;
+      w = wn;
+    }
+    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 4, nb, la);
+    V.lincomb({x, w}, x, la);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 5, nb, la);
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), ms, sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  if (throw_on_cap)
+    for (int i = 0; i < nb; ++i)
+      if (want[i] && !hm[i].done) throw Error(OMX_ERR_NOT_CONVERGED, "MINRES did not converge within the iteration cap");
+  OMX_HIP_CHECK(hipMemcpyAsync(xout, x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+  V.project(xout, act);
+  V.project(xout, act);
+  return it;
+}
+
+
+}  // namespace omx

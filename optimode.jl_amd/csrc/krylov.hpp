@@ -1,0 +1,43 @@
+// Batched FP64 vector helpers and preconditioned MINRES on the complement of a vector,
+// shared by the adjoint (eig_adjt), GVD and the FP64 eigenpair refinement.
+#pragma once
+#include <initializer_list>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace omx {
+
+// sum of nb partial values
+double sum_part(const double* p, int nb);
+
+// Vector helpers over nb instances of K complex FP64 entries at stride ld, against the
+// unit vectors EV (vector i at EV + i ld).  Coefficients of lincomb live on the device
+// (coef: 4 complex per instance), so that no step waits for the host.
+struct VecOps {
+  hipStream_t s;
+  int nb;
+  long ld, K;
+  int nblk;
+  const c64* EV;
+  DevMem part, coef;
+  VecOps(hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_);
+  // part[i nblk + b] = partial sums of Re <A_i, B_i>
+  void vdot(const c64* A, const c64* B, const int* act);
+  // out_i = sum_j coef[4 i + j] in_j_i (out may alias an input)
+  void lincomb(std::initializer_list<const c64*> ins, c64* outp, const int* act);
+  // x_i <- x_i - ev_i <ev_i, x_i>
+  void project(c64* x, const int* act);
+};
+
+// Solves (M(k_i) - shift_i) x_i = r_i on the complement of ev_i for the instances with
+// want[i] (MINRES, Paige & Saunders 1975, preconditioned by P projected onto the
+// complement; M - shift is Hermitian, singular along ev when shift is its eigenvalue, and
+// indefinite above the lowest band).  r is projected on entry and overwritten; xout
+// receives the solution orthogonalised against ev twice.  Stops when the preconditioned
+// residual estimate falls below rtol times its start, or after maxit iterations (an error
+// only when throw_on_cap).  Returns the iteration count.
+int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, const double* dk, c64* r, c64* xout,
+                      const std::vector<int>& want, double rtol, int maxit = 1000, bool throw_on_cap = true);
+
+}  // namespace omx

@@ -2,10 +2,15 @@
 // solvers/dftk.jl:304-506) over device-resident blocks, Newton solve_k (src/solve.jl:77-115,
 // solvers/gpu.jl:213-242) in an FP32 phase followed by an FP64 refinement phase.
 #include "solver.hpp"
+#include "krylov.hpp"
+#include "refine.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -540,6 +545,8 @@ template <typename T> class Batch {
 
 Solver::Solver(int dev, int nx_, int ny_, double Dx_, double Dy_, const double* epsinv)
     : device(dev), nx(nx_), ny(ny_), Dx(Dx_), Dy(Dy_) {
+  const char* tr = std::getenv("OMX_TRACE");
+  trace = tr && tr[0] && tr[0] != '0';
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw Error(OMX_ERR_NO_DEVICE, "no HIP device available");
   if (dev < 0 || dev >= count) throw Error(OMX_ERR_NO_DEVICE, "device index out of range");
@@ -607,6 +614,12 @@ void Solver::set_option(const std::string& key, double v) {
     opt.plain = v != 0 ? 1 : 0;
   else if (key == "warm_start")
     opt.warm_start = v != 0 ? 1 : 0;
+  else if (key == "refine")
+    opt.refine = v != 0 ? 1 : 0;
+  else if (key == "jd_rtol")
+    opt.jd_rtol = v;
+  else if (key == "jd_maxit")
+    opt.jd_maxit = int(v);
   else if (key == "reg_ops") {
     opt.reg_ops = v != 0 ? 1 : 0;
     select_operator(*this, p32);
@@ -846,6 +859,13 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
     iters += b.run(act, bands, nchk, msize, tols, maxiter, stats);
     b.newton(hns);
     ++newton_total;
+    if (S.trace) {
+      std::fprintf(stderr, "[omx] %s newton step %d:", sizeof(T) == 4 ? "fp32" : "fp64", step);
+      for (int i = 0; i < ninst; ++i)
+        if (act[i])
+          std::fprintf(stderr, " b%d(it %d tol %.0e f %.1e)", i, b.hst[i].niter, tols[i], hns[i].f);
+      std::fprintf(stderr, "\n");
+    }
     // a band may only finish on an eigenpair converged to the requested tolerance: when
     // |Δω²| met k_tol on a looser (inexact-Newton) eigensolve, re-solve at the same k
     for (int i = 0; i < ninst && strict; ++i)
@@ -855,6 +875,348 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
       }
   }
   return iters;
+}
+
+using cd = std::complex<double>;
+
+// Real symmetric eigenproblem by cyclic Jacobi (n <= 40): A (row-major n x n) is destroyed;
+// w ascending, V columns the eigenvectors (row-major, V[r * n + c]).
+void jacobi_sym(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& V) {
+  V.assign(size_t(n) * n, 0.0);
+  for (int i = 0; i < n; ++i) V[size_t(i) * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0, tot = 0;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) {
+        const double a = A[size_t(p) * n + q] * A[size_t(p) * n + q];
+        tot += a;
+        if (p != q) off += a;
+      }
+    if (off <= 1e-32 * tot) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[size_t(p) * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double app = A[size_t(p) * n + p], aqq = A[size_t(q) * n + q];
+        const double theta = 0.5 * (aqq - app) / apq;
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[size_t(k) * n + p], akq = A[size_t(k) * n + q];
+          A[size_t(k) * n + p] = c * akp - sn * akq;
+          A[size_t(k) * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[size_t(p) * n + k], aqk = A[size_t(q) * n + k];
+          A[size_t(p) * n + k] = c * apk - sn * aqk;
+          A[size_t(q) * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[size_t(k) * n + p], vkq = V[size_t(k) * n + q];
+          V[size_t(k) * n + p] = c * vkp - sn * vkq;
+          V[size_t(k) * n + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return A[size_t(a) * n + a] < A[size_t(b) * n + b]; });
+  std::vector<double> V2(size_t(n) * n);
+  w.resize(n);
+  for (int c = 0; c < n; ++c) {
+    w[c] = A[size_t(idx[c]) * n + idx[c]];
+    for (int r = 0; r < n; ++r) V2[size_t(r) * n + c] = V[size_t(r) * n + idx[c]];
+  }
+  V.swap(V2);
+}
+
+// Generalised Hermitian eigenproblem G c = lam B c for n <= RF_MAXC (G, B column-major
+// with leading dimension RF_MAXC): B = L L^H, H = L^-1 G L^-H, H's eigenvectors from its
+// real 2n x 2n form (each complex pair appears twice; complex Gram-Schmidt keeps one
+// vector per pair), c = L^-H y.  lam ascending, C column-major n x n.  False when B is
+// not numerically positive definite.
+bool dense_geig(int n, const c64* Gp, const c64* Bp, std::vector<double>& lam, std::vector<cd>& C) {
+  auto G = [&](int a, int b) { return 0.5 * (cd(Gp[a + RF_MAXC * b].x, Gp[a + RF_MAXC * b].y) +
+                                             std::conj(cd(Gp[b + RF_MAXC * a].x, Gp[b + RF_MAXC * a].y))); };
+  auto B = [&](int a, int b) { return 0.5 * (cd(Bp[a + RF_MAXC * b].x, Bp[a + RF_MAXC * b].y) +
+                                             std::conj(cd(Bp[b + RF_MAXC * a].x, Bp[b + RF_MAXC * a].y))); };
+  std::vector<cd> L(size_t(n) * n, 0.0);  // column-major lower
+  for (int j = 0; j < n; ++j) {
+    cd d = B(j, j);
+    for (int k = 0; k < j; ++k) d -= L[j + size_t(n) * k] * std::conj(L[j + size_t(n) * k]);
+    if (!(d.real() > 1e-14)) return false;
+    const double ljj = std::sqrt(d.real());
+    L[j + size_t(n) * j] = ljj;
+    for (int i = j + 1; i < n; ++i) {
+      cd v = B(i, j);
+      for (int k = 0; k < j; ++k) v -= L[i + size_t(n) * k] * std::conj(L[j + size_t(n) * k]);
+      L[i + size_t(n) * j] = v / ljj;
+    }
+  }
+  // W = L^-1 G (column by column), H = L^-1 W^H  (H = H^H)
+  std::vector<cd> W(size_t(n) * n), H(size_t(n) * n);
+  auto fsolve = [&](std::vector<cd>& x) {  // x <- L^-1 x
+    for (int i = 0; i < n; ++i) {
+      cd v = x[i];
+      for (int k = 0; k < i; ++k) v -= L[i + size_t(n) * k] * x[k];
+      x[i] = v / L[i + size_t(n) * i];
+    }
+  };
+  std::vector<cd> col(n);
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) col[i] = G(i, j);
+    fsolve(col);
+    for (int i = 0; i < n; ++i) W[i + size_t(n) * j] = col[i];
+  }
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) col[i] = std::conj(W[j + size_t(n) * i]);
+    fsolve(col);
+    for (int i = 0; i < n; ++i) H[i + size_t(n) * j] = col[i];
+  }
+  const int n2 = 2 * n;
+  std::vector<double> A(size_t(n2) * n2), w, V;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const cd h = 0.5 * (H[i + size_t(n) * j] + std::conj(H[j + size_t(n) * i]));
+      A[size_t(i) * n2 + j] = h.real();
+      A[size_t(i + n) * n2 + j + n] = h.real();
+      A[size_t(i) * n2 + j + n] = -h.imag();
+      A[size_t(i + n) * n2 + j] = h.imag();
+    }
+  jacobi_sym(n2, A, w, V);
+  std::vector<cd> Y;  // accepted complex eigenvectors, n each
+  lam.clear();
+  for (int c = 0; c < n2 && int(lam.size()) < n; ++c) {
+    std::vector<cd> y(n);
+    for (int r = 0; r < n; ++r) y[r] = cd(V[size_t(r) * n2 + c], V[size_t(r + n) * n2 + c]);
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t q = 0; q < lam.size(); ++q) {
+        cd d = 0;
+        for (int r = 0; r < n; ++r) d += std::conj(Y[q * n + r]) * y[r];
+        for (int r = 0; r < n; ++r) y[r] -= d * Y[q * n + r];
+      }
+    double nr = 0;
+    for (int r = 0; r < n; ++r) nr += std::norm(y[r]);
+    nr = std::sqrt(nr);
+    if (nr < 0.5) continue;  // the other half of a pair already taken
+    for (int r = 0; r < n; ++r) Y.push_back(y[r] / nr);
+    lam.push_back(w[c]);
+  }
+  if (int(lam.size()) < n) return false;
+  // C = L^-H Y: back substitution with L^H (upper)
+  C.assign(size_t(n) * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    for (int i = n - 1; i >= 0; --i) {
+      cd v = Y[size_t(j) * n + i];
+      for (int k = i + 1; k < n; ++k) v -= std::conj(L[k + size_t(n) * i]) * C[k + size_t(n) * j];
+      C[i + size_t(n) * j] = v / L[i + size_t(n) * i];
+    }
+  }
+  return true;
+}
+
+// FP64 Newton phase by eigenpair refinement instead of LOBPCG.  Instance i holds band
+// bands[i] at its own k_i with a block of msize[i] FP64 columns (from the FP32 phase).
+// Per Newton step: Rayleigh-Ritz of the block at k_i, then Jacobi-Davidson steps for the
+// band's Ritz pair (rho, x): MINRES on (M - rho) t = r in the complement of x (relative
+// tolerance jd_rtol, at most jd_maxit iterations), t orthogonalised against the block and Rayleigh-Ritz on [block, t],
+// until |M x - rho x| <= tol; then the Newton update of k from rho and HMkH(x).  The
+// band's Ritz value is the bands[i]-th of the block, as LOBPCG's locking order gives it.
+int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
+                 const std::vector<int>& msize, double tol, int max_steps, SolveStats& stats, int& newton_total,
+                 double& max_resid) {
+  const int ninst = b.ninst;
+  const long K = S.K, ld = S.ld, istride = b.istride;
+  hipStream_t s = S.stream;
+  const int nblk_g = int(std::max<long>(1, std::min<long>((K + 31) / 32, std::max(16, 1024 / ninst))));
+  const int nblk_r = int(std::min<long>(256, (K + 255) / 256));
+  DevMem n1d(sizeof(int) * ninst), zero(sizeof(int) * ninst), cold(sizeof(int) * ninst), lamd(sizeof(double) * ninst),
+      Tm(sizeof(c64) * ninst * ld), T2(sizeof(c64) * ninst * ld), ATm(sizeof(c64) * ninst * ld),
+      XB(sizeof(c64) * ninst * ld), RBm(sizeof(c64) * ninst * ld), part(sizeof(c64) * size_t(ninst) * nblk_g * RF_PAIRS),
+      Gd(sizeof(c64) * ninst * RF_PAIRS), Bd(sizeof(c64) * ninst * RF_PAIRS), Cd(sizeof(c64) * ninst * RF_PAIRS),
+      normp(sizeof(double) * ninst * nblk_r), scl(sizeof(double) * ninst), dots(sizeof(double) * ninst * b.dps);
+  OMX_HIP_CHECK(hipMemsetAsync(zero.as<void>(), 0, sizeof(int) * ninst, s));
+  OMX_HIP_CHECK(hipMemsetAsync(Tm.as<void>(), 0, sizeof(c64) * ninst * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(T2.as<void>(), 0, sizeof(c64) * ninst * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(ATm.as<void>(), 0, sizeof(c64) * ninst * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(XB.as<void>(), 0, sizeof(c64) * ninst * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(RBm.as<void>(), 0, sizeof(c64) * ninst * ld, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(cold.as<void>(), bands.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
+  const OpPlan<double>& plan = S.p64.plan;
+  std::vector<int> n1h(ninst, 0), jdh(ninst, 0);
+  std::vector<double> lam_band(ninst, 0.0), rn(ninst, 0.0);
+  std::vector<c64> hG(size_t(ninst) * RF_PAIRS), hB(size_t(ninst) * RF_PAIRS), hC(size_t(ninst) * RF_PAIRS);
+  auto up_n1 = [&](const std::vector<int>& v) {
+    OMX_HIP_CHECK(hipMemcpyAsync(n1d.as<void>(), v.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
+  };
+  // Rayleigh-Ritz of [X | t] (t only where with_t[i]) for the instances with n1h[i] > 0;
+  // X, AX <- [X | t] C, [AX | At] C with C the n1h[i] lowest Ritz vectors
+  auto rr = [&](const std::vector<int>& with_t) {
+    bool any_t = false;
+    for (int i = 0; i < ninst; ++i) any_t = any_t || (n1h[i] > 0 && with_t[i]);
+    const RCols Ux{b.p(b.X[b.cur]), ld, istride, any_t ? Tm.as<c64>() : nullptr};
+    const RCols Ua{b.p(b.AX[b.cur]), ld, istride, any_t ? ATm.as<c64>() : nullptr};
+    launch_bgram(Ux, Ua, 0, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Gd.as<c64>(), s);
+    launch_bgram(Ux, Ux, 0, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Bd.as<c64>(), s);
+    OMX_HIP_CHECK(hipMemcpyAsync(hG.data(), Gd.as<void>(), sizeof(c64) * hG.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipMemcpyAsync(hB.data(), Bd.as<void>(), sizeof(c64) * hB.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    std::fill(hC.begin(), hC.end(), c64{0, 0});
+    std::vector<double> lam;
+    std::vector<cd> C;
+    for (int i = 0; i < ninst; ++i) {
+      const int m = n1h[i];
+      if (m == 0) continue;
+      c64* Ci = hC.data() + size_t(i) * RF_PAIRS;
+      const int n = m + ((any_t && with_t[i]) ? 1 : 0);
+      if (!dense_geig(n, hG.data() + size_t(i) * RF_PAIRS, hB.data() + size_t(i) * RF_PAIRS, lam, C))
+        throw Error(OMX_ERR_CHOLESKY, "refinement: the Rayleigh-Ritz basis lost rank");
+      for (int j = 0; j < m; ++j)
+        for (int l = 0; l < n; ++l) Ci[l + RF_MAXC * j] = c64{C[l + size_t(n) * j].real(), C[l + size_t(n) * j].imag()};
+      lam_band[i] = lam[bands[i]];
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(Cd.as<void>(), hC.data(), sizeof(c64) * hC.size(), hipMemcpyHostToDevice, s));
+    // in place: each row's inputs are read before its outputs are written (one thread per
+    // row); instances outside this Rayleigh-Ritz are left as they are
+    launch_bcombine(Ux, Cd.as<c64>(), n1d.as<int>(), 0, b.p(b.X[b.cur]), istride, ninst, K, s);
+    launch_bcombine(Ua, Cd.as<c64>(), n1d.as<int>(), 0, b.p(b.AX[b.cur]), istride, ninst, K, s);
+  };
+  // band column's x (XB), residual (RBm) and residual norms for the instances in n1h
+  auto band_resid = [&]() {
+    OMX_HIP_CHECK(hipMemcpyAsync(lamd.as<void>(), lam_band.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
+    launch_band_resid(b.p(b.X[b.cur]), b.p(b.AX[b.cur]), ld, istride, cold.as<int>(), lamd.as<double>(),
+                      n1d.as<int>(), ninst, K, nblk_r, XB.as<c64>(), RBm.as<c64>(), normp.as<double>(), s);
+    std::vector<double> h(size_t(ninst) * nblk_r);
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), normp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < ninst; ++i) rn[i] = n1h[i] ? std::sqrt(sum_part(h.data() + size_t(i) * nblk_r, nblk_r)) : 0.0;
+  };
+  const int jd_max = 8;
+  int outer = 0;
+  for (int step = 0; step < max_steps; ++step) {
+    bool any = false;
+    for (int i = 0; i < ninst; ++i) {
+      n1h[i] = hns[i].done ? 0 : msize[i];
+      any = any || n1h[i] > 0;
+    }
+    if (!any) break;
+    up_n1(n1h);
+    // AX = M X at the current k, then Rayleigh-Ritz of the block
+    b.apply(OP_M, b.X[b.cur], b.AX[b.cur], zero.as<int>(), n1d.as<int>());
+    for (int i = 0; i < ninst; ++i) stats.matvec += n1h[i];
+    rr(std::vector<int>(ninst, 0));
+    ++outer;
+    std::vector<int> conv(ninst, 0);
+    for (int jd = 0;; ++jd) {
+      band_resid();
+      bool need = false;
+      for (int i = 0; i < ninst; ++i) {
+        conv[i] = n1h[i] > 0 && rn[i] <= tol;
+        jdh[i] = (n1h[i] > 0 && !conv[i]) ? 1 : 0;
+        need = need || jdh[i];
+      }
+      if (!need || jd >= jd_max) break;
+      // t: (M - rho) t = r on the complement of x (the sign of t is immaterial)
+      VecOps V(s, ninst, ld, K, XB.as<c64>());
+      const int its = minres_complement(S, V, lam_band, b.kd.as<double>(), RBm.as<c64>(), Tm.as<c64>(), jdh,
+                                        S.opt.jd_rtol, S.opt.jd_maxit, false);
+      for (int i = 0; i < ninst; ++i) {
+        stats.matvec += long(its) * jdh[i];
+        stats.precon += long(its) * jdh[i];
+      }
+      // t <- t - X (X^H t), twice, then unit norm (instances with jdh only)
+      std::vector<int> n1j(ninst);
+      for (int i = 0; i < ninst; ++i) n1j[i] = jdh[i] ? msize[i] : 0;
+      up_n1(n1j);
+      for (int pass = 0; pass < 2; ++pass) {
+        const RCols Xc{b.p(b.X[b.cur]), ld, istride, nullptr}, Tc{nullptr, ld, ld, Tm.as<c64>()};
+        launch_bgram(Xc, Tc, 1, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Gd.as<c64>(), s);
+        OMX_HIP_CHECK(hipMemcpyAsync(hG.data(), Gd.as<void>(), sizeof(c64) * hG.size(), hipMemcpyDeviceToHost, s));
+        OMX_HIP_CHECK(hipStreamSynchronize(s));
+        std::fill(hC.begin(), hC.end(), c64{0, 0});
+        for (int i = 0; i < ninst; ++i) {
+          if (!jdh[i]) continue;
+          c64* Ci = hC.data() + size_t(i) * RF_PAIRS;
+          for (int l = 0; l < msize[i]; ++l) {
+            const c64 g = hG[size_t(i) * RF_PAIRS + l];  // (l, 0): <x_l, t>
+            Ci[l] = c64{-g.x, -g.y};
+          }
+          Ci[msize[i]] = c64{1, 0};
+        }
+        OMX_HIP_CHECK(hipMemcpyAsync(Cd.as<void>(), hC.data(), sizeof(c64) * hC.size(), hipMemcpyHostToDevice, s));
+        const RCols XT{b.p(b.X[b.cur]), ld, istride, Tm.as<c64>()};
+        launch_bcombine(XT, Cd.as<c64>(), n1d.as<int>(), 1, T2.as<c64>(), ld, ninst, K, s);
+        std::swap(Tm, T2);
+      }
+      {
+        V.vdot(Tm.as<c64>(), Tm.as<c64>(), n1d.as<int>() /* nonzero = active */);
+        std::vector<double> h(size_t(ninst) * V.nblk), sc(ninst, 1.0);
+        OMX_HIP_CHECK(hipMemcpyAsync(h.data(), V.part.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        OMX_HIP_CHECK(hipStreamSynchronize(s));
+        for (int i = 0; i < ninst; ++i)
+          if (jdh[i]) {
+            const double nt = std::sqrt(sum_part(h.data() + size_t(i) * V.nblk, V.nblk));
+            sc[i] = nt > 0 ? 1.0 / nt : 0.0;
+          }
+        OMX_HIP_CHECK(hipMemcpyAsync(scl.as<void>(), sc.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
+        launch_scale_cols(Tm.as<c64>(), ld, scl.as<double>(), n1d.as<int>(), ninst, K, s);
+      }
+      // At = M t
+      {
+        const VecSel sel{ninst, 1, zero.as<int>(), n1d.as<int>()};
+        BView<double> Tv{Tm.as<c64>(), ld, ld}, ATv{ATm.as<c64>(), ld, ld};
+        launch_operator<double>(OP_M, plan, Tv, ATv, b.p(b.D), sel, b.kd.as<double>(), nullptr, 0, s);
+        for (int i = 0; i < ninst; ++i) stats.matvec += jdh[i];
+      }
+      // Rayleigh-Ritz on [X | t] for the instances still refining
+      n1h = n1j;
+      rr(jdh);
+      ++outer;
+    }
+    // restore the step's instance set (those converged in the JD loop were dropped from n1h)
+    for (int i = 0; i < ninst; ++i) n1h[i] = hns[i].done ? 0 : msize[i];
+    up_n1(n1h);
+    band_resid();
+    // Newton update from rho and HMkH(x) of the band vector
+    {
+      const VecSel sel{ninst, 1, zero.as<int>(), n1d.as<int>()};
+      BView<double> XBv{XB.as<c64>(), ld, ld};
+      launch_operator<double>(OP_MKH, plan, XBv, XBv, b.p(b.D), sel, b.kd.as<double>(), dots.as<double>(), b.dps, s);
+      std::vector<double> h(size_t(ninst) * b.dps);
+      OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dots.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+      OMX_HIP_CHECK(hipStreamSynchronize(s));
+      std::vector<double> kh(ninst);
+      for (int i = 0; i < ninst; ++i) {
+        NewtonState& ns = hns[i];
+        kh[i] = ns.k;
+        if (!n1h[i]) continue;
+        const double slope = 2.0 * sum_part(h.data() + size_t(i) * b.dps, b.dps);
+        ns.lam = lam_band[i];
+        ns.f = lam_band[i] - ns.omega2;
+        ns.slope = slope;
+        ns.nsolves += 1;
+        max_resid = std::max(max_resid, rn[i]);
+        if (rn[i] <= tol && std::fabs(ns.f) <= ns.k_tol) {
+          ns.done = 1;
+        } else if (ns.nsolves >= ns.max_solves) {
+          ns.done = 2;
+        } else {
+          const double kn = ns.k - ns.f / slope;
+          ns.k = (std::isfinite(kn) && kn > 0) ? kn : ns.k;
+        }
+        kh[i] = ns.k;
+      }
+      OMX_HIP_CHECK(hipMemcpyAsync(b.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
+    }
+    ++newton_total;
+    if (S.trace) {
+      std::fprintf(stderr, "[omx] fp64 refine step %d:", step);
+      for (int i = 0; i < ninst; ++i)
+        if (n1h[i]) std::fprintf(stderr, " b%d(res %.1e f %.1e)", i, rn[i], hns[i].f);
+      std::fprintf(stderr, "\n");
+    }
+  }
+  return outer;
 }
 
 }  // namespace
@@ -873,6 +1235,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   OMX_HIP_CHECK(hipEventCreate(&e0));
   OMX_HIP_CHECK(hipEventCreate(&e1));
   OMX_HIP_CHECK(hipEventRecord(e0, stream));
+  const auto t_start = std::chrono::steady_clock::now();
+  auto t_fp32_end = t_start;
   SolveStats stats;
   const int ninst = nev, M = nev;
   // k_guess (MaxwellEigenmodes.jl:63): omega * sqrt(1 / min_a min eps^-1[a,a,:,:])
@@ -1007,6 +1371,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,
                                  stream));
     OMX_HIP_CHECK(hipStreamSynchronize(stream));
+    t_fp32_end = std::chrono::steady_clock::now();
     for (auto& s : hns) {
       if (s.done == 1) s.done = 0;  // refine in FP64
       s.k_tol = k_tol;
@@ -1021,10 +1386,21 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
     }
   }
+  double max_resid = 0;
   if (!opt.plain) {
     const int left = std::max(1, max_eigsolves - newton_total);
-    stats.it64 +=
-        newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
+    if (opt.refine && !opt.fp64_only)
+      stats.it64 += refine_phase(*this, b64, hns, bands, msize, eig_tol, left, stats, newton_total, max_resid);
+    else
+      stats.it64 +=
+          newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
+  }
+  if (trace) {
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[omx] solve_k phases: fp32 %.2f ms, fp64 %.2f ms\n",
+                 std::chrono::duration<double, std::milli>(t_fp32_end - t_start).count(),
+                 std::chrono::duration<double, std::milli>(t1 - t_fp32_end).count());
   }
   bool ok = true;
   for (int i = 0; i < ninst; ++i) {
@@ -1053,7 +1429,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     info->lobpcg_iters_fp64 = stats.it64;
     info->n_matvec = stats.matvec;
     info->n_precon = stats.precon;
-    info->max_residual = 0;
+    info->max_residual = max_resid;
     info->device_ms = ms;
   }
   if (!ok) throw Error(OMX_ERR_NOT_CONVERGED, "solve_k: Newton iteration did not converge within max_eigsolves");

@@ -147,6 +147,10 @@ struct Options {
                                 // random start at k_guess per band, every eigensolve to eig_tol
   int warm_start = 0;           // solve_k without kguess/Hguess starts every band from the previous
                                 // solve's k and eigenvectors (kept on the device)
+  int refine = 1;               // FP64 phase by Rayleigh-Ritz + Jacobi-Davidson refinement of each
+                                // band's vector (0: FP64 LOBPCG on the whole block)
+  double jd_rtol = 1e-3;        // relative tolerance of the refinement's correction solves
+  int jd_maxit = 60;            // iteration cap of one correction solve
 };
 
 // Device-resident EME mode set: E and H of nev modes, planar per mode and component
@@ -199,7 +203,9 @@ class Solver {
   std::vector<double> last_kmags;   // k of the eigenvectors in last_evecs
   int last_adjoint_iters = 0;
   ModeSlot modes[MODE_SLOTS];
+  bool trace = false;               // OMX_TRACE=1: per-Newton-step iteration log on stderr
   DevMem scratch_a, scratch_b, scratch_c;
+  DevMem krylov_ws;                  // MINRES vectors and scratch (grown on demand)
   DevMem field_scratch, argmax_v, argmax_i;  // canonicalize_phase
   Profiler prof;
 

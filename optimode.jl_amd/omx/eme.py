@@ -169,13 +169,15 @@ class EMEResult:
 
 
 def eme(epsinvs, lengths, omega, grid, solver, nev=2, conjugate=False, reg=1e-9, reciprocity=True,
-        passivity="invert", warmstart=True, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=60, maxiter=100):
+        passivity="invert", warmstart=True, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=60, maxiter=100,
+        continue_chain=False):
     """``eme(ε⁻¹s, ∂ε_∂ωs, lengths, ω, grid, solver; nev, conjugate, reg, reciprocity,
     passivity)`` (``eme.jl:90-104``) with ``warmstart`` (``eme.jl:58-76``): solve every
     cell, build its power-normalised modes on the GPU, couple neighbours through interface
     S-matrices and cascade ``prop₀ ⋆ iface₀₁ ⋆ prop₁ ⋆ …`` (``_assemble``, ``eme.jl:125-136``).
     ``epsinvs`` may be arrays or zero-argument callables producing them (e.g. GPU
-    smoothing of the cell's cross-section)."""
+    smoothing of the cell's cross-section).  ``continue_chain=True`` warm-starts the first
+    cell from the solver's previous solve as well (a chain processed in pieces)."""
     n = len(epsinvs)
     if n < 1 or len(lengths) != n:
         raise ValueError("need one length per cell and at least one cell")
@@ -184,7 +186,7 @@ def eme(epsinvs, lengths, omega, grid, solver, nev=2, conjugate=False, reg=1e-9,
     prev_slot = None
     for i in range(n):
         ei = epsinvs[i]() if callable(epsinvs[i]) else epsinvs[i]
-        solver.options["warm_start"] = 1.0 if (warmstart and i > 0) else 0.0
+        solver.options["warm_start"] = 1.0 if (warmstart and (i > 0 or continue_chain)) else 0.0
         t0 = time.perf_counter()
         km, _, info = solve_k(omega, ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
                               max_eigsolves=max_eigsolves, maxiter=maxiter, return_info=True, fetch_evecs=False)
