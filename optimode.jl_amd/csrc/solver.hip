@@ -616,6 +616,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.warm_start = v != 0 ? 1 : 0;
   else if (key == "refine")
     opt.refine = v != 0 ? 1 : 0;
+  else if (key == "fp32_newton")
+    opt.fp32_newton = v != 0 ? 1 : 0;
   else if (key == "jd_rtol")
     opt.jd_rtol = v;
   else if (key == "jd_maxit")
@@ -1364,8 +1366,9 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
       stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
     }
-    stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
-                               newton_total, false);
+    if (opt.fp32_newton)
+      stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
+                                 newton_total, false);
     launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
                                   K, stream);
     OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,

@@ -149,6 +149,8 @@ struct Options {
                                 // solve's k and eigenvectors (kept on the device)
   int refine = 1;               // FP64 phase by Rayleigh-Ritz + Jacobi-Davidson refinement of each
                                 // band's vector (0: FP64 LOBPCG on the whole block)
+  int fp32_newton = 1;          // FP32 Newton steps before the FP64 phase (0: the FP64 phase starts
+                                // from the shared first FP32 eigensolve)
   double jd_rtol = 1e-3;        // relative tolerance of the refinement's correction solves
   int jd_maxit = 60;            // iteration cap of one correction solve
 };

@@ -16,7 +16,7 @@ using ChainRulesCore: NoTangent, AbstractZero
 using DielectricSmoothing: Grid, N
 import MaxwellEigenmodes: solve_ω², solve_k, AbstractEigensolver, ModeSolver
 
-export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E
+export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E, hip_build_modes, hip_overlap_matrix
 
 const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
 
@@ -218,6 +218,45 @@ field ε⁻¹ fft(kx_tc(ev_j)) of mode j, size (3, Nx, Ny).
 function hip_ng_gvd_E(solver::HIPSolver, kmags::AbstractVector{<:Real}, evecs, ω::Real,
                       ε⁻¹::AbstractArray{Float64,4}, ∂ε_∂ω, ∂²ε_∂ω², grid::Grid{2,Float64})
     return _ng_gvd(solver, kmags, evecs, ω, ε⁻¹, ∂ε_∂ω, ∂²ε_∂ω², grid, true)
+end
+
+"""
+    hip_build_modes(solver, slot, kmags, ε⁻¹, grid; evecs=nothing, conjugate=false, normalize=true)
+
+`build_mode` (lib/EigenmodeExpansion/src/modes.jl:116-122) for every mode into device mode
+set `slot` (0..3): E = i ε⁻¹ fft(kx_tc(ev)), H = fft(tc(ev)), both divided by
+sqrt(inner_product(m, m)).  `evecs = nothing` uses the eigenvectors of the handle's last
+`solve_k`.  Returns the complex norms.
+"""
+function hip_build_modes(solver::HIPSolver, slot::Integer, kmags::AbstractVector{<:Real},
+                         ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64}; evecs=nothing,
+                         conjugate::Bool=false, normalize::Bool=true)
+    h = _handle!(solver, grid, ε⁻¹)
+    km = Vector{Float64}(kmags)
+    E = evecs === nothing ? nothing : reduce(hcat, [Vector{ComplexF64}(v) for v in evecs])
+    norms = Vector{ComplexF64}(undef, length(km))
+    GC.@preserve E begin
+        st = ccall((:omx_build_modes, libomx), Cint,
+                   (Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Cint, Cint, Ptr{ComplexF64}),
+                   h, slot, length(km), km, E === nothing ? Ptr{ComplexF64}(C_NULL) : E, conjugate, normalize, norms)
+    end
+    _check(st, h)
+    return norms
+end
+
+"""
+    hip_overlap_matrix(solver, slot_a, na, slot_b, nb; conjugate=false)
+
+`overlap_matrix(modes_a, modes_b; conjugate)` (modes.jl:156-170) of two device mode sets.
+"""
+function hip_overlap_matrix(solver::HIPSolver, slot_a::Integer, na::Integer, slot_b::Integer, nb::Integer;
+                            conjugate::Bool=false)
+    h = solver.handle
+    O = Matrix{ComplexF64}(undef, na, nb)
+    st = ccall((:omx_overlap_matrix, libomx), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{ComplexF64}),
+               h, slot_a, slot_b, conjugate, O)
+    _check(st, h)
+    return O
 end
 
 end # module
