@@ -132,6 +132,77 @@ def eme_workload(a):
     print(json.dumps(out), flush=True)
 
 
+def batch_problems(seed=3, n=64):
+    """The mixed-grid batch: seeded ridges (w ~ U(0.8, 2.5) um, h ~ U(0.3, 0.9) um,
+    sidewall ~ U(60, 90) deg, core eps ~ U(3.5, 4.9)) on SiO2 under air, grids uniform in
+    {128^2, 192^2, 256^2, 384^2} over an 8 x 4 um cell, omega ~ U(0.5, 1.0), nev uniform in
+    4..16; full 3x3 smoothed tensors (xy terms at the slanted faces), smoothed on the GPU."""
+    import numpy as np
+    import omx
+    from omx import geometry as G
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(n):
+        N = int(rng.choice([128, 192, 256, 384]))
+        g = omx.Grid(8.0, 4.0, N, N)
+        w, h, sw, ec = rng.uniform(0.8, 2.5), rng.uniform(0.3, 0.9), rng.uniform(60, 90), rng.uniform(3.5, 4.9)
+        om, nev = rng.uniform(0.5, 1.0), int(rng.integers(4, 17))
+        probs.append(dict(grid=g, epsinv=G.ridge_waveguide(g, w, h, sw, ec, backend="gpu"), omega=om, nev=nev,
+                          k_tol=1e-10, eig_tol=1e-8, max_eigsolves=60, maxiter=200))
+    return probs
+
+
+def batch_workload(a):
+    """64 independent solve_k problems of uneven grid, omega and mode count in one
+    omx_solve_k_batch call (4 streams), against the same problems one after another."""
+    import numpy as np
+    import omx
+    probs = batch_problems()
+    nmodes = sum(p["nev"] for p in probs)
+    omx.solve_k_batch(probs[:4], nstreams=4, precision=a.precision)  # warm-up
+    t0 = time.perf_counter()
+    res = omx.solve_k_batch(probs, nstreams=4, precision=a.precision)
+    elapsed = time.perf_counter() - t0
+    conv = sum(p["nev"] for p, r in zip(probs, res) if r["status"] == 0 and r["info"]["converged"])
+    # the same problems one at a time on one stream
+    t1 = time.perf_counter()
+    res1 = omx.solve_k_batch(probs, nstreams=1, precision=a.precision)
+    seq = time.perf_counter() - t1
+    # two problems against a cold FP64 solve at a tighter tolerance
+    errs = []
+    for j in range(2):
+        p = probs[j]
+        kr, _ = omx.solve_k(p["omega"], p["epsinv"], p["grid"], omx.HIPSolver(precision="fp64"), nev=p["nev"],
+                            k_tol=1e-12, eig_tol=1e-10, fetch_evecs=False)
+        errs.append(float(np.max(np.abs(res[j]["kmags"] ** 2 - kr ** 2) / kr ** 2)))
+    same = all(np.array_equal(r["kmags"], r1["kmags"]) for r, r1 in zip(res, res1))
+    out = {
+        "metric": "converged modes/s, 64-instance mixed-grid batch (128^2..384^2, 4-16 modes)",
+        "value": round(conv / elapsed, 3),
+        "unit": "modes/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * elapsed, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if a.precision == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "omx_solve_k_batch over 64 seeded ridge problems (grids 128^2..384^2, omega U(0.5,1), "
+                        "nev 4..16, k_tol 1e-10, eig_tol 1e-8), 4 streams",
+            "modes_total": nmodes,
+            "modes_converged": conv,
+            "one_stream_ms": round(1e3 * seq, 1),
+            "concurrency_speedup": round(seq / elapsed, 2),
+            "results_identical_to_one_stream": same,
+            "max_rel_err_k2_vs_cold_fp64_first_two": max(errs),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,8 +211,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-algorithm comparison solve")
-    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme"],
-                    help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain")
+    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "batch"],
+                    help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain; "
+                         "batch: the 64-instance mixed-grid batch")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every solve here fits one MI355X and "
@@ -150,6 +222,13 @@ def main():
     if a.workload == "eme":
         eme_workload(a)
         return
+    if a.workload == "batch":
+        # The first run of this workload ended in a GPU memory fault (dispatch dump:
+        # k_combine<float,16,CMB_TWO>, 512 x 14 blocks, i.e. a 256^2 nev-14 FP32 P update)
+        # whose cause has not been shown; it stays off until it is.
+        print("the batch workload is disabled: its one GPU run faulted and the cause is not yet found "
+              "(omx_solve_k_batch is covered at small sizes by tests/test_gpu_batch.py)", file=sys.stderr)
+        sys.exit(2)
 
     import numpy as np
     import omx

@@ -50,12 +50,28 @@ typedef struct omx_info {
   int32_t converged;         /* 1 when every requested mode met k_tol / tol */
   int32_t n_newton;          /* Newton steps (eigensolves), summed over bands */
   int32_t lobpcg_iters_fp32; /* LOBPCG iterations in FP32, summed over bands and steps */
-  int32_t lobpcg_iters_fp64; /* LOBPCG iterations in FP64 (refinement) */
+  int32_t lobpcg_iters_fp64; /* FP64 phase: Rayleigh-Ritz steps of the eigenpair refinement
+                                (FP64 LOBPCG iterations with option refine = 0) */
   int64_t n_matvec;          /* operator applications (vectors) */
   int64_t n_precon;          /* preconditioner applications (vectors) */
   double max_residual;       /* final FP64 residual norm ||M x - w2 x||, max over modes */
   double device_ms;          /* GPU time of the solve (HIP events) */
 } omx_info;
+
+/* One problem of omx_solve_k_batch: inputs, then outputs written by the call. */
+typedef struct omx_problem {
+  int32_t nx, ny;            /* grid */
+  double Dx, Dy;             /* cell (um) */
+  const double* epsinv;      /* (3,3,Nx,Ny) column-major, symmetric per pixel */
+  double omega;
+  int32_t nev;
+  int32_t max_eigsolves, maxiter;
+  double k_tol, eig_tol;
+  double* kmags_out;         /* nev */
+  double* evecs_out;         /* NULL, or nev vectors of 2N complex */
+  omx_info info;             /* out */
+  int32_t status;            /* out: OMX_OK or the problem's error */
+} omx_problem;
 
 /* library */
 const char* omx_version(void);
@@ -102,6 +118,15 @@ int omx_group_index(omx_solver* h, double omega, int nev, const double* kmags, c
 int omx_ng_gvd(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
                const double* deps_domega, const double* d2eps_domega2, double* ng_out, double* gvd_out,
                double* E_out);
+
+/* Batched solve_k (no single reference counterpart: the 64-instance mixed-grid batch of
+   independent solve_k calls, src/solve.jl:182-196, that sweeps and EME run one by one).
+   Problems may differ in grid, ω and nev; they are taken largest first from a shared queue
+   by min(nstreams, 8) host workers, each with its own HIP stream and handles, so that
+   several solves occupy the GPU at once.  Per-problem results and status go into probs;
+   the call returns OMX_OK when every problem succeeded, else the first failing status.
+   fp64_only != 0 runs every solve in FP64. */
+int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only);
 
 /* EME per-slice mode set (lib/EigenmodeExpansion/src/modes.jl:105-122, mode_fields and
    build_mode): E = i ε⁻¹ fft(kx_tc(ev_j)) and H = fft(tc(ev_j)) of nev <= 16 modes at

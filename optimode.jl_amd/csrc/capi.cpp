@@ -101,6 +101,19 @@ int omx_smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nsha
   }
 }
 
+int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only) {
+  try {
+    omx::solve_k_batch(device, nprob, probs, nstreams, fp64_only);
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  }
+}
+
 const char* omx_last_error(const omx_solver* h) {
   if (!h || !h->s) return g_error.c_str();
   return h->s->last_error.c_str();

@@ -744,19 +744,28 @@ void Solver::group_index(double omega, int nev, const double* kmags, const doubl
 namespace {
 
 // A batch of the requested shape from the handle's workspace cache (created on first use;
-// at most four shapes are kept, the oldest is freed first).  Returned with cur = 0.
-template <typename T> Batch<T>& workspace(Solver& S, int ninst, int M, int maxiter) {
+// at most four shapes are kept, least recently used freed first).  Returned with cur = 0.
+// The caller holds the returned pointer for as long as it uses the batch: a later request
+// in the same solve may drop the entry from the cache, and the batch then lives until the
+// caller lets it go.
+template <typename T> std::shared_ptr<Batch<T>> workspace(Solver& S, int ninst, int M, int maxiter) {
   const int tb = int(sizeof(T));
-  for (auto& w : S.workspaces)
+  for (size_t j = 0; j < S.workspaces.size(); ++j) {
+    const Solver::Workspace& w = S.workspaces[j];
     if (w.tbytes == tb && w.ninst == ninst && w.M == M && w.maxiter >= maxiter) {
-      Batch<T>& b = *static_cast<Batch<T>*>(w.batch.get());
-      b.cur = 0;
+      Solver::Workspace hit = w;
+      S.workspaces.erase(S.workspaces.begin() + long(j));
+      S.workspaces.push_back(hit);
+      std::shared_ptr<Batch<T>> b(hit.batch, static_cast<Batch<T>*>(hit.batch.get()));
+      b->cur = 0;
       return b;
     }
+  }
   if (S.workspaces.size() >= 4) S.workspaces.erase(S.workspaces.begin());
   std::shared_ptr<void> p(new Batch<T>(S, ninst, M, maxiter), [](void* q) { delete static_cast<Batch<T>*>(q); });
-  S.workspaces.push_back(Solver::Workspace{tb, ninst, M, maxiter, p});
-  return *static_cast<Batch<T>*>(p.get());
+  const size_t bytes = size_t(2 * tb) * size_t(ninst) * M * (11 * size_t(S.ld) + 3 * size_t(S.N));
+  S.workspaces.push_back(Solver::Workspace{tb, ninst, M, maxiter, p, bytes});
+  return std::shared_ptr<Batch<T>>(p, static_cast<Batch<T>*>(p.get()));
 }
 
 template <typename T>
@@ -784,10 +793,12 @@ void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_
   OMX_HIP_CHECK(hipEventRecord(e0, stream));
   SolveStats stats;
   std::vector<int> act{1}, bands{0}, nchk{nev};
-  Batch<double>& b64 = workspace<double>(*this, 1, nev, std::max(maxiter, opt.maxiter_fp64));
+  const auto b64p = workspace<double>(*this, 1, nev, std::max(maxiter, opt.maxiter_fp64));
+  Batch<double>& b64 = *b64p;
   OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
   if (!opt.fp64_only) {
-    Batch<float>& b32 = workspace<float>(*this, 1, nev, maxiter);
+    const auto b32p = workspace<float>(*this, 1, nev, maxiter);
+    Batch<float>& b32 = *b32p;
     OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), &k, sizeof(double), hipMemcpyHostToDevice, stream));
     if (H_io)
       upload_block<float>(*this, b32, H_io, nev, 0, 1);
@@ -1271,7 +1282,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     msize[i] = (opt.guard < 0 || opt.plain) ? M : std::min(M, i + 1 + opt.guard);
   }
   const int maxit64 = std::max(maxiter, opt.maxiter_fp64);
-  Batch<double>& b64 = workspace<double>(*this, ninst, M, maxit64);
+  const auto b64p = workspace<double>(*this, ninst, M, maxit64);
+  Batch<double>& b64 = *b64p;
   int newton_total = 0;
   const int nguess_used = (Hguess && nguess > 0) ? std::min(nguess, M) : 0;
   const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
@@ -1357,7 +1369,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
         if (i != b) hns[i] = keep[i];
     }
   } else if (!opt.fp64_only) {
-    Batch<float>& b32 = workspace<float>(*this, ninst, M, maxiter);
+    const auto b32p = workspace<float>(*this, ninst, M, maxiter);
+    Batch<float>& b32 = *b32p;
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
     if (warm) {
       warm_block(b32);

@@ -216,7 +216,13 @@ class Solver {
   struct Workspace {
     int tbytes, ninst, M, maxiter;
     std::shared_ptr<void> batch;
+    size_t bytes;  // device memory the batch holds
   };
+  size_t workspace_bytes() const {
+    size_t b = 0;
+    for (const auto& w : workspaces) b += w.bytes;
+    return b;
+  }
   std::vector<Workspace> workspaces;
 
   template <typename T> PrecisionData<T>& pd();
@@ -230,6 +236,9 @@ template <typename T> void upload_tensor6(const Solver& S, const double* E, DevM
 void check_symmetric(long N, const double* E, const char* what);
 // per pixel, out = sym(L M R) for column-major 3x3 fields
 void pixel_product3(long N, const double* L, const double* M, const double* R, double* out);
+
+// Many solve_k problems of uneven grid, omega and mode count (batch.cpp).
+void solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only);
 
 // Kottke-smoothed eps^-1 of convex polygons (smooth.hip); no handle needed.
 void smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshape, const int* nverts,

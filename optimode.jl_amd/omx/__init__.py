@@ -25,8 +25,8 @@ import numpy as np
 from . import _lib
 from ._lib import OmxError, OmxInfo, OMX_PREC_FP32, OMX_PREC_FP64  # noqa: F401
 
-__all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "rrule_solve_k", "group_index",
-           "ng_gvd", "k_guess", "OmxError", "library_path", "exported_symbols", "kernel_stats"]
+__all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "solve_k_batch", "rrule_solve_k",
+           "group_index", "ng_gvd", "k_guess", "OmxError", "library_path", "exported_symbols", "kernel_stats"]
 
 
 def library_path():
@@ -275,6 +275,47 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
     if return_info:
         return km, evecs, info.as_dict()
     return km, evecs
+
+
+def solve_k_batch(problems, nstreams=4, precision="mixed", device=0, fetch_evecs=False):
+    """Many independent ``solve_k`` problems in one call (``omx_solve_k_batch``): each problem
+    is a dict with ``grid``, ``epsinv``, ``omega``, ``nev`` and optionally ``k_tol``,
+    ``eig_tol`` (1e-8), ``max_eigsolves`` (60), ``maxiter`` (100).  Problems may differ in
+    grid, ω and nev; ``nstreams`` host workers, each with its own HIP stream, run them
+    concurrently.  Returns one dict per problem: ``kmags``, ``evecs`` (when ``fetch_evecs``),
+    ``info`` and ``status`` (0 = OMX_OK)."""
+    n = len(problems)
+    arr = (_lib.OmxProblem * max(1, n))()
+    keep = []
+    outs = []
+    for j, p in enumerate(problems):
+        g = p["grid"]
+        ec = _epsinv_c(p["epsinv"], g)
+        km = np.zeros(int(p["nev"]))
+        ev = np.zeros((int(p["nev"]), 2 * g.N), dtype=np.complex128) if fetch_evecs else None
+        keep += [ec, km, ev]
+        q = arr[j]
+        q.nx, q.ny, q.Dx, q.Dy = g.Nx, g.Ny, float(g.Dx), float(g.Dy)
+        q.epsinv = _lib.dptr(ec)
+        q.omega = float(p["omega"])
+        q.nev = int(p["nev"])
+        q.max_eigsolves = int(p.get("max_eigsolves", 60))
+        q.maxiter = int(p.get("maxiter", 100))
+        q.k_tol = float(p.get("k_tol", 1e-8))
+        q.eig_tol = float(p.get("eig_tol", 1e-8))
+        q.kmags_out = _lib.dptr(km)
+        q.evecs_out = _lib.dptr(ev)
+        outs.append((km, ev))
+    L = _lib.lib()
+    st = L.omx_solve_k_batch(int(device), n, arr, int(nstreams), 1 if precision == "fp64" else 0)
+    if st in (_lib.OMX_ERR_NO_DEVICE, _lib.OMX_ERR_INVALID_ARG) and all(arr[j].status == _lib.OMX_ERR_INTERNAL
+                                                                          for j in range(n)):
+        _lib.check(st, None)
+    res = []
+    for j, (km, ev) in enumerate(outs):
+        res.append({"kmags": km, "evecs": None if ev is None else [ev[i].copy() for i in range(len(km))],
+                    "info": arr[j].info.as_dict(), "status": int(arr[j].status)})
+    return res
 
 
 def group_index(k, ev, omega, epsinv, deps_domega, grid, solver):

@@ -43,6 +43,22 @@ class OmxInfo(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class OmxProblem(ctypes.Structure):
+    """``omx_problem`` of include/omx.h."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("Dx", ctypes.c_double), ("Dy", ctypes.c_double),
+        ("epsinv", ctypes.POINTER(ctypes.c_double)),
+        ("omega", ctypes.c_double),
+        ("nev", ctypes.c_int32), ("max_eigsolves", ctypes.c_int32), ("maxiter", ctypes.c_int32),
+        ("k_tol", ctypes.c_double), ("eig_tol", ctypes.c_double),
+        ("kmags_out", ctypes.POINTER(ctypes.c_double)),
+        ("evecs_out", ctypes.POINTER(ctypes.c_double)),
+        ("info", OmxInfo),
+        ("status", ctypes.c_int32),
+    ]
+
+
 # (name, restype, argtypes) for every entry point of include/omx.h
 SIGNATURES = [
     ("omx_version", ctypes.c_char_p, []),
@@ -71,6 +87,8 @@ SIGNATURES = [
     ("omx_solve_k_adjoint", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
                                            ctypes.POINTER(ctypes.c_uint8), _dp, _dp]),
     ("omx_get_kernel_stats", ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
+    ("omx_solve_k_batch", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OmxProblem), ctypes.c_int,
+                                         ctypes.c_int]),
     ("omx_smooth_epsinv", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp, ctypes.c_int,
                                          _dp]),
