@@ -8,7 +8,7 @@ imports, calls or falls back to anything in here.
 
 Every function cites the reference ``file:line`` it restates.  How each one is pinned
 (closed forms, independent dense assembly, brute force, finite differences) is stated
-by its tests in ``tests/test_oracle.py`` and ``tests/test_oracle_analysis.py``; a function
+by its tests in ``tests/test_oracle*.py``; a function
 without such a check says "parity unpinned" in its docstring.
 """
 from .maxwell import *  # noqa: F401,F403
@@ -17,3 +17,4 @@ from .solve import solve_omega2, solve_k, solve_k_single  # noqa: F401
 from .adjoint import (eig_adjt, epsinv_bar, grad_k_mag_m_n, solve_k_pullback,  # noqa: F401
                       sliceinv_3x3, sliceinv_3x3_pullback, group_index)
 from .analysis import ng_gvd, gaussian_wg_eps_dispersive  # noqa: F401
+from . import periodic  # noqa: F401

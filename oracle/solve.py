@@ -3,6 +3,17 @@ import numpy as np
 
 from .maxwell import HelmholtzOracle, k_guess, canonicalize_phase
 from .lobpcg import lobpcg
+from .periodic import Grid3, HelmholtzOracle3, canonicalize_phase3
+
+
+def _helmholtz(k, epsinv, grid):
+    """The operator bundle of the grid's dimension (the reference dispatches on
+    ``Grid{2}`` / ``Grid{3}``)."""
+    return HelmholtzOracle3(k, epsinv, grid) if isinstance(grid, Grid3) else HelmholtzOracle(k, epsinv, grid)
+
+
+def _canonicalize(ev, k, epsinv, grid):
+    return (canonicalize_phase3 if isinstance(grid, Grid3) else canonicalize_phase)(ev, k, epsinv, grid)
 
 
 def solve_omega2(k, epsinv, grid, H0, tol=1e-8, maxiter=100, n_conv_check=None, seed=0):
@@ -12,7 +23,7 @@ def solve_omega2(k, epsinv, grid, H0, tol=1e-8, maxiter=100, n_conv_check=None, 
     ``H0`` is the ``(2N, nev)`` starting block (``ms.H⃗``).  Returns
     ``(omega2[nev], X (2N, nev), info)``; ``X`` is what the reference writes back to
     ``ms.H⃗``."""
-    op = HelmholtzOracle(k, epsinv, grid)
+    op = _helmholtz(k, epsinv, grid)
     res = lobpcg(op.M, np.array(H0, dtype=np.complex128), precon=op.P, tol=tol, maxiter=maxiter,
                  n_conv_check=n_conv_check, seed=seed)
     return res["lam"].copy(), res["X"].copy(), res
@@ -40,7 +51,7 @@ def solve_k_single(omega, epsinv, grid, state, eigind, nev, k_tol=1e-8, eig_tol=
         state["k"] = x
         ev = X[:, eigind].copy()
         f = lam[eigind] - w2
-        slope = 2.0 * HelmholtzOracle(x, epsinv, grid).HMkH(ev)
+        slope = 2.0 * _helmholtz(x, epsinv, grid).HMkH(ev)
         if abs(f) <= k_tol:
             return x, ev, nevals, iters
         x_new = x - f / slope
@@ -79,7 +90,7 @@ def solve_k(omega, epsinv, grid, nev=1, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=
                                                max_eigsolves, maxiter,
                                                n_conv_check=(eigind + 1) if lock_band_only else None)
         kmags.append(k)
-        evecs.append(canonicalize_phase(ev, k, epsinv, grid))
+        evecs.append(_canonicalize(ev, k, epsinv, grid))
         infos.append(dict(n_eigsolves=nsolves, lobpcg_iters=iters))
     if return_info:
         return np.array(kmags), evecs, infos
