@@ -82,6 +82,14 @@ const char* omx_status_string(int status);
    and eps^-1 (symmetric 3x3 per pixel), uploaded once.  Replaces the HelmholtzMap /
    ModeSolver construction (lib/MaxwellEigenmodes/src/maxwell.jl:922-972). */
 int omx_create(omx_solver** out, int device, int nx, int ny, double Dx, double Dy, const double* epsinv);
+/* Handle on a 3D cell periodic along z (Grid{3}: Dx x Dy x Dz um, Nx x Ny x Nz pixels; Dz is
+   the period Λ of lib/MaxwellEigenmodes/src/grads/period.jl:1-27).  eps^-1 is (3,3,Nx,Ny,Nz)
+   column-major, eigenvectors have 2 Nx Ny Nz complex entries, flat index
+   c + 2(ix + Nx(iy + Ny iz)).  The operator entry points, omx_solve_omega2, omx_solve_k (the
+   solve_k of solve_k_periodic, period.jl:144-148), omx_hmh and omx_group_index accept
+   such handles; the adjoint, ng_gvd and the EME mode sets return OMX_ERR_UNSUPPORTED. */
+int omx_create_3d(omx_solver** out, int device, int nx, int ny, int nz, double Dx, double Dy, double Dz,
+                  const double* epsinv);
 int omx_destroy(omx_solver* h);
 /* update_ε⁻¹ (maxwell.jl:1114-1136) */
 int omx_set_epsinv(omx_solver* h, const double* epsinv);
@@ -96,6 +104,14 @@ int omx_apply_preconditioner(omx_solver* h, double k, const double* H_in, double
                              int precision);
 /* HMₖH (maxwell.jl:267-275) per vector: out[j] = -Re<H_j, kx_ct(ifft(eps^-1 fft(zx_tc H_j)))> */
 int omx_hmkh(omx_solver* h, double k, const double* H, int m, double* out, int precision);
+
+/* Period sensitivity on a 3D handle, FP64, per mode j at k = kmags[j]:
+   dw2dk_out[j] = 2 HMₖH(ev_j) (∂ω²/∂k) and dw2dL_out[j] = ∂ω²_∂Λ(ev_j) = 2 HMₖH_weighted(ev_j,
+   g_z/Λ) (lib/MaxwellEigenmodes/src/grads/period.jl:69-74, :88-90).  With them the pull-back
+   of solve_k_periodic for cotangents on kmags is ω̄ = Σ 2ω k̄_j/∂ω²∂k_j and
+   Λ̄ = Σ k̄_j (-∂ω²∂Λ_j/∂ω²∂k_j) (period.jl:224-232). */
+int omx_period_derivatives(omx_solver* h, int nev, const double* kmags, const double* evecs, double* dw2dk_out,
+                           double* dw2dL_out);
 
 /* HMH (src/maxwell.jl:288-291) per vector, FP64: out[j] = Re<H_j, kx_ct(ifft(A fft(kx_tc H_j)))>
    with normalised ifft, for a symmetric per-pixel field A (3,3,Nx,Ny) column-major, or

@@ -170,6 +170,7 @@ __global__ __launch_bounds__(AB) void k_ct_combine(const c64* F1, const c64* F2,
 
 void Solver::adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
                      const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar) {
+  require_2d("the solve_k adjoint");
   if (nev < 1 || nev > 64) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..64");
   if (!kmags || !evecs || !omegabar || !epsinvbar) throw Error(OMX_ERR_INVALID_ARG, "NULL argument");
   if (!(omega > 0) || !std::isfinite(omega)) throw Error(OMX_ERR_INVALID_ARG, "omega must be positive and finite");
@@ -284,6 +285,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
 // E_out (optional): E = Ei . fft(kx_tc(ev)) per mode, (3,Nx,Ny) complex column-major.
 void Solver::ng_gvd(double omega, int nev, const double* kmags, const double* evecs, const double* deps,
                     const double* ddeps, double* ng_out, double* gvd_out, double* E_out) {
+  require_2d("ng_gvd");
   if (nev < 1 || nev > 64) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..64");
   if (!kmags || !evecs || !deps || !ddeps || !ng_out || !gvd_out) throw Error(OMX_ERR_INVALID_ARG, "NULL argument");
   if (!(omega > 0) || !std::isfinite(omega)) throw Error(OMX_ERR_INVALID_ARG, "omega must be positive and finite");

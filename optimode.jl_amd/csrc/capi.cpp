@@ -80,6 +80,29 @@ int omx_create(omx_solver** out, int device, int nx, int ny, double Dx, double D
   }
 }
 
+int omx_create_3d(omx_solver** out, int device, int nx, int ny, int nz, double Dx, double Dy, double Dz,
+                  const double* epsinv) {
+  if (!out) return OMX_ERR_INVALID_ARG;
+  *out = nullptr;
+  try {
+    auto h = std::make_unique<omx_solver>();
+    h->s = std::make_unique<omx::Solver>(device, nx, ny, nz, Dx, Dy, Dz, epsinv);
+    *out = h.release();
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  }
+}
+
+int omx_period_derivatives(omx_solver* h, int nev, const double* kmags, const double* evecs, double* dw2dk_out,
+                           double* dw2dL_out) {
+  return guarded(h, [&](omx::Solver& s) { s.period_derivatives(nev, kmags, evecs, dw2dk_out, dw2dL_out); });
+}
+
 int omx_destroy(omx_solver* h) {
   if (!h) return OMX_ERR_INVALID_ARG;
   delete h;

@@ -116,6 +116,7 @@ int grid_for(long n) { return int(std::min<long>(1024, (n + EB - 1) / EB)); }
 
 void Solver::build_modes(int slot, int nev, const double* kmags, const double* evecs, int conjugate, int normalize,
                          double* norms_out) {
+  require_2d("EME mode sets");
   if (slot < 0 || slot >= MODE_SLOTS) throw Error(OMX_ERR_INVALID_ARG, "mode slot must be in 0..3");
   if (nev < 1 || nev > OV_MAX) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..16");
   if (!kmags) throw Error(OMX_ERR_INVALID_ARG, "kmags is NULL");

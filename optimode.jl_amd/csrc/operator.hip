@@ -695,10 +695,15 @@ void launch_op(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const 
 template <typename T>
 void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
                      const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  if (plan.nz > 1) {
+    launch_operator3d<T>(op, plan, X, Y, D, sel, kinst, dot_partial, dps, stream);
+    return;
+  }
   switch (op) {
     case OP_M: launch_op<T, OP_M>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
     case OP_P: launch_op<T, OP_P>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
     case OP_MKH: launch_op<T, OP_MKH>(plan, X, Y, D, sel, kinst, dot_partial, dps, stream); break;
+    default: throw Error(OMX_ERR_UNSUPPORTED, "the period-weighted MkH needs a 3D cell");
   }
   OMX_HIP_CHECK(hipGetLastError());
 }
@@ -706,6 +711,10 @@ void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, c
 template <typename T>
 void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                    const double* kinst, double scale, hipStream_t stream) {
+  if (plan.nz > 1) {
+    launch_fields3d<T>(fop, plan, X, D, sel, kinst, scale, stream);
+    return;
+  }
   switch (plan.vpt) {
     case 8: launch_fields_vpt<T, 8>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
     case 16: launch_fields_vpt<T, 16>(fop, plan, X, D, sel, kinst, T(scale), stream); break;

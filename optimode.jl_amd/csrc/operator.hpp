@@ -25,7 +25,9 @@
 
 namespace omx {
 
-enum OpKind : int { OP_M = 0, OP_P = 1, OP_MKH = 2 };
+// OP_MKHW (3D cells only): MkH with the zx_tc field weighted plane wave by plane wave by
+// g_z / Dz, i.e. <H| dM/dDz |H> with Dz the period along z (grads/period.jl:69-74, :88-90)
+enum OpKind : int { OP_M = 0, OP_P = 1, OP_MKH = 2, OP_MKHW = 3 };
 
 template <typename T> struct Basis {
   T mag, m0, m1, m2, n0, n1, n2;
@@ -90,6 +92,14 @@ template <typename T> struct OpPlan {
   int vpt;                        // Stockham path: values per thread bound (8, 16, 32)
   int reg;                        // 1: the operator uses the register-FFT path
   int dot_blocks;                 // OP_MKH dot partials per vector (blocks of pass 3)
+  // 3D cells periodic along z (nz > 1; 2D: nz = 1): plane iz has g_z = fft_index(iz)/Dz,
+  // vectors are nz consecutive planes of 2 nx ny values, the intermediate field is
+  // [v][iz][iy][ix][c], eps6 is (nz, ny, nx, 6)
+  int nz;
+  double invdz;
+  FFTFactors fz;
+  const cplx<T>* twz;             // device: exp(-2 pi i t/nz)
+  int zcols_per_block;            // x-columns per block of the z pass
 };
 
 // Register path geometry: rows per block of passes 1 and 3 (the column-major intermediate
@@ -122,6 +132,15 @@ enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2, FIELD_DK = 3, FIE
 template <typename T>
 void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                    const double* kinst, double scale, hipStream_t stream);
+
+// 3D cells (plan.nz > 1): the same operator and the FIELD_E helper with 3D transforms;
+// launch_operator / launch_fields forward here.
+template <typename T>
+void launch_operator3d(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+                       const double* kinst, double* dot_partial, int dot_partial_stride, hipStream_t stream);
+template <typename T>
+void launch_fields3d(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
+                     const double* kinst, double scale, hipStream_t stream);
 
 // Host helper: factor L into radices {8,4,3,5,2} (no lone factor 2 unless L is 2*odd).
 bool fft_factor(int L, FFTFactors& f);

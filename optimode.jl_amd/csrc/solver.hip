@@ -64,32 +64,34 @@ Profiler::~Profiler() {
 template <> PrecisionData<float>& Solver::pd<float>() { return p32; }
 template <> PrecisionData<double>& Solver::pd<double>() { return p64; }
 
-// A symmetric per-pixel 3x3 field (column-major (3,3,Nx,Ny), FP64 host) in the two device
-// layouts the operator reads: 6 unique entries xx, yy, zz, xy, xz, yz per pixel, row-major
-// (iy, ix) and column-major (ix, iy).
+// A symmetric per-pixel 3x3 field (column-major (3,3,Nx,Ny) or (3,3,Nx,Ny,Nz), FP64 host) in
+// the two device layouts the operator reads: 6 unique entries xx, yy, zz, xy, xz, yz per
+// pixel, row-major (iz, iy, ix) and column-major per plane (iz, ix, iy).
 template <typename T>
 void upload_tensor6(const Solver& S, const double* E, DevMem& rowm, DevMem& colm) {
   const long N = S.N;
   std::vector<T> e6(size_t(6) * N);
-  for (int iy = 0; iy < S.ny; ++iy)
-    for (int ix = 0; ix < S.nx; ++ix) {
-      const long p = ix + long(S.nx) * iy;
-      const double* e = E + 9 * p;
-      T* o = e6.data() + 6 * p;
-      o[0] = T(e[0]);  // xx
-      o[1] = T(e[4]);  // yy
-      o[2] = T(e[8]);  // zz
-      o[3] = T(e[3]);  // (0,1)
-      o[4] = T(e[6]);  // (0,2)
-      o[5] = T(e[7]);  // (1,2)
-    }
+  for (long p = 0; p < N; ++p) {
+    const double* e = E + 9 * p;
+    T* o = e6.data() + 6 * p;
+    o[0] = T(e[0]);  // xx
+    o[1] = T(e[4]);  // yy
+    o[2] = T(e[8]);  // zz
+    o[3] = T(e[3]);  // (0,1)
+    o[4] = T(e[6]);  // (0,2)
+    o[5] = T(e[7]);  // (1,2)
+  }
   rowm.ensure(sizeof(T) * 6 * N);
   OMX_HIP_CHECK(hipMemcpy(rowm.as<void>(), e6.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
-  // column-major copy: pixel (ix, iy) at 6 (iy + ny ix)
+  // column-major copy per plane: pixel (ix, iy, iz) at 6 (iy + ny (ix + nx iz)) (the
+  // register path reads it on 2D cells)
   std::vector<T> e6c(size_t(6) * N);
-  for (int ix = 0; ix < S.nx; ++ix)
-    for (int iy = 0; iy < S.ny; ++iy)
-      for (int a = 0; a < 6; ++a) e6c[6 * (iy + size_t(S.ny) * ix) + a] = e6[6 * (ix + size_t(S.nx) * iy) + a];
+  const long nxy = long(S.nx) * S.ny;
+  for (int iz = 0; iz < S.nz; ++iz)
+    for (int ix = 0; ix < S.nx; ++ix)
+      for (int iy = 0; iy < S.ny; ++iy)
+        for (int a = 0; a < 6; ++a)
+          e6c[6 * (iy + size_t(S.ny) * ix + nxy * iz) + a] = e6[6 * (ix + size_t(S.nx) * iy + nxy * iz) + a];
   colm.ensure(sizeof(T) * 6 * N);
   OMX_HIP_CHECK(hipMemcpy(colm.as<void>(), e6c.data(), sizeof(T) * 6 * N, hipMemcpyHostToDevice));
 }
@@ -168,6 +170,18 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
     }
     return tw;
   };
+  pl.nz = S.nz;
+  pl.invdz = S.nz > 1 ? 1.0 / S.Dz : 0.0;
+  pl.twz = nullptr;
+  pl.zcols_per_block = 1;
+  if (S.nz > 1) {
+    if (!fft_factor(S.nz, pl.fz))
+      throw Error(OMX_ERR_BAD_SHAPE, "grid sizes must factor into 2, 3 and 5 (got Nz = " + std::to_string(S.nz) + ")");
+    auto twz = twiddles(S.nz);
+    P.twz.alloc(sizeof(cplx<T>) * S.nz);
+    OMX_HIP_CHECK(hipMemcpy(P.twz.template as<void>(), twz.data(), sizeof(cplx<T>) * S.nz, hipMemcpyHostToDevice));
+    pl.twz = P.twz.template as<cplx<T>>();
+  }
   auto twx = twiddles(S.nx), twy = twiddles(S.ny);
   P.twx.alloc(sizeof(cplx<T>) * S.nx);
   P.twy.alloc(sizeof(cplx<T>) * S.ny);
@@ -191,15 +205,25 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
       break;
     }
   }
+  if (S.nz > 1)
+    for (int tz : {16, 8, 4, 2}) {
+      if (S.nx % tz == 0 && 3 * tz * S.nz <= budget) {
+        pl.zcols_per_block = tz;
+        break;
+      }
+    }
   const int v1 = choose_vpt(3 * pl.rows_per_block, pl.fx);
   const int v2 = choose_vpt(3 * pl.cols_per_block, pl.fy);
-  if (v1 < 0 || v2 < 0) throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
-  pl.vpt = std::max(v1, v2);
+  const int v3 = S.nz > 1 ? choose_vpt(3 * pl.zcols_per_block, pl.fz) : 8;
+  if (v1 < 0 || v2 < 0 || v3 < 0) throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
+  pl.vpt = std::max({v1, v2, v3});
   const size_t lds13 = sizeof(cplx<T>) * (size_t(S.nx) + 3 * size_t(pl.rows_per_block) * S.nx);
   const size_t lds2 = sizeof(cplx<T>) * (size_t(S.ny) + 3 * size_t(pl.cols_per_block) * S.ny);
-  if (lds13 > 160 * 1024 || lds2 > 160 * 1024) throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
-  // register-FFT operator for power-of-two sizes 64 .. 512
-  P.reg_ok = (reg_fft_ok(S.nx) && reg_fft_ok(S.ny) && S.ny % reg_rows<T>() == 0 &&
+  const size_t lds3 = sizeof(cplx<T>) * (size_t(S.nz) + 3 * size_t(pl.zcols_per_block) * S.nz);
+  if (lds13 > 160 * 1024 || lds2 > 160 * 1024 || lds3 > 160 * 1024)
+    throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
+  // register-FFT operator for power-of-two sizes 64 .. 512 (2D cells)
+  P.reg_ok = (S.nz == 1 && reg_fft_ok(S.nx) && reg_fft_ok(S.ny) && S.ny % reg_rows<T>() == 0 &&
               S.nx % (REG_COL_THREADS / (S.ny / 8)) == 0)
                  ? 1
                  : 0;
@@ -211,7 +235,7 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
 template <typename T> void select_operator(Solver& S, PrecisionData<T>& P) {
   OpPlan<T>& pl = P.plan;
   pl.reg = (P.reg_ok && S.opt.reg_ops) ? 1 : 0;
-  pl.dot_blocks = S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block);
+  pl.dot_blocks = S.nz * (S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block));
 }
 
 template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
@@ -544,7 +568,10 @@ template <typename T> class Batch {
 // ---------------------------------------------------------------------------------------
 
 Solver::Solver(int dev, int nx_, int ny_, double Dx_, double Dy_, const double* epsinv)
-    : device(dev), nx(nx_), ny(ny_), Dx(Dx_), Dy(Dy_) {
+    : Solver(dev, nx_, ny_, 1, Dx_, Dy_, 0.0, epsinv) {}
+
+Solver::Solver(int dev, int nx_, int ny_, int nz_, double Dx_, double Dy_, double Dz_, const double* epsinv)
+    : device(dev), nx(nx_), ny(ny_), Dx(Dx_), Dy(Dy_), nz(nz_), Dz(Dz_) {
   const char* tr = std::getenv("OMX_TRACE");
   trace = tr && tr[0] && tr[0] != '0';
   int count = 0;
@@ -555,11 +582,13 @@ Solver::Solver(int dev, int nx_, int ny_, double Dx_, double Dy_, const double* 
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     throw Error(OMX_ERR_NO_DEVICE, std::string("omx is built for gfx950, device is ") + prop.gcnArchName);
   if (nx < 2 || ny < 2) throw Error(OMX_ERR_BAD_SHAPE, "grid must be at least 2 x 2");
+  if (nz < 1) throw Error(OMX_ERR_BAD_SHAPE, "Nz must be positive");
   if (!(Dx > 0) || !(Dy > 0) || !std::isfinite(Dx) || !std::isfinite(Dy))
     throw Error(OMX_ERR_INVALID_ARG, "cell size must be positive and finite");
+  if (nz > 1 && !(Dz > 0 && std::isfinite(Dz))) throw Error(OMX_ERR_INVALID_ARG, "the period Dz must be positive");
   OMX_HIP_CHECK(hipSetDevice(dev));
   OMX_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  N = long(nx) * ny;
+  N = long(nx) * ny * nz;
   K = 2 * N;
   ld = (K + 63) / 64 * 64;
   build_plan<float>(*this, p32);
@@ -640,7 +669,8 @@ void Solver::set_option(const std::string& key, double v) {
 // Host FP64 blocks (m vectors of K complex, contiguous) -> operator -> host.
 void Solver::apply(OpKind op, int prec, double k, const double* Hin, double* Hout, int m, double* dots) {
   if (m <= 0) throw Error(OMX_ERR_INVALID_ARG, "m must be positive");
-  if (!Hin || (op != OP_MKH && !Hout) || (op == OP_MKH && !dots)) throw Error(OMX_ERR_INVALID_ARG, "NULL buffer");
+  const bool dot = op == OP_MKH || op == OP_MKHW;
+  if (!Hin || (!dot && !Hout) || (dot && !dots)) throw Error(OMX_ERR_INVALID_ARG, "NULL buffer");
   if (!std::isfinite(k)) throw Error(OMX_ERR_INVALID_ARG, "k must be finite");
   OMX_HIP_CHECK(hipSetDevice(device));
   const size_t nbytes64 = sizeof(c64) * size_t(m) * K;
@@ -662,12 +692,12 @@ void Solver::apply(OpKind op, int prec, double k, const double* Hin, double* Hou
     launch_convert<double, float>(scratch_a.as<c64>(), K, 0, x32.as<c32>(), K, 0, 1, m, K, stream);
     BView<float> X{x32.as<c32>(), K, 0}, Y{y32.as<c32>(), K, 0};
     launch_operator<float>(op, p32.plan, X, Y, Dm.as<c32>(), sel, kbuf.as<double>(), dotbuf.as<double>(), dps, stream);
-    if (op != OP_MKH) launch_convert<float, double>(y32.as<c32>(), K, 0, scratch_b.as<c64>(), K, 0, 1, m, K, stream);
+    if (!dot) launch_convert<float, double>(y32.as<c32>(), K, 0, scratch_b.as<c64>(), K, 0, 1, m, K, stream);
     OMX_HIP_CHECK(hipStreamSynchronize(stream));
   } else {
     throw Error(OMX_ERR_INVALID_ARG, "precision must be OMX_PREC_FP32 or OMX_PREC_FP64");
   }
-  if (op == OP_MKH) {
+  if (dot) {
     std::vector<double> h(size_t(m) * dps);
     OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotbuf.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
     OMX_HIP_CHECK(hipStreamSynchronize(stream));
@@ -1458,6 +1488,27 @@ void Solver::get_evecs(double* evecs) {
     OMX_HIP_CHECK(hipMemcpyAsync(evecs + 2 * size_t(j) * K, last_evecs.as<c64>() + size_t(j) * ld, sizeof(c64) * K,
                                  hipMemcpyDeviceToHost, stream));
   OMX_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Solver::require_2d(const char* what) const {
+  if (nz > 1) throw Error(OMX_ERR_UNSUPPORTED, std::string(what) + " is built for 2D cells only");
+}
+
+// Period sensitivity on a 3D cell (lib/MaxwellEigenmodes/src/grads/period.jl:88-90, with
+// HMkH_weighted :69-74): per mode j at its own k_j, dw2dk[j] = 2 HMkH(ev_j) and
+// dw2dL[j] = 2 HMkH_weighted(ev_j, g_z / Dz), both from the FP64 operator on the GPU.
+void Solver::period_derivatives(int nev, const double* kmags, const double* evecs, double* dw2dk, double* dw2dL) {
+  if (nz <= 1) throw Error(OMX_ERR_UNSUPPORTED, "period derivatives need a 3D cell (Nz > 1)");
+  if (nev < 1 || !kmags || !evecs || !dw2dk || !dw2dL) throw Error(OMX_ERR_INVALID_ARG, "bad period_derivatives arguments");
+  for (int j = 0; j < nev; ++j) {
+    if (!std::isfinite(kmags[j])) throw Error(OMX_ERR_INVALID_ARG, "k must be finite");
+    const double* ev = evecs + 2 * size_t(j) * K;
+    double hk = 0, hl = 0;
+    apply(OP_MKH, OMX_PREC_FP64, kmags[j], ev, nullptr, 1, &hk);
+    apply(OP_MKHW, OMX_PREC_FP64, kmags[j], ev, nullptr, 1, &hl);
+    dw2dk[j] = 2.0 * hk;
+    dw2dL[j] = 2.0 * hl;
+  }
 }
 
 }  // namespace omx

@@ -58,7 +58,7 @@ class DevMem {
 
 template <typename T> struct PrecisionData {
   OpPlan<T> plan{};
-  DevMem eps6, eps6c, twx, twy;
+  DevMem eps6, eps6c, twx, twy, twz;
   int reg_ok = 0;  // the grid admits the register-FFT operator
 };
 
@@ -166,6 +166,8 @@ struct ModeSlot {
 class Solver {
  public:
   Solver(int device, int nx, int ny, double Dx, double Dy, const double* epsinv);
+  // 3D cell periodic along z (Grid{3}): nz planes, period Dz; epsinv is (3,3,Nx,Ny,Nz)
+  Solver(int device, int nx, int ny, int nz, double Dx, double Dy, double Dz, const double* epsinv);
   ~Solver();
 
   void set_epsinv(const double* epsinv);
@@ -186,6 +188,9 @@ class Solver {
   void overlap(int slot_a, int slot_b, int conjugate, double* out);
   void get_mode_fields(int slot, double* E_out, double* H_out);
   void set_option(const std::string& key, double value);
+  // per mode j: dw2dk[j] = 2 HMkH(ev_j) and dw2dL[j] = 2 <ev_j| dM/dDz |ev_j> (3D cells)
+  void period_derivatives(int nev, const double* kmags, const double* evecs, double* dw2dk, double* dw2dL);
+  void require_2d(const char* what) const;
 
   std::mutex mu;
   std::string last_error;
@@ -195,6 +200,8 @@ class Solver {
   hipStream_t stream = nullptr;
   int nx, ny;
   double Dx, Dy;
+  int nz = 1;       // planes along z (1: a 2D cell)
+  double Dz = 0.0;  // period along z (3D cells)
   long N, K, ld;
   PrecisionData<float> p32;
   PrecisionData<double> p64;

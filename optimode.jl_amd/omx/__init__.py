@@ -25,8 +25,9 @@ import numpy as np
 from . import _lib
 from ._lib import OmxError, OmxInfo, OMX_PREC_FP32, OMX_PREC_FP64  # noqa: F401
 
-__all__ = ["Grid", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "solve_k_batch", "rrule_solve_k",
-           "group_index", "ng_gvd", "k_guess", "OmxError", "library_path", "exported_symbols", "kernel_stats"]
+__all__ = ["Grid", "Grid3", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "solve_k_batch", "rrule_solve_k",
+           "group_index", "ng_gvd", "k_guess", "solve_k_periodic", "period_derivatives", "rrule_solve_k_periodic",
+           "OmxError", "library_path", "exported_symbols", "kernel_stats"]
 
 
 def library_path():
@@ -51,6 +52,29 @@ class Grid:
         return f"Grid({self.Dx}, {self.Dy}, {self.Nx}, {self.Ny})"
 
 
+class Grid3:
+    """3D cell ``Dx x Dy x Dz`` (um) on ``Nx x Ny x Nz`` pixels, periodic along z with period
+    ``Dz`` (``Grid{3}``, lib/DielectricSmoothing/src/grid.jl:46-71)."""
+
+    def __init__(self, Dx, Dy, Dz, Nx, Ny, Nz):
+        self.Dx, self.Dy, self.Dz = float(Dx), float(Dy), float(Dz)
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+
+    @property
+    def N(self):
+        return self.Nx * self.Ny * self.Nz
+
+    def key(self):
+        return (self.Nx, self.Ny, self.Nz, self.Dx, self.Dy, self.Dz)
+
+    def __repr__(self):
+        return f"Grid3({self.Dx}, {self.Dy}, {self.Dz}, {self.Nx}, {self.Ny}, {self.Nz})"
+
+
+def _grid_key(grid):
+    return grid.key() if isinstance(grid, Grid3) else (grid.Nx, grid.Ny, 1, grid.Dx, grid.Dy, 0.0)
+
+
 def k_guess(omega, epsinv):
     """``k_guess`` (src/MaxwellEigenmodes.jl:63)."""
     return float(omega) * np.sqrt(1.0 / min(np.min(epsinv[a, a]) for a in range(3)))
@@ -58,10 +82,11 @@ def k_guess(omega, epsinv):
 
 def _epsinv_c(epsinv, grid):
     e = np.asarray(epsinv, dtype=np.float64)
-    if e.shape != (3, 3, grid.Nx, grid.Ny):
-        raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"epsinv must have shape (3,3,{grid.Nx},{grid.Ny}), got {e.shape}")
-    # column-major (3,3,Nx,Ny) flat order: a + 3b + 9(ix + Nx iy)
-    return np.ascontiguousarray(e.transpose(3, 2, 1, 0))
+    shape = (3, 3, grid.Nx, grid.Ny) + ((grid.Nz,) if isinstance(grid, Grid3) else ())
+    if e.shape != shape:
+        raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"epsinv must have shape {shape}, got {e.shape}")
+    # column-major flat order: a + 3b + 9(ix + Nx iy [+ Nx Ny iz])
+    return np.ascontiguousarray(e.transpose(tuple(range(e.ndim - 1, -1, -1))))
 
 
 class _Handle:
@@ -70,7 +95,11 @@ class _Handle:
     def __init__(self, device, grid, epsinv_c):
         L = _lib.lib()
         h = ctypes.c_void_p()
-        st = L.omx_create(ctypes.byref(h), int(device), grid.Nx, grid.Ny, grid.Dx, grid.Dy, _lib.dptr(epsinv_c))
+        if isinstance(grid, Grid3):
+            st = L.omx_create_3d(ctypes.byref(h), int(device), grid.Nx, grid.Ny, grid.Nz, grid.Dx, grid.Dy, grid.Dz,
+                                 _lib.dptr(epsinv_c))
+        else:
+            st = L.omx_create(ctypes.byref(h), int(device), grid.Nx, grid.Ny, grid.Dx, grid.Dy, _lib.dptr(epsinv_c))
         if st != _lib.OMX_OK:
             raise OmxError(st, L.omx_last_error(None).decode() or L.omx_status_string(st).decode())
         self.h = h
@@ -133,7 +162,7 @@ class HIPSolver:
 
     def handle(self, grid, epsinv):
         h = self._h
-        same_grid = h is not None and (h.grid.Nx, h.grid.Ny, h.grid.Dx, h.grid.Dy) == (grid.Nx, grid.Ny, grid.Dx, grid.Dy)
+        same_grid = h is not None and _grid_key(h.grid) == _grid_key(grid)
         if not same_grid:
             ec = _epsinv_c(epsinv, grid)
             if h is not None:
@@ -392,5 +421,55 @@ def rrule_solve_k(omega, epsinv, grid, solver, nev=1, **kw):
                                             _lib.dptr(eibar_c))
         _lib.check(st, h.h)
         return (None, float(wbar[0]), np.ascontiguousarray(eibar_c.transpose(3, 2, 1, 0)), None, None)
+
+    return (kmags, evecs), pullback
+
+
+def _grid_with_period(grid, L):
+    """``_grid_with_period`` (lib/MaxwellEigenmodes/src/grads/period.jl:123-125)."""
+    return Grid3(grid.Dx, grid.Dy, float(L), grid.Nx, grid.Ny, grid.Nz)
+
+
+def solve_k_periodic(omega, epsinv, L, grid, solver, nev=1, **kw):
+    """``solve_k_periodic(ω, ε⁻¹, Λ, grid, solver; nev, ...)``
+    (lib/MaxwellEigenmodes/src/grads/period.jl:144-148): ``solve_k`` on the 3D cell with its
+    period set to ``Λ`` (ε⁻¹ held fixed in index space)."""
+    if not isinstance(grid, Grid3):
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "solve_k_periodic needs a Grid3")
+    return solve_k(omega, epsinv, _grid_with_period(grid, L), solver, nev=nev, **kw)
+
+
+def period_derivatives(kmags, evecs, epsinv, grid, solver):
+    """Per mode: ``(∂ω²/∂k, ∂ω²/∂Λ)`` = ``(2 HMₖH(ev), ∂ω²_∂Λ(ev))`` on a 3D cell
+    (lib/MaxwellEigenmodes/src/grads/period.jl:88-90), FP64 on the GPU."""
+    if not isinstance(grid, Grid3):
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "period derivatives need a Grid3")
+    h = solver.handle(grid, epsinv)
+    ks = np.atleast_1d(np.asarray(kmags, dtype=np.float64))
+    E = _cols(np.stack(list(evecs), 1), 2 * grid.N)
+    dk, dL = np.empty(len(ks)), np.empty(len(ks))
+    _lib.check(_lib.lib().omx_period_derivatives(h.h, len(ks), _lib.dptr(ks), _lib.dptr(E), _lib.dptr(dk),
+                                                 _lib.dptr(dL)), h.h)
+    return dk, dL
+
+
+def rrule_solve_k_periodic(omega, epsinv, L, grid, solver, nev=1, **kw):
+    """``rrule(solve_k_periodic, ω, ε⁻¹, Λ, grid, solver; nev)`` (period.jl:236-249) ->
+    ``((kmags, evecs), pullback)``.  The pull-back maps cotangents on ``kmags`` to
+    ``(ω̄, Λ̄)`` (the ``ēvecs = Zero`` branch of ``_solve_k_period_grads``, period.jl:182-234:
+    ``ω̄ = Σ 2ω k̄/∂ω²∂k``, ``Λ̄ = Σ k̄ (-∂ω²∂Λ/∂ω²∂k)``).  Eigenvector cotangents and ε̄⁻¹ on
+    3D cells are not built: a non-zero ``ēvecs`` raises."""
+    g = _grid_with_period(grid, L)
+    kmags, evecs = solve_k(omega, epsinv, g, solver, nev=nev, **kw)
+
+    def pullback(cot):
+        kbar, evbar = cot
+        if evbar is not None and any(v is not None and np.any(v) for v in evbar):
+            raise OmxError(_lib.OMX_ERR_UNSUPPORTED, "eigenvector cotangents on 3D cells are not supported")
+        kb = np.zeros(nev) if kbar is None else np.array([0.0 if v is None else float(v) for v in kbar])
+        dk, dL = period_derivatives(kmags, evecs, epsinv, g, solver)
+        wbar = float(np.sum(2.0 * float(omega) * kb / dk))
+        Lbar = float(np.sum(kb * (-dL / dk)))
+        return wbar, Lbar
 
     return (kmags, evecs), pullback

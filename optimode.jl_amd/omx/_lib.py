@@ -66,6 +66,9 @@ SIGNATURES = [
     ("omx_status_string", ctypes.c_char_p, [ctypes.c_int]),
     ("omx_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double, _dp]),
+    ("omx_create_3d", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
+    ("omx_period_derivatives", ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
     ("omx_destroy", ctypes.c_int, [_vp]),
     ("omx_set_epsinv", ctypes.c_int, [_vp, _dp]),
     ("omx_last_error", ctypes.c_char_p, [_vp]),
