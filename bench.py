@@ -132,6 +132,106 @@ def eme_workload(a):
     print(json.dumps(out), flush=True)
 
 
+PER_GRID = (4.0, 3.0, 0.45, 128, 96, 24)   # Dx, Dy, period Dz (um), Nx, Ny, Nz
+PER_OMEGA = 0.5
+PER_NEV = 4
+
+
+def periodic_cpu_baseline(ncores):
+    """The FP64 oracle on the host: the fundamental Bloch mode of the same 3D cell."""
+    os.environ["ORACLE_FFT_WORKERS"] = str(ncores)
+    import oracle as O  # the checker: timed here as the CPU baseline only
+    from oracle import periodic as P
+    from omx import geometry as G
+    g = P.Grid3(*PER_GRID)
+    ei = G.bragg_grating(g)
+    t = time.perf_counter()
+    km, _ = O.solve_k(PER_OMEGA, ei, g, nev=1, k_tol=1e-10, eig_tol=1e-8, maxiter=300)
+    dt = time.perf_counter() - t
+    return {"value": 1.0 / dt, "unit": "modes/s", "cores": ncores, "kind": "port",
+            "sample": f"FP64 NumPy oracle solve_k on the same 128x96x24 cell, nev=1 (fundamental), eig_tol 1e-8, "
+                      f"k_tol 1e-10: {dt:.1f} s for 1 mode (k={km[0]:.8f})"}
+
+
+def periodic_workload(a):
+    """solve_k on one period of a 3D Bragg-grating waveguide (128 x 96 x 24 voxels, period
+    0.45 um, 4 Bloch modes): the z-periodic path (five-kernel 3D operator).  Prints one JSON
+    line with modes/s, the FP32 3D operator's roofline and the CPU baseline."""
+    import numpy as np
+    import omx
+    from omx import geometry as G
+
+    grid = omx.Grid3(*PER_GRID)
+    ei = G.bragg_grating(grid)
+    solver = omx.HIPSolver(precision=a.precision, profile=True)
+    kw = dict(nev=PER_NEV, k_tol=1e-10, eig_tol=1e-8, max_eigsolves=60, maxiter=300, fetch_evecs=False,
+              return_info=True)
+    for _ in range(a.warmup):
+        omx.solve_k(PER_OMEGA, ei, grid, solver, **kw)
+    h = solver.handle(grid, ei)
+    omx._lib.check(omx._lib.lib().omx_set_option(h.h, b"profile", 1.0), h.h)
+    infos, kms = [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        km, _, info = omx.solve_k(PER_OMEGA, ei, grid, solver, **kw)
+        kms.append(km)
+        infos.append(info)
+    elapsed = time.perf_counter() - t0
+    op_ms, op_n, op_bytes = omx.kernel_stats(solver, "op_fp32" if a.precision == "mixed" else "op_fp64")
+    # accuracy: against a cold FP64 solve at tighter tolerances, |Δω²| = slope |Δk|
+    ref = omx.HIPSolver(precision="fp64")
+    kr, evr = omx.solve_k(PER_OMEGA, ei, grid, ref, nev=PER_NEV, k_tol=1e-12, eig_tol=1e-10, maxiter=300)
+    dk, dL = omx.period_derivatives(kr, evr, ei, grid, ref)
+    errs = [float(np.max(np.abs(dk * (km - kr)) / PER_OMEGA ** 2)) for km in kms]
+    conv = sum(int(np.sum(np.abs(dk * (km - kr)) / PER_OMEGA ** 2 <= REL_TOL)) for km in kms)
+    roofline = None
+    if op_n > 0 and op_ms > 0:
+        achieved = op_bytes / (op_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "3D Helmholtz operator application (rows, columns, z-pencils, columns, rows)",
+                    "launches": op_n, "avg_us": round(1e3 * op_ms / op_n, 2), "avg_alg_bytes": round(op_bytes / op_n)}
+    ncores = 1
+    try:
+        ncores = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        ncores = max(1, min(16, os.cpu_count() or 1))
+    out = {
+        "metric": "converged Bloch modes/s, 3D Bragg-grating waveguide 128x96x24 (omega^2 rel err <=1e-8)",
+        "value": round(conv / elapsed, 3),
+        "unit": "modes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if a.precision == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "solve_k_periodic: one period (0.45 um) of a Gaussian-core Bragg grating (eps 4.0 core, "
+                        "2.1 background, width 1.0 um modulated +-50%, height 0.4 um), 4x3 um cross-section, "
+                        "128x96x24 voxels, omega=0.5, 4 modes, k_tol 1e-10, eig_tol 1e-8",
+            "grid": list(PER_GRID[3:]),
+            "nev": PER_NEV,
+            "precision": a.precision,
+            "kmags": [float(x) for x in kr],
+            "dw2_dperiod": [float(x) for x in dL],
+            "modes_converged": conv,
+            "modes_total": PER_NEV * a.steps,
+            "max_rel_err_omega2_vs_fp64": max(errs),
+            "lobpcg_iters_fp32_per_step": float(np.mean([i["lobpcg_iters_fp32"] for i in infos])),
+            "fp64_rr_steps_per_step": float(np.mean([i["lobpcg_iters_fp64"] for i in infos])),
+            "newton_steps_per_step": float(np.mean([i["n_newton"] for i in infos])),
+            "matvecs_per_step": float(np.mean([i["n_matvec"] for i in infos])),
+        },
+        "roofline": roofline,
+        "cpu_baseline": None if a.no_cpu_baseline else periodic_cpu_baseline(ncores),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def batch_problems(seed=3, n=64):
     """The mixed-grid batch: seeded ridges (w ~ U(0.8, 2.5) um, h ~ U(0.3, 0.9) um,
     sidewall ~ U(60, 90) deg, core eps ~ U(3.5, 4.9)) on SiO2 under air, grids uniform in
@@ -211,9 +311,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-algorithm comparison solve")
-    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "batch"],
+    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "periodic", "batch"],
                     help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain; "
-                         "batch: the 64-instance mixed-grid batch")
+                         "periodic: solve_k on a 128x96x24 3D Bragg-grating cell; batch: the 64-instance "
+                         "mixed-grid batch")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every solve here fits one MI355X and "
@@ -221,6 +322,9 @@ def main():
         sys.exit(2)
     if a.workload == "eme":
         eme_workload(a)
+        return
+    if a.workload == "periodic":
+        periodic_workload(a)
         return
     if a.workload == "batch":
         # The first run of this workload ended in a GPU memory fault (dispatch dump:

@@ -214,3 +214,20 @@ def coupler_slice(grid, gap, w1, w2, h=0.8, eps_core=EPS_SI3N4, eps_clad=EPS_SIO
     x2 = gap / 2 + w2 / 2
     return smooth_epsinv([Rect(x1, 0.0, w1, h, eps_core), Rect(x2, 0.0, w2, h, eps_core)], eps_clad, grid, nsub,
                          backend)
+
+
+def bragg_grating(grid, eps_core=4.0, eps_bg=2.1, w0=1.0, h0=0.4, mod=0.5):
+    """ε⁻¹ ``(3,3,Nx,Ny,Nz)`` of one period of a Bragg-grating waveguide on a ``Grid3``: an
+    isotropic Gaussian index core ``ε = ε_bg + (ε_core - ε_bg) exp(-(x²/w(z)² + y²/h0²))``
+    whose width is modulated along the period, ``w(z) = w0 (1 + mod cos(2π z/Δz))``, on
+    the pixel centres ``-D/2 + i D/N`` of each axis (grid.jl:108-127)."""
+    xs = -grid.Dx / 2 + np.arange(grid.Nx) * (grid.Dx / grid.Nx)
+    ys = -grid.Dy / 2 + np.arange(grid.Ny) * (grid.Dy / grid.Ny)
+    zs = -grid.Dz / 2 + np.arange(grid.Nz) * (grid.Dz / grid.Nz)
+    w = w0 * (1.0 + mod * np.cos(2 * np.pi * zs / grid.Dz))
+    eps = eps_bg + (eps_core - eps_bg) * np.exp(-(xs[:, None, None] ** 2 / w[None, None, :] ** 2
+                                                  + ys[None, :, None] ** 2 / h0 ** 2))
+    out = np.zeros((3, 3, grid.Nx, grid.Ny, grid.Nz))
+    for a in range(3):
+        out[a, a] = 1.0 / eps
+    return out
