@@ -5,6 +5,7 @@
 #   solve_ω²   lib/MaxwellEigenmodes/src/solve.jl:38-63, solvers/gpu.jl:201-211
 #   solve_k    lib/MaxwellEigenmodes/src/solve.jl:182-196, solvers/gpu.jl:213-242
 #   rrule      lib/MaxwellEigenmodes/src/solve.jl:213-289, solvers/gpu.jl:292-365
+#   3D cells   solve_k on Grid{3}, rrule(solve_k_periodic) for k cotangents (src/grads/period.jl:144-249)
 # Host data stays Julia-owned Float64 / ComplexF64 column-major arrays; the library copies
 # them to the device.  Non-zero statuses become `error(...)` with the library's message.
 module MaxwellEigenmodesHIPExt
@@ -14,9 +15,10 @@ using Logging: AbstractLogger, NullLogger
 using ChainRulesCore
 using ChainRulesCore: NoTangent, AbstractZero
 using DielectricSmoothing: Grid, N
-import MaxwellEigenmodes: solve_ω², solve_k, AbstractEigensolver, ModeSolver
+import MaxwellEigenmodes: solve_ω², solve_k, solve_k_periodic, AbstractEigensolver, ModeSolver
 
-export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E, hip_build_modes, hip_overlap_matrix
+export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E, hip_build_modes, hip_overlap_matrix,
+       hip_period_derivatives
 
 const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
 
@@ -44,12 +46,12 @@ mutable struct HIPSolver{L<:AbstractLogger} <: AbstractEigensolver{L}
     device::Int
     precision::Symbol
     handle::Ptr{Cvoid}
-    key::Tuple{Int,Int,Float64,Float64}
+    key::Tuple{Int,Int,Int,Float64,Float64,Float64}
 end
 
 function HIPSolver(; device::Integer=0, precision::Symbol=:mixed, logger::L=NullLogger()) where {L}
     precision in (:mixed, :fp64) || error("HIPSolver precision must be :mixed or :fp64")
-    s = HIPSolver{L}(logger, Int(device), precision, C_NULL, (0, 0, 0.0, 0.0))
+    s = HIPSolver{L}(logger, Int(device), precision, C_NULL, (0, 0, 0, 0.0, 0.0, 0.0))
     finalizer(s) do x
         x.handle == C_NULL || ccall((:omx_destroy, libomx), Cint, (Ptr{Cvoid},), x.handle)
         x.handle = C_NULL
@@ -63,13 +65,33 @@ _check(st, h) = st == 0 || error("HIPSolver (omx status $st): " * _lasterr(h))
 # one handle per (grid, device); ε⁻¹ is re-uploaded when it changes (update_ε⁻¹)
 function _handle!(s::HIPSolver, grid::Grid{2}, ε⁻¹::AbstractArray{<:Real,4})
     ei = Array{Float64,4}(ε⁻¹)
-    key = (grid.Nx, grid.Ny, Float64(grid.Δx), Float64(grid.Δy))
+    key = (grid.Nx, grid.Ny, 1, Float64(grid.Δx), Float64(grid.Δy), 0.0)
     if s.handle == C_NULL || s.key != key
         s.handle == C_NULL || ccall((:omx_destroy, libomx), Cint, (Ptr{Cvoid},), s.handle)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         st = ccall((:omx_create, libomx), Cint,
                    (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Cdouble, Cdouble, Ptr{Cdouble}),
                    h, s.device, grid.Nx, grid.Ny, grid.Δx, grid.Δy, ei)
+        _check(st, C_NULL)
+        s.handle, s.key = h[], key
+    else
+        _check(ccall((:omx_set_epsinv, libomx), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), s.handle, ei), s.handle)
+    end
+    _check(ccall((:omx_set_option, libomx), Cint, (Ptr{Cvoid}, Cstring, Cdouble),
+                 s.handle, "fp64_only", s.precision === :fp64 ? 1.0 : 0.0), s.handle)
+    return s.handle
+end
+
+# a 3D cell periodic along z (Grid{3}; Δz is the period Λ of grads/period.jl)
+function _handle!(s::HIPSolver, grid::Grid{3}, ε⁻¹::AbstractArray{<:Real,5})
+    ei = Array{Float64,5}(ε⁻¹)
+    key = (grid.Nx, grid.Ny, grid.Nz, Float64(grid.Δx), Float64(grid.Δy), Float64(grid.Δz))
+    if s.handle == C_NULL || s.key != key
+        s.handle == C_NULL || ccall((:omx_destroy, libomx), Cint, (Ptr{Cvoid},), s.handle)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:omx_create_3d, libomx), Cint,
+                   (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Cint, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}),
+                   h, s.device, grid.Nx, grid.Ny, grid.Nz, grid.Δx, grid.Δy, grid.Δz, ei)
         _check(st, C_NULL)
         s.handle, s.key = h[], key
     else
@@ -95,7 +117,8 @@ function solve_ω²(ms::ModeSolver{2,T}, solver::HIPSolver; nev=1, maxiter=100, 
     return ω², [H[:, j] for j in 1:nev]
 end
 
-function solve_k(ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64}, solver::HIPSolver; nev=1,
+function solve_k(ω::Real, ε⁻¹::Union{AbstractArray{Float64,4},AbstractArray{Float64,5}},
+                 grid::Union{Grid{2,Float64},Grid{3,Float64}}, solver::HIPSolver; nev=1,
                  max_eigsolves=60, maxiter=100, k_tol=1e-8, eig_tol=1e-8, log=false, kguess=nothing,
                  Hguess=nothing, f_filter=nothing, overwrite=false)
     h = _handle!(solver, grid, ε⁻¹)
@@ -154,6 +177,50 @@ function ChainRulesCore.rrule(::typeof(solve_k), ω::Real, ε⁻¹::AbstractArra
         return (NoTangent(), ω̄[], ε̄⁻¹, NoTangent(), NoTangent())
     end
     return (kmags, evecs), hip_solve_k_pullback
+end
+
+"""
+    hip_period_derivatives(solver, kmags, evecs, ε⁻¹, grid::Grid{3}) -> (∂ω²∂k, ∂ω²∂Λ)
+
+Per mode, `2 HMₖH(ev)` and `∂ω²_∂Λ(ev)` (src/grads/period.jl:69-90) on the GPU.
+"""
+function hip_period_derivatives(solver::HIPSolver, kmags::AbstractVector{<:Real}, evecs,
+                                ε⁻¹::AbstractArray{Float64,5}, grid::Grid{3,Float64})
+    h = _handle!(solver, grid, ε⁻¹)
+    km = Vector{Float64}(kmags)
+    E = reduce(hcat, [Vector{ComplexF64}(v) for v in evecs])
+    dk = Vector{Float64}(undef, length(km))
+    dL = Vector{Float64}(undef, length(km))
+    st = ccall((:omx_period_derivatives, libomx), Cint,
+               (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Ptr{Cdouble}, Ptr{Cdouble}),
+               h, length(km), km, E, dk, dL)
+    _check(st, h)
+    return dk, dL
+end
+
+# rrule(solve_k_periodic, ...) (src/grads/period.jl:236-249) for cotangents on kmags: the
+# ēvecs = Zero branch of _solve_k_period_grads (:182-234).  Eigenvector cotangents and ε̄⁻¹
+# on 3D cells are not built on this backend.
+function ChainRulesCore.rrule(::typeof(solve_k_periodic), ω::Real, ε⁻¹::AbstractArray{Float64,5}, Λ::Real,
+                              grid::Grid{3,Float64}, solver::HIPSolver; nev=1, kwargs...)
+    g = Grid(grid.Δx, grid.Δy, Float64(Λ), grid.Nx, grid.Ny, grid.Nz)
+    kmags, evecs = solve_k(ω, ε⁻¹, g, solver; nev, kwargs...)
+    function hip_solve_k_periodic_pullback(ΔΩ)
+        k̄mags, ēvecs = ΔΩ
+        ēvecs isa AbstractZero || all(v -> v isa AbstractZero, ēvecs) ||
+            error("HIPSolver: eigenvector cotangents of solve_k_periodic are not supported")
+        kb = zeros(Float64, nev)
+        if !(k̄mags isa AbstractZero)
+            for (j, v) in enumerate(k̄mags)
+                kb[j] = v isa AbstractZero ? 0.0 : Float64(v)
+            end
+        end
+        dk, dL = hip_period_derivatives(solver, kmags, evecs, ε⁻¹, g)
+        ω̄ = sum(2 * ω .* kb ./ dk)
+        Λ̄ = sum(kb .* (-dL ./ dk))
+        return (NoTangent(), ω̄, NoTangent(), Λ̄, NoTangent(), NoTangent())
+    end
+    return (kmags, evecs), hip_solve_k_periodic_pullback
 end
 
 """
