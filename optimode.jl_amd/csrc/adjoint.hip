@@ -209,7 +209,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
   std::vector<double> slope(nb), khb(nb, 0.0);
   for (int i = 0; i < nb; ++i) slope[i] = 2.0 * sum_part(hh.data() + size_t(i) * dps, dps);
 
-  VecOps V(s, nb, ld, K, EV.as<c64>());
+  VecOps V(*this, s, nb, ld, K, EV.as<c64>());
   if (any_ev) {
     OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), hasv.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
     const int* act = actd.as<int>();
@@ -373,7 +373,7 @@ void Solver::ng_gvd(double omega, int nev, const double* kmags, const double* ev
                      EV.as<c64>(), n3, ld, nx, ny, 1.0 / Dx, 1.0 / Dy, dk.as<double>(), cf.as<double>(), R.as<c64>());
   OMX_HIP_CHECK(hipGetLastError());
   // ev' on the complement of ev
-  VecOps V(s, nb, ld, K, EV.as<c64>());
+  VecOps V(*this, s, nb, ld, K, EV.as<c64>());
   const std::vector<int> all(nb, 1);
   last_adjoint_iters = minres_complement(*this, V, std::vector<double>(nb, omega * omega), dk.as<double>(),
                                          R.as<c64>(), Hp.as<c64>(), all, 1e-12);

@@ -216,9 +216,12 @@ double sum_part(const double* p, int nb) {
   return s;
 }
 
-VecOps::VecOps(hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_)
+VecOps::VecOps(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_)
     : s(s_), nb(nb_), ld(ld_), K(K_), nblk(int(std::min<long>(256, (K_ + AB - 1) / AB))), EV(EV_),
-      part(sizeof(c64) * size_t(nb_) * 256), coef(sizeof(c64) * 4 * nb_) {}
+      part(S.vec_part), coef(S.vec_coef) {
+  part.ensure(sizeof(c64) * size_t(nb_) * 256);
+  coef.ensure(sizeof(c64) * 4 * size_t(nb_));
+}
 
 void VecOps::vdot(const c64* A, const c64* B, const int* act) {
   hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());

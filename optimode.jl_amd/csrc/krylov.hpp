@@ -20,8 +20,9 @@ struct VecOps {
   long ld, K;
   int nblk;
   const c64* EV;
-  DevMem part, coef;
-  VecOps(hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_);
+  DevMem& part;  // the handle's buffers, grown on demand (no allocation per solve)
+  DevMem& coef;
+  VecOps(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_);
   // part[i nblk + b] = partial sums of Re <A_i, B_i>
   void vdot(const c64* A, const c64* B, const int* act);
   // out_i = sum_j coef[4 i + j] in_j_i (out may alias an input)

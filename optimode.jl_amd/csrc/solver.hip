@@ -254,7 +254,7 @@ template <typename T> class Batch {
   int ninst, M;
   long K, ld, istride;
   DevMem X[2], AX[2], R[2], AR, P[2], AP[2], D;
-  DevMem st, shape, gramp, red, normp, Cx, Cp, Cw, lam, rhist, dotp, active, band, kd, ns;
+  DevMem st, shape, gramp, red, normp, Cx, Cp, Cw, lam, rhist, dotp, active, band, kd, ns, selc0, selnc;
   DenseLaunch dl{};
   PinnedMem snap[2];  // LState snapshots of consecutive iterations
   hipEvent_t snap_ev[2] = {nullptr, nullptr};
@@ -301,6 +301,8 @@ template <typename T> class Batch {
     band.alloc(sizeof(int) * ninst);
     kd.alloc(sizeof(double) * ninst);
     ns.alloc(sizeof(NewtonState) * ninst);
+    selc0.alloc(sizeof(int) * ninst);
+    selnc.alloc(sizeof(int) * ninst);
     dl.ninst = ninst;
     dl.pm = pm;
     dl.nchunk = 1;  // dense kernels read the reduced Gram matrices
@@ -530,7 +532,7 @@ template <typename T> class Batch {
   std::vector<double> hmkh_cols(int inst, int ncol) {
     std::vector<int> c0(ninst, 0), nc(ninst, 0);
     nc[inst] = ncol;
-    DevMem dc0(sizeof(int) * ninst), dnc(sizeof(int) * ninst);
+    DevMem &dc0 = selc0, &dnc = selnc;
     OMX_HIP_CHECK(hipMemcpyAsync(dc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
     OMX_HIP_CHECK(hipMemcpyAsync(dnc.as<void>(), nc.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
     apply(OP_MKH, X[cur], X[cur], dc0.as<int>(), dnc.as<int>(), dotp.as<double>());
@@ -1073,11 +1075,29 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
   hipStream_t s = S.stream;
   const int nblk_g = int(std::max<long>(1, std::min<long>((K + 31) / 32, std::max(16, 1024 / ninst))));
   const int nblk_r = int(std::min<long>(256, (K + 255) / 256));
-  DevMem n1d(sizeof(int) * ninst), zero(sizeof(int) * ninst), cold(sizeof(int) * ninst), lamd(sizeof(double) * ninst),
-      Tm(sizeof(c64) * ninst * ld), T2(sizeof(c64) * ninst * ld), ATm(sizeof(c64) * ninst * ld),
-      XB(sizeof(c64) * ninst * ld), RBm(sizeof(c64) * ninst * ld), part(sizeof(c64) * size_t(ninst) * nblk_g * RF_PAIRS),
-      Gd(sizeof(c64) * ninst * RF_PAIRS), Bd(sizeof(c64) * ninst * RF_PAIRS), Cd(sizeof(c64) * ninst * RF_PAIRS),
-      normp(sizeof(double) * ninst * nblk_r), scl(sizeof(double) * ninst), dots(sizeof(double) * ninst * b.dps);
+  RefineWorkspace& W = S.refine_ws;
+  if (W.ninst < ninst || W.ld != ld || W.nblk_g < nblk_g || W.nblk_r < nblk_r || W.dps < b.dps) {
+    const int n = std::max(ninst, W.ld == ld ? W.ninst : 0);
+    const int ng = std::max(nblk_g, W.nblk_g), nr = std::max(nblk_r, W.nblk_r), nd = std::max(b.dps, W.dps);
+    W.n1d.alloc(sizeof(int) * n);
+    W.zero.alloc(sizeof(int) * n);
+    W.cold.alloc(sizeof(int) * n);
+    W.lamd.alloc(sizeof(double) * n);
+    for (DevMem* v : {&W.Tm, &W.T2, &W.ATm, &W.XB, &W.RBm}) v->alloc(sizeof(c64) * size_t(n) * ld);
+    W.part.alloc(sizeof(c64) * size_t(n) * ng * RF_PAIRS);
+    for (DevMem* v : {&W.Gd, &W.Bd, &W.Cd}) v->alloc(sizeof(c64) * size_t(n) * RF_PAIRS);
+    W.normp.alloc(sizeof(double) * size_t(n) * nr);
+    W.scl.alloc(sizeof(double) * n);
+    W.dots.alloc(sizeof(double) * size_t(n) * nd);
+    W.ninst = n;
+    W.ld = ld;
+    W.nblk_g = ng;
+    W.nblk_r = nr;
+    W.dps = nd;
+  }
+  DevMem &n1d = W.n1d, &zero = W.zero, &cold = W.cold, &lamd = W.lamd, &Tm = W.Tm, &T2 = W.T2, &ATm = W.ATm,
+         &XB = W.XB, &RBm = W.RBm, &part = W.part, &Gd = W.Gd, &Bd = W.Bd, &Cd = W.Cd, &normp = W.normp,
+         &scl = W.scl, &dots = W.dots;
   OMX_HIP_CHECK(hipMemsetAsync(zero.as<void>(), 0, sizeof(int) * ninst, s));
   OMX_HIP_CHECK(hipMemsetAsync(Tm.as<void>(), 0, sizeof(c64) * ninst * ld, s));
   OMX_HIP_CHECK(hipMemsetAsync(T2.as<void>(), 0, sizeof(c64) * ninst * ld, s));
@@ -1160,7 +1180,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       }
       if (!need || jd >= jd_max) break;
       // t: (M - rho) t = r on the complement of x (the sign of t is immaterial)
-      VecOps V(s, ninst, ld, K, XB.as<c64>());
+      VecOps V(S, s, ninst, ld, K, XB.as<c64>());
       const int its = minres_complement(S, V, lam_band, b.kd.as<double>(), RBm.as<c64>(), Tm.as<c64>(), jdh,
                                         S.opt.jd_rtol, S.opt.jd_maxit, false);
       for (int i = 0; i < ninst; ++i) {

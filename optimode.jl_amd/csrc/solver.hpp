@@ -163,6 +163,14 @@ struct ModeSlot {
   DevMem E, H;
 };
 
+// Buffers of the FP64 eigenpair refinement, kept on the handle between solves (grown on
+// demand, so that a repeated solve allocates and frees nothing there).
+struct RefineWorkspace {
+  int ninst = 0, nblk_g = 0, nblk_r = 0, dps = 0;
+  long ld = 0;
+  DevMem n1d, zero, cold, lamd, Tm, T2, ATm, XB, RBm, part, Gd, Bd, Cd, normp, scl, dots;
+};
+
 class Solver {
  public:
   Solver(int device, int nx, int ny, double Dx, double Dy, const double* epsinv);
@@ -215,6 +223,8 @@ class Solver {
   bool trace = false;               // OMX_TRACE=1: per-Newton-step iteration log on stderr
   DevMem scratch_a, scratch_b, scratch_c;
   DevMem krylov_ws;                  // MINRES vectors and scratch (grown on demand)
+  DevMem vec_part, vec_coef;          // partial sums and coefficients of the vector helpers
+  RefineWorkspace refine_ws;          // buffers of the FP64 refinement phase
   DevMem field_scratch, argmax_v, argmax_i;  // canonicalize_phase
   Profiler prof;
 
