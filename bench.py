@@ -155,7 +155,7 @@ def periodic_cpu_baseline(ncores):
 
 def periodic_workload(a):
     """solve_k on one period of a 3D Bragg-grating waveguide (128 x 96 x 24 voxels, period
-    0.45 um, 4 Bloch modes): the z-periodic path (five-kernel 3D operator).  Prints one JSON
+    0.45 um, 4 Bloch modes): the z-periodic path (three-kernel yz-slab 3D operator).  Prints one JSON
     line with modes/s, the FP32 3D operator's roofline and the CPU baseline."""
     import numpy as np
     import omx
@@ -189,7 +189,7 @@ def periodic_workload(a):
         achieved = op_bytes / (op_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "3D Helmholtz operator application (rows, columns, z-pencils, columns, rows)",
+                    "kernel": "3D Helmholtz operator application (x rows, yz slabs in LDS, x rows)",
                     "launches": op_n, "avg_us": round(1e3 * op_ms / op_n, 2), "avg_alg_bytes": round(op_bytes / op_n)}
     ncores = 1
     try:

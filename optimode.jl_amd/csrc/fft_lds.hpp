@@ -182,4 +182,70 @@ __device__ void fft_lds(cplx<T>* buf, int nb, int sb, int se, const FFTFactors& 
   }
 }
 
+// The same transform over nb sequences with a two-level start: sequence b starts at
+// buf + (b / nb2) sb1 + (b % nb2) sb2 (element stride se).  One call then covers, e.g.,
+// every z-pencil of a [group][iz][iy] slab (nb2 = ny, sb1 = ny nz, sb2 = 1, se = ny).
+template <typename T, int DIR, int R, int VPT>
+__device__ __forceinline__ void fft_pass2(cplx<T>* buf, int nb, int nb2, int sb1, int sb2, int L, int se, int Ns,
+                                          const cplx<T>* tw) {
+  constexpr int MB = (VPT + R - 1) / R;
+  const int T_ = L / R;
+  const int nbf = nb * T_;
+  const int tstride = L / (Ns * R);
+  cplx<T> v[MB][R];
+  int outidx[MB];
+#pragma unroll
+  for (int u = 0; u < MB; ++u) {
+    const int bf = threadIdx.x + u * blockDim.x;
+    outidx[u] = -1;
+    if (bf < nbf) {
+      const int b = bf / T_;
+      const int j = bf - b * T_;
+      const int jq = j / Ns;
+      const int k = j - jq * Ns;
+      const int b1 = b / nb2;
+      const int start = b1 * sb1 + (b - b1 * nb2) * sb2;
+      const cplx<T>* base = buf + start;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[u][r] = base[(j + r * T_) * se];
+      const int ts = k * tstride;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cplx<T> w = tw[ts * r];
+        if (DIR > 0) w.y = -w.y;
+        v[u][r] = v[u][r] * w;
+      }
+      Dft<R, DIR, T>::run(v[u]);
+      outidx[u] = start + (jq * Ns * R + k) * se;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < MB; ++u) {
+    if (outidx[u] >= 0) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[outidx[u] + q * Ns * se] = v[u][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int DIR, int VPT>
+__device__ void fft_lds2(cplx<T>* buf, int nb, int nb2, int sb1, int sb2, int se, const FFTFactors& f,
+                         const cplx<T>* tw) {
+  int Ns = 1;
+  for (int p = 0; p < f.n; ++p) {
+    const int R = f.r[p];
+    switch (R) {
+      case 8: fft_pass2<T, DIR, 8, VPT>(buf, nb, nb2, sb1, sb2, f.L, se, Ns, tw); break;
+      case 4: fft_pass2<T, DIR, 4, VPT>(buf, nb, nb2, sb1, sb2, f.L, se, Ns, tw); break;
+      case 2: fft_pass2<T, DIR, 2, VPT>(buf, nb, nb2, sb1, sb2, f.L, se, Ns, tw); break;
+      case 3: fft_pass2<T, DIR, 3, VPT>(buf, nb, nb2, sb1, sb2, f.L, se, Ns, tw); break;
+      case 5: fft_pass2<T, DIR, 5, VPT>(buf, nb, nb2, sb1, sb2, f.L, se, Ns, tw); break;
+      default: break;
+    }
+    Ns *= R;
+  }
+}
+
 }  // namespace omx

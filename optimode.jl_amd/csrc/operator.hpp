@@ -100,6 +100,9 @@ template <typename T> struct OpPlan {
   FFTFactors fz;
   const cplx<T>* twz;             // device: exp(-2 pi i t/nz)
   int zcols_per_block;            // x-columns per block of the z pass
+  // > 0: the yz-slab path (whole x-columns of ny x nz values per block in LDS, intermediate
+  // x-major [v][ix][iz][iy][c]), x-columns per block; 0: five passes
+  int slab_cols;
 };
 
 // Register path geometry: rows per block of passes 1 and 3 (the column-major intermediate
@@ -132,6 +135,11 @@ enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2, FIELD_DK = 3, FIE
 template <typename T>
 void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                    const double* kinst, double scale, hipStream_t stream);
+
+// yz-slab kernel geometry of the 3D operator: threads per block and values-per-thread bound
+// of its FFT calls
+constexpr int SLAB_THREADS = 512;
+constexpr int SLAB_VPT = 16;
 
 // 3D cells (plan.nz > 1): the same operator and the FIELD_E helper with 3D transforms;
 // launch_operator / launch_fields forward here.

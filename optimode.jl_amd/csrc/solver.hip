@@ -137,6 +137,17 @@ constexpr double EPS64 = 2.220446049250313e-16;
 
 template <typename T> constexpr double eps_of() { return sizeof(T) == 4 ? EPS32 : EPS64; }
 
+// every radix pass of a batched LDS FFT (nb sequences of f.L) fits `threads` threads
+// holding at most vpt values each (the bound fft_pass relies on)
+bool fft_fits(int nb, const FFTFactors& f, int threads, int vpt) {
+  for (int p = 0; p < f.n; ++p) {
+    const int R = f.r[p];
+    const long nbf = long(nb) * (f.L / R);
+    if ((nbf + threads - 1) / threads > (vpt + R - 1) / R) return false;
+  }
+  return true;
+}
+
 int choose_vpt(int nb, const FFTFactors& f) {
   for (int v : {8, 16, 32}) {
     bool ok = true;
@@ -220,6 +231,18 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
   const size_t lds13 = sizeof(cplx<T>) * (size_t(S.nx) + 3 * size_t(pl.rows_per_block) * S.nx);
   const size_t lds2 = sizeof(cplx<T>) * (size_t(S.ny) + 3 * size_t(pl.cols_per_block) * S.ny);
   const size_t lds3 = sizeof(cplx<T>) * (size_t(S.nz) + 3 * size_t(pl.zcols_per_block) * S.nz);
+  // yz-slab path: whole x-columns (3 ny nz values) per block when they fit in LDS and in
+  // the slab kernel's FFT bound
+  P.slab_ok = 0;
+  if (S.nz > 1 && sizeof(cplx<T>) * (size_t(S.nx) + 3 * size_t(pl.rows_per_block) * (S.nx + 1)) <= 160 * 1024)
+    for (int ts : {2, 1}) {
+      const size_t ls = sizeof(cplx<T>) * (size_t(S.ny) + S.nz + 3 * size_t(ts) * S.ny * S.nz);
+      if (S.nx % ts == 0 && ls <= 160 * 1024 && fft_fits(3 * ts * S.nz, pl.fy, SLAB_THREADS, SLAB_VPT) &&
+          fft_fits(3 * ts * S.ny, pl.fz, SLAB_THREADS, SLAB_VPT)) {
+        P.slab_ok = ts;
+        break;
+      }
+    }
   if (lds13 > 160 * 1024 || lds2 > 160 * 1024 || lds3 > 160 * 1024)
     throw Error(OMX_ERR_BAD_SHAPE, "grid too large for the LDS FFT");
   // register-FFT operator for power-of-two sizes 64 .. 512 (2D cells)
@@ -235,6 +258,7 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
 template <typename T> void select_operator(Solver& S, PrecisionData<T>& P) {
   OpPlan<T>& pl = P.plan;
   pl.reg = (P.reg_ok && S.opt.reg_ops) ? 1 : 0;
+  pl.slab_cols = S.opt.slab3d ? P.slab_ok : 0;
   pl.dot_blocks = S.nz * (S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block));
 }
 
@@ -653,8 +677,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_rtol = v;
   else if (key == "jd_maxit")
     opt.jd_maxit = int(v);
-  else if (key == "reg_ops") {
-    opt.reg_ops = v != 0 ? 1 : 0;
+  else if (key == "reg_ops" || key == "slab3d") {
+    (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
     select_operator(*this, p32);
     select_operator(*this, p64);
     workspaces.clear();  // batches size their dot partials by the operator in use

@@ -59,7 +59,8 @@ class DevMem {
 template <typename T> struct PrecisionData {
   OpPlan<T> plan{};
   DevMem eps6, eps6c, twx, twy, twz;
-  int reg_ok = 0;  // the grid admits the register-FFT operator
+  int reg_ok = 0;   // the grid admits the register-FFT operator
+  int slab_ok = 0;  // 3D cells: x-columns per block of the yz-slab path (0: five passes only)
 };
 
 struct SolveStats {
@@ -143,6 +144,7 @@ struct Options {
   uint64_t seed = 0x5eed;
   int fp64_only = 0;            // skip the FP32 phase
   int reg_ops = 1;              // register-FFT operator where the grid admits it
+  int slab3d = 1;               // 3D cells: the yz-slab operator where the cell admits it
   int plain = 0;                // the reference's algorithm: FP64 only, nev-column block per band,
                                 // random start at k_guess per band, every eigensolve to eig_tol
   int warm_start = 0;           // solve_k without kguess/Hguess starts every band from the previous

@@ -35,9 +35,10 @@ def _relerr(a, b):
 SIZES = [(12, 10, 6), (16, 16, 8), (24, 16, 12), (20, 18, 5), (64, 64, 16)]
 
 
+@pytest.mark.parametrize("slab", [1, 0], ids=["slab", "five_pass"])
 @pytest.mark.parametrize("shape", SIZES, ids=lambda s: "x".join(map(str, s)))
-def test_operator_family_matches_oracle_3d(shape):
-    print(f"3D operator {shape}", flush=True)
+def test_operator_family_matches_oracle_3d(shape, slab):
+    print(f"3D operator {shape} slab {slab}", flush=True)
     og, g = _grids(shape)
     ei = _aniso3(og)
     rng = np.random.default_rng(2)
@@ -49,6 +50,8 @@ def test_operator_family_matches_oracle_3d(shape):
     refP = P.apply_P3(H, k, ei, og, basis)
     refK = np.array([P.HMkH3(H[:, j], k, ei, og, basis) for j in range(m)])
     solver = omx.HIPSolver()
+    h = solver.handle(g, ei)
+    omx._lib.check(omx._lib.lib().omx_set_option(h.h, b"slab3d", float(slab)), h.h)
     for prec, tol in ((omx.OMX_PREC_FP64, 1e-12), (omx.OMX_PREC_FP32, 3e-6)):
         Mh = omx.HelmholtzMap(k, ei, g, solver, precision=prec)
         assert _relerr(Mh * H, refM) < tol
