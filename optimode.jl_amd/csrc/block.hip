@@ -320,7 +320,26 @@ __global__ __launch_bounds__(BLK) void k_combine(ColList<T> Y, ColList<T> Y2, Co
       a1[q] = cplx<T>{T(0), T(0)};
       a2[q] = cplx<T>{T(0), T(0)};
     }
-    for (int a = 0; a < np; ++a) {
+    // four input columns per step: their loads are issued together before the products
+    int a = 0;
+    for (; a + 4 <= np; a += 4) {
+      cplx<T> y[4], y2[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        y[u] = py[a + u][row];
+        y2[u] = cplx<T>{T(0), T(0)};
+        if (MODE == CMB_TWO || MODE == CMB_RESID) y2[u] = py2[a + u][row];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int q = 0; q < QM; ++q) {
+          const cplx<T> c = Cs[a + u][q];
+          a1[q] = a1[q] + y[u] * c;
+          if (MODE == CMB_TWO || MODE == CMB_RESID) a2[q] = a2[q] + y2[u] * c;
+        }
+    }
+    for (; a < np; ++a) {
       const cplx<T> y = py[a][row];
       cplx<T> y2{T(0), T(0)};
       if (MODE == CMB_TWO || MODE == CMB_RESID) y2 = py2[a][row];
@@ -350,15 +369,23 @@ __global__ __launch_bounds__(BLK) void k_combine(ColList<T> Y, ColList<T> Y2, Co
     }
   }
   if (MODE == CMB_RESID) {
+    // per column: a wavefront reduction (fixed butterfly order), then the BLK/64 wavefront
+    // sums in order through LDS
+    constexpr int NW = BLK / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
     for (int q = 0; q < QM; ++q) {
-      red[threadIdx.x] = nrm[q];
-      __syncthreads();
-      for (int s = BLK / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) norm_partial[(long(inst) * gridDim.x + blockIdx.x) * QM + q] = red[0];
-      __syncthreads();
+      double v = nrm[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) red[wave * QM + q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < QM) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) t += red[w * QM + threadIdx.x];
+      norm_partial[(long(inst) * gridDim.x + blockIdx.x) * QM + threadIdx.x] = t;
     }
   }
 }
