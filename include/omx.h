@@ -205,6 +205,14 @@ int omx_smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nsha
                       const double* verts, const double* eps_shapes, const double* eps_background, int nsub,
                       double* epsinv_out);
 
+/* sliceinv_3x3 (lib/MaxwellEigenmodes/src/linalg.jl:126-149): the per-pixel inverse Y = A^-1
+   of npix 3x3 tensors, column-major (3,3,Nx,Ny[,Nz]) (pixel p at 9p + a + 3b), FP64 on the
+   GPU; and its pull-back (rrule, src/grads/linalg.jl:40-47) Abar = -Y^T Ybar Y^T, which
+   turns an eps^-1 cotangent (omx_solve_k_adjoint) into an eps cotangent.  No handle:
+   errors are reported through omx_last_error(NULL). */
+int omx_sliceinv_3x3(int device, int64_t npix, const double* A, double* Y_out);
+int omx_sliceinv_3x3_pullback(int device, int64_t npix, const double* Y, const double* Ybar, double* Abar_out);
+
 /* Live kernel statistics (no counterpart in the reference: instrumentation for
    measurement).  With omx_set_option(h, "profile", 1) every operator application inside
    a solve is bracketed by HIP events on the handle's stream; `family` is "op_fp32" or

@@ -15,7 +15,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "omx", "libomx.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["operator.hip", "operator3d.hip", "block.hip", "dense.hip", "fields.hip", "solver.hip", "adjoint.hip", "krylov.hip", "refine.hip", "smooth.hip", "eme.hip",
+SOURCES = ["operator.hip", "operator3d.hip", "block.hip", "dense.hip", "fields.hip", "solver.hip", "adjoint.hip", "krylov.hip", "refine.hip", "smooth.hip", "tensor.hip", "eme.hip",
            "batch.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -124,6 +124,32 @@ int omx_smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nsha
   }
 }
 
+int omx_sliceinv_3x3(int device, int64_t npix, const double* A, double* Y_out) {
+  try {
+    omx::sliceinv_3x3(device, long(npix), A, Y_out);
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  }
+}
+
+int omx_sliceinv_3x3_pullback(int device, int64_t npix, const double* Y, const double* Ybar, double* Abar_out) {
+  try {
+    omx::sliceinv_3x3_pullback(device, long(npix), Y, Ybar, Abar_out);
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  }
+}
+
 int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only) {
   try {
     omx::solve_k_batch(device, nprob, probs, nstreams, fp64_only);

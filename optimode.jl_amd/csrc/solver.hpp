@@ -264,4 +264,8 @@ void smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshape,
                    const double* verts, const double* eps_shapes, const double* eps_background, int nsub,
                    double* epsinv_out);
 
+// per-pixel 3x3 inverse and its pull-back Abar = -Y^T Ybar Y^T (tensor.hip); no handle
+void sliceinv_3x3(int device, long npix, const double* A, double* Y_out);
+void sliceinv_3x3_pullback(int device, long npix, const double* Y, const double* Ybar, double* Abar_out);
+
 }  // namespace omx

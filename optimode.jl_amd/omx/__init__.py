@@ -27,7 +27,7 @@ from ._lib import OmxError, OmxInfo, OMX_PREC_FP32, OMX_PREC_FP64  # noqa: F401
 
 __all__ = ["Grid", "Grid3", "HIPSolver", "HelmholtzMap", "solve_omega2", "solve_k", "solve_k_batch", "rrule_solve_k",
            "group_index", "ng_gvd", "k_guess", "solve_k_periodic", "period_derivatives", "rrule_solve_k_periodic",
-           "OmxError", "library_path", "exported_symbols", "kernel_stats"]
+           "sliceinv_3x3", "rrule_sliceinv_3x3", "OmxError", "library_path", "exported_symbols", "kernel_stats"]
 
 
 def library_path():
@@ -473,3 +473,42 @@ def rrule_solve_k_periodic(omega, epsinv, L, grid, solver, nev=1, **kw):
         return wbar, Lbar
 
     return (kmags, evecs), pullback
+
+
+def _tensor_field(A):
+    a = np.asarray(A, dtype=np.float64)
+    if a.ndim < 2 or a.shape[:2] != (3, 3):
+        raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"expected a (3,3,...) tensor field, got {a.shape}")
+    # column-major flat order: pixel p at 9p + a + 3b
+    return np.ascontiguousarray(a.transpose(tuple(range(a.ndim - 1, -1, -1)))), a.shape
+
+
+def _tensor_back(flat, shape):
+    return np.ascontiguousarray(flat.transpose(tuple(range(flat.ndim - 1, -1, -1)))).reshape(shape)
+
+
+def sliceinv_3x3(A, device=0):
+    """``sliceinv_3x3(A)`` (lib/MaxwellEigenmodes/src/linalg.jl:126-149): the per-pixel inverse of
+    a ``(3,3,Nx,Ny[,Nz])`` tensor field, FP64 on the GPU."""
+    a, shape = _tensor_field(A)
+    y = np.empty_like(a)
+    _lib.check(_lib.lib().omx_sliceinv_3x3(int(device), a.size // 9, _lib.dptr(a), _lib.dptr(y)))
+    return _tensor_back(y, shape)
+
+
+def rrule_sliceinv_3x3(A, device=0):
+    """``rrule(sliceinv_3x3, A)`` (src/grads/linalg.jl:40-47) -> ``(Y, pullback)`` with
+    ``pullback(Ȳ) = Ā = -Yᵀ Ȳ Yᵀ`` per pixel, on the GPU."""
+    Y = sliceinv_3x3(A, device)
+
+    def pullback(Ybar):
+        y, shape = _tensor_field(Y)
+        b, _ = _tensor_field(Ybar)
+        if b.shape != y.shape:
+            raise OmxError(_lib.OMX_ERR_BAD_SHAPE, "the cotangent must have the shape of the inverse")
+        out = np.empty_like(y)
+        _lib.check(_lib.lib().omx_sliceinv_3x3_pullback(int(device), y.size // 9, _lib.dptr(y), _lib.dptr(b),
+                                                        _lib.dptr(out)))
+        return _tensor_back(out, shape)
+
+    return Y, pullback

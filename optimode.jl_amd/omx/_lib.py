@@ -92,6 +92,8 @@ SIGNATURES = [
     ("omx_get_kernel_stats", ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
     ("omx_solve_k_batch", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OmxProblem), ctypes.c_int,
                                          ctypes.c_int]),
+    ("omx_sliceinv_3x3", ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+    ("omx_sliceinv_3x3_pullback", ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]),
     ("omx_smooth_epsinv", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp, ctypes.c_int,
                                          _dp]),
