@@ -18,7 +18,7 @@ using DielectricSmoothing: Grid, N
 import MaxwellEigenmodes: solve_ω², solve_k, solve_k_periodic, AbstractEigensolver, ModeSolver
 
 export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E, hip_build_modes, hip_overlap_matrix,
-       hip_period_derivatives
+       hip_period_derivatives, hip_sliceinv_3x3, hip_sliceinv_3x3_pullback
 
 const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
 
@@ -221,6 +221,37 @@ function ChainRulesCore.rrule(::typeof(solve_k_periodic), ω::Real, ε⁻¹::Abs
         return (NoTangent(), ω̄, NoTangent(), Λ̄, NoTangent(), NoTangent())
     end
     return (kmags, evecs), hip_solve_k_periodic_pullback
+end
+
+"""
+    hip_sliceinv_3x3(A; device=0)
+
+`sliceinv_3x3(A)` (src/linalg.jl:126-149) on the GPU: the per-pixel inverse of a
+(3,3,Nx,Ny[,Nz]) tensor field.
+"""
+function hip_sliceinv_3x3(A::AbstractArray{Float64}; device::Integer=0)
+    a = Array{Float64}(A)
+    y = similar(a)
+    st = ccall((:omx_sliceinv_3x3, libomx), Cint, (Cint, Int64, Ptr{Cdouble}, Ptr{Cdouble}),
+               device, length(a) ÷ 9, a, y)
+    _check(st, C_NULL)
+    return y
+end
+
+"""
+    hip_sliceinv_3x3_pullback(Y, Ȳ; device=0)
+
+Pull-back of `sliceinv_3x3` (src/grads/linalg.jl:40-47): Ā = -Yᵀ Ȳ Yᵀ per pixel, Y = A⁻¹.
+"""
+function hip_sliceinv_3x3_pullback(Y::AbstractArray{Float64}, Ȳ::AbstractArray{Float64}; device::Integer=0)
+    size(Y) == size(Ȳ) || error("hip_sliceinv_3x3_pullback: Y and Ȳ differ in shape")
+    y = Array{Float64}(Y)
+    yb = Array{Float64}(Ȳ)
+    ab = similar(y)
+    st = ccall((:omx_sliceinv_3x3_pullback, libomx), Cint, (Cint, Int64, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+               device, length(y) ÷ 9, y, yb, ab)
+    _check(st, C_NULL)
+    return ab
 end
 
 """
