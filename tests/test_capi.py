@@ -65,3 +65,33 @@ def test_no_device_gives_instructive_error():
         omx.HIPSolver().handle(g, ei)
     assert exc.value.status == _lib.OMX_ERR_NO_DEVICE
     assert "device" in str(exc.value).lower()
+
+
+def test_epsinv_layout_conversion_3d():
+    g = omx.Grid3(2.0, 1.0, 0.5, 3, 2, 4)
+    e = np.arange(3 * 3 * 3 * 2 * 4, dtype=float).reshape(3, 3, 3, 2, 4)
+    c = omx._epsinv_c(e, g).ravel()
+    for a in range(3):
+        for b in range(3):
+            for ix in range(3):
+                for iy in range(2):
+                    for iz in range(4):
+                        assert c[a + 3 * b + 9 * (ix + 3 * (iy + 2 * iz))] == e[a, b, ix, iy, iz]
+    with pytest.raises(omx.OmxError):
+        omx._epsinv_c(e[..., 0], g)
+
+
+def test_tensor_field_round_trip():
+    e = np.random.default_rng(0).standard_normal((3, 3, 4, 3, 2))
+    flat, shape = omx._tensor_field(e)
+    assert flat.ravel()[1 + 3 * 2 + 9 * (3 + 4 * (1 + 3 * 1))] == e[1, 2, 3, 1, 1]
+    assert np.array_equal(omx._tensor_back(flat, shape), e)
+
+
+@pytest.mark.parametrize("args", [["--gpus", "2"], ["--workload", "batch"]])
+def test_bench_refuses_multi_gpu_and_disabled_workloads(args):
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and r.stderr.strip() and not r.stdout.strip()
