@@ -21,6 +21,7 @@ __all__ = [
     "Grid3", "gvec3", "gz_field", "period_weight", "grid_with_period", "mag_mn3", "to_grid3", "to_flat3",
     "apply_M3", "apply_P3", "HMkH3", "HMkH_weighted", "dw2_dL", "E_field3", "canonicalize_phase3",
     "HelmholtzOracle3", "solve_k_periodic", "solve_k_periodic_kbar_pullback", "bragg_wg_epsi",
+    "epsinv_bar3", "wbar_z", "solve_k_periodic_pullback",
 ]
 
 
@@ -246,3 +247,84 @@ def bragg_wg_epsi(grid, eps_core=4.0, eps_bg=2.1, w0=1.0, h0=0.6, mod=0.25):
     for a in range(3):
         out[a, a] = 1.0 / eps
     return out
+
+
+def epsinv_bar3(d, lamd):
+    """3D ``ε⁻¹_bar`` (``grads/solve.jl:464-487``): diagonal ``-Re(λd_a conj d_a)``; the upper
+    off-diagonal entries ``(0,1)``, ``(1,2)``, ``(0,2)`` hold ``-Re(λd_j conj d_i + λd_i conj d_j)``
+    and the lower ones stay zero (unlike the mirrored 2D form).  ``d``, ``lamd``:
+    ``(3, Nx, Ny, Nz)``; returns ``(3, 3, Nx, Ny, Nz)``."""
+    out = np.zeros((3, 3) + d.shape[1:])
+    for a in range(3):
+        out[a, a] = -np.real(lamd[a] * np.conj(d[a]))
+    for i, j in ((0, 1), (1, 2), (0, 2)):
+        out[i, j] = -np.real(lamd[j] * np.conj(d[i]) + lamd[i] * np.conj(d[j]))
+    return out
+
+
+def wbar_z(magbar, mbar, nbar, mag, m, n):
+    """``_wbar_z`` (``grads/period.jl:97-119``): the z-component of the per-plane-wave
+    cotangent of ``w = k_vec - g`` from those of ``(mag, m, n)``:
+    ``w̄ = māg ŵ - (m̄.ŵ/mag) m - (n̄.ŵ/mag) n`` with ``ŵ = m x n``."""
+    wx = m[1] * n[2] - m[2] * n[1]
+    wy = m[2] * n[0] - m[0] * n[2]
+    wz = m[0] * n[1] - m[1] * n[0]
+    mw = (mbar[0] * wx + mbar[1] * wy + mbar[2] * wz) / mag
+    nw = (nbar[0] * wx + nbar[1] * wy + nbar[2] * wz) / mag
+    return magbar * wz - mw * m[2] - nw * n[2]
+
+
+def solve_k_periodic_pullback(omega, epsinv, L, grid, kmags, evecs, kbar, evbar, adj_rtol=1e-13):
+    """``_solve_k_period_grads`` (``grads/period.jl:182-234``): ``(k̄mags, ēvecs)`` ->
+    ``(ω̄, ε̄⁻¹, Λ̄)``.  Eigenvector channel: ``λ`` from ``eig_adjt`` (re-orthogonalised), ``ε̄⁻¹``
+    from ``d = fft(kx_tc ev)/N`` and ``λd = fft(kx_tc λ)``, and the per-plane-wave ``w̄_z`` from
+    ``λẽ = bfft(ε⁻¹ λd/N)``, ``ẽ = bfft(ε⁻¹ d)``: ``k̄ₕ = -Σ w̄_z``, ``Λ̄_H = -Σ w̄_z g_z/Λ``.
+    Eigenvalue channel with ``k̄tot = k̄ + k̄ₕ``: ``λ = (k̄tot/∂ω²∂k) ev``, ``ω̄ += 2ω k̄tot/∂ω²∂k``,
+    ``Λ̄ += Λ̄_H + k̄tot (-∂ω²∂Λ/∂ω²∂k)``.  ``kbar``/``evbar`` entries may be ``None``."""
+    from .adjoint import eig_adjt
+    g = grid_with_period(grid, L)
+    N = g.N
+    wt = period_weight(g)
+    ei_bar = np.zeros_like(epsinv, dtype=float)
+    w_bar, L_bar = 0.0, 0.0
+    for i in range(len(kmags)):
+        k = float(kmags[i])
+        ev = np.asarray(evecs[i], dtype=np.complex128)
+        kb = 0.0 if (kbar is None or kbar[i] is None) else float(kbar[i])
+        eb = None if (evbar is None or evbar[i] is None) else np.asarray(evbar[i], dtype=np.complex128)
+        op = HelmholtzOracle3(k, epsinv, g)
+        mag, m, n = op.basis
+        dk = 2.0 * op.HMkH(ev)
+        dL = 2.0 * HMkH_weighted(ev, k, epsinv, g, wt, op.basis)
+        evg = to_grid3(ev, g)
+        kh, LH = 0.0, 0.0
+
+        def kxtc(h):
+            return (h[1][None] * m - h[0][None] * n) * mag[None]
+
+        def eid3(x):
+            return np.einsum("baxyz,bxyz->axyz", epsinv, x)
+
+        if eb is not None and np.any(eb):
+            lam = eig_adjt(op.M, omega ** 2, ev, 0.0, eb, P=op.P, rtol=adj_rtol)
+            lam = lam - np.vdot(ev, lam) * ev
+            lg = to_grid3(lam, g)
+            d = _fft3(kxtc(evg / N))
+            lamd = _fft3(kxtc(lg))
+            ei_bar += epsinv_bar3(d, lamd)
+            lamd = lamd / N
+            lam_e = _bfft3(eid3(lamd))
+            e_t = _bfft3(eid3(d))
+            kxbar_m = np.real(lam_e * np.conj(evg[1])[None] + e_t * np.conj(lg[1])[None])
+            kxbar_n = -np.real(lam_e * np.conj(evg[0])[None] + e_t * np.conj(lg[0])[None])
+            magbar = (n * kxbar_n).sum(0) + (m * kxbar_m).sum(0)
+            wz = wbar_z(magbar, kxbar_m * mag[None], kxbar_n * mag[None], mag, m, n)
+            kh = -np.sum(wz)
+            LH = -np.sum(wz * wt)
+        ktot = kb + kh
+        d = _fft3(kxtc(evg / N))
+        lamd = _fft3(kxtc((ktot / dk) * evg))
+        ei_bar += epsinv_bar3(d, lamd)
+        w_bar += 2.0 * omega * ktot / dk
+        L_bar += LH + ktot * (-dL / dk)
+    return w_bar, ei_bar, L_bar

@@ -120,3 +120,70 @@ def test_period_pullback_matches_finite_differences_of_resolves():
     fd_L = (P.solve_k_periodic(omega, ei, L + h, g, **kw)[0][0] - P.solve_k_periodic(omega, ei, L - h, g, **kw)[0][0]) / (2 * h)
     assert abs(wbar - fd_w) < 1e-6 * abs(fd_w)
     assert abs(Lbar - fd_L) < 1e-6 * abs(fd_L)
+
+
+def test_epsinv_bar3_convention():
+    d = RNG.standard_normal((3, 2, 2, 2)) + 1j * RNG.standard_normal((3, 2, 2, 2))
+    ld = RNG.standard_normal((3, 2, 2, 2)) + 1j * RNG.standard_normal((3, 2, 2, 2))
+    e = P.epsinv_bar3(d, ld)
+    e2 = O.epsinv_bar(d.reshape(3, 2, 4), ld.reshape(3, 2, 4)).reshape(3, 3, 2, 2, 2)
+    # grads/solve.jl:464-487: the 2D values on and above the diagonal, zeros below
+    for a in range(3):
+        for b in range(3):
+            assert np.allclose(e[a, b], e2[a, b] if a <= b else 0.0)
+
+
+def test_wbar_z_sums_to_the_k_gradient_in_2d():
+    # Σ w̄_z over the plane waves is ∇ₖmag_m_n with dk = z (grads/period.jl:97-119), checked
+    # against the pinned 2D restatement on a single plane
+    g2 = O.Grid(2.0, 1.5, 6, 5)
+    mag, m, n = O.mag_mn(0.7, g2)
+    magbar = RNG.standard_normal(mag.shape)
+    mbar, nbar = RNG.standard_normal(m.shape), RNG.standard_normal(n.shape)
+    ref = O.grad_k_mag_m_n(magbar, mbar, nbar, mag, m, n)
+    assert abs(np.sum(P.wbar_z(magbar, mbar, nbar, mag, m, n)) - ref) < 1e-12 * max(1.0, abs(ref))
+
+
+def _bragg_small():
+    g = P.Grid3(2.4, 2.0, 0.45, 8, 8, 4)
+    return g, P.bragg_wg_epsi(g, eps_core=4.0, eps_bg=2.1, w0=0.5, h0=0.4, mod=0.5)
+
+
+def _k1(omega, ei, L, g):
+    km, ev = P.solve_k_periodic(omega, ei, L, g, nev=1, k_tol=1e-14, eig_tol=1e-12, maxiter=500)
+    return km[0], ev[0]
+
+
+def test_period_pullback_full_matches_finite_differences():
+    g, ei = _bragg_small()
+    omega, L = 0.5, g.Dz
+    k0, ev0 = _k1(omega, ei, L, g)
+    # eigenvalue channel alone: dk/dω, dk/dΛ and a symmetric eps^-1 direction
+    wbar, eibar, Lbar = P.solve_k_periodic_pullback(omega, ei, L, g, [k0], [ev0], [1.0], None)
+    h = 1e-4
+    fd_w = (_k1(omega + h, ei, L, g)[0] - _k1(omega - h, ei, L, g)[0]) / (2 * h)
+    fd_L = (_k1(omega, ei, L + h, g)[0] - _k1(omega, ei, L - h, g)[0]) / (2 * h)
+    assert abs(wbar - fd_w) < 1e-6 * abs(fd_w) and abs(Lbar - fd_L) < 1e-5 * abs(fd_L)
+    rng = np.random.default_rng(9)
+    d = np.zeros_like(ei)
+    for a in range(3):
+        d[a, a] = rng.standard_normal(g.shape) * 1e-2
+    s = rng.standard_normal(g.shape) * 1e-2
+    d[0, 1] = d[1, 0] = s
+    he = 1e-4  # the off-diagonal direction has a large curvature: keep the step small
+    fd_e = (_k1(omega, ei + he * d, L, g)[0] - _k1(omega, ei - he * d, L, g)[0]) / (2 * he)
+    assert abs(np.sum(eibar * d) - fd_e) < 1e-5 * abs(fd_e)
+    # eigenvector channel: J = |<ev, c>|^2 (phase invariant)
+    c = rng.standard_normal(2 * g.N) + 1j * rng.standard_normal(2 * g.N)
+
+    def J(w, e, Lp):
+        return abs(np.vdot(_k1(w, e, Lp, g)[1], c)) ** 2
+
+    evbar = 2 * c * np.vdot(c, ev0)
+    wbar2, eibar2, Lbar2 = P.solve_k_periodic_pullback(omega, ei, L, g, [k0], [ev0], [None], [evbar])
+    fdj_w = (J(omega + h, ei, L) - J(omega - h, ei, L)) / (2 * h)
+    fdj_L = (J(omega, ei, L + h) - J(omega, ei, L - h)) / (2 * h)
+    fdj_e = (J(omega, ei + he * d, L) - J(omega, ei - he * d, L)) / (2 * he)
+    assert abs(wbar2 - fdj_w) < 1e-4 * abs(fdj_w)
+    assert abs(Lbar2 - fdj_L) < 1e-4 * abs(fdj_L)
+    assert abs(np.sum(eibar2 * d) - fdj_e) < 1e-4 * abs(fdj_e)
