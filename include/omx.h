@@ -186,11 +186,22 @@ int omx_get_evecs(omx_solver* h, double* evecs_out);
 /* Pull-back of rrule(solve_k) (src/solve.jl:213-289, solvers/gpu.jl:292-365): given the
    primal (kmags, evecs) and cotangents kbar (nev, NULL = zero) and evbar (2N x nev complex,
    NULL = zero; evbar_mask[j] = 0 marks band j's cotangent as zero), returns
-   omegabar and epsinvbar (3,3,Nx,Ny) with the 2D mirrored off-diagonal convention
-   (grads/solve.jl:490-515). */
+   omegabar and epsinvbar: (3,3,Nx,Ny) with the 2D mirrored off-diagonal convention
+   (grads/solve.jl:490-515), or on a 3D handle (3,3,Nx,Ny,Nz) with each off-diagonal sum
+   above the diagonal and zeros below (grads/solve.jl:464-487). */
 int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
                         const double* kbar, const double* evbar, const uint8_t* evbar_mask,
                         double* omegabar_out, double* epsinvbar_out);
+
+/* Pull-back of rrule(solve_k_periodic, ω, ε⁻¹, Λ, grid, solver) (grads/period.jl:163-249,
+   _solve_k_period_grads) on a 3D handle whose cell period Dz is Λ: the arguments and the
+   ω̄, ε̄⁻¹ outputs of omx_solve_k_adjoint on that handle, plus the period cotangent
+   Lbar_out = Σ_j Λ̄_H,j + k̄tot_j (-∂ω²∂Λ_j / ∂ω²∂k_j), where k̄tot_j = k̄_j + k̄ₕ,j, and
+   k̄ₕ and Λ̄_H = -Σ_G w̄_z g_z/Λ come from the eigenvector channel (eig_adjt) through the
+   plane-wave basis (period.jl:97-119).  OMX_ERR_UNSUPPORTED on a 2D handle. */
+int omx_solve_k_periodic_adjoint(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+                                 const double* kbar, const double* evbar, const uint8_t* evbar_mask,
+                                 double* omegabar_out, double* epsinvbar_out, double* Lbar_out);
 
 /* Kottke sub-pixel smoothing on the GPU (lib/DielectricSmoothing/src/kottke.jl:44-95,
    smooth.jl:115-170, smooth_eps_single and smooth_eps): eps^-1 (3,3,Nx,Ny) column-major

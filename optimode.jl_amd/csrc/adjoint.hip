@@ -47,20 +47,27 @@ __global__ __launch_bounds__(AB) void k_epsbar(const c64* Dd, const c64* Dl, lon
 
 // k̄ from the basis sensitivities (maxwell.jl:560-566 ∇ₖmag_m_n with dk = z, as used at
 // solve.jl:258-270): per G, kx̄_m = Re(le conj(ev2) + e conj(l2)), kx̄_n = -Re(le conj(ev1)
-// + e conj(l1)), māg = n.kx̄_n + m.kx̄_m; partial sums of -(k̄_mag + k̄_m + k̄_n).
+// + e conj(l1)), māg = n.kx̄_n + m.kx̄_m; partial sums of -(k̄_mag + k̄_m + k̄_n), i.e. of
+// -w̄_z, the z-component of the plane wave's cotangent of w = k_vec - G (grads/period.jl:
+// 97-119).  On 3D cells (nz > 1) plane iz has w_z = k - g_z and partL receives the partial
+// sums of -w̄_z g_z / Dz (Λ̄_H, period.jl:209-210).
 __global__ __launch_bounds__(AB) void k_khbar(const c64* LE, const c64* EE, const c64* EV, const c64* LA, long n3,
-                                              long ld, int nx, int ny, double invdx, double invdy,
-                                              const double* kinst, const int* act, double* part) {
+                                              long ld, int nx, int ny, int nz, double invdx, double invdy,
+                                              double invdz, const double* kinst, const int* act, double* part,
+                                              double* partL) {
   const int i = blockIdx.y;
-  __shared__ double red[AB];
-  double acc = 0;
+  __shared__ double red[AB], redL[AB];
+  double acc = 0, accL = 0;
   if (act[i]) {
     const double k = kinst[i];
-    const long N = long(nx) * ny;
+    const long nxy = long(nx) * ny, N = nxy * nz;
     for (long p = long(blockIdx.x) * AB + threadIdx.x; p < N; p += long(gridDim.x) * AB) {
-      const int ix = int(p % nx), iy = int(p / nx);
+      const int iz = int(p / nxy);
+      const long pxy = p - long(iz) * nxy;
+      const int ix = int(pxy % nx), iy = int(pxy / nx);
       const double gx = double(fft_index(ix, nx)) * invdx, gy = double(fft_index(iy, ny)) * invdy;
-      const double wx = -gx, wy = -gy, wz = k;
+      const double gz = nz > 1 ? double(fft_index(iz, nz)) * invdz : 0.0;
+      const double wx = -gx, wy = -gy, wz = k - gz;
       const double mag = sqrt(wx * wx + wy * wy + wz * wz);
       double n0, n1;
       const double rho2 = wx * wx + wy * wy;
@@ -99,15 +106,23 @@ __global__ __launch_bounds__(AB) void k_khbar(const c64* LE, const c64* EE, cons
       t -= mag * (km[0] * mq0 + km[1] * mq1 + km[2] * mq2);
       t -= mag * (kn[0] * nq0 + kn[1] * nq1 + kn[2] * nq2);
       acc -= t;
+      accL -= t * gz * invdz;
     }
   }
   red[threadIdx.x] = acc;
+  redL[threadIdx.x] = accL;
   __syncthreads();
   for (int s = AB / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if (threadIdx.x < s) {
+      red[threadIdx.x] += red[threadIdx.x + s];
+      redL[threadIdx.x] += redL[threadIdx.x + s];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+  if (threadIdx.x == 0) {
+    part[long(i) * gridDim.x + blockIdx.x] = red[0];
+    if (partL) partL[long(i) * gridDim.x + blockIdx.x] = redL[0];
+  }
 }
 
 // out[i*nb + b] partials of Re sum_p X_p^H A_p Y_p over the pixels (3-component fields in
@@ -169,8 +184,9 @@ __global__ __launch_bounds__(AB) void k_ct_combine(const c64* F1, const c64* F2,
 }  // namespace
 
 void Solver::adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
-                     const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar) {
-  require_2d("the solve_k adjoint");
+                     const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar,
+                     double* Lbar) {
+  if (Lbar && nz <= 1) throw Error(OMX_ERR_UNSUPPORTED, "the period cotangent needs a 3D cell (Nz > 1)");
   if (nev < 1 || nev > 64) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..64");
   if (!kmags || !evecs || !omegabar || !epsinvbar) throw Error(OMX_ERR_INVALID_ARG, "NULL argument");
   if (!(omega > 0) || !std::isfinite(omega)) throw Error(OMX_ERR_INVALID_ARG, "omega must be positive and finite");
@@ -183,7 +199,7 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
   const int nblk = int(std::min<long>(256, (K + AB - 1) / AB));
   DevMem dk(sizeof(double) * nb), EV(sizeof(c64) * nb * ld), EB(sizeof(c64) * nb * ld), LA(sizeof(c64) * nb * ld),
       Dd(sizeof(c64) * nb * n3), Dl(sizeof(c64) * nb * n3), eb6(sizeof(double) * 6 * N), actd(sizeof(int) * nb),
-      part(sizeof(c64) * nb * nblk), cfac(sizeof(double) * nb);
+      part(sizeof(c64) * nb * nblk), partL(sizeof(double) * nb * nblk), cfac(sizeof(double) * nb);
   OMX_HIP_CHECK(hipMemsetAsync(eb6.as<void>(), 0, sizeof(double) * 6 * N, s));
   OMX_HIP_CHECK(hipMemcpyAsync(dk.as<void>(), kmags, sizeof(double) * nb, hipMemcpyHostToDevice, s));
   for (int i = 0; i < nb; ++i)
@@ -206,8 +222,16 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
   std::vector<double> hh(size_t(nb) * dps);
   OMX_HIP_CHECK(hipMemcpyAsync(hh.data(), hdots.as<void>(), sizeof(double) * hh.size(), hipMemcpyDeviceToHost, s));
   OMX_HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<double> slope(nb), khb(nb, 0.0);
+  std::vector<double> slope(nb), khb(nb, 0.0), dL(nb, 0.0), LHb(nb, 0.0);
   for (int i = 0; i < nb; ++i) slope[i] = 2.0 * sum_part(hh.data() + size_t(i) * dps, dps);
+  if (Lbar) {
+    // dω²/dΛ = 2 HMkH_weighted(ev, g_z / Dz) (grads/period.jl:88-90)
+    launch_operator<double>(OP_MKHW, p64.plan, EVv, EVv, Dd.as<c64>(), sel1, dk.as<double>(), hdots.as<double>(), dps,
+                            s);
+    OMX_HIP_CHECK(hipMemcpyAsync(hh.data(), hdots.as<void>(), sizeof(double) * hh.size(), hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < nb; ++i) dL[i] = 2.0 * sum_part(hh.data() + size_t(i) * dps, dps);
+  }
 
   VecOps V(*this, s, nb, ld, K, EV.as<c64>());
   if (any_ev) {
@@ -230,21 +254,29 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
     launch_fields<double>(FIELD_EINV, p64.plan, LAv, Dl.as<c64>(), sel1, dk.as<double>(), 1.0 / double(N), s);
     launch_fields<double>(FIELD_EINV, p64.plan, EVv, Dd.as<c64>(), sel1, dk.as<double>(), 1.0, s);
     hipLaunchKernelGGL(k_khbar, dim3(nblk, nb), dim3(AB), 0, s, Dl.as<c64>(), Dd.as<c64>(), EV.as<c64>(), LA.as<c64>(),
-                       n3, ld, nx, ny, 1.0 / Dx, 1.0 / Dy, dk.as<double>(), act, part.as<double>());
-    std::vector<double> hp(size_t(nb) * nblk);
+                       n3, ld, nx, ny, nz, 1.0 / Dx, 1.0 / Dy, nz > 1 ? 1.0 / Dz : 0.0, dk.as<double>(), act,
+                       part.as<double>(), Lbar ? partL.as<double>() : nullptr);
+    std::vector<double> hp(size_t(nb) * nblk), hl(size_t(nb) * nblk);
     OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), part.as<void>(), sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
+    if (Lbar)
+      OMX_HIP_CHECK(hipMemcpyAsync(hl.data(), partL.as<void>(), sizeof(double) * hl.size(), hipMemcpyDeviceToHost, s));
     OMX_HIP_CHECK(hipStreamSynchronize(s));
     for (int i = 0; i < nb; ++i)
-      if (hasv[i]) khb[i] = sum_part(hp.data() + size_t(i) * nblk, nblk);
+      if (hasv[i]) {
+        khb[i] = sum_part(hp.data() + size_t(i) * nblk, nblk);
+        if (Lbar) LHb[i] = sum_part(hl.data() + size_t(i) * nblk, nblk);
+      }
   }
   // eigenvalue channel: lambda = ((kbar + khbar)/slope) ev -> epsbar += epsinv_bar(d, s N d)
+  // and, on 3D cells, Λ̄ += Λ̄_H + k̄tot (-∂ω²∂Λ / ∂ω²∂k) (grads/period.jl:224-232)
   std::vector<double> cf(nb);
-  double wb = 0;
+  double wb = 0, lb = 0;
   for (int i = 0; i < nb; ++i) {
     const double kb = kbar ? kbar[i] : 0.0;
     const double sc = (kb + khb[i]) / slope[i];
     cf[i] = sc * double(N);
     wb += 2.0 * omega * sc;
+    lb += LHb[i] - sc * dL[i];
   }
   OMX_HIP_CHECK(hipMemcpyAsync(cfac.as<void>(), cf.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
   OMX_HIP_CHECK(hipMemcpyAsync(actd.as<void>(), all.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
@@ -255,18 +287,24 @@ void Solver::adjoint(double omega, int nev, const double* kmags, const double* e
   std::vector<double> h6(6 * size_t(N));
   OMX_HIP_CHECK(hipMemcpyAsync(h6.data(), eb6.as<void>(), sizeof(double) * h6.size(), hipMemcpyDeviceToHost, s));
   OMX_HIP_CHECK(hipStreamSynchronize(s));
-  // (3,3,Nx,Ny) column-major, symmetric
+  // (3,3,Nx,Ny) column-major, mirrored (2D convention, grads/solve.jl:490-515); on 3D cells
+  // (3,3,Nx,Ny,Nz) with the off-diagonal sums above the diagonal and zeros below (:464-487)
+  const bool mirror = nz <= 1;
   for (long p = 0; p < N; ++p) {
     const double* a = h6.data() + 6 * p;
     double* o = epsinvbar + 9 * p;
     o[0] = a[0];
     o[4] = a[1];
     o[8] = a[2];
-    o[3] = o[1] = a[3];
-    o[6] = o[2] = a[4];
-    o[7] = o[5] = a[5];
+    o[3] = a[3];
+    o[6] = a[4];
+    o[7] = a[5];
+    o[1] = mirror ? a[3] : 0.0;
+    o[2] = mirror ? a[4] : 0.0;
+    o[5] = mirror ? a[5] : 0.0;
   }
   *omegabar = wb;
+  if (Lbar) *Lbar = lb;
 }
 
 // ng_gvd / ng_gvd_E (lib/ModeAnalysis/src/group_index.jl:126-162, :55-95) per mode, FP64:

@@ -244,6 +244,15 @@ int omx_solve_k_adjoint(omx_solver* h, double omega, int nev, const double* kmag
   });
 }
 
+int omx_solve_k_periodic_adjoint(omx_solver* h, double omega, int nev, const double* kmags, const double* evecs,
+                                 const double* kbar, const double* evbar, const uint8_t* evbar_mask,
+                                 double* omegabar_out, double* epsinvbar_out, double* Lbar_out) {
+  return guarded(h, [&](omx::Solver& s) {
+    if (!Lbar_out) throw omx::Error(OMX_ERR_INVALID_ARG, "NULL Lbar_out");
+    s.adjoint(omega, nev, kmags, evecs, kbar, evbar, evbar_mask, omegabar_out, epsinvbar_out, Lbar_out);
+  });
+}
+
 int omx_get_kernel_stats(omx_solver* h, const char* family, double* total_ms, int64_t* launches, double* bytes) {
   if (!family || !total_ms || !launches || !bytes) return OMX_ERR_INVALID_ARG;
   return guarded(h, [&](omx::Solver& s) {

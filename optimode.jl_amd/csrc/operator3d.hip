@@ -123,9 +123,10 @@ __global__ __launch_bounds__(B3) void k3_cols(OpPlan<T> plan, cplx<T>* D, VecSel
   }
 }
 
-// z-pencils of TZ consecutive ix at one iy: FFT along z, the per-voxel eps^-1 (eps_ave for
-// OP_P), then (MODE 0) the inverse FFT along z; MODE 1 is the FIELD_E helper: forward
-// transform and eps^-1 only, times `scale`.
+// z-pencils of TZ consecutive ix at one iy.  MODE 0 (operator): FFT along z, the per-voxel
+// eps^-1 (eps_ave for OP_P), inverse FFT along z.  Field helpers: MODE 1 (FIELD_E) forward
+// FFT and eps^-1 times `scale`; MODE 2 (FIELD_D) forward FFT times `scale`; MODE 3
+// (FIELD_EINV) eps^-1 times `scale`, then the backward FFT.
 template <typename T, int OP, int VPT, int MODE>
 __global__ __launch_bounds__(B3) void k3_pencils(OpPlan<T> plan, cplx<T>* D, VecSel sel, T scale) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(B3) void k3_pencils(OpPlan<T> plan, cplx<T>* D, Vec
     buf[(t * 3 + c) * nz + iz] = Dv[((long(iz) * ny + iy) * nx + ix0 + t) * 3 + c];
   }
   __syncthreads();
-  fft_lds<T, MODE == 1 ? -1 : Dirs3<OP>::first, VPT>(buf, 3 * TZ, nz, 1, plan.fz, tw);
+  if (MODE != 3) fft_lds<T, MODE == 0 ? Dirs3<OP>::first : -1, VPT>(buf, 3 * TZ, nz, 1, plan.fz, tw);
   for (int e = threadIdx.x; e < nz * TZ; e += blockDim.x) {
     const int iz = e / TZ;
     const int t = e - iz * TZ;
@@ -157,14 +158,18 @@ __global__ __launch_bounds__(B3) void k3_pencils(OpPlan<T> plan, cplx<T>* D, Vec
     cplx<T>* b1 = buf + (t * 3 + 1) * nz + iz;
     cplx<T>* b2 = buf + (t * 3 + 2) * nz + iz;
     const cplx<T> d0 = *b0, d1 = *b1, d2 = *b2;
-    if (OP == OP_P) {
+    if (MODE == 2) {
+      *b0 = d0 * scale;
+      *b1 = d1 * scale;
+      *b2 = d2 * scale;
+    } else if (OP == OP_P) {
       const T ea = T(3) / (ep[0] + ep[1] + ep[2]);
       *b0 = d0 * ea;
       *b1 = d1 * ea;
       *b2 = d2 * ea;
     } else {
       const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
-      const T s = MODE == 1 ? scale : T(1);
+      const T s = MODE == 0 ? T(1) : scale;
       *b0 = (d0 * exx + d1 * exy + d2 * exz) * s;
       *b1 = (d0 * exy + d1 * eyy + d2 * eyz) * s;
       *b2 = (d0 * exz + d1 * eyz + d2 * ezz) * s;
@@ -172,6 +177,7 @@ __global__ __launch_bounds__(B3) void k3_pencils(OpPlan<T> plan, cplx<T>* D, Vec
   }
   __syncthreads();
   if (MODE == 0) fft_lds<T, Dirs3<OP>::second, VPT>(buf, 3 * TZ, nz, 1, plan.fz, tw);
+  if (MODE == 3) fft_lds<T, +1, VPT>(buf, 3 * TZ, nz, 1, plan.fz, tw);
   for (int e = threadIdx.x; e < nz * TZ * 3; e += blockDim.x) {
     const int c = e % 3;
     const int q = e / 3;
@@ -254,6 +260,38 @@ __global__ __launch_bounds__(B3) void k3_rows3(OpPlan<T> plan, BView<T> Hin, BVi
       __syncthreads();
     }
     if (threadIdx.x == 0) dot_partial[long(v) * dps + long(iz) * gridDim.x + blockIdx.x] = -red[0];
+  }
+}
+
+// rows of plane iz of the field helpers' intermediate: backward FFT along x, in place
+template <typename T, int VPT>
+__global__ __launch_bounds__(B3) void k3_rows_inv(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = plan.nx, ny = plan.ny, nz = plan.nz, RB = plan.rows_per_block;
+  cplx<T>* buf = tw + nx;
+  const int vz = blockIdx.y;
+  const int v = vz / nz;
+  const int inst = v / sel.mper, col = v - (v / sel.mper) * sel.mper;
+  if (!selected3(sel, inst, col)) return;
+  const int iy0 = blockIdx.x * RB;
+  for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = plan.twx[t];
+  cplx<T>* Dv = D + long(vz) * 3 * nx * ny;
+  for (int e = threadIdx.x; e < RB * nx * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int r = q / nx;
+    const int ix = q - r * nx;
+    buf[(r * 3 + c) * nx + ix] = Dv[(long(iy0 + r) * nx + ix) * 3 + c];
+  }
+  __syncthreads();
+  fft_lds<T, +1, VPT>(buf, 3 * RB, nx, 1, plan.fx, tw);
+  for (int e = threadIdx.x; e < RB * nx * 3; e += blockDim.x) {
+    const int c = e % 3;
+    const int q = e / 3;
+    const int r = q / nx;
+    const int ix = q - r * nx;
+    Dv[(long(iy0 + r) * nx + ix) * 3 + c] = buf[(r * 3 + c) * nx + ix];
   }
 }
 
@@ -513,12 +551,21 @@ void op3(const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel
 }
 
 template <typename T, int VPT>
-void field_e3_vpt(const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel, const double* kinst, T scale,
-                  hipStream_t s) {
+void fields3_vpt(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
+                 const double* kinst, T scale, hipStream_t s) {
   const Geo3 g = geometry3(plan, sel);
+  if (fop == FIELD_EINV) {
+    hipLaunchKernelGGL((k3_pencils<T, OP_M, VPT, 3>), g.g3, dim3(B3), g.lds3, s, plan, D, sel, scale);
+    hipLaunchKernelGGL((k3_cols<T, +1, VPT>), g.g2, dim3(B3), g.lds2, s, plan, D, sel);
+    hipLaunchKernelGGL((k3_rows_inv<T, VPT>), g.g1, dim3(B3), g.lds13, s, plan, D, sel);
+    return;
+  }
   hipLaunchKernelGGL((k3_rows1<T, OP_M, VPT>), g.g1, dim3(B3), g.lds13, s, plan, X, D, sel, kinst);
   hipLaunchKernelGGL((k3_cols<T, -1, VPT>), g.g2, dim3(B3), g.lds2, s, plan, D, sel);
-  hipLaunchKernelGGL((k3_pencils<T, OP_M, VPT, 1>), g.g3, dim3(B3), g.lds3, s, plan, D, sel, scale);
+  if (fop == FIELD_E)
+    hipLaunchKernelGGL((k3_pencils<T, OP_M, VPT, 1>), g.g3, dim3(B3), g.lds3, s, plan, D, sel, scale);
+  else
+    hipLaunchKernelGGL((k3_pencils<T, OP_M, VPT, 2>), g.g3, dim3(B3), g.lds3, s, plan, D, sel, scale);
 }
 
 }  // namespace
@@ -535,15 +582,17 @@ void launch_operator3d(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y,
   OMX_HIP_CHECK(hipGetLastError());
 }
 
-// FIELD_E only on 3D cells: D = eps^-1 . fft3(kx_tc(H)) * scale, [v][iz][iy][ix][c]
+// Field helpers on 3D cells, layout [v][iz][iy][ix][c]: FIELD_E (D = eps^-1 . fft3(kx_tc(H))
+// * scale), FIELD_D (D = fft3(kx_tc(H)) * scale) and FIELD_EINV (D <- bfft3(eps^-1 . D) * scale)
 template <typename T>
 void launch_fields3d(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                      const double* kinst, double scale, hipStream_t stream) {
-  if (fop != FIELD_E) throw Error(OMX_ERR_UNSUPPORTED, "this field helper is built for 2D cells only");
+  if (fop != FIELD_E && fop != FIELD_D && fop != FIELD_EINV)
+    throw Error(OMX_ERR_UNSUPPORTED, "this field helper is built for 2D cells only");
   switch (plan.vpt) {
-    case 8: field_e3_vpt<T, 8>(plan, X, D, sel, kinst, T(scale), stream); break;
-    case 16: field_e3_vpt<T, 16>(plan, X, D, sel, kinst, T(scale), stream); break;
-    case 32: field_e3_vpt<T, 32>(plan, X, D, sel, kinst, T(scale), stream); break;
+    case 8: fields3_vpt<T, 8>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
+    case 16: fields3_vpt<T, 16>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
+    case 32: fields3_vpt<T, 32>(fop, plan, X, D, sel, kinst, T(scale), stream); break;
     default: throw Error(OMX_ERR_INTERNAL, "operator plan has an unsupported values-per-thread bound");
   }
   OMX_HIP_CHECK(hipGetLastError());

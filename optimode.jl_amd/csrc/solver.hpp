@@ -189,8 +189,10 @@ class Solver {
   void hmh(double k, const double* A, const double* H, int m, double* out);
   void group_index(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
                    double* ng_out);
+  // Lbar: NULL, or (3D cells) the period cotangent of solve_k_periodic
   void adjoint(double omega, int nev, const double* kmags, const double* evecs, const double* kbar,
-               const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar);
+               const double* evbar, const uint8_t* evbar_mask, double* omegabar, double* epsinvbar,
+               double* Lbar = nullptr);
   void ng_gvd(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
               const double* d2eps_domega2, double* ng_out, double* gvd_out, double* E_out);
   void build_modes(int slot, int nev, const double* kmags, const double* evecs, int conjugate, int normalize,

@@ -5,7 +5,7 @@
 #   solve_ω²   lib/MaxwellEigenmodes/src/solve.jl:38-63, solvers/gpu.jl:201-211
 #   solve_k    lib/MaxwellEigenmodes/src/solve.jl:182-196, solvers/gpu.jl:213-242
 #   rrule      lib/MaxwellEigenmodes/src/solve.jl:213-289, solvers/gpu.jl:292-365
-#   3D cells   solve_k on Grid{3}, rrule(solve_k_periodic) for k cotangents (src/grads/period.jl:144-249)
+#   3D cells   solve_k on Grid{3} and its rrule, rrule(solve_k_periodic) (src/grads/period.jl:144-249)
 # Host data stays Julia-owned Float64 / ComplexF64 column-major arrays; the library copies
 # them to the device.  Non-zero statuses become `error(...)` with the library's message.
 module MaxwellEigenmodesHIPExt
@@ -144,27 +144,32 @@ function solve_k(ω::Real, ε⁻¹::Union{AbstractArray{Float64,4},AbstractArray
     return kmags, [evecs[:, j] for j in 1:nev]
 end
 
-function ChainRulesCore.rrule(::typeof(solve_k), ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64},
-                              solver::HIPSolver; nev=1, kwargs...)
+# k̄ / ēvecs cotangents (AbstractZero entries are zero) packed for the C-ABI pull-backs
+function _pack_cotangents(ΔΩ, nev, n2)
+    k̄mags, ēvecs = ΔΩ
+    kb = zeros(Float64, nev)
+    if !(k̄mags isa AbstractZero)
+        for (j, v) in enumerate(k̄mags)
+            kb[j] = v isa AbstractZero ? 0.0 : Float64(v)
+        end
+    end
+    eb = zeros(ComplexF64, n2, nev)
+    mask = zeros(UInt8, nev)
+    if !(ēvecs isa AbstractZero)
+        for (j, v) in enumerate(ēvecs)
+            v isa AbstractZero && continue
+            eb[:, j] .= v
+            mask[j] = 0x01
+        end
+    end
+    return kb, eb, mask
+end
+
+function ChainRulesCore.rrule(::typeof(solve_k), ω::Real, ε⁻¹::Union{AbstractArray{Float64,4},AbstractArray{Float64,5}},
+                              grid::Union{Grid{2,Float64},Grid{3,Float64}}, solver::HIPSolver; nev=1, kwargs...)
     kmags, evecs = solve_k(ω, ε⁻¹, grid, solver; nev, kwargs...)
     function hip_solve_k_pullback(ΔΩ)
-        k̄mags, ēvecs = ΔΩ
-        n2 = 2 * N(grid)
-        kb = zeros(Float64, nev)
-        if !(k̄mags isa AbstractZero)
-            for (j, v) in enumerate(k̄mags)
-                kb[j] = v isa AbstractZero ? 0.0 : Float64(v)
-            end
-        end
-        eb = zeros(ComplexF64, n2, nev)
-        mask = zeros(UInt8, nev)
-        if !(ēvecs isa AbstractZero)
-            for (j, v) in enumerate(ēvecs)
-                v isa AbstractZero && continue
-                eb[:, j] .= v
-                mask[j] = 0x01
-            end
-        end
+        kb, eb, mask = _pack_cotangents(ΔΩ, nev, 2 * N(grid))
         E = reduce(hcat, evecs)
         ω̄ = Ref(0.0)
         ε̄⁻¹ = zeros(Float64, size(ε⁻¹))
@@ -198,27 +203,26 @@ function hip_period_derivatives(solver::HIPSolver, kmags::AbstractVector{<:Real}
     return dk, dL
 end
 
-# rrule(solve_k_periodic, ...) (src/grads/period.jl:236-249) for cotangents on kmags: the
-# ēvecs = Zero branch of _solve_k_period_grads (:182-234).  Eigenvector cotangents and ε̄⁻¹
-# on 3D cells are not built on this backend.
+# rrule(solve_k_periodic, ...) (src/grads/period.jl:236-249): both channels of
+# _solve_k_period_grads (:163-234) on the GPU -- ω̄, ε̄⁻¹ (3D convention: off-diagonal sums
+# above the diagonal) and Λ̄.
 function ChainRulesCore.rrule(::typeof(solve_k_periodic), ω::Real, ε⁻¹::AbstractArray{Float64,5}, Λ::Real,
                               grid::Grid{3,Float64}, solver::HIPSolver; nev=1, kwargs...)
     g = Grid(grid.Δx, grid.Δy, Float64(Λ), grid.Nx, grid.Ny, grid.Nz)
     kmags, evecs = solve_k(ω, ε⁻¹, g, solver; nev, kwargs...)
     function hip_solve_k_periodic_pullback(ΔΩ)
-        k̄mags, ēvecs = ΔΩ
-        ēvecs isa AbstractZero || all(v -> v isa AbstractZero, ēvecs) ||
-            error("HIPSolver: eigenvector cotangents of solve_k_periodic are not supported")
-        kb = zeros(Float64, nev)
-        if !(k̄mags isa AbstractZero)
-            for (j, v) in enumerate(k̄mags)
-                kb[j] = v isa AbstractZero ? 0.0 : Float64(v)
-            end
-        end
-        dk, dL = hip_period_derivatives(solver, kmags, evecs, ε⁻¹, g)
-        ω̄ = sum(2 * ω .* kb ./ dk)
-        Λ̄ = sum(kb .* (-dL ./ dk))
-        return (NoTangent(), ω̄, NoTangent(), Λ̄, NoTangent(), NoTangent())
+        kb, eb, mask = _pack_cotangents(ΔΩ, nev, 2 * N(g))
+        E = reduce(hcat, evecs)
+        ω̄ = Ref(0.0)
+        Λ̄ = Ref(0.0)
+        ε̄⁻¹ = zeros(Float64, size(ε⁻¹))
+        h = _handle!(solver, g, ε⁻¹)
+        st = ccall((:omx_solve_k_periodic_adjoint, libomx), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Ptr{Cdouble}, Ptr{ComplexF64},
+                    Ptr{UInt8}, Ref{Cdouble}, Ptr{Cdouble}, Ref{Cdouble}),
+                   h, ω, nev, kmags, E, kb, eb, mask, ω̄, ε̄⁻¹, Λ̄)
+        _check(st, h)
+        return (NoTangent(), ω̄[], ε̄⁻¹, Λ̄[], NoTangent(), NoTangent())
     end
     return (kmags, evecs), hip_solve_k_periodic_pullback
 end

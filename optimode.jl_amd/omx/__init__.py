@@ -397,32 +397,44 @@ def rrule_solve_k(omega, epsinv, grid, solver, nev=1, **kw):
     ``evbar`` may be ``None`` or contain ``None`` entries (zero cotangents)."""
     kmags, evecs = solve_k(omega, epsinv, grid, solver, nev=nev, **kw)
     h = solver.handle(grid, epsinv)
-    n2 = 2 * grid.N
 
     def pullback(cot):
-        kbar, evbar = cot
-        kb = np.zeros(nev)
-        if kbar is not None:
-            for j, v in enumerate(kbar):
-                kb[j] = 0.0 if v is None else float(v)
-        eb = np.zeros((nev, n2), dtype=np.complex128)
-        mask = np.zeros(nev, dtype=np.uint8)
-        if evbar is not None:
-            for j, v in enumerate(evbar):
-                if v is not None:
-                    eb[j] = np.asarray(v, dtype=np.complex128)
-                    mask[j] = 1
-        E = np.ascontiguousarray(np.stack(evecs), dtype=np.complex128)
-        wbar = np.zeros(1)
-        eibar_c = np.zeros((grid.Ny, grid.Nx, 3, 3))
-        st = _lib.lib().omx_solve_k_adjoint(h.h, float(omega), int(nev), _lib.dptr(np.ascontiguousarray(kmags)),
-                                            _lib.dptr(E), _lib.dptr(kb), _lib.dptr(eb),
-                                            mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _lib.dptr(wbar),
-                                            _lib.dptr(eibar_c))
-        _lib.check(st, h.h)
-        return (None, float(wbar[0]), np.ascontiguousarray(eibar_c.transpose(3, 2, 1, 0)), None, None)
+        wbar, eibar, _ = _adjoint(h, grid, omega, nev, kmags, evecs, cot, period=False)
+        return (None, wbar, eibar, None, None)
 
     return (kmags, evecs), pullback
+
+
+def _adjoint(h, grid, omega, nev, kmags, evecs, cot, period):
+    """Shared body of the solve_k / solve_k_periodic pull-backs: cotangents ``(kbar, evbar)``
+    (``None`` entries are zero) -> ``(ω̄, ε̄⁻¹, Λ̄ or None)`` through the C-ABI."""
+    kbar, evbar = cot
+    kb = np.zeros(nev)
+    if kbar is not None:
+        for j, v in enumerate(kbar):
+            kb[j] = 0.0 if v is None else float(v)
+    eb = np.zeros((nev, 2 * grid.N), dtype=np.complex128)
+    mask = np.zeros(nev, dtype=np.uint8)
+    if evbar is not None:
+        for j, v in enumerate(evbar):
+            if v is not None:
+                eb[j] = np.asarray(v, dtype=np.complex128)
+                mask[j] = 1
+    E = np.ascontiguousarray(np.stack(evecs), dtype=np.complex128)
+    wbar, Lbar = np.zeros(1), np.zeros(1)
+    # column-major (3,3,Nx,Ny[,Nz]): C order reversed
+    dims = (grid.Nx, grid.Ny) + ((grid.Nz,) if isinstance(grid, Grid3) else ())
+    eibar_c = np.zeros(tuple(reversed(dims)) + (3, 3))
+    args = [h.h, float(omega), int(nev), _lib.dptr(np.ascontiguousarray(kmags, dtype=np.float64)), _lib.dptr(E),
+            _lib.dptr(kb), _lib.dptr(eb), mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _lib.dptr(wbar),
+            _lib.dptr(eibar_c)]
+    if period:
+        st = _lib.lib().omx_solve_k_periodic_adjoint(*args, _lib.dptr(Lbar))
+    else:
+        st = _lib.lib().omx_solve_k_adjoint(*args)
+    _lib.check(st, h.h)
+    eibar = np.ascontiguousarray(eibar_c.transpose(tuple(range(eibar_c.ndim - 1, -1, -1))))
+    return float(wbar[0]), eibar, (float(Lbar[0]) if period else None)
 
 
 def _grid_with_period(grid, L):
@@ -455,22 +467,17 @@ def period_derivatives(kmags, evecs, epsinv, grid, solver):
 
 def rrule_solve_k_periodic(omega, epsinv, L, grid, solver, nev=1, **kw):
     """``rrule(solve_k_periodic, ω, ε⁻¹, Λ, grid, solver; nev)`` (period.jl:236-249) ->
-    ``((kmags, evecs), pullback)``.  The pull-back maps cotangents on ``kmags`` to
-    ``(ω̄, Λ̄)`` (the ``ēvecs = Zero`` branch of ``_solve_k_period_grads``, period.jl:182-234:
-    ``ω̄ = Σ 2ω k̄/∂ω²∂k``, ``Λ̄ = Σ k̄ (-∂ω²∂Λ/∂ω²∂k)``).  Eigenvector cotangents and ε̄⁻¹ on
-    3D cells are not built: a non-zero ``ēvecs`` raises."""
+    ``((kmags, evecs), pullback)``.  ``pullback((kbar, evbar))`` returns
+    ``(None, ω̄, ε̄⁻¹, Λ̄, None, None)`` (``_solve_k_period_grads``, period.jl:163-234): both
+    channels, the eigenvector one through ``eig_adjt`` and the plane-wave basis, on the GPU
+    in FP64; ε̄⁻¹ is ``(3,3,Nx,Ny,Nz)`` with the off-diagonal sums above the diagonal."""
     g = _grid_with_period(grid, L)
     kmags, evecs = solve_k(omega, epsinv, g, solver, nev=nev, **kw)
+    h = solver.handle(g, epsinv)
 
     def pullback(cot):
-        kbar, evbar = cot
-        if evbar is not None and any(v is not None and np.any(v) for v in evbar):
-            raise OmxError(_lib.OMX_ERR_UNSUPPORTED, "eigenvector cotangents on 3D cells are not supported")
-        kb = np.zeros(nev) if kbar is None else np.array([0.0 if v is None else float(v) for v in kbar])
-        dk, dL = period_derivatives(kmags, evecs, epsinv, g, solver)
-        wbar = float(np.sum(2.0 * float(omega) * kb / dk))
-        Lbar = float(np.sum(kb * (-dL / dk)))
-        return wbar, Lbar
+        wbar, eibar, Lbar = _adjoint(h, g, omega, nev, kmags, evecs, cot, period=True)
+        return (None, wbar, eibar, Lbar, None, None)
 
     return (kmags, evecs), pullback
 

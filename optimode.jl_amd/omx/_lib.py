@@ -89,6 +89,8 @@ SIGNATURES = [
     ("omx_get_evecs", ctypes.c_int, [_vp, _dp]),
     ("omx_solve_k_adjoint", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
                                            ctypes.POINTER(ctypes.c_uint8), _dp, _dp]),
+    ("omx_solve_k_periodic_adjoint", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                                    ctypes.POINTER(ctypes.c_uint8), _dp, _dp, _dp]),
     ("omx_get_kernel_stats", ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
     ("omx_solve_k_batch", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OmxProblem), ctypes.c_int,
                                          ctypes.c_int]),

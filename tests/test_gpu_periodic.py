@@ -118,9 +118,12 @@ def test_solve_k_periodic_pullback_matches_oracle_and_finite_differences():
     kw = dict(nev=2, k_tol=1e-13, eig_tol=1e-11, maxiter=400)
     (km, evs), pb = omx.rrule_solve_k_periodic(omega, ei, L, g, solver, **kw)
     kbar = [1.0, -0.5]
-    wbar, Lbar = pb((kbar, None))
+    _, wbar, eibar, Lbar, _, _ = pb((kbar, None))
     wref, Lref = P.solve_k_periodic_kbar_pullback(omega, ei, L, og, km, evs, kbar)
     assert abs(wbar - wref) < 1e-9 * abs(wref) and abs(Lbar - Lref) < 1e-9 * abs(Lref)
+    _, eref, _ = P.solve_k_periodic_pullback(omega, ei, L, og, km, evs, kbar, None)
+    assert eibar.shape == (3, 3) + og.shape
+    assert np.max(np.abs(eibar - eref)) < 1e-9 * np.max(np.abs(eref))
     h = 1e-4
 
     def f(w, Lp):
@@ -134,13 +137,87 @@ def test_solve_k_periodic_pullback_matches_oracle_and_finite_differences():
     assert abs(Lbar - fd_L) < 1e-5 * abs(fd_L)
 
 
+def _full_pullback_case():
+    og, g = _grids((12, 12, 6), D=(2.0, 1.6, 0.45))
+    ei = P.bragg_wg_epsi(og, eps_core=4.0, eps_bg=2.1, w0=0.5, h0=0.4, mod=0.5)
+    return og, g, ei
+
+
+def test_solve_k_periodic_full_pullback_matches_oracle():
+    # both channels: k̄ on band 1, ēv = 2 c <c, ev> on both bands (the cotangent of |<ev, c>|^2)
+    print("3D full period pull-back", flush=True)
+    og, g, ei = _full_pullback_case()
+    omega, L = 0.5, 0.45
+    solver = omx.HIPSolver(precision="fp64")
+    kw = dict(nev=2, k_tol=1e-13, eig_tol=1e-11, maxiter=400)
+    (km, evs), pb = omx.rrule_solve_k_periodic(omega, ei, L, g, solver, **kw)
+    rng = np.random.default_rng(21)
+    c = rng.standard_normal(2 * og.N) + 1j * rng.standard_normal(2 * og.N)
+    evbar = [2 * c * np.vdot(c, evs[j]) for j in range(2)]
+    for kbar, eb in (([0.7, None], evbar), (None, [None, evbar[1]])):
+        _, wbar, eibar, Lbar, _, _ = pb((kbar, eb))
+        wref, eref, Lref = P.solve_k_periodic_pullback(omega, ei, L, og, km, evs, kbar or [None, None], eb)
+        print(f"  wbar {wbar} ref {wref}; Lbar {Lbar} ref {Lref}", flush=True)
+        assert abs(wbar - wref) < 1e-7 * abs(wref)
+        assert abs(Lbar - Lref) < 1e-7 * abs(Lref)
+        assert np.max(np.abs(eibar - eref)) < 1e-7 * np.max(np.abs(eref))
+        # the 3D convention: nothing below the diagonal
+        for a, b in ((1, 0), (2, 0), (2, 1)):
+            assert not np.any(eibar[a, b])
+
+
+def test_solve_k_periodic_eigenvector_pullback_matches_finite_differences():
+    # J = |<ev_2, c>|^2 through GPU re-solves, in ω and in Λ
+    og, g, ei = _full_pullback_case()
+    omega, L = 0.5, 0.45
+    solver = omx.HIPSolver(precision="fp64")
+    kw = dict(nev=2, k_tol=1e-13, eig_tol=1e-12, maxiter=500)
+    (km, evs), pb = omx.rrule_solve_k_periodic(omega, ei, L, g, solver, **kw)
+    c = np.random.default_rng(22).standard_normal(2 * og.N) + 0j
+    _, wbar, _, Lbar, _, _ = pb((None, [None, 2 * c * np.vdot(c, evs[1])]))
+
+    def J(w, Lp):
+        _, ev = omx.solve_k_periodic(w, ei, Lp, g, solver, **kw)
+        return abs(np.vdot(ev[1], c)) ** 2
+
+    h = 1e-4
+    fd_w = (J(omega + h, L) - J(omega - h, L)) / (2 * h)
+    fd_L = (J(omega, L + h) - J(omega, L - h)) / (2 * h)
+    print(f"  wbar {wbar} fd {fd_w}; Lbar {Lbar} fd {fd_L}", flush=True)
+    assert abs(wbar - fd_w) < 1e-4 * abs(fd_w)
+    assert abs(Lbar - fd_L) < 1e-4 * abs(fd_L)
+
+
+def test_solve_k_adjoint_on_a_3d_handle_matches_oracle():
+    # rrule(solve_k) on a 3D cell: ω̄ and ε̄⁻¹ of the periodic pull-back, no period cotangent
+    og, g, ei = _full_pullback_case()
+    solver = omx.HIPSolver(precision="fp64")
+    (km, evs), pb = omx.rrule_solve_k(0.5, ei, g, solver, nev=1, k_tol=1e-13, eig_tol=1e-11, maxiter=400)
+    c = np.random.default_rng(23).standard_normal(2 * og.N) + 0j
+    eb = [2 * c * np.vdot(c, evs[0])]
+    _, wbar, eibar, _, _ = pb(([1.0], eb))
+    wref, eref, _ = P.solve_k_periodic_pullback(0.5, ei, og.Dz, og, km, evs, [1.0], eb)
+    assert abs(wbar - wref) < 1e-7 * abs(wref)
+    assert np.max(np.abs(eibar - eref)) < 1e-7 * np.max(np.abs(eref))
+
+
 def test_3d_handles_reject_2d_only_paths():
     og, g = _grids((8, 8, 4), D=(2.4, 2.0, 0.45))
     ei = P.bragg_wg_epsi(og, eps_core=4.0, eps_bg=2.1, w0=0.5, h0=0.4, mod=0.5)
     solver = omx.HIPSolver()
     km, evs = omx.solve_k(0.5, ei, g, solver, nev=1, maxiter=300)
+    d = np.zeros_like(ei)
     with pytest.raises(omx.OmxError) as e:
-        omx.rrule_solve_k(0.5, ei, g, solver, nev=1)[1](([1.0], None))
+        omx.ng_gvd(0.5, km[0], evs[0], ei, d, d, g, solver)
+    assert e.value.status == omx._lib.OMX_ERR_UNSUPPORTED
+    # the period cotangent needs a 3D handle
+    og2 = O.Grid(2.4, 2.0, 8, 8)
+    g2 = omx.Grid(2.4, 2.0, 8, 8)
+    ei2 = O.gaussian_wg_epsi([4.0, 2.1, 0.5, 0.4], og2)
+    k2, ev2 = omx.solve_k(0.5, ei2, g2, solver, nev=1, maxiter=300)
+    h2 = solver.handle(g2, ei2)
+    with pytest.raises(omx.OmxError) as e:
+        omx._adjoint(h2, g2, 0.5, 1, k2, ev2, ([1.0], None), period=True)
     assert e.value.status == omx._lib.OMX_ERR_UNSUPPORTED
     with pytest.raises(omx.OmxError) as e:  # Nz = 7: a prime factor the FFT does not take
         omx.solve_k(0.5, np.repeat(ei[..., :1], 7, axis=-1), omx.Grid3(2.4, 2.0, 0.9, 8, 8, 7), solver, nev=1)
