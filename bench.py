@@ -311,6 +311,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-algorithm comparison solve")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison solves")
     ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "periodic", "batch"],
                     help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain; "
                          "periodic: solve_k on a 128x96x24 3D Bragg-grating cell; batch: the 64-instance "
@@ -388,6 +389,7 @@ def main():
                  "modes_converged": nconv, "lobpcg_iters_fp64": infop["lobpcg_iters_fp64"],
                  "newton_steps": infop["n_newton"], "matvecs": infop["n_matvec"],
                  "speedup": round(value / (nconv / dtp), 2) if nconv else None}
+    learned = None if a.no_learned else learned_start(a, OMEGA, ei, grid, kw, slope_ref, k_ref, value, infos)
     roofline = None
     if op_n > 0 and op_ms > 0:
         achieved = op_bytes / (op_ms * 1e-3) / 1e9
@@ -428,10 +430,41 @@ def main():
             "parallelism": "single GPU; bands batched as independent Newton instances",
         },
         "plain": plain,
+        "learned_start": learned,
         "roofline": roofline,
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(out), flush=True)
+
+
+def learned_start(a, omega, ei, grid, kw, slope_ref, k_ref, value, infos):
+    """The same solve started from the trained warm-start network (omx/data/warmnet_tfln256.npz,
+    trained on 512 solver-converged rib draws): the guess (features, network forward, start
+    vectors) is inside the timed region.  Outside the headline's timed region."""
+    import numpy as np
+    import omx
+    from omx import warmstart as WS
+    path = os.path.join(WS.DATA_DIR, "warmnet_tfln256.npz")
+    if not os.path.exists(path):
+        return None
+    st = WS.LearnedStart.load(path)
+    solver = omx.HIPSolver(precision=a.precision)
+    for _ in range(max(1, a.warmup)):
+        WS.solve_k_learned(omega, ei, grid, solver, st, **kw)
+    conv, li = 0, []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        km, _, info = WS.solve_k_learned(omega, ei, grid, solver, st, **kw)
+        conv += int(np.sum(np.abs(slope_ref * (km - k_ref)) / omega ** 2 <= REL_TOL))
+        li.append(info)
+    dt = time.perf_counter() - t0
+    return {"modes_per_s": round(conv / dt, 3), "ms_per_step": round(1e3 * dt / a.steps, 2),
+            "modes_converged": conv, "vs_plain_start": round((conv / dt) / value, 3) if value else None,
+            "device_ms_per_step": round(float(np.mean([i["device_ms"] for i in li])), 2),
+            "device_ms_per_step_plain_start": round(float(np.mean([i["device_ms"] for i in infos])), 2),
+            "lobpcg_iters_fp32_per_step": float(np.mean([i["lobpcg_iters_fp32"] for i in li])),
+            "lobpcg_iters_fp32_per_step_plain_start": float(np.mean([i["lobpcg_iters_fp32"] for i in infos])),
+            "newton_steps_per_step": float(np.mean([i["n_newton"] for i in li]))}
 
 
 if __name__ == "__main__":

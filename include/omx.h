@@ -94,7 +94,8 @@ int omx_destroy(omx_solver* h);
 /* update_ε⁻¹ (maxwell.jl:1114-1136) */
 int omx_set_epsinv(omx_solver* h, const double* epsinv);
 const char* omx_last_error(const omx_solver* h);
-/* tuning knobs: "fp32_tol", "fp32_switch", "maxiter_fp64", "guard", "seed" */
+/* tuning knobs: "fp32_tol", "fp32_switch", "maxiter_fp64", "guard", "seed"; start modes
+   "warm_start" and "band_guess" (omx_solve_k) */
 int omx_set_option(omx_solver* h, const char* key, double value);
 
 /* HelmholtzMap * H for m vectors at k_vec = (0,0,k): _M! (maxwell.jl:471-481) */
@@ -176,7 +177,9 @@ int omx_solve_omega2(omx_solver* h, double k, int nev, double tol, int maxiter, 
    on the device (fetch them with omx_get_evecs).  With option "warm_start" set and both
    kguess and Hguess NULL, every band starts at the previous solve's k of that band with
    the previous eigenvectors (on the device) as its block -- the EME slice chain
-   (lib/EigenmodeExpansion/src/eme.jl:58-76). */
+   (lib/EigenmodeExpansion/src/eme.jl:58-76).  With option "band_guess" set, kguess holds nev
+   values and Hguess at least one vector: band j starts at kguess[j] with the Hguess block
+   and the shared first eigensolve is skipped (the learned warm start, omx_warmnet_*). */
 int omx_solve_k(omx_solver* h, double omega, int nev, double k_tol, double eig_tol, int max_eigsolves,
                 int maxiter, const double* kguess, const double* Hguess, int nguess, double* kmags_out,
                 double* evecs_out, omx_info* info);
@@ -223,6 +226,32 @@ int omx_smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nsha
    errors are reported through omx_last_error(NULL). */
 int omx_sliceinv_3x3(int device, int64_t npix, const double* A, double* Y_out);
 int omx_sliceinv_3x3_pullback(int device, int64_t npix, const double* Y, const double* Ybar, double* Abar_out);
+
+/* Learned warm start (no counterpart in the reference, which starts every solve from random
+   vectors and k_guess, src/MaxwellEigenmodes.jl:63): a small convolutional network on the GPU,
+   FP32, mapping a hw x hw map of 7 channels (the 6 unique eps^-1 components and omega) to
+   nev effective-index guesses and 4 nev channels of low-resolution start vectors (re/im of the
+   two plane-wave components per band).  Tensors are [batch][channel][y][x] row-major floats.
+   Layers (3x3 convolutions): 7->16 @hw, pool, 16->32 @hw/2, pool, 32->32 @hw/4 (-> pixel mean
+   -> dense -> nev), [up(.) | 32] 64->32 @hw/2, [up(.) | 16] 48->32 @hw, 32->4nev @hw.
+   Loss: mean (y - t_fields)^2 + wk mean (k - t_k)^2.  hw: multiple of 4 in 4..256; nev 1..16.
+   Errors are reported through omx_last_error(NULL); calls on one network are serialised. */
+typedef struct omx_warmnet omx_warmnet;
+int omx_warmnet_create(int device, int hw, int nev, uint64_t seed, omx_warmnet** out);
+int omx_warmnet_destroy(omx_warmnet* h);
+int omx_warmnet_num_params(omx_warmnet* h, int64_t* n);
+int omx_warmnet_get_params(omx_warmnet* h, float* out);
+int omx_warmnet_set_params(omx_warmnet* h, const float* in);
+/* fields_out (batch x 4nev x hw x hw) and k_out (batch x nev) may each be NULL */
+int omx_warmnet_forward(omx_warmnet* h, int batch, const float* x, float* fields_out, float* k_out);
+/* loss of one batch and its gradient with respect to the flat parameters (grad_out may be NULL) */
+int omx_warmnet_loss_grad(omx_warmnet* h, int batch, const float* x, const float* t_fields, const float* t_k,
+                          float wk, double* loss_out, float* grad_out);
+/* Adam (β 0.9, 0.999) over n samples held on the device: `epochs` seeded shuffles, minibatches
+   of `batch` (the remainder of each shuffle is skipped); loss_hist_out[e] (may be NULL) is the
+   mean minibatch loss of epoch e. */
+int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
+                      int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out);
 
 /* Live kernel statistics (no counterpart in the reference: instrumentation for
    measurement).  With omx_set_option(h, "profile", 1) every operator application inside

@@ -5,6 +5,12 @@
 #include <mutex>
 
 #include "solver.hpp"
+#include "warmnet.hpp"
+
+struct omx_warmnet {
+  std::unique_ptr<omx::WarmNet> n;
+  std::mutex mu;
+};
 
 struct omx_solver {
   std::unique_ptr<omx::Solver> s;
@@ -32,6 +38,29 @@ template <typename F> int guarded(omx_solver* h, F&& f) {
     h->s->last_error = "unknown error";
     return OMX_ERR_INTERNAL;
   }
+}
+
+// handle-free errors (and those of warm-start networks) go to the thread's last error
+template <typename F> int guarded_free(F&& f) {
+  try {
+    f();
+    return OMX_OK;
+  } catch (const omx::Error& e) {
+    g_error = e.what();
+    return e.code();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return OMX_ERR_INTERNAL;
+  } catch (...) {
+    g_error = "unknown error";
+    return OMX_ERR_INTERNAL;
+  }
+}
+
+template <typename F> int guarded_net(omx_warmnet* h, F&& f) {
+  if (!h || !h->n) return OMX_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return guarded_free([&] { f(*h->n); });
 }
 
 }  // namespace
@@ -251,6 +280,51 @@ int omx_solve_k_periodic_adjoint(omx_solver* h, double omega, int nev, const dou
     if (!Lbar_out) throw omx::Error(OMX_ERR_INVALID_ARG, "NULL Lbar_out");
     s.adjoint(omega, nev, kmags, evecs, kbar, evbar, evbar_mask, omegabar_out, epsinvbar_out, Lbar_out);
   });
+}
+
+int omx_warmnet_create(int device, int hw, int nev, uint64_t seed, omx_warmnet** out) {
+  if (!out) return OMX_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guarded_free([&] {
+    auto h = std::make_unique<omx_warmnet>();
+    h->n = std::make_unique<omx::WarmNet>(device, hw, nev, seed);
+    *out = h.release();
+  });
+}
+
+int omx_warmnet_destroy(omx_warmnet* h) {
+  delete h;
+  return OMX_OK;
+}
+
+int omx_warmnet_num_params(omx_warmnet* h, int64_t* n) {
+  if (!n) return OMX_ERR_INVALID_ARG;
+  return guarded_net(h, [&](omx::WarmNet& w) { *n = w.nparams(); });
+}
+
+int omx_warmnet_get_params(omx_warmnet* h, float* out) {
+  if (!out) return OMX_ERR_INVALID_ARG;
+  return guarded_net(h, [&](omx::WarmNet& w) { w.get_params(out); });
+}
+
+int omx_warmnet_set_params(omx_warmnet* h, const float* in) {
+  if (!in) return OMX_ERR_INVALID_ARG;
+  return guarded_net(h, [&](omx::WarmNet& w) { w.set_params(in); });
+}
+
+int omx_warmnet_forward(omx_warmnet* h, int batch, const float* x, float* fields_out, float* k_out) {
+  return guarded_net(h, [&](omx::WarmNet& w) { w.forward(batch, x, fields_out, k_out); });
+}
+
+int omx_warmnet_loss_grad(omx_warmnet* h, int batch, const float* x, const float* t_fields, const float* t_k,
+                          float wk, double* loss_out, float* grad_out) {
+  if (!loss_out) return OMX_ERR_INVALID_ARG;
+  return guarded_net(h, [&](omx::WarmNet& w) { *loss_out = w.loss_grad(batch, x, t_fields, t_k, wk, grad_out); });
+}
+
+int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
+                      int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out) {
+  return guarded_net(h, [&](omx::WarmNet& w) { w.train(n, x, t_fields, t_k, wk, epochs, batch, lr, seed, loss_hist_out); });
 }
 
 int omx_get_kernel_stats(omx_solver* h, const char* family, double* total_ms, int64_t* launches, double* bytes) {

@@ -669,6 +669,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.plain = v != 0 ? 1 : 0;
   else if (key == "warm_start")
     opt.warm_start = v != 0 ? 1 : 0;
+  else if (key == "band_guess")
+    opt.band_guess = v != 0 ? 1 : 0;
   else if (key == "refine")
     opt.refine = v != 0 ? 1 : 0;
   else if (key == "fp32_newton")
@@ -1403,6 +1405,26 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   // the device), and the shared first eigensolve is skipped
   const bool warm = opt.warm_start && !opt.plain && !kguess && !Hguess && last_nev > 0 &&
                     int(last_kmags.size()) >= last_nev;
+  // per-band guesses (the learned warm start): band i starts at kguess[i] with the Hguess
+  // block (filled up with random columns), and the shared first eigensolve is skipped
+  const bool banded = opt.band_guess && !opt.plain && kguess && nguess_used > 0;
+  if (banded)
+    for (int i = 0; i < ninst; ++i)
+      if (!(kguess[i] > 0) || !std::isfinite(kguess[i]))
+        throw Error(OMX_ERR_INVALID_ARG, "band_guess needs a positive finite kguess per band");
+  auto guided_block = [&](auto& b) {
+    using TT = typename std::remove_reference<decltype(b)>::type::value_type;
+    if (nguess_used < M) launch_randn<TT>(b.p(b.X[b.cur]), ld, b.istride, ninst, M, K, opt.seed, stream);
+    upload_block<TT>(*this, b, Hguess, nguess_used, 0, ninst);
+    std::vector<double> kh(ninst);
+    for (int i = 0; i < ninst; ++i) {
+      kh[i] = kguess[i];
+      hns[i].k = kh[i];
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice,
+                                 stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(stream));
+  };
   auto warm_block = [&](auto& b) {
     using TT = typename std::remove_reference<decltype(b)>::type::value_type;
     const int nw = std::min(last_nev, M);
@@ -1448,6 +1470,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
     if (warm) {
       warm_block(b32);
+    } else if (banded) {
+      guided_block(b32);
     } else {
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
@@ -1470,6 +1494,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     for (auto& s : hns) s.k_tol = k_tol;
     if (warm) {
       warm_block(b64);
+    } else if (banded) {
+      guided_block(b64);
     } else {
       launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, 1);

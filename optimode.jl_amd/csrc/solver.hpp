@@ -149,6 +149,9 @@ struct Options {
                                 // random start at k_guess per band, every eigensolve to eig_tol
   int warm_start = 0;           // solve_k without kguess/Hguess starts every band from the previous
                                 // solve's k and eigenvectors (kept on the device)
+  int band_guess = 0;           // solve_k with kguess (one k per band) and Hguess: every band starts at
+                                // its own guess with the Hguess block, no shared first eigensolve
+                                // (the learned warm start)
   int refine = 1;               // FP64 phase by Rayleigh-Ritz + Jacobi-Davidson refinement of each
                                 // band's vector (0: FP64 LOBPCG on the whole block)
   int fp32_newton = 1;          // FP32 Newton steps before the FP64 phase (0: the FP64 phase starts

@@ -1,0 +1,74 @@
+// The learned warm start of solve_k: a small fully convolutional network that maps a
+// downsampled eps^-1 map plus omega to per-band effective-index guesses and low-resolution
+// start vectors.  It has no counterpart in the reference (OptiMode starts every solve from
+// random vectors and k_guess, src/MaxwellEigenmodes.jl:63); it is trained on the GPU by the
+// kernels in warmnet.hip on modes the solver itself converged.
+//
+// Tensors are FP32, [batch][channel][y][x] with y, x in 0..hw-1.  Network (3x3 convolutions,
+// zero padding, stride 1):
+//   c1: 7 -> 16 at hw, ReLU                         (input: 6 eps^-1 components + omega)
+//   pool: 2x2 average                               -> hw/2
+//   c2: 16 -> 32 at hw/2, ReLU
+//   pool                                            -> hw/4
+//   c3: 32 -> 32 at hw/4, ReLU
+//   head: mean over pixels of c3 (32) -> dense -> nev            (effective-index guesses)
+//   c4: [up2(c3) | c2] 64 -> 32 at hw/2, ReLU       (nearest-neighbour upsampling)
+//   c5: [up2(c4) | c1] 48 -> 32 at hw, ReLU
+//   c6: 32 -> 4 nev at hw (linear)                  (start vectors: re/im of 2 components)
+// Loss: mean over field entries of (y - T)^2 + wk * mean over bands of (k - Tk)^2.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace omx {
+
+class WarmNet {
+ public:
+  static constexpr int NIN = 7;
+  WarmNet(int device, int hw, int nev, uint64_t seed);
+  ~WarmNet();
+  WarmNet(const WarmNet&) = delete;
+  WarmNet& operator=(const WarmNet&) = delete;
+  int hw() const { return hw_; }
+  int nev() const { return nev_; }
+  long nparams() const { return nparams_; }
+  void get_params(float* out);
+  void set_params(const float* in);
+  // x: B x 7 x hw x hw -> fields B x 4nev x hw x hw (may be NULL), k B x nev (may be NULL)
+  void forward(int B, const float* x, float* fields, float* kout);
+  // loss and its gradient with respect to the flat parameter vector (grad may be NULL)
+  double loss_grad(int B, const float* x, const float* tf, const float* tk, float wk, float* grad);
+  // Adam over n samples held on the device, minibatches of `batch` in a seeded shuffle;
+  // hist[e] = mean minibatch loss of epoch e
+  void train(int n, const float* x, const float* tf, const float* tk, float wk, int epochs, int batch, float lr,
+             uint64_t seed, double* hist);
+
+ private:
+  struct Conv {
+    int ci, co, res;
+    long w, b;  // offsets into the parameter vector
+  };
+  int device_, hw_, nev_;
+  long nparams_ = 0;
+  Conv c_[6];
+  long dw_ = 0, db_ = 0;  // dense head
+  hipStream_t s_ = nullptr;
+  DevMem P_, G_, M1_, M2_;  // parameters, gradient, Adam moments
+  int B_ = 0;               // batch capacity of the activation buffers
+  DevMem x_, a1_, p1_, a2_, p2_, a3_, g_, k_, cat4_, a4_, cat5_, a5_, y_;
+  DevMem dy_, da5_, dcat5_, da4_, dcat4_, da3_, dp2_, da2_, dp1_, da1_, dk_, dg_, part_, lossd_;
+  long adam_t_ = 0;
+
+  void ensure_batch(int B);
+  void fwd(int B);
+  // gradient of the loss for targets already on the device; accumulates the loss sum into
+  // lossd_[slot] (device double)
+  void bwd(int B, const float* tf_dev, const float* tk_dev, float wk, int slot);
+  void conv(const Conv& c, const float* in, float* out, int B, bool relu);
+  void conv_bwd(const Conv& c, const float* in, const float* dout, float* din, int B);
+};
+
+}  // namespace omx

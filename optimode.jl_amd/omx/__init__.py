@@ -156,6 +156,7 @@ class HIPSolver:
             self.options["reg_ops"] = 1.0 if reg_ops else 0.0
         self.options["plain"] = 1.0 if plain else 0.0
         self.options["warm_start"] = 1.0 if warm_start else 0.0
+        self.options["band_guess"] = 0.0  # set per call by solve_k(..., band_guess=True)
         self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
         self.profile = bool(profile)
         self._h = None
@@ -275,18 +276,32 @@ def solve_omega2(k, epsinv, grid, solver, nev=1, maxiter=100, tol=1e-8, evecs_gu
 
 
 def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k_tol=1e-8, eig_tol=1e-8,
-            kguess=None, Hguess=None, return_info=False, fetch_evecs=True):
+            kguess=None, Hguess=None, return_info=False, fetch_evecs=True, band_guess=False):
     """``solve_k(ω, ε⁻¹, grid, solver; nev, max_eigsolves, maxiter, k_tol, eig_tol, kguess,
-    Hguess)`` -> ``(kmags, evecs)``: unit-norm, phase-canonicalised eigenvectors."""
+    Hguess)`` -> ``(kmags, evecs)``: unit-norm, phase-canonicalised eigenvectors.
+    ``band_guess=True`` (the learned warm start): ``kguess`` holds one k per band and
+    ``Hguess`` the start block; the shared first eigensolve is skipped."""
     h = solver.handle(grid, epsinv)
     n2 = 2 * grid.N
     km = np.empty(nev)
     kg = None
     if kguess is not None:
-        kg = np.array([float(np.ravel([kguess])[0])])
+        kv = np.ravel(np.asarray(kguess, dtype=np.float64))
+        if band_guess:
+            if kv.size != nev or Hguess is None:
+                raise OmxError(_lib.OMX_ERR_INVALID_ARG, "band_guess needs one kguess per band and Hguess")
+            kg = np.ascontiguousarray(kv)
+        else:
+            kg = kv[:1].copy()
+    elif band_guess:
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "band_guess needs one kguess per band and Hguess")
     Hg = None
     ng = 0
-    if Hguess is not None:
+    if isinstance(Hguess, np.ndarray) and Hguess.ndim == 2 and Hguess.shape[1] == n2 and Hguess.shape[0] != n2:
+        # rows are the start vectors (as omx.warmstart builds them): no per-column copies
+        Hg = np.ascontiguousarray(Hguess[:nev], dtype=np.complex128)
+        ng = Hg.shape[0]
+    elif Hguess is not None:
         cols = [np.asarray(Hguess)] if np.ndim(Hguess) == 1 else (
             [np.asarray(v) for v in Hguess] if isinstance(Hguess, (list, tuple)) else
             [np.asarray(Hguess)[:, j] for j in range(np.shape(Hguess)[1])])
@@ -296,9 +311,16 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
             ng = len(cols)
     ev = np.empty((nev, n2), dtype=np.complex128) if fetch_evecs else None
     info = OmxInfo()
-    st = _lib.lib().omx_solve_k(h.h, float(omega), int(nev), float(k_tol), float(eig_tol), int(max_eigsolves),
-                                int(maxiter), _lib.dptr(kg), _lib.dptr(Hg), ng, _lib.dptr(km), _lib.dptr(ev),
-                                ctypes.byref(info))
+    L = _lib.lib()
+    if band_guess:
+        _lib.check(L.omx_set_option(h.h, b"band_guess", 1.0), h.h)
+    try:
+        st = L.omx_solve_k(h.h, float(omega), int(nev), float(k_tol), float(eig_tol), int(max_eigsolves),
+                           int(maxiter), _lib.dptr(kg), _lib.dptr(Hg), ng, _lib.dptr(km), _lib.dptr(ev),
+                           ctypes.byref(info))
+    finally:
+        if band_guess:
+            L.omx_set_option(h.h, b"band_guess", 0.0)
     _lib.check(st, h.h)
     evecs = [ev[j].copy() for j in range(nev)] if fetch_evecs else None
     if return_info:

@@ -24,6 +24,7 @@ OMX_PREC_FP32 = 0
 OMX_PREC_FP64 = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -96,6 +97,16 @@ SIGNATURES = [
                                          ctypes.c_int]),
     ("omx_sliceinv_3x3", ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
     ("omx_sliceinv_3x3_pullback", ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]),
+    ("omx_warmnet_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                          ctypes.POINTER(_vp)]),
+    ("omx_warmnet_destroy", ctypes.c_int, [_vp]),
+    ("omx_warmnet_num_params", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    ("omx_warmnet_get_params", ctypes.c_int, [_vp, _fp]),
+    ("omx_warmnet_set_params", ctypes.c_int, [_vp, _fp]),
+    ("omx_warmnet_forward", ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp]),
+    ("omx_warmnet_loss_grad", ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_float, _dp, _fp]),
+    ("omx_warmnet_train", ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_float, ctypes.c_uint64, _dp]),
     ("omx_smooth_epsinv", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp, ctypes.c_int,
                                          _dp]),
@@ -144,3 +155,12 @@ def dptr(a):
 
 def as_c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def fptr(a):
+    """Pointer to a C-contiguous float32 numpy array (or None)."""
+    if a is None:
+        return None
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(_fp)

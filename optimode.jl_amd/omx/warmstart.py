@@ -1,0 +1,312 @@
+"""Learned warm start of ``solve_k``: a small convolutional network (kernels in
+``csrc/warmnet.hip``, trained on the GPU) predicts each band's effective index and a
+low-resolution start block from the smoothed ε⁻¹ map plus ω.  The plain start stays the
+fallback, and the converged result does not depend on the start (the same k_tol / eig_tol
+stopping rules apply).  No counterpart in the reference, which starts every solve from
+random vectors at ``k_guess`` (``src/MaxwellEigenmodes.jl:63``).
+
+Representation:
+- input: 7 maps of ``hw x hw`` -- the 6 unique ε⁻¹ components averaged over pixel blocks
+  and the normalised ω, axes ``[channel][ix][iy]``;
+- start vectors: per band, the ``hw x hw`` lowest plane waves of the eigenvector's two
+  components (FFT order), transformed with an unnormalised forward FFT into a smooth
+  localised map (4 channels: re/im of the two components);
+- effective index ``n = k/ω`` per band, normalised as ``(n - N_SHIFT) / N_SCALE``.
+
+Training data come from the solver itself (``generate``): TFLN ribs with random top width,
+film, etch depth, sidewall angle and ω around the 256x256 rib workload.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from . import Grid, OmxError
+
+N_SHIFT, N_SCALE = 1.8, 0.2
+OMEGA0, OMEGA_SCALE = 0.65, 0.05
+DIAG0, DIAG_SCALE, OFF_SCALE = 0.55, 0.25, 0.05
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+class WarmNet:
+    """The network on the GPU (``omx_warmnet_*``): FP32 forward, loss gradient and Adam."""
+
+    def __init__(self, hw=64, nev=8, seed=0, device=0):
+        self.hw, self.nev = int(hw), int(nev)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().omx_warmnet_create(int(device), self.hw, self.nev, int(seed), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().omx_warmnet_num_params(self.h, ctypes.byref(n)))
+        self.nparams = int(n.value)
+
+    def close(self):
+        if self.h:
+            _lib.lib().omx_warmnet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        _lib.check(_lib.lib().omx_warmnet_get_params(self.h, _lib.fptr(out)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"expected {self.nparams} parameters, got {p.size}")
+        _lib.check(_lib.lib().omx_warmnet_set_params(self.h, _lib.fptr(p)))
+
+    def _x(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 4 or x.shape[1:] != (7, self.hw, self.hw):
+            raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"expected (B, 7, {self.hw}, {self.hw}) inputs, got {x.shape}")
+        return x
+
+    def forward(self, x):
+        """``(fields (B, 4nev, hw, hw), k (B, nev))``."""
+        x = self._x(x)
+        B = x.shape[0]
+        y = np.empty((B, 4 * self.nev, self.hw, self.hw), dtype=np.float32)
+        k = np.empty((B, self.nev), dtype=np.float32)
+        _lib.check(_lib.lib().omx_warmnet_forward(self.h, B, _lib.fptr(x), _lib.fptr(y), _lib.fptr(k)))
+        return y, k
+
+    def loss_grad(self, x, tf, tk, wk=1.0):
+        x = self._x(x)
+        B = x.shape[0]
+        tf = np.ascontiguousarray(tf, dtype=np.float32)
+        tk = np.ascontiguousarray(tk, dtype=np.float32)
+        if tf.shape != (B, 4 * self.nev, self.hw, self.hw) or tk.shape != (B, self.nev):
+            raise OmxError(_lib.OMX_ERR_BAD_SHAPE, "target shapes do not match the batch")
+        loss = ctypes.c_double()
+        g = np.empty(self.nparams, dtype=np.float32)
+        _lib.check(_lib.lib().omx_warmnet_loss_grad(self.h, B, _lib.fptr(x), _lib.fptr(tf), _lib.fptr(tk), float(wk),
+                                                    ctypes.byref(loss), _lib.fptr(g)))
+        return loss.value, g
+
+    def train(self, x, tf, tk, wk=1.0, epochs=100, batch=16, lr=1e-3, seed=0):
+        """Adam over the whole set on the device; returns the per-epoch mean loss."""
+        x = self._x(x)
+        n = x.shape[0]
+        tf = np.ascontiguousarray(tf, dtype=np.float32)
+        tk = np.ascontiguousarray(tk, dtype=np.float32)
+        if tf.shape != (n, 4 * self.nev, self.hw, self.hw) or tk.shape != (n, self.nev):
+            raise OmxError(_lib.OMX_ERR_BAD_SHAPE, "target shapes do not match the data set")
+        hist = np.empty(int(epochs))
+        _lib.check(_lib.lib().omx_warmnet_train(self.h, n, _lib.fptr(x), _lib.fptr(tf), _lib.fptr(tk), float(wk),
+                                                int(epochs), int(batch), float(lr), int(seed), _lib.dptr(hist)))
+        return hist
+
+
+def _blocks(grid, hw):
+    if grid.Nx % hw or grid.Ny % hw:
+        raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"grid {grid.Nx}x{grid.Ny} is not a multiple of the {hw} map")
+    return grid.Nx // hw, grid.Ny // hw
+
+
+def features(epsinv, grid, omega, hw=64):
+    """``(7, hw, hw)`` float32 network input: block means of ε⁻¹ xx, yy, zz, xy, xz, yz and ω."""
+    fx, fy = _blocks(grid, hw)
+    e = np.asarray(epsinv, dtype=np.float64)
+    comps = [e[0, 0], e[1, 1], e[2, 2], e[0, 1], e[0, 2], e[1, 2]]
+    out = np.empty((7, hw, hw), dtype=np.float32)
+    for c, a in enumerate(comps):
+        m = a.reshape(hw, fx, hw, fy).mean(axis=(1, 3))
+        out[c] = (m - DIAG0) / DIAG_SCALE if c < 3 else m / OFF_SCALE
+    out[6] = (float(omega) - OMEGA0) / OMEGA_SCALE
+    return out
+
+
+def _sel(n, hw):
+    return np.r_[0:hw // 2, n - hw // 2:n]
+
+
+def field_targets(evecs, grid, hw=64):
+    """``(4 nev, hw, hw)`` float32: per band, the ``hw x hw`` lowest plane waves of both
+    components (FFT order) mapped by an unnormalised forward FFT, as re/im channels."""
+    sx, sy = _sel(grid.Nx, hw), _sel(grid.Ny, hw)
+    out = np.empty((4 * len(evecs), hw, hw), dtype=np.float32)
+    for j, ev in enumerate(evecs):
+        A = np.asarray(ev).reshape((2, grid.Nx, grid.Ny), order="F")
+        u = np.fft.fft2(A[:, sx][:, :, sy], axes=(1, 2))
+        out[4 * j + 0], out[4 * j + 1] = u[0].real, u[0].imag
+        out[4 * j + 2], out[4 * j + 3] = u[1].real, u[1].imag
+    return out
+
+
+def _flat_index(grid, hw):
+    """Flat positions (2, hw, hw) of the ``hw x hw`` lowest plane waves of both components
+    in a vector of 2N entries (index c + 2(ix + Nx iy))."""
+    sx, sy = _sel(grid.Nx, hw), _sel(grid.Ny, hw)
+    return np.arange(2)[:, None, None] + 2 * (sx[None, :, None] + grid.Nx * sy[None, None, :])
+
+
+def start_vectors(fields, grid, hw=64):
+    """Inverse of ``field_targets``: unit-norm vectors of 2N complex (zero above the ``hw``
+    lowest plane waves), one per band, as the rows of a ``(nev, 2N)`` array."""
+    nev = fields.shape[0] // 4
+    f = np.asarray(fields, dtype=np.float64).reshape(nev, 2, 2, hw, hw)
+    C = np.fft.ifft2(f[:, :, 0] + 1j * f[:, :, 1], axes=(2, 3))  # (nev, 2, hw, hw)
+    nrm = np.sqrt(np.sum(np.abs(C) ** 2, axis=(1, 2, 3)))
+    C /= np.where(nrm > 0, nrm, 1.0)[:, None, None, None]
+    out = np.zeros((nev, 2 * grid.N), dtype=np.complex128)
+    out[:, _flat_index(grid, hw).ravel()] = C.reshape(nev, -1)
+    return out
+
+
+def neff_targets(kmags, omega):
+    return ((np.asarray(kmags, dtype=np.float64) / float(omega) - N_SHIFT) / N_SCALE).astype(np.float32)
+
+
+class LearnedStart:
+    """A trained network plus its map size: ``guess(ε⁻¹, grid, ω)`` -> ``(kguess, Hguess)``."""
+
+    def __init__(self, net):
+        self.net = net
+
+    @property
+    def hw(self):
+        return self.net.hw
+
+    @property
+    def nev(self):
+        return self.net.nev
+
+    def guess(self, epsinv, grid, omega):
+        y, k = self.net.forward(features(epsinv, grid, omega, self.hw)[None])
+        kg = float(omega) * (N_SHIFT + N_SCALE * k[0].astype(np.float64))
+        return kg, start_vectors(y[0], grid, self.hw)
+
+    def save(self, path):
+        np.savez(path, params=self.net.params, hw=self.hw, nev=self.nev)
+
+    @classmethod
+    def load(cls, path, device=0):
+        with np.load(path, allow_pickle=False) as z:
+            net = WarmNet(int(z["hw"]), int(z["nev"]), device=device)
+            net.params = z["params"]
+        return cls(net)
+
+
+def solve_k_learned(omega, epsinv, grid, solver, start, nev=None, mode="seed", k0_scale=1.03, return_info=False,
+                    **kw):
+    """``solve_k`` started from the network's guesses; the stopping rules and the converged
+    result are those of ``solve_k``.  ``mode="seed"``: the shared first eigensolve starts at
+    ``k0_scale`` times the predicted k of band 1 from the predicted block (instead of k_guess
+    and random vectors; slightly above band 1 so that every band's first Newton step moves
+    down in k, as from k_guess); ``mode="band"``: every band starts at its own predicted k
+    with the predicted block and the shared first eigensolve is skipped (``band_guess``)."""
+    from . import solve_k
+    nev = start.nev if nev is None else int(nev)
+    if nev > start.nev:
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, f"the network predicts {start.nev} bands, {nev} requested")
+    kg, H = start.guess(epsinv, grid, omega)
+    kg = np.maximum(kg[:nev], 1e-3)
+    if mode == "band":
+        return solve_k(omega, epsinv, grid, solver, nev=nev, kguess=kg, Hguess=H[:nev], band_guess=True,
+                       return_info=return_info, **kw)
+    if mode != "seed":
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "mode must be 'seed' or 'band'")
+    return solve_k(omega, epsinv, grid, solver, nev=nev, kguess=k0_scale * kg[0], Hguess=H[:nev],
+                   return_info=return_info, **kw)
+
+
+# --- training data from the solver ------------------------------------------------------
+
+def random_rib(rng):
+    """One draw of the rib distribution: TFLN rib parameters and ω around the 256x256 rib
+    (top width 1.2 um, film 0.6 um, etch 0.3 um, 60 degree sidewalls, ω = 1/1.55)."""
+    film = rng.uniform(0.5, 0.7)
+    return dict(top_w=rng.uniform(0.8, 1.6), film=film, etch=film * rng.uniform(0.3, 0.7),
+                sidewall_deg=rng.uniform(55.0, 75.0), omega=rng.uniform(0.58, 0.72))
+
+
+def rib_epsinv(grid, prm, backend="gpu"):
+    from . import geometry as G
+    return G.tfln_rib(grid, top_w=prm["top_w"], film=prm["film"], etch=prm["etch"], sidewall_deg=prm["sidewall_deg"],
+                      backend=backend)
+
+
+def generate(seeds, grid, nev, solver, hw=64, k_tol=1e-10, eig_tol=1e-8, log=None):
+    """Inputs and targets for the draws of ``seeds``: ``(X, TF, TK, infos)``."""
+    from . import solve_k
+    X = np.empty((len(seeds), 7, hw, hw), dtype=np.float32)
+    TF = np.empty((len(seeds), 4 * nev, hw, hw), dtype=np.float32)
+    TK = np.empty((len(seeds), nev), dtype=np.float32)
+    infos = []
+    for i, s in enumerate(seeds):
+        prm = random_rib(np.random.default_rng(int(s)))
+        ei = rib_epsinv(grid, prm)
+        km, ev, info = solve_k(prm["omega"], ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
+                               return_info=True)
+        X[i] = features(ei, grid, prm["omega"], hw)
+        TF[i] = field_targets(ev, grid, hw)
+        TK[i] = neff_targets(km, prm["omega"])
+        infos.append(info)
+        if log and (i + 1) % 32 == 0:
+            log(f"generated {i + 1}/{len(seeds)}")
+    return X, TF, TK, infos
+
+
+def compare(seeds, grid, nev, start, solver, k_tol=1e-10, eig_tol=1e-8, mode="seed"):
+    """Held-out draws solved from the plain start and from the learned start: per draw the
+    two k vectors, LOBPCG iterations and wall times."""
+    from . import solve_k
+    rows = []
+    for s in seeds:
+        prm = random_rib(np.random.default_rng(int(s)))
+        ei = rib_epsinv(grid, prm)
+        w = prm["omega"]
+        t0 = time.perf_counter()
+        kp, _, ip = solve_k(w, ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol, return_info=True,
+                            fetch_evecs=False)
+        t1 = time.perf_counter()
+        kl, _, il = solve_k_learned(w, ei, grid, solver, start, nev=nev, mode=mode, k_tol=k_tol, eig_tol=eig_tol,
+                                    return_info=True, fetch_evecs=False)
+        t2 = time.perf_counter()
+        rows.append(dict(seed=int(s), k_plain=kp, k_learned=kl, it_plain=ip["lobpcg_iters_fp32"] + ip["lobpcg_iters_fp64"],
+                         it_learned=il["lobpcg_iters_fp32"] + il["lobpcg_iters_fp64"], newton_plain=ip["n_newton"],
+                         newton_learned=il["n_newton"], ms_plain=1e3 * (t1 - t0), ms_learned=1e3 * (t2 - t1),
+                         conv_plain=ip["converged"], conv_learned=il["converged"]))
+    return rows
+
+
+def train_default(n=512, epochs=600, out=None, log=print):
+    """The shipped network (``data/warmnet_tfln256.npz``): 512 rib draws (seeds 1000..1511)
+    solved at 256x256 with 8 bands, 600 epochs of Adam (lr 2e-3, minibatch 16, wk 1) on the
+    GPU."""
+    from . import HIPSolver
+    grid = Grid(8.0, 4.0, 256, 256)
+    nev, hw = 8, 64
+    t = time.perf_counter()
+    X, TF, TK, _ = generate(range(1000, 1000 + n), grid, nev, HIPSolver(), hw=hw, log=log)
+    log(f"generated {n} draws in {time.perf_counter() - t:.1f} s")
+    net = WarmNet(hw, nev, seed=0)
+    t = time.perf_counter()
+    hist = net.train(X, TF, TK, wk=1.0, epochs=epochs, batch=16, lr=2e-3, seed=1)
+    log(f"trained {epochs} epochs in {time.perf_counter() - t:.1f} s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    st = LearnedStart(net)
+    if out:
+        st.save(out)
+    return st, hist
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="train the warm-start network on solver-converged rib draws")
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--out", default=os.path.join(DATA_DIR, "warmnet_tfln256.npz"))
+    a = ap.parse_args()
+    train_default(a.n, a.epochs, a.out)

@@ -1,0 +1,78 @@
+"""The warm-start network on the GPU: forward and loss gradient against the CPU restatement
+(itself pinned by torch autograd and finite differences), deterministic training, and the
+learned start against the plain start on held-out rib geometries (same converged k, fewer
+LOBPCG iterations)."""
+import numpy as np
+import pytest
+
+import omx
+from omx import warmstart as WS
+from oracle import warmnet as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_case(hw, nev, B, seed):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(W.layout(hw, nev)[3]) * 0.2).astype(np.float32)
+    x = rng.standard_normal((B, 7, hw, hw)).astype(np.float32)
+    tf = rng.standard_normal((B, 4 * nev, hw, hw)).astype(np.float32)
+    tk = rng.standard_normal((B, nev)).astype(np.float32)
+    return p, x, tf, tk
+
+
+@pytest.mark.parametrize("hw,nev,B", [(16, 3, 3), (32, 2, 5), (64, 8, 2)])
+def test_forward_and_gradient_match_oracle(hw, nev, B):
+    print(f"warmnet parity hw {hw} nev {nev} B {B}", flush=True)
+    p, x, tf, tk = _rand_case(hw, nev, B, 1)
+    net = WS.WarmNet(hw, nev)
+    assert net.nparams == p.size
+    net.params = p
+    assert np.array_equal(net.params, p)
+    y, k = net.forward(x)
+    yr, kr = W.forward(p.astype(np.float64), x, hw, nev)
+    assert np.max(np.abs(y - yr)) < 1e-5 * np.max(np.abs(yr))
+    assert np.max(np.abs(k - kr)) < 1e-5 * np.max(np.abs(kr))
+    wk = 0.5
+    L, g = net.loss_grad(x, tf, tk, wk)
+    Lr, gr = W.loss_grad(p.astype(np.float64), x, tf, tk, wk, hw, nev)
+    assert abs(L - Lr) < 1e-5 * abs(Lr)
+    assert np.max(np.abs(g - gr)) < 1e-4 * np.max(np.abs(gr))
+
+
+def test_training_reduces_loss_and_is_deterministic():
+    hw, nev = 16, 2
+    rng = np.random.default_rng(4)
+    teacher = (rng.standard_normal(W.layout(hw, nev)[3]) * 0.2)
+    x = rng.standard_normal((48, 7, hw, hw)).astype(np.float32)
+    tf, tk = W.forward(teacher, x, hw, nev)
+    runs = []
+    for _ in range(2):
+        net = WS.WarmNet(hw, nev, seed=3)
+        hist = net.train(x, tf.astype(np.float32), tk.astype(np.float32), wk=1.0, epochs=60, batch=8, lr=3e-3, seed=5)
+        runs.append((hist, net.params))
+    print(f"  loss {runs[0][0][0]:.4g} -> {runs[0][0][-1]:.4g}", flush=True)
+    assert runs[0][0][-1] < 0.2 * runs[0][0][0]
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][0], runs[1][0])
+
+
+def test_learned_start_matches_plain_and_saves_iterations():
+    # rib draws at 128x128, 4 bands: train on 96 solver-converged draws, compare on 6 held out
+    grid = omx.Grid(8.0, 4.0, 128, 128)
+    nev, hw = 4, 32
+    solver = omx.HIPSolver()
+    X, TF, TK, _ = WS.generate(range(1000, 1096), grid, nev, solver, hw=hw)
+    net = WS.WarmNet(hw, nev, seed=0)
+    hist = net.train(X, TF, TK, wk=1.0, epochs=200, batch=16, lr=2e-3, seed=1)
+    print(f"  training loss {hist[0]:.4g} -> {hist[-1]:.4g}", flush=True)
+    assert hist[-1] < 0.5 * hist[0]
+    rows = WS.compare(range(30000, 30006), grid, nev, WS.LearnedStart(net), solver, mode="seed")
+    it_p = sum(r["it_plain"] for r in rows)
+    it_l = sum(r["it_learned"] for r in rows)
+    for r in rows:
+        print(f"  seed {r['seed']}: it {r['it_plain']} -> {r['it_learned']}, newton {r['newton_plain']} -> "
+              f"{r['newton_learned']}, ms {r['ms_plain']:.1f} -> {r['ms_learned']:.1f}", flush=True)
+        assert r["conv_plain"] and r["conv_learned"]
+        assert np.max(np.abs(r["k_learned"] - r["k_plain"]) / r["k_plain"]) < 1e-8
+    print(f"  LOBPCG iterations plain {it_p}, learned {it_l}", flush=True)
+    assert it_l < it_p

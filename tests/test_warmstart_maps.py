@@ -1,0 +1,48 @@
+"""Host-side maps of the learned warm start (no GPU): features and the start-vector
+representation, pinned by closed forms."""
+import numpy as np
+
+import omx
+from omx import warmstart as WS
+
+
+def test_start_vectors_invert_field_targets_on_low_plane_waves():
+    grid = omx.Grid(8.0, 4.0, 64, 32)
+    hw = 16
+    rng = np.random.default_rng(0)
+    evs = []
+    for _ in range(3):
+        A = np.zeros((2, 64, 32), dtype=complex)
+        sx, sy = WS._sel(64, hw), WS._sel(32, hw)
+        A[:, sx[:, None], sy[None, :]] = rng.standard_normal((2, hw, hw)) + 1j * rng.standard_normal((2, hw, hw))
+        v = A.ravel(order="F")
+        evs.append(v / np.linalg.norm(v))
+    back = WS.start_vectors(WS.field_targets(evs, grid, hw).astype(np.float64), grid, hw)
+    for v, b in zip(evs, back):
+        assert np.allclose(v, b, atol=1e-12)
+    # a vector with energy above the kept plane waves comes back as its normalised projection
+    A = np.zeros((2, 64, 32), dtype=complex)
+    A[0, 0, 0], A[1, 20, 3] = 0.6, 0.8  # (20, 3) lies outside the 16 x 16 lowest waves
+    t = WS.field_targets([A.ravel(order="F")], grid, hw)
+    b = WS.start_vectors(t.astype(np.float64), grid, hw)[0].reshape((2, 64, 32), order="F")
+    assert abs(b[0, 0, 0] - 1.0) < 1e-12 and np.count_nonzero(np.abs(b) > 1e-12) == 1
+
+
+def test_features_block_means_and_omega():
+    grid = omx.Grid(4.0, 4.0, 32, 32)
+    e = np.zeros((3, 3, 32, 32))
+    for a in range(3):
+        e[a, a] = 0.55 + 0.25 * (a + 1)
+    e[0, 1] = e[1, 0] = 0.05 * np.arange(32)[:, None] / 31
+    f = WS.features(e, grid, 0.7, hw=8)
+    assert f.shape == (7, 8, 8) and f.dtype == np.float32
+    for a in range(3):
+        assert np.allclose(f[a], a + 1)
+    assert np.allclose(f[3][:, 0], (np.arange(32).reshape(8, 4).mean(1) / 31))
+    assert np.allclose(f[6], (0.7 - WS.OMEGA0) / WS.OMEGA_SCALE)
+
+
+def test_neff_targets_round_trip():
+    k = np.array([1.3, 1.2, 1.05])
+    t = WS.neff_targets(k, 0.65)
+    assert np.allclose(0.65 * (WS.N_SHIFT + WS.N_SCALE * t.astype(float)), k, rtol=1e-6)
