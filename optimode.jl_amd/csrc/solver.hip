@@ -671,6 +671,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.warm_start = v != 0 ? 1 : 0;
   else if (key == "band_guess")
     opt.band_guess = v != 0 ? 1 : 0;
+  else if (key == "guess_lowres")
+    opt.guess_lowres = std::max(0, int(v));
   else if (key == "refine")
     opt.refine = v != 0 ? 1 : 0;
   else if (key == "fp32_newton")
@@ -826,6 +828,26 @@ template <typename T> std::shared_ptr<Batch<T>> workspace(Solver& S, int ninst, 
   return std::shared_ptr<Batch<T>>(p, static_cast<Batch<T>*>(p.get()));
 }
 
+// dst column j (stride ld) <- the low plane waves of src vector j: entry c + 2(gx + hw gy) of
+// a vector of 2 hw^2 goes to c + 2(ix + nx iy) with ix = gx (gx < hw/2) or nx - hw + gx, and
+// iy likewise; the other entries of the columns are left as they are
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter_lowres(const c64* src, int ncols, int hw, int nx, int ny,
+                                                        cplx<T>* dst, long ld) {
+  const long per = 2L * hw * hw, n = per * ncols;
+  for (long e = long(blockIdx.x) * 256 + threadIdx.x; e < n; e += long(gridDim.x) * 256) {
+    const int col = int(e / per);
+    const long r = e - long(col) * per;
+    const int c = int(r & 1);
+    const long q = r >> 1;
+    const int gx = int(q % hw), gy = int(q / hw);
+    const int ix = gx < hw / 2 ? gx : nx - hw + gx;
+    const int iy = gy < hw / 2 ? gy : ny - hw + gy;
+    const c64 v = src[e];
+    dst[long(col) * ld + 2 * (long(ix) + long(nx) * iy) + c] = cplx<T>{T(v.x), T(v.y)};
+  }
+}
+
 template <typename T>
 void upload_block(Solver& S, Batch<T>& b, const double* H, int ncols, int inst_from, int inst_to) {
   // H: ncols host vectors (K complex FP64 each) -> columns 0..ncols-1 of instances [from, to)
@@ -834,6 +856,24 @@ void upload_block(Solver& S, Batch<T>& b, const double* H, int ncols, int inst_f
   for (int i = inst_from; i < inst_to; ++i)
     launch_convert<double, T>(tmp.as<c64>(), S.K, 0, b.p(b.X[b.cur]) + i * b.istride, b.ld, 0, 1, ncols, S.K,
                               S.stream);
+  OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+}
+
+// as upload_block for start vectors given by their hw x hw lowest plane waves (2 hw^2
+// complex each, entry c + 2(gx + hw gy), FFT order in gx and gy); the rest of each column is 0
+template <typename T>
+void upload_block_lowres(Solver& S, Batch<T>& b, const double* H, int ncols, int hw, int inst_from, int inst_to) {
+  const long per = 2L * hw * hw;
+  DevMem tmp(sizeof(c64) * size_t(ncols) * per);
+  OMX_HIP_CHECK(hipMemcpyAsync(tmp.as<void>(), H, sizeof(c64) * size_t(ncols) * per, hipMemcpyHostToDevice, S.stream));
+  for (int i = inst_from; i < inst_to; ++i) {
+    cplx<T>* dst = b.p(b.X[b.cur]) + i * b.istride;
+    OMX_HIP_CHECK(hipMemsetAsync(dst, 0, sizeof(cplx<T>) * size_t(ncols) * b.ld, S.stream));
+    const long n = per * ncols;
+    hipLaunchKernelGGL((k_scatter_lowres<T>), dim3(unsigned(std::min<long>((n + 255) / 256, 4096))), dim3(256), 0,
+                       S.stream, tmp.as<c64>(), ncols, hw, S.nx, S.ny, dst, b.ld);
+  }
+  OMX_HIP_CHECK(hipGetLastError());
   OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
 }
 
@@ -1362,6 +1402,17 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   Batch<double>& b64 = *b64p;
   int newton_total = 0;
   const int nguess_used = (Hguess && nguess > 0) ? std::min(nguess, M) : 0;
+  const int ghw = opt.guess_lowres;
+  if (nguess_used && ghw > 0 && (nz > 1 || ghw % 2 != 0 || ghw > nx || ghw > ny))
+    throw Error(OMX_ERR_INVALID_ARG, "guess_lowres must be even and at most Nx, Ny (2D cells)");
+  // start vectors into instances [from, to): full vectors, or their lowest plane waves
+  auto put_guess = [&](auto& b, int from, int to) {
+    using TT = typename std::remove_reference<decltype(b)>::type::value_type;
+    if (ghw > 0)
+      upload_block_lowres<TT>(*this, b, Hguess, nguess_used, ghw, from, to);
+    else
+      upload_block<TT>(*this, b, Hguess, nguess_used, from, to);
+  };
   const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
 
   // first eigensolve, shared by every band: the lowest nev pairs at k0 (instance 0);
@@ -1415,7 +1466,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   auto guided_block = [&](auto& b) {
     using TT = typename std::remove_reference<decltype(b)>::type::value_type;
     if (nguess_used < M) launch_randn<TT>(b.p(b.X[b.cur]), ld, b.istride, ninst, M, K, opt.seed, stream);
-    upload_block<TT>(*this, b, Hguess, nguess_used, 0, ninst);
+    put_guess(b, 0, ninst);
     std::vector<double> kh(ninst);
     for (int i = 0; i < ninst; ++i) {
       kh[i] = kguess[i];
@@ -1453,7 +1504,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     for (int b = 0; b < ninst; ++b) {
       launch_randn<double>(b64.p(b64.X[b64.cur]) + b * b64.istride, ld, b64.istride, 1, M, K, opt.seed + 7919u * b,
                            stream);
-      if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, b, b + 1);
+      if (nguess_used) put_guess(b64, b, b + 1);
       const std::vector<NewtonState> keep = hns;
       for (int i = 0; i < ninst; ++i)
         if (i != b) hns[i].done = 3;  // not this band's turn
@@ -1474,7 +1525,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       guided_block(b32);
     } else {
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
-      if (nguess_used) upload_block<float>(*this, b32, Hguess, nguess_used, 0, 1);
+      if (nguess_used) put_guess(b32, 0, 1);
       stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
     }
     if (opt.fp32_newton)
@@ -1498,7 +1549,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       guided_block(b64);
     } else {
       launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
-      if (nguess_used) upload_block<double>(*this, b64, Hguess, nguess_used, 0, 1);
+      if (nguess_used) put_guess(b64, 0, 1);
       stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
     }
   }

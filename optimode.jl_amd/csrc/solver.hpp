@@ -152,6 +152,8 @@ struct Options {
   int band_guess = 0;           // solve_k with kguess (one k per band) and Hguess: every band starts at
                                 // its own guess with the Hguess block, no shared first eigensolve
                                 // (the learned warm start)
+  int guess_lowres = 0;         // > 0: solve_k's Hguess vectors hold only their guess_lowres^2
+                                // lowest plane waves (2 hw^2 complex each), scattered on the device
   int refine = 1;               // FP64 phase by Rayleigh-Ritz + Jacobi-Davidson refinement of each
                                 // band's vector (0: FP64 LOBPCG on the whole block)
   int fp32_newton = 1;          // FP32 Newton steps before the FP64 phase (0: the FP64 phase starts

@@ -157,6 +157,7 @@ class HIPSolver:
         self.options["plain"] = 1.0 if plain else 0.0
         self.options["warm_start"] = 1.0 if warm_start else 0.0
         self.options["band_guess"] = 0.0  # set per call by solve_k(..., band_guess=True)
+        self.options["guess_lowres"] = 0.0  # set per call by solve_k(..., guess_hw=hw)
         self.options["fp64_only"] = 1.0 if precision == "fp64" else 0.0
         self.profile = bool(profile)
         self._h = None
@@ -276,11 +277,13 @@ def solve_omega2(k, epsinv, grid, solver, nev=1, maxiter=100, tol=1e-8, evecs_gu
 
 
 def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k_tol=1e-8, eig_tol=1e-8,
-            kguess=None, Hguess=None, return_info=False, fetch_evecs=True, band_guess=False):
+            kguess=None, Hguess=None, return_info=False, fetch_evecs=True, band_guess=False, guess_hw=None):
     """``solve_k(ω, ε⁻¹, grid, solver; nev, max_eigsolves, maxiter, k_tol, eig_tol, kguess,
     Hguess)`` -> ``(kmags, evecs)``: unit-norm, phase-canonicalised eigenvectors.
     ``band_guess=True`` (the learned warm start): ``kguess`` holds one k per band and
-    ``Hguess`` the start block; the shared first eigensolve is skipped."""
+    ``Hguess`` the start block; the shared first eigensolve is skipped.  ``guess_hw=hw``:
+    the rows of ``Hguess`` hold only the ``hw x hw`` lowest plane waves of each start vector
+    (``2 hw^2`` entries, index ``c + 2(gx + hw gy)`` in FFT order), scattered on the GPU."""
     h = solver.handle(grid, epsinv)
     n2 = 2 * grid.N
     km = np.empty(nev)
@@ -297,7 +300,13 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
         raise OmxError(_lib.OMX_ERR_INVALID_ARG, "band_guess needs one kguess per band and Hguess")
     Hg = None
     ng = 0
-    if isinstance(Hguess, np.ndarray) and Hguess.ndim == 2 and Hguess.shape[1] == n2 and Hguess.shape[0] != n2:
+    if guess_hw is not None:
+        hw = int(guess_hw)
+        Hg = np.ascontiguousarray(np.atleast_2d(Hguess)[:nev], dtype=np.complex128)
+        if Hg.shape[1] != 2 * hw * hw:
+            raise OmxError(_lib.OMX_ERR_BAD_SHAPE, f"guess_hw={hw} needs rows of {2 * hw * hw} entries")
+        ng = Hg.shape[0]
+    elif isinstance(Hguess, np.ndarray) and Hguess.ndim == 2 and Hguess.shape[1] == n2 and Hguess.shape[0] != n2:
         # rows are the start vectors (as omx.warmstart builds them): no per-column copies
         Hg = np.ascontiguousarray(Hguess[:nev], dtype=np.complex128)
         ng = Hg.shape[0]
@@ -314,6 +323,8 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
     L = _lib.lib()
     if band_guess:
         _lib.check(L.omx_set_option(h.h, b"band_guess", 1.0), h.h)
+    if guess_hw is not None:
+        _lib.check(L.omx_set_option(h.h, b"guess_lowres", float(guess_hw)), h.h)
     try:
         st = L.omx_solve_k(h.h, float(omega), int(nev), float(k_tol), float(eig_tol), int(max_eigsolves),
                            int(maxiter), _lib.dptr(kg), _lib.dptr(Hg), ng, _lib.dptr(km), _lib.dptr(ev),
@@ -321,6 +332,8 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
     finally:
         if band_guess:
             L.omx_set_option(h.h, b"band_guess", 0.0)
+        if guess_hw is not None:
+            L.omx_set_option(h.h, b"guess_lowres", 0.0)
     _lib.check(st, h.h)
     evecs = [ev[j].copy() for j in range(nev)] if fetch_evecs else None
     if return_info:

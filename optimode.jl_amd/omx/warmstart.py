@@ -119,11 +119,11 @@ def features(epsinv, grid, omega, hw=64):
     """``(7, hw, hw)`` float32 network input: block means of ε⁻¹ xx, yy, zz, xy, xz, yz and ω."""
     fx, fy = _blocks(grid, hw)
     e = np.asarray(epsinv, dtype=np.float64)
-    comps = [e[0, 0], e[1, 1], e[2, 2], e[0, 1], e[0, 2], e[1, 2]]
+    a, b = np.array([0, 1, 2, 0, 0, 1]), np.array([0, 1, 2, 1, 2, 2])
+    m = e[a, b].reshape(6, hw, fx, hw, fy).mean(axis=(2, 4))
     out = np.empty((7, hw, hw), dtype=np.float32)
-    for c, a in enumerate(comps):
-        m = a.reshape(hw, fx, hw, fy).mean(axis=(1, 3))
-        out[c] = (m - DIAG0) / DIAG_SCALE if c < 3 else m / OFF_SCALE
+    out[:3] = (m[:3] - DIAG0) / DIAG_SCALE
+    out[3:6] = m[3:] / OFF_SCALE
     out[6] = (float(omega) - OMEGA0) / OMEGA_SCALE
     return out
 
@@ -165,6 +165,18 @@ def start_vectors(fields, grid, hw=64):
     return out
 
 
+def lowres_vectors(fields, hw=64):
+    """The start vectors of ``start_vectors`` by their ``hw x hw`` lowest plane waves only:
+    rows of ``2 hw^2`` complex, entry ``c + 2(gx + hw gy)`` (FFT order), unit norm -- the
+    ``guess_hw`` form of ``solve_k``, scattered into the start block on the GPU."""
+    nev = fields.shape[0] // 4
+    f = np.asarray(fields, dtype=np.float64).reshape(nev, 2, 2, hw, hw)
+    C = np.fft.ifft2(f[:, :, 0] + 1j * f[:, :, 1], axes=(2, 3))  # (nev, 2, gx, gy)
+    nrm = np.sqrt(np.sum(np.abs(C) ** 2, axis=(1, 2, 3)))
+    C /= np.where(nrm > 0, nrm, 1.0)[:, None, None, None]
+    return np.ascontiguousarray(C.transpose(0, 3, 2, 1)).reshape(nev, -1)
+
+
 def neff_targets(kmags, omega):
     return ((np.asarray(kmags, dtype=np.float64) / float(omega) - N_SHIFT) / N_SCALE).astype(np.float32)
 
@@ -183,10 +195,12 @@ class LearnedStart:
     def nev(self):
         return self.net.nev
 
-    def guess(self, epsinv, grid, omega):
+    def guess(self, epsinv, grid, omega, lowres=False):
+        """``(kguess per band, start vectors)``: full vectors ``(nev, 2N)``, or with ``lowres``
+        their lowest plane waves ``(nev, 2 hw^2)`` for ``solve_k(..., guess_hw=hw)``."""
         y, k = self.net.forward(features(epsinv, grid, omega, self.hw)[None])
         kg = float(omega) * (N_SHIFT + N_SCALE * k[0].astype(np.float64))
-        return kg, start_vectors(y[0], grid, self.hw)
+        return kg, (lowres_vectors(y[0], self.hw) if lowres else start_vectors(y[0], grid, self.hw))
 
     def save(self, path):
         np.savez(path, params=self.net.params, hw=self.hw, nev=self.nev)
@@ -211,15 +225,15 @@ def solve_k_learned(omega, epsinv, grid, solver, start, nev=None, mode="seed", k
     nev = start.nev if nev is None else int(nev)
     if nev > start.nev:
         raise OmxError(_lib.OMX_ERR_INVALID_ARG, f"the network predicts {start.nev} bands, {nev} requested")
-    kg, H = start.guess(epsinv, grid, omega)
+    if mode not in ("seed", "band"):
+        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "mode must be 'seed' or 'band'")
+    kg, H = start.guess(epsinv, grid, omega, lowres=True)
     kg = np.maximum(kg[:nev], 1e-3)
     if mode == "band":
         return solve_k(omega, epsinv, grid, solver, nev=nev, kguess=kg, Hguess=H[:nev], band_guess=True,
-                       return_info=return_info, **kw)
-    if mode != "seed":
-        raise OmxError(_lib.OMX_ERR_INVALID_ARG, "mode must be 'seed' or 'band'")
+                       guess_hw=start.hw, return_info=return_info, **kw)
     return solve_k(omega, epsinv, grid, solver, nev=nev, kguess=k0_scale * kg[0], Hguess=H[:nev],
-                   return_info=return_info, **kw)
+                   guess_hw=start.hw, return_info=return_info, **kw)
 
 
 # --- training data from the solver ------------------------------------------------------

@@ -138,7 +138,7 @@ def test_solve_k_periodic_pullback_matches_oracle_and_finite_differences():
 
 
 def _full_pullback_case():
-    og, g = _grids((8, 8, 4), D=(2.4, 2.0, 0.45))
+    og, g = _grids((12, 12, 6), D=(2.0, 1.6, 0.45))
     ei = P.bragg_wg_epsi(og, eps_core=4.0, eps_bg=2.1, w0=0.5, h0=0.4, mod=0.5)
     return og, g, ei
 

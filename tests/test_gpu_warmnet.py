@@ -76,3 +76,21 @@ def test_learned_start_matches_plain_and_saves_iterations():
         assert np.max(np.abs(r["k_learned"] - r["k_plain"]) / r["k_plain"]) < 1e-8
     print(f"  LOBPCG iterations plain {it_p}, learned {it_l}", flush=True)
     assert it_l < it_p
+
+
+def test_lowres_guess_equals_full_vector_guess():
+    # the same start block given as full vectors or by its lowest plane waves (scattered on
+    # the GPU) gives the same solve, bit for bit
+    grid = omx.Grid(8.0, 4.0, 64, 64)
+    prm = WS.random_rib(np.random.default_rng(7))
+    ei = WS.rib_epsinv(grid, prm)
+    f = np.random.default_rng(8).standard_normal((12, 16, 16))
+    full, low = WS.start_vectors(f, grid, 16), WS.lowres_vectors(f, 16)
+    solver = omx.HIPSolver()
+    kw = dict(nev=3, k_tol=1e-10, eig_tol=1e-8, return_info=True, fetch_evecs=False)
+    for band in (False, True):
+        kg = [1.2 * prm["omega"] * 2.0] if not band else list(prm["omega"] * np.array([2.0, 1.9, 1.8]))
+        ka, _, ia = omx.solve_k(prm["omega"], ei, grid, solver, kguess=kg, Hguess=full, band_guess=band, **kw)
+        kb, _, ib = omx.solve_k(prm["omega"], ei, grid, solver, kguess=kg, Hguess=low, band_guess=band, guess_hw=16,
+                                **kw)
+        assert np.array_equal(ka, kb) and ia["lobpcg_iters_fp32"] == ib["lobpcg_iters_fp32"]

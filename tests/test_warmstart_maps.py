@@ -46,3 +46,19 @@ def test_neff_targets_round_trip():
     k = np.array([1.3, 1.2, 1.05])
     t = WS.neff_targets(k, 0.65)
     assert np.allclose(0.65 * (WS.N_SHIFT + WS.N_SCALE * t.astype(float)), k, rtol=1e-6)
+
+
+def test_lowres_vectors_are_the_kept_plane_waves_of_start_vectors():
+    grid = omx.Grid(8.0, 4.0, 64, 32)
+    hw = 16
+    f = np.random.default_rng(1).standard_normal((12, hw, hw))
+    full = WS.start_vectors(f, grid, hw)
+    low = WS.lowres_vectors(f, hw)
+    assert low.shape == (3, 2 * hw * hw)
+    # entry c + 2(gx + hw gy) of a low vector sits at c + 2(ix + Nx iy) of the full one
+    sx, sy = WS._sel(64, hw), WS._sel(32, hw)
+    for c in range(2):
+        for gx in (0, 3, hw - 1):
+            for gy in (0, 5, hw - 2):
+                assert np.allclose(low[:, c + 2 * (gx + hw * gy)], full[:, c + 2 * (sx[gx] + 64 * sy[gy])])
+    assert np.allclose(np.linalg.norm(low, axis=1), 1.0)
