@@ -95,7 +95,9 @@ int omx_destroy(omx_solver* h);
 int omx_set_epsinv(omx_solver* h, const double* epsinv);
 const char* omx_last_error(const omx_solver* h);
 /* tuning knobs: "fp32_tol", "fp32_switch", "maxiter_fp64", "guard", "seed"; start modes
-   "warm_start" and "band_guess" (omx_solve_k) */
+   "warm_start" and "band_guess" (omx_solve_k); "guess_lowres" = hw > 0 (even, <= Nx, Ny):
+   omx_solve_k's Hguess vectors hold only their hw x hw lowest plane waves, 2 hw^2 complex
+   each at c + 2(gx + hw gy) in FFT order, and are scattered into the start block on the GPU */
 int omx_set_option(omx_solver* h, const char* key, double value);
 
 /* HelmholtzMap * H for m vectors at k_vec = (0,0,k): _M! (maxwell.jl:471-481) */

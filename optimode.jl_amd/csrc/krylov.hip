@@ -1,4 +1,4 @@
-// Batched FP64 vector helpers and preconditioned MINRES (see krylov.hpp).
+// Batched vector helpers and preconditioned MINRES (see krylov.hpp).
 #include "krylov.hpp"
 
 #include <algorithm>
@@ -11,17 +11,19 @@ namespace {
 
 constexpr int AB = 256;
 
-// out[(i*nb + b)] = Re sum_rows conj(a_i) b_i over block b's rows (vector i at base + i*ld)
-__global__ __launch_bounds__(AB) void k_vdot(const c64* A, const c64* B, long ld, long K, const int* act,
+// out[(i*nb + b)] = Re sum_rows conj(a_i) b_i over block b's rows (vector i at base + i*ld),
+// accumulated in FP64
+template <typename T>
+__global__ __launch_bounds__(AB) void k_vdot(const cplx<T>* A, const cplx<T>* B, long ld, long K, const int* act,
                                              double* part) {
   const int i = blockIdx.y;
   __shared__ double red[AB];
   double s = 0;
   if (act[i]) {
-    const c64* a = A + i * ld;
-    const c64* b = B + i * ld;
+    const cplx<T>* a = A + i * ld;
+    const cplx<T>* b = B + i * ld;
     for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB)
-      s += a[r].x * b[r].x + a[r].y * b[r].y;
+      s += double(a[r].x) * double(b[r].x) + double(a[r].y) * double(b[r].y);
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -32,18 +34,20 @@ __global__ __launch_bounds__(AB) void k_vdot(const c64* A, const c64* B, long ld
   if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
 }
 
-// complex dot <a_i, b_i> = sum conj(a) b, partials (re, im)
-__global__ __launch_bounds__(AB) void k_cdot(const c64* A, const c64* B, long ld, long K, const int* act,
+// complex dot <a_i, b_i> = sum conj(a) b, partials (re, im) in FP64
+template <typename T>
+__global__ __launch_bounds__(AB) void k_cdot(const cplx<T>* A, const cplx<T>* B, long ld, long K, const int* act,
                                              c64* part) {
   const int i = blockIdx.y;
   __shared__ double rr[AB], ri[AB];
   double sr = 0, si = 0;
   if (act[i]) {
-    const c64* a = A + i * ld;
-    const c64* b = B + i * ld;
+    const cplx<T>* a = A + i * ld;
+    const cplx<T>* b = B + i * ld;
     for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-      sr += a[r].x * b[r].x + a[r].y * b[r].y;
-      si += a[r].x * b[r].y - a[r].y * b[r].x;
+      const double ax = a[r].x, ay = a[r].y, bx = b[r].x, by = b[r].y;
+      sr += ax * bx + ay * by;
+      si += ax * by - ay * bx;
     }
   }
   rr[threadIdx.x] = sr;
@@ -59,39 +63,40 @@ __global__ __launch_bounds__(AB) void k_cdot(const c64* A, const c64* B, long ld
   if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = c64{rr[0], ri[0]};
 }
 
-// out_i = sum_j c[i*4 + j] * in_j_i for up to 4 inputs (complex coefficients); out may
-// alias an input (each element is read before it is written by the same thread)
-struct Lin {
-  const c64* in[4];
+// out_i = sum_j c[i*4 + j] * in_j_i for up to 4 inputs (complex FP64 coefficients, FP64
+// arithmetic); out may alias an input (each element is read before it is written by the
+// same thread)
+template <typename T> struct Lin {
+  const cplx<T>* in[4];
   int nin;
-  c64* out;
+  cplx<T>* out;
   const c64* coef;  // per instance, 4 complex
   long ld;
 };
 
-template <int NIN>
-__global__ __launch_bounds__(AB) void k_lincomb(Lin L, long K, const int* act) {
+template <typename T, int NIN>
+__global__ __launch_bounds__(AB) void k_lincomb(Lin<T> L, long K, const int* act) {
   const int i = blockIdx.y;
   if (!act[i]) return;
   c64 cs[NIN];
-  const c64* in[NIN];
+  const cplx<T>* in[NIN];
 #pragma unroll
   for (int j = 0; j < NIN; ++j) {
     cs[j] = L.coef[4 * i + j];
     in[j] = L.in[j] + i * L.ld;
   }
-  c64* out = L.out + i * L.ld;
+  cplx<T>* out = L.out + i * L.ld;
   for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-    c64 v[NIN];
+    cplx<T> v[NIN];
 #pragma unroll
     for (int j = 0; j < NIN; ++j) v[j] = in[j][r];
     double sx = 0, sy = 0;
 #pragma unroll
     for (int j = 0; j < NIN; ++j) {
-      sx += cs[j].x * v[j].x - cs[j].y * v[j].y;
-      sy += cs[j].x * v[j].y + cs[j].y * v[j].x;
+      sx += cs[j].x * double(v[j].x) - cs[j].y * double(v[j].y);
+      sy += cs[j].x * double(v[j].y) + cs[j].y * double(v[j].x);
     }
-    out[r] = c64{sx, sy};
+    out[r] = cplx<T>{T(sx), T(sy)};
   }
 }
 
@@ -216,19 +221,20 @@ double sum_part(const double* p, int nb) {
   return s;
 }
 
-VecOps::VecOps(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_)
+template <typename T>
+VecOpsT<T>::VecOpsT(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const C* EV_)
     : s(s_), nb(nb_), ld(ld_), K(K_), nblk(int(std::min<long>(256, (K_ + AB - 1) / AB))), EV(EV_),
       part(S.vec_part), coef(S.vec_coef) {
   part.ensure(sizeof(c64) * size_t(nb_) * 256);
   coef.ensure(sizeof(c64) * 4 * size_t(nb_));
 }
 
-void VecOps::vdot(const c64* A, const c64* B, const int* act) {
-  hipLaunchKernelGGL(k_vdot, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
+template <typename T> void VecOpsT<T>::vdot(const C* A, const C* B, const int* act) {
+  hipLaunchKernelGGL(k_vdot<T>, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
 }
 
-void VecOps::lincomb(std::initializer_list<const c64*> ins, c64* outp, const int* act) {
-  Lin L{};
+template <typename T> void VecOpsT<T>::lincomb(std::initializer_list<const C*> ins, C* outp, const int* act) {
+  Lin<T> L{};
   int j = 0;
   for (auto p : ins) L.in[j++] = p;
   L.nin = j;
@@ -237,45 +243,47 @@ void VecOps::lincomb(std::initializer_list<const c64*> ins, c64* outp, const int
   L.ld = ld;
   const dim3 g(nblk, nb);
   switch (L.nin) {
-    case 1: hipLaunchKernelGGL(k_lincomb<1>, g, dim3(AB), 0, s, L, K, act); break;
-    case 2: hipLaunchKernelGGL(k_lincomb<2>, g, dim3(AB), 0, s, L, K, act); break;
-    case 3: hipLaunchKernelGGL(k_lincomb<3>, g, dim3(AB), 0, s, L, K, act); break;
-    default: hipLaunchKernelGGL(k_lincomb<4>, g, dim3(AB), 0, s, L, K, act); break;
+    case 1: hipLaunchKernelGGL((k_lincomb<T, 1>), g, dim3(AB), 0, s, L, K, act); break;
+    case 2: hipLaunchKernelGGL((k_lincomb<T, 2>), g, dim3(AB), 0, s, L, K, act); break;
+    case 3: hipLaunchKernelGGL((k_lincomb<T, 3>), g, dim3(AB), 0, s, L, K, act); break;
+    default: hipLaunchKernelGGL((k_lincomb<T, 4>), g, dim3(AB), 0, s, L, K, act); break;
   }
 }
 
-void VecOps::project(c64* x, const int* act) {
-  hipLaunchKernelGGL(k_cdot, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.as<c64>());
+template <typename T> void VecOpsT<T>::project(C* x, const int* act) {
+  hipLaunchKernelGGL(k_cdot<T>, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.as<c64>());
   hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(AB), 0, s, part.as<c64>(), nblk, coef.as<c64>());
   lincomb({x, EV}, x, act);
 }
 
-int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, const double* dk, c64* r, c64* xout,
-                      const std::vector<int>& want, double rtol, int maxit, bool throw_on_cap) {
+template <typename T>
+int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shifts, const double* dk, cplx<T>* r,
+                        cplx<T>* xout, const std::vector<int>& want, double rtol, int maxit, bool throw_on_cap) {
+  using C = cplx<T>;
   hipStream_t s = S.stream;
   const int nb = V.nb;
   const long ld = S.ld, n3 = 3 * S.N;
   // workspace kept on the handle (grown on demand): 8 vectors and the operator scratch per
   // instance, the MINRES states and two flag arrays
-  const size_t vb = sizeof(c64) * nb * ld;
+  const size_t vb = sizeof(C) * nb * ld;
   const int iblk = ((nb + 63) / 64) * 64;  // ints per flag array
-  const size_t need = 8 * vb + sizeof(c64) * nb * n3 + sizeof(MinresState) * nb + 2 * sizeof(int) * iblk;
+  const size_t need = 8 * vb + sizeof(C) * nb * n3 + sizeof(MinresState) * nb + 2 * sizeof(int) * iblk;
   S.krylov_ws.ensure(need);
   char* wp = S.krylov_ws.as<char>();
-  c64 *x = reinterpret_cast<c64*>(wp), *r1 = r, *r2 = reinterpret_cast<c64*>(wp + vb),
-      *y = reinterpret_cast<c64*>(wp + 2 * vb), *v = reinterpret_cast<c64*>(wp + 3 * vb),
-      *w = reinterpret_cast<c64*>(wp + 4 * vb), *w1 = reinterpret_cast<c64*>(wp + 5 * vb),
-      *w2 = reinterpret_cast<c64*>(wp + 6 * vb), *avp = reinterpret_cast<c64*>(wp + 7 * vb);
-  c64* dm = reinterpret_cast<c64*>(wp + 8 * vb);
-  MinresState* ms = reinterpret_cast<MinresState*>(wp + 8 * vb + sizeof(c64) * nb * n3);
+  C *x = reinterpret_cast<C*>(wp), *r1 = r, *r2 = reinterpret_cast<C*>(wp + vb),
+    *y = reinterpret_cast<C*>(wp + 2 * vb), *v = reinterpret_cast<C*>(wp + 3 * vb),
+    *w = reinterpret_cast<C*>(wp + 4 * vb), *w1 = reinterpret_cast<C*>(wp + 5 * vb),
+    *w2 = reinterpret_cast<C*>(wp + 6 * vb), *avp = reinterpret_cast<C*>(wp + 7 * vb);
+  C* dm = reinterpret_cast<C*>(wp + 8 * vb);
+  MinresState* ms = reinterpret_cast<MinresState*>(wp + 8 * vb + sizeof(C) * nb * n3);
   int* actp = reinterpret_cast<int*>(reinterpret_cast<char*>(ms) + sizeof(MinresState) * nb);
   int* livep = actp + iblk;
   OMX_HIP_CHECK(hipMemcpyAsync(actp, want.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
   const int* act = actp;
-  OMX_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(c64) * nb * ld, s));
-  OMX_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(c64) * nb * ld, s));
-  OMX_HIP_CHECK(hipMemsetAsync(w2, 0, sizeof(c64) * nb * ld, s));
-  OMX_HIP_CHECK(hipMemsetAsync(w1, 0, sizeof(c64) * nb * ld, s));
+  OMX_HIP_CHECK(hipMemsetAsync(x, 0, vb, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w, 0, vb, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w2, 0, vb, s));
+  OMX_HIP_CHECK(hipMemsetAsync(w1, 0, vb, s));
   std::vector<MinresState> hm(nb);
   for (int i = 0; i < nb; ++i) {
     std::memset(&hm[i], 0, sizeof(MinresState));
@@ -285,17 +293,18 @@ int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, c
   }
   OMX_HIP_CHECK(hipMemcpyAsync(ms, hm.data(), sizeof(MinresState) * nb, hipMemcpyHostToDevice, s));
   const int nblk = V.nblk;
-  double* part = V.part.as<double>();
-  c64* coef = V.coef.as<c64>();
+  double* part = V.part.template as<double>();
+  c64* coef = V.coef.template as<c64>();
   V.project(r1, act);
-  BView<double> AVv{avp, ld, ld}, Yv{y, ld, ld}, R1v{r1, ld, ld};
+  BView<T> AVv{avp, ld, ld}, Yv{y, ld, ld}, R1v{r1, ld, ld};
   const VecSel sel{nb, 1, nullptr, nullptr};
+  const OpPlan<T>& plan = S.pd<T>().plan;
   // y = P r1 (projected)
-  launch_operator<double>(OP_P, S.p64.plan, R1v, Yv, dm, sel, dk, nullptr, 0, s);
+  launch_operator<T>(OP_P, plan, R1v, Yv, dm, sel, dk, nullptr, 0, s);
   V.project(y, act);
   V.vdot(r1, y, act);
   hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 0, nb, act);
-  OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, vb, hipMemcpyDeviceToDevice, s));
   std::vector<int> live(want);
   int it = 0;
   for (; it < maxit; ++it) {
@@ -313,8 +322,8 @@ int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, c
     }
     // v = y / beta (coefficients left by the previous scalar step)
     V.lincomb({y}, v, la);
-    BView<double> Vv{v, ld, ld};
-    launch_operator<double>(OP_M, S.p64.plan, Vv, AVv, dm, sel, dk, nullptr, 0, s);
+    BView<T> Vv{v, ld, ld};
+    launch_operator<T>(OP_M, plan, Vv, AVv, dm, sel, dk, nullptr, 0, s);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 1, nb, la);
     V.lincomb({avp, v, r1}, y, la);
     V.vdot(v, y, la);
@@ -322,8 +331,8 @@ int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, c
     V.lincomb({y, r2}, y, la);
     std::swap(r1, r2);  // r1 <- r2
     std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
-    BView<double> R2b{r2, ld, ld}, Yb{y, ld, ld};
-    launch_operator<double>(OP_P, S.p64.plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
+    BView<T> R2b{r2, ld, ld}, Yb{y, ld, ld};
+    launch_operator<T>(OP_P, plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
     V.project(y, la);
     V.vdot(r2, y, la);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 3, nb, la);
@@ -331,7 +340,7 @@ int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, c
     // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
     V.lincomb({v, w2, w}, w1, la);
     {
-      c64* wn = w1;
+      C* wn = w1;
       w1 = w2;
       w2 = w;
       w = wn;
@@ -345,11 +354,18 @@ int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, c
   if (throw_on_cap)
     for (int i = 0; i < nb; ++i)
       if (want[i] && !hm[i].done) throw Error(OMX_ERR_NOT_CONVERGED, "MINRES did not converge within the iteration cap");
-  OMX_HIP_CHECK(hipMemcpyAsync(xout, x, sizeof(c64) * nb * ld, hipMemcpyDeviceToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(xout, x, vb, hipMemcpyDeviceToDevice, s));
   V.project(xout, act);
   V.project(xout, act);
   return it;
 }
+
+template struct VecOpsT<double>;
+template struct VecOpsT<float>;
+template int minres_complement_t<double>(Solver&, VecOpsT<double>&, const std::vector<double>&, const double*, c64*,
+                                         c64*, const std::vector<int>&, double, int, bool);
+template int minres_complement_t<float>(Solver&, VecOpsT<float>&, const std::vector<double>&, const double*,
+                                        cplx<float>*, cplx<float>*, const std::vector<int>&, double, int, bool);
 
 
 }  // namespace omx

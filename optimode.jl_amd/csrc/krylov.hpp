@@ -1,5 +1,5 @@
-// Batched FP64 vector helpers and preconditioned MINRES on the complement of a vector,
-// shared by the adjoint (eig_adjt), GVD and the FP64 eigenpair refinement.
+// Batched vector helpers and preconditioned MINRES on the complement of a vector, shared by
+// the adjoint (eig_adjt), GVD (FP64) and the eigenpair refinement's correction solves (FP32).
 #pragma once
 #include <initializer_list>
 #include <vector>
@@ -11,34 +11,46 @@ namespace omx {
 // sum of nb partial values
 double sum_part(const double* p, int nb);
 
-// Vector helpers over nb instances of K complex FP64 entries at stride ld, against the
-// unit vectors EV (vector i at EV + i ld).  Coefficients of lincomb live on the device
-// (coef: 4 complex per instance), so that no step waits for the host.
-struct VecOps {
+// Vector helpers over nb instances of K complex entries (FP64 or FP32; dots and
+// coefficients in FP64) at stride ld, against the unit vectors EV (vector i at EV + i ld).
+// Coefficients of lincomb live on the device (coef: 4 complex per instance), so that no
+// step waits for the host.
+template <typename T> struct VecOpsT {
+  using C = cplx<T>;
   hipStream_t s;
   int nb;
   long ld, K;
   int nblk;
-  const c64* EV;
+  const C* EV;
   DevMem& part;  // the handle's buffers, grown on demand (no allocation per solve)
   DevMem& coef;
-  VecOps(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const c64* EV_);
+  VecOpsT(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const C* EV_);
   // part[i nblk + b] = partial sums of Re <A_i, B_i>
-  void vdot(const c64* A, const c64* B, const int* act);
+  void vdot(const C* A, const C* B, const int* act);
   // out_i = sum_j coef[4 i + j] in_j_i (out may alias an input)
-  void lincomb(std::initializer_list<const c64*> ins, c64* outp, const int* act);
+  void lincomb(std::initializer_list<const C*> ins, C* outp, const int* act);
   // x_i <- x_i - ev_i <ev_i, x_i>
-  void project(c64* x, const int* act);
+  void project(C* x, const int* act);
 };
+using VecOps = VecOpsT<double>;
 
 // Solves (M(k_i) - shift_i) x_i = r_i on the complement of ev_i for the instances with
 // want[i] (MINRES, Paige & Saunders 1975, preconditioned by P projected onto the
 // complement; M - shift is Hermitian, singular along ev when shift is its eigenvalue, and
-// indefinite above the lowest band).  r is projected on entry and overwritten; xout
-// receives the solution orthogonalised against ev twice.  Stops when the preconditioned
-// residual estimate falls below rtol times its start, or after maxit iterations (an error
-// only when throw_on_cap).  Returns the iteration count.
-int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, const double* dk, c64* r, c64* xout,
-                      const std::vector<int>& want, double rtol, int maxit = 1000, bool throw_on_cap = true);
+// indefinite above the lowest band), with vectors and operator applications in T and the
+// recurrences in FP64.  r is projected on entry and overwritten; xout receives the
+// solution orthogonalised against ev twice.  Stops when the preconditioned residual
+// estimate falls below rtol times its start, or after maxit iterations (an error only when
+// throw_on_cap).  Returns the iteration count.
+template <typename T>
+int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shifts, const double* dk, cplx<T>* r,
+                        cplx<T>* xout, const std::vector<int>& want, double rtol, int maxit = 1000,
+                        bool throw_on_cap = true);
+
+inline int minres_complement(Solver& S, VecOps& V, const std::vector<double>& shifts, const double* dk, c64* r,
+                             c64* xout, const std::vector<int>& want, double rtol, int maxit = 1000,
+                             bool throw_on_cap = true) {
+  return minres_complement_t<double>(S, V, shifts, dk, r, xout, want, rtol, maxit, throw_on_cap);
+}
 
 }  // namespace omx

@@ -681,6 +681,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_rtol = v;
   else if (key == "jd_maxit")
     opt.jd_maxit = int(v);
+  else if (key == "jd_fp32")
+    opt.jd_fp32 = v != 0 ? 1 : 0;
   else if (key == "reg_ops" || key == "slab3d") {
     (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
     select_operator(*this, p32);
@@ -1150,6 +1152,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
     W.cold.alloc(sizeof(int) * n);
     W.lamd.alloc(sizeof(double) * n);
     for (DevMem* v : {&W.Tm, &W.T2, &W.ATm, &W.XB, &W.RBm}) v->alloc(sizeof(c64) * size_t(n) * ld);
+    for (DevMem* v : {&W.XBf, &W.RBf, &W.Tf}) v->alloc(sizeof(cplx<float>) * size_t(n) * ld);
     W.part.alloc(sizeof(c64) * size_t(n) * ng * RF_PAIRS);
     for (DevMem* v : {&W.Gd, &W.Bd, &W.Cd}) v->alloc(sizeof(c64) * size_t(n) * RF_PAIRS);
     W.normp.alloc(sizeof(double) * size_t(n) * nr);
@@ -1247,8 +1250,21 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       if (!need || jd >= jd_max) break;
       // t: (M - rho) t = r on the complement of x (the sign of t is immaterial)
       VecOps V(S, s, ninst, ld, K, XB.as<c64>());
-      const int its = minres_complement(S, V, lam_band, b.kd.as<double>(), RBm.as<c64>(), Tm.as<c64>(), jdh,
-                                        S.opt.jd_rtol, S.opt.jd_maxit, false);
+      int its;
+      if (S.opt.jd_fp32) {
+        // the correction only needs jd_rtol (1e-3) relative accuracy: FP32 vectors and
+        // operator, FP64 recurrences; the residual that decides convergence stays FP64
+        using cf = cplx<float>;
+        launch_convert<double, float>(XB.as<c64>(), ld, ld, W.XBf.as<cf>(), ld, ld, ninst, 1, K, s);
+        launch_convert<double, float>(RBm.as<c64>(), ld, ld, W.RBf.as<cf>(), ld, ld, ninst, 1, K, s);
+        VecOpsT<float> V32(S, s, ninst, ld, K, W.XBf.as<cf>());
+        its = minres_complement_t<float>(S, V32, lam_band, b.kd.as<double>(), W.RBf.as<cf>(), W.Tf.as<cf>(), jdh,
+                                         S.opt.jd_rtol, S.opt.jd_maxit, false);
+        launch_convert<float, double>(W.Tf.as<cf>(), ld, ld, Tm.as<c64>(), ld, ld, ninst, 1, K, s);
+      } else {
+        its = minres_complement(S, V, lam_band, b.kd.as<double>(), RBm.as<c64>(), Tm.as<c64>(), jdh, S.opt.jd_rtol,
+                                S.opt.jd_maxit, false);
+      }
       for (int i = 0; i < ninst; ++i) {
         stats.matvec += long(its) * jdh[i];
         stats.precon += long(its) * jdh[i];

@@ -160,6 +160,7 @@ struct Options {
                                 // from the shared first FP32 eigensolve)
   double jd_rtol = 1e-3;        // relative tolerance of the refinement's correction solves
   int jd_maxit = 60;            // iteration cap of one correction solve
+  int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
 };
 
 // Device-resident EME mode set: E and H of nev modes, planar per mode and component
@@ -176,6 +177,7 @@ struct RefineWorkspace {
   int ninst = 0, nblk_g = 0, nblk_r = 0, dps = 0;
   long ld = 0;
   DevMem n1d, zero, cold, lamd, Tm, T2, ATm, XB, RBm, part, Gd, Bd, Cd, normp, scl, dots;
+  DevMem XBf, RBf, Tf;  // FP32 copies for the correction solves (option jd_fp32)
 };
 
 class Solver {
