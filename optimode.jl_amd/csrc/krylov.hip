@@ -307,19 +307,10 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, vb, hipMemcpyDeviceToDevice, s));
   std::vector<int> live(want);
   int it = 0;
-  for (; it < maxit; ++it) {
-    const int* la = livep;
-    if (it % 4 == 0) {
-      OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), ms, sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
-      OMX_HIP_CHECK(hipStreamSynchronize(s));
-      bool anyl = false;
-      for (int i = 0; i < nb; ++i) {
-        live[i] = want[i] && !hm[i].done;
-        anyl = anyl || live[i];
-      }
-      if (!anyl) break;
-      OMX_HIP_CHECK(hipMemcpyAsync(livep, live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
-    }
+  const int* la = livep;
+  // one MINRES iteration; its buffer rotations (r1 <- r2 <- y <- r1 and w <- w1 <- w2 <- w)
+  // are 3-cycles, so three iterations leave every pointer where it was
+  auto iteration = [&]() {
     // v = y / beta (coefficients left by the previous scalar step)
     V.lincomb({y}, v, la);
     BView<T> Vv{v, ld, ld};
@@ -348,7 +339,40 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 4, nb, la);
     V.lincomb({x, w}, x, la);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 5, nb, la);
+  };
+  // Groups of three iterations, replayed from one captured HIP graph (option "graphs"):
+  // the kernels are short and the host's launch rate, not the GPU, set their pace.  The
+  // graph holds kernels only (no memset or copy nodes); the host reads the MINRES states
+  // between groups.
+  constexpr int GROUP = 3;
+  hipGraphExec_t gexec = nullptr;
+  for (; it < maxit; it += GROUP) {
+    OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), ms, sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    bool anyl = false;
+    for (int i = 0; i < nb; ++i) {
+      live[i] = want[i] && !hm[i].done;
+      anyl = anyl || live[i];
+    }
+    if (!anyl) break;
+    OMX_HIP_CHECK(hipMemcpyAsync(livep, live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+    const int nit = std::min(GROUP, maxit - it);
+    if (S.opt.graphs && nit == GROUP) {
+      if (!gexec) {
+        hipGraph_t g = nullptr;
+        OMX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int u = 0; u < GROUP; ++u) iteration();
+        OMX_HIP_CHECK(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        OMX_HIP_CHECK(e);
+      }
+      OMX_HIP_CHECK(hipGraphLaunch(gexec, s));
+    } else {
+      for (int u = 0; u < nit; ++u) iteration();
+    }
   }
+  if (gexec) (void)hipGraphExecDestroy(gexec);
   OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), ms, sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));
   OMX_HIP_CHECK(hipStreamSynchronize(s));
   if (throw_on_cap)

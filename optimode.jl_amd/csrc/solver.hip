@@ -683,6 +683,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_maxit = int(v);
   else if (key == "jd_fp32")
     opt.jd_fp32 = v != 0 ? 1 : 0;
+  else if (key == "graphs")
+    opt.graphs = v != 0 ? 1 : 0;
   else if (key == "reg_ops" || key == "slab3d") {
     (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
     select_operator(*this, p32);
