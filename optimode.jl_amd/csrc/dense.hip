@@ -289,8 +289,9 @@ __device__ int d_ortho_against(c64* X, int rows, int cx, const c64* Y, int cy, d
 // so A' = L A L^H with L block diagonal.  A round is two LDS passes: the m/2 rotations
 // (one thread each), then the (m/2)^2 2x2 blocks of A (read from one buffer, written to
 // the other) and the m x m/2 row pairs of V (in place).  A sweep is m - 1 rounds.  Sweeps
-// stop after one whose rotations all have |sin| <= 1e-9 (the off-diagonal part left is
-// then below 1e-18 |A|) or that performs none.  Returns the buffer holding the final
+// stop after one whose rotations all have |sin| <= stop (the off-diagonal part left is then
+// below about stop^2 |A|: 1e-18 for the FP64 batches' 1e-9, 1e-9 for the FP32 batches' 3e-5)
+// or that performs none.  Returns the buffer holding the final
 // matrix (eigenvalues on its diagonal); V holds the eigenvectors in its columns.
 __device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {
   const int ia = (t == 0) ? 0 : 1 + (t - 1 + r) % (m - 1);
@@ -330,7 +331,7 @@ __device__ __forceinline__ double jac_rot(const c64* A, int p, int q, double& c,
   return sn;
 }
 
-__device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
+__device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n, double stop) {
   __shared__ int rot[3], big[3];
   __shared__ int rp[DMAX / 2], rq[DMAX / 2];
   __shared__ double rc[DMAX / 2];
@@ -365,7 +366,7 @@ __device__ c64* d_jacobi(c64* A0, c64* A1, c64* V, int n) {
         c64 sv;
         const double sn = jac_rot(Ac, p, q, c, sv);
         if (sn >= 0.0) rot[sl] = 1;
-        if (sn > 1e-9) big[sl] = 1;
+        if (sn > stop) big[sl] = 1;
         rp[t] = p;
         rq[t] = q;
         rc[t] = c;
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(RBLK) void k_dense_rr(DenseLaunch dl, LState* st, i
     for (int q = threadIdx.x; q < nact; q += blockDim.x) li[q] = A[q * DLD + q].x;
     return;
   }
-  const c64* Ad = d_jacobi(A, A2, V, py);
+  const c64* Ad = d_jacobi(A, A2, V, py, dl.jac_stop);
   for (int j = threadIdx.x; j < py; j += blockDim.x) {
     const double wj = Ad[j * DLD + j].x;
     int r = 0;

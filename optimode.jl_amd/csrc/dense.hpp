@@ -61,6 +61,7 @@ struct DenseLaunch {
   int nblk;          // combine blocks per instance (residual-norm partials)
   int qmax;          // combine column bound
   int maxiter;
+  double jac_stop;   // Rayleigh-Ritz Jacobi: stop after a sweep whose rotations all have |sin| <= jac_stop
 };
 
 // Rayleigh-Ritz: G (py x py) from gram partials -> cX (py x nact), lam (nact).

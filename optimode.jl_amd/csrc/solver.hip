@@ -331,6 +331,7 @@ template <typename T> class Batch {
     dl.pm = pm;
     dl.nchunk = 1;  // dense kernels read the reduced Gram matrices
     dl.qmax = qmax;
+    dl.jac_stop = 1e-9;
     hst.resize(ninst);
     for (int b = 0; b < 2; ++b) {
       snap[b].alloc(sizeof(LState) * ninst);
@@ -418,6 +419,7 @@ template <typename T> class Batch {
           const std::vector<int>& msize, const std::vector<double>& tols, int maxiter, SolveStats& stats) {
     if (maxiter > maxiter_alloc) maxiter = maxiter_alloc;
     dl.maxiter = maxiter;
+    dl.jac_stop = sizeof(T) == 4 ? S.opt.jac_stop32 : 1e-9;
     std::vector<LState> h(ninst);
     for (int i = 0; i < ninst; ++i) {
       LState& s = h[i];
@@ -685,6 +687,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_fp32 = v != 0 ? 1 : 0;
   else if (key == "graphs")
     opt.graphs = v != 0 ? 1 : 0;
+  else if (key == "jac_stop32")
+    opt.jac_stop32 = v;
   else if (key == "reg_ops" || key == "slab3d") {
     (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
     select_operator(*this, p32);
