@@ -290,7 +290,7 @@ __device__ int d_ortho_against(c64* X, int rows, int cx, const c64* Y, int cy, d
 // (one thread each), then the (m/2)^2 2x2 blocks of A (read from one buffer, written to
 // the other) and the m x m/2 row pairs of V (in place).  A sweep is m - 1 rounds.  Sweeps
 // stop after one whose rotations all have |sin| <= stop (the off-diagonal part left is then
-// below about stop^2 |A|: 1e-18 for the FP64 batches' 1e-9, 1e-9 for the FP32 batches' 3e-5)
+// below about stop^2 |A|: 1e-18 for the FP64 batches' 1e-9, 1e-8 for the FP32 batches' 1e-4)
 // or that performs none.  Returns the buffer holding the final
 // matrix (eigenvalues on its diagonal); V holds the eigenvectors in its columns.
 __device__ __forceinline__ void jac_pair(int t, int r, int m, int& p, int& q) {

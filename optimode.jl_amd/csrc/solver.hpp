@@ -161,7 +161,8 @@ struct Options {
   double jd_rtol = 1e-3;        // relative tolerance of the refinement's correction solves
   int jd_maxit = 60;            // iteration cap of one correction solve
   int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
-XX
+  double jac_stop32 = 1e-4;     // Jacobi stop (max |sin| of a sweep) of the FP32 batches' Rayleigh-Ritz
+                                // (off-diagonal left ~1e-8 |A|, below FP32 data; FP64 batches: 1e-9)
   int graphs = 0;               // MINRES iterations replayed from a captured HIP graph (groups of 3;
                                 // same results, no measured gain: the gaps are GPU-side)
 };
