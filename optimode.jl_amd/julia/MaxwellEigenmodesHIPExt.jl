@@ -6,11 +6,13 @@
 #   solve_k    lib/MaxwellEigenmodes/src/solve.jl:182-196, solvers/gpu.jl:213-242
 #   rrule      lib/MaxwellEigenmodes/src/solve.jl:213-289, solvers/gpu.jl:292-365
 #   3D cells   solve_k on Grid{3} and its rrule, rrule(solve_k_periodic) (src/grads/period.jl:144-249)
+#   learned    HIPWarmNet + hip_solve_k_learned: the learned warm start (no reference counterpart)
 # Host data stays Julia-owned Float64 / ComplexF64 column-major arrays; the library copies
 # them to the device.  Non-zero statuses become `error(...)` with the library's message.
 module MaxwellEigenmodesHIPExt
 
 using LinearAlgebra
+using FFTW: ifft
 using Logging: AbstractLogger, NullLogger
 using ChainRulesCore
 using ChainRulesCore: NoTangent, AbstractZero
@@ -18,7 +20,8 @@ using DielectricSmoothing: Grid, N
 import MaxwellEigenmodes: solve_ω², solve_k, solve_k_periodic, AbstractEigensolver, ModeSolver
 
 export HIPSolver, hip_group_index, hip_ng_gvd, hip_ng_gvd_E, hip_build_modes, hip_overlap_matrix,
-       hip_period_derivatives, hip_sliceinv_3x3, hip_sliceinv_3x3_pullback
+       hip_period_derivatives, hip_sliceinv_3x3, hip_sliceinv_3x3_pullback, HIPWarmNet, hip_warmnet_params,
+       hip_warmnet_params!, hip_warmnet_forward, hip_warmnet_train!, hip_solve_k_learned
 
 const libomx = get(ENV, "OMX_LIBRARY", "libomx.so")
 
@@ -359,6 +362,149 @@ function hip_overlap_matrix(solver::HIPSolver, slot_a::Integer, na::Integer, slo
                h, slot_a, slot_b, conjugate, O)
     _check(st, h)
     return O
+end
+
+# ---------------------------------------------------------------------------------------
+# Learned warm start (omx_warmnet_*): a small CNN on the GPU predicting per-band effective
+# indices and low-plane-wave start vectors from the ε⁻¹ map and ω.  Arrays follow the C
+# layout [batch][channel][a][b] (b fastest), i.e. Julia arrays of size (b, a, channel, batch);
+# the maps here use a = block index along x, b = block index along y.  The normalisation
+# constants are those of omx/warmstart.py, with which the shipped network was trained.
+# ---------------------------------------------------------------------------------------
+
+const WS_N = (1.8, 0.2)          # effective index = N[1] + N[2] * output
+const WS_OMEGA = (0.65, 0.05)
+const WS_DIAG = (0.55, 0.25)
+const WS_OFF = 0.05
+
+mutable struct HIPWarmNet
+    handle::Ptr{Cvoid}
+    hw::Int
+    nev::Int
+end
+
+function _wcheck(st)
+    st == 0 || error("omx warm-start network: ", unsafe_string(ccall((:omx_last_error, libomx), Cstring, (Ptr{Cvoid},), C_NULL)))
+end
+
+"""
+    HIPWarmNet(hw=64, nev=8; seed=0, device=0)
+
+The warm-start network on the GPU; load trained parameters with `hip_warmnet_params!`
+(the shipped `omx/data/warmnet_tfln256.npz` holds them under `params`).
+"""
+function HIPWarmNet(hw::Integer=64, nev::Integer=8; seed::Integer=0, device::Integer=0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    _wcheck(ccall((:omx_warmnet_create, libomx), Cint, (Cint, Cint, Cint, UInt64, Ref{Ptr{Cvoid}}),
+                  device, hw, nev, seed, h))
+    n = HIPWarmNet(h[], Int(hw), Int(nev))
+    finalizer(n) do x
+        x.handle == C_NULL || ccall((:omx_warmnet_destroy, libomx), Cint, (Ptr{Cvoid},), x.handle)
+        x.handle = C_NULL
+    end
+    return n
+end
+
+function hip_warmnet_params(net::HIPWarmNet)
+    n = Ref{Int64}(0)
+    _wcheck(ccall((:omx_warmnet_num_params, libomx), Cint, (Ptr{Cvoid}, Ref{Int64}), net.handle, n))
+    p = Vector{Float32}(undef, n[])
+    _wcheck(ccall((:omx_warmnet_get_params, libomx), Cint, (Ptr{Cvoid}, Ptr{Float32}), net.handle, p))
+    return p
+end
+
+function hip_warmnet_params!(net::HIPWarmNet, p::AbstractVector{<:Real})
+    q = Vector{Float32}(p)
+    length(q) == length(hip_warmnet_params(net)) || error("wrong parameter count")
+    _wcheck(ccall((:omx_warmnet_set_params, libomx), Cint, (Ptr{Cvoid}, Ptr{Float32}), net.handle, q))
+    return net
+end
+
+"""
+    hip_warmnet_forward(net, x) -> (fields, k)
+
+x: (hw, hw, 7, B) Float32 -> fields (hw, hw, 4nev, B), k (nev, B).
+"""
+function hip_warmnet_forward(net::HIPWarmNet, x::AbstractArray{<:Real,4})
+    X = Array{Float32}(x)
+    size(X)[1:3] == (net.hw, net.hw, 7) || error("expected (hw, hw, 7, B) inputs")
+    B = size(X, 4)
+    Y = Array{Float32}(undef, net.hw, net.hw, 4 * net.nev, B)
+    K = Array{Float32}(undef, net.nev, B)
+    _wcheck(ccall((:omx_warmnet_forward, libomx), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}),
+                  net.handle, B, X, Y, K))
+    return Y, K
+end
+
+"""
+    hip_warmnet_train!(net, x, t_fields, t_k; wk=1, epochs=100, batch=16, lr=1e-3, seed=0) -> loss per epoch
+
+Adam on the GPU over the whole set (x: (hw, hw, 7, n), t_fields: (hw, hw, 4nev, n), t_k: (nev, n)).
+"""
+function hip_warmnet_train!(net::HIPWarmNet, x, tf, tk; wk::Real=1.0, epochs::Integer=100, batch::Integer=16,
+                            lr::Real=1e-3, seed::Integer=0)
+    X, TF, TK = Array{Float32}(x), Array{Float32}(tf), Array{Float32}(tk)
+    n = size(X, 4)
+    hist = Vector{Float64}(undef, epochs)
+    _wcheck(ccall((:omx_warmnet_train, libomx), Cint,
+                  (Ptr{Cvoid}, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Cfloat, Cint, Cint, Cfloat, UInt64,
+                   Ptr{Float64}), net.handle, n, X, TF, TK, wk, epochs, batch, lr, seed, hist))
+    return hist
+end
+
+# network input of one (ε⁻¹, ω): block means of ε⁻¹ xx yy zz xy xz yz, then ω
+function _ws_features(ε⁻¹::AbstractArray{<:Real,4}, grid::Grid{2}, ω::Real, hw::Int)
+    fx, fy = grid.Nx ÷ hw, grid.Ny ÷ hw
+    (fx * hw == grid.Nx && fy * hw == grid.Ny) || error("grid is not a multiple of the $hw map")
+    X = Array{Float32}(undef, hw, hw, 7, 1)
+    comps = ((1, 1), (2, 2), (3, 3), (1, 2), (1, 3), (2, 3))
+    for (c, (a, b)) in enumerate(comps), ix in 1:hw, iy in 1:hw
+        m = sum(@view ε⁻¹[a, b, (ix-1)*fx+1:ix*fx, (iy-1)*fy+1:iy*fy]) / (fx * fy)
+        X[iy, ix, c, 1] = c <= 3 ? (m - WS_DIAG[1]) / WS_DIAG[2] : m / WS_OFF
+    end
+    X[:, :, 7, 1] .= (ω - WS_OMEGA[1]) / WS_OMEGA[2]
+    return X
+end
+
+"""
+    hip_solve_k_learned(ω, ε⁻¹, grid, solver, net; nev=net.nev, k0_scale=1.03, kwargs...)
+
+`solve_k` whose shared first eigensolve starts at `k0_scale` times the network's band-1 k
+from the network's start block (given by its hw x hw lowest plane waves and scattered on
+the GPU); the stopping rules and the converged result are those of `solve_k`.
+"""
+function hip_solve_k_learned(ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64}, solver::HIPSolver,
+                             net::HIPWarmNet; nev::Integer=net.nev, k0_scale::Real=1.03, max_eigsolves=60,
+                             maxiter=100, k_tol=1e-8, eig_tol=1e-8)
+    nev <= net.nev || error("the network predicts $(net.nev) bands")
+    hw = net.hw
+    Y, K = hip_warmnet_forward(net, _ws_features(ε⁻¹, grid, ω, hw))
+    k0 = max(k0_scale * ω * (WS_N[1] + WS_N[2] * K[1, 1]), 1e-3)
+    L = Array{ComplexF64}(undef, 2, hw, hw, nev)       # entry c + 2(gx + hw gy) of each vector
+    for j in 1:nev
+        for c in 1:2
+            u = [complex(Float64(Y[iy, ix, 4(j-1)+2c-1, 1]), Float64(Y[iy, ix, 4(j-1)+2c, 1])) for ix in 1:hw, iy in 1:hw]
+            L[c, :, :, j] .= ifft(u)
+        end
+        v = @view L[:, :, :, j]
+        v ./= max(norm(v), eps())
+    end
+    h = _handle!(solver, grid, ε⁻¹)
+    kmags = Vector{Float64}(undef, nev)
+    evecs = Matrix{ComplexF64}(undef, 2 * N(grid), nev)
+    info = Ref(OmxInfo())
+    kg = Float64[k0]
+    _check(ccall((:omx_set_option, libomx), Cint, (Ptr{Cvoid}, Cstring, Cdouble), h, "guess_lowres", hw), h)
+    st = try
+        ccall((:omx_solve_k, libomx), Cint,
+              (Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble, Cint, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Cint,
+               Ptr{Cdouble}, Ptr{ComplexF64}, Ref{OmxInfo}),
+              h, ω, nev, k_tol, eig_tol, max_eigsolves, maxiter, kg, L, nev, kmags, evecs, info)
+    finally
+        ccall((:omx_set_option, libomx), Cint, (Ptr{Cvoid}, Cstring, Cdouble), h, "guess_lowres", 0.0)
+    end
+    _check(st, h)
+    return kmags, [evecs[:, j] for j in 1:nev]
 end
 
 end # module
