@@ -311,7 +311,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-algorithm comparison solve")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison solves")
+    ap.add_argument("--no-learned", action="store_true", help="skip the comparison solves from the other start")
+    ap.add_argument("--start", default="auto", choices=["auto", "learned", "plain"],
+                    help="start of the timed solves: the shipped warm-start network (auto: when it loads) or plain")
     ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "periodic", "batch"],
                     help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain; "
                          "periodic: solve_k on a 128x96x24 3D Bragg-grating cell; batch: the 64-instance "
@@ -350,8 +352,20 @@ def main():
     solver = omx.HIPSolver(precision=a.precision, profile=True)
     kw = dict(nev=NEV, k_tol=1e-10, eig_tol=1e-8, max_eigsolves=60, maxiter=200, fetch_evecs=False,
               return_info=True)
+    # the timed solves start from the shipped warm-start network (the guess is inside the
+    # timed region); without it, or with --start plain, from k_guess and random vectors
+    st = None if a.start == "plain" else load_learned()
+    if a.start == "learned" and st is None:
+        raise RuntimeError("--start learned: the warm-start network could not be loaded")
+
+    def step(sv):
+        if st is not None:
+            from omx import warmstart as WS
+            return WS.solve_k_learned(OMEGA, ei, grid, sv, st, **kw)
+        return omx.solve_k(OMEGA, ei, grid, sv, **kw)
+
     for _ in range(a.warmup):
-        omx.solve_k(OMEGA, ei, grid, solver, **kw)
+        step(solver)
     # reset the live kernel statistics so that they cover the timed region only
     h = solver.handle(grid, ei)
     omx._lib.check(omx._lib.lib().omx_set_option(h.h, b"profile", 1.0), h.h)
@@ -361,7 +375,7 @@ def main():
     errs = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        km, _, info = omx.solve_k(OMEGA, ei, grid, solver, **kw)
+        km, _, info = step(solver)
         rel = np.abs(slope_ref * (km - k_ref)) / OMEGA ** 2
         converged += int(np.sum(rel <= REL_TOL))
         errs.append(float(rel.max()))
@@ -389,7 +403,11 @@ def main():
                  "modes_converged": nconv, "lobpcg_iters_fp64": infop["lobpcg_iters_fp64"],
                  "newton_steps": infop["n_newton"], "matvecs": infop["n_matvec"],
                  "speedup": round(value / (nconv / dtp), 2) if nconv else None}
-    learned = None if a.no_learned else learned_start(a, OMEGA, ei, grid, kw, slope_ref, k_ref, value, infos)
+    # the other start on the same kernels, outside the timed region
+    other = None
+    if not a.no_learned:
+        other = (plain_start(a, OMEGA, ei, grid, kw, slope_ref, k_ref, value, infos) if st is not None else
+                 learned_start(a, OMEGA, ei, grid, kw, slope_ref, k_ref, value, infos))
     roofline = None
     if op_n > 0 and op_ms > 0:
         achieved = op_bytes / (op_ms * 1e-3) / 1e9
@@ -428,26 +446,60 @@ def main():
             "newton_steps_per_step": float(np.mean([i["n_newton"] for i in infos])),
             "matvecs_per_step": float(np.mean([i["n_matvec"] for i in infos])),
             "parallelism": "single GPU; bands batched as independent Newton instances",
+            "start": "learned (omx/data/warmnet_tfln256.npz, guess inside the timed region)" if st is not None
+                     else "plain (k_guess, random vectors)",
         },
         "plain": plain,
-        "learned_start": learned,
+        ("plain_start" if st is not None else "learned_start"): other,
         "roofline": roofline,
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(out), flush=True)
 
 
-def learned_start(a, omega, ei, grid, kw, slope_ref, k_ref, value, infos):
-    """The same solve started from the trained warm-start network (omx/data/warmnet_tfln256.npz,
-    trained on 512 solver-converged rib draws): the guess (features, network forward, start
-    vectors) is inside the timed region.  Outside the headline's timed region."""
-    import numpy as np
-    import omx
+def load_learned():
+    """The shipped warm-start network (trained on 512 solver-converged rib draws), or None."""
     from omx import warmstart as WS
     path = os.path.join(WS.DATA_DIR, "warmnet_tfln256.npz")
     if not os.path.exists(path):
         return None
-    st = WS.LearnedStart.load(path)
+    return WS.LearnedStart.load(path)
+
+
+def plain_start(a, omega, ei, grid, kw, slope_ref, k_ref, value, infos):
+    """The same solve from the plain start (k_guess, random vectors) on a fresh handle,
+    outside the headline's timed region."""
+    import numpy as np
+    import omx
+    solver = omx.HIPSolver(precision=a.precision)
+    for _ in range(max(1, a.warmup)):
+        omx.solve_k(omega, ei, grid, solver, **kw)
+    conv, pi = 0, []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        km, _, info = omx.solve_k(omega, ei, grid, solver, **kw)
+        conv += int(np.sum(np.abs(slope_ref * (km - k_ref)) / omega ** 2 <= REL_TOL))
+        pi.append(info)
+    dt = time.perf_counter() - t0
+    return {"modes_per_s": round(conv / dt, 3), "ms_per_step": round(1e3 * dt / a.steps, 2),
+            "modes_converged": conv, "learned_vs_plain_start": round(value / (conv / dt), 3) if conv else None,
+            "device_ms_per_step": round(float(np.mean([i["device_ms"] for i in pi])), 2),
+            "device_ms_per_step_learned_start": round(float(np.mean([i["device_ms"] for i in infos])), 2),
+            "lobpcg_iters_fp32_per_step": float(np.mean([i["lobpcg_iters_fp32"] for i in pi])),
+            "lobpcg_iters_fp32_per_step_learned_start": float(np.mean([i["lobpcg_iters_fp32"] for i in infos])),
+            "newton_steps_per_step": float(np.mean([i["n_newton"] for i in pi]))}
+
+
+def learned_start(a, omega, ei, grid, kw, slope_ref, k_ref, value, infos):
+    """The same solve started from the trained warm-start network, the guess inside the
+    timed region, when the headline ran from the plain start.  Outside the headline's timed
+    region."""
+    import numpy as np
+    import omx
+    from omx import warmstart as WS
+    st = load_learned()
+    if st is None:
+        return None
     solver = omx.HIPSolver(precision=a.precision)
     for _ in range(max(1, a.warmup)):
         WS.solve_k_learned(omega, ei, grid, solver, st, **kw)

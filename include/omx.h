@@ -255,6 +255,21 @@ int omx_warmnet_loss_grad(omx_warmnet* h, int batch, const float* x, const float
 int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
                       int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out);
 
+/* The guess omx_solve_k_learned starts from, for inspection: k_out (nev floats, the
+   normalised effective indices) and, unless NULL, lowres_out (nev vectors of 2 hw^2 complex,
+   entry c + 2(gx + hw gy), unit norm). */
+int omx_warmnet_guess(omx_warmnet* net, omx_solver* h, double omega, float* k_out, double* lowres_out);
+
+/* omx_solve_k from the network's guess with everything on the device but the nev predicted
+   indices: features from the handle's eps^-1, one network forward, the inverse 2D DFT of the
+   predicted maps into the start block's hw x hw lowest plane waves; the shared first
+   eigensolve starts at k0_scale times the predicted band-1 k (1.03 is the tested choice).
+   Arguments and outputs otherwise as omx_solve_k; 2D handles whose Nx, Ny are multiples of
+   the network's hw, nev <= the network's band count, both on one device. */
+int omx_solve_k_learned(omx_solver* h, omx_warmnet* net, double omega, int nev, double k0_scale, double k_tol,
+                        double eig_tol, int max_eigsolves, int maxiter, double* kmags_out, double* evecs_out,
+                        omx_info* info);
+
 /* Live kernel statistics (no counterpart in the reference: instrumentation for
    measurement).  With omx_set_option(h, "profile", 1) every operator application inside
    a solve is bracketed by HIP events on the handle's stream; `family` is "op_fp32" or

@@ -327,6 +327,32 @@ int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fiel
   return guarded_net(h, [&](omx::WarmNet& w) { w.train(n, x, t_fields, t_k, wk, epochs, batch, lr, seed, loss_hist_out); });
 }
 
+int omx_warmnet_guess(omx_warmnet* net, omx_solver* h, double omega, float* k_out, double* lowres_out) {
+  if (!net || !net->n || !k_out) return OMX_ERR_INVALID_ARG;
+  return guarded(h, [&](omx::Solver& s) {
+    std::lock_guard<std::mutex> lk(net->mu);
+    s.require_2d("the learned start");
+    if (net->n->device() != s.device) throw omx::Error(OMX_ERR_INVALID_ARG, "the network and the solver are on different devices");
+    OMX_HIP_CHECK(hipSetDevice(s.device));
+    const omx::c64* low = net->n->guess_device(s.p64.plan.eps6, s.nx, s.ny, omega, k_out);
+    if (lowres_out) {
+      const size_t n = size_t(net->n->nev()) * 2 * net->n->hw() * net->n->hw();
+      OMX_HIP_CHECK(hipMemcpy(lowres_out, low, sizeof(omx::c64) * n, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int omx_solve_k_learned(omx_solver* h, omx_warmnet* net, double omega, int nev, double k0_scale, double k_tol,
+                        double eig_tol, int max_eigsolves, int maxiter, double* kmags_out, double* evecs_out,
+                        omx_info* info) {
+  if (!net || !net->n) return OMX_ERR_INVALID_ARG;
+  return guarded(h, [&](omx::Solver& s) {
+    std::lock_guard<std::mutex> lk(net->mu);
+    omx::solve_k_learned(s, *net->n, omega, nev, k0_scale, k_tol, eig_tol, max_eigsolves, maxiter, kmags_out,
+                         evecs_out, info);
+  });
+}
+
 int omx_get_kernel_stats(omx_solver* h, const char* family, double* total_ms, int64_t* launches, double* bytes) {
   if (!family || !total_ms || !launches || !bytes) return OMX_ERR_INVALID_ARG;
   return guarded(h, [&](omx::Solver& s) {

@@ -872,14 +872,19 @@ void upload_block(Solver& S, Batch<T>& b, const double* H, int ncols, int inst_f
 template <typename T>
 void upload_block_lowres(Solver& S, Batch<T>& b, const double* H, int ncols, int hw, int inst_from, int inst_to) {
   const long per = 2L * hw * hw;
-  DevMem tmp(sizeof(c64) * size_t(ncols) * per);
-  OMX_HIP_CHECK(hipMemcpyAsync(tmp.as<void>(), H, sizeof(c64) * size_t(ncols) * per, hipMemcpyHostToDevice, S.stream));
+  DevMem tmp;
+  const c64* src = S.guess_dev;
+  if (!src) {
+    tmp.alloc(sizeof(c64) * size_t(ncols) * per);
+    OMX_HIP_CHECK(hipMemcpyAsync(tmp.as<void>(), H, sizeof(c64) * size_t(ncols) * per, hipMemcpyHostToDevice, S.stream));
+    src = tmp.as<c64>();
+  }
   for (int i = inst_from; i < inst_to; ++i) {
     cplx<T>* dst = b.p(b.X[b.cur]) + i * b.istride;
     OMX_HIP_CHECK(hipMemsetAsync(dst, 0, sizeof(cplx<T>) * size_t(ncols) * b.ld, S.stream));
     const long n = per * ncols;
     hipLaunchKernelGGL((k_scatter_lowres<T>), dim3(unsigned(std::min<long>((n + 255) / 256, 4096))), dim3(256), 0,
-                       S.stream, tmp.as<c64>(), ncols, hw, S.nx, S.ny, dst, b.ld);
+                       S.stream, src, ncols, hw, S.nx, S.ny, dst, b.ld);
   }
   OMX_HIP_CHECK(hipGetLastError());
   OMX_HIP_CHECK(hipStreamSynchronize(S.stream));

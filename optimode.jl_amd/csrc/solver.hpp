@@ -229,6 +229,9 @@ class Solver {
   PrecisionData<float> p32;
   PrecisionData<double> p64;
   Options opt;
+  // set only during solve_k_learned: the start vectors' lowest plane waves already on the
+  // device (used in place of Hguess when guess_lowres > 0)
+  const c64* guess_dev = nullptr;
   std::vector<double> epsinv_host;  // (3,3,Nx,Ny) column-major copy
   DevMem last_evecs;                // FP64, nev columns of ld complex
   int last_nev = 0;

@@ -312,6 +312,89 @@ __global__ __launch_bounds__(TB) void k_gather(const float* src, const int* idx,
   }
 }
 
+// x[(c hw + bx) hw + by] = normalised mean of eps6 component c over pixel block (bx, by)
+// (fx x fy pixels), c < 6; channel 6 = normalised omega
+__global__ __launch_bounds__(TB) void k_ws_features(const double* eps6, int nx, int ny, int hw, double omega,
+                                                    float* x) {
+  const int fx = nx / hw, fy = ny / hw;
+  const long n = 7L * hw * hw;
+  for (long e = long(blockIdx.x) * TB + threadIdx.x; e < n; e += long(gridDim.x) * TB) {
+    const int c = int(e / (long(hw) * hw));
+    const int q = int(e - long(c) * hw * hw);
+    const int bx = q / hw, by = q - (q / hw) * hw;
+    double v;
+    if (c == 6) {
+      v = (omega - WS_OMEGA0) / WS_OMEGA_SCALE;
+    } else {
+      double sum = 0.0;
+      for (int iy = by * fy; iy < (by + 1) * fy; ++iy)
+        for (int ix = bx * fx; ix < (bx + 1) * fx; ++ix) sum += eps6[6 * (long(ix) + long(nx) * iy) + c];
+      const double m = sum / double(fx * fy);
+      v = c < 3 ? (m - WS_DIAG0) / WS_DIAG_SCALE : m / WS_OFF_SCALE;
+    }
+    x[e] = float(v);
+  }
+}
+
+// inverse DFT along the second map axis: t[(jc hw + x) hw + gy] = sum_y u_jc[x][y] e^{+2 pi i gy y / hw}
+// with u_jc = y[4j + 2c] + i y[4j + 2c + 1] (jc = 2j + c)
+__global__ __launch_bounds__(TB) void k_ws_idft1(const float* yf, int nev, int hw, c64* t) {
+  const long hh = long(hw) * hw, n = 2L * nev * hh;
+  for (long e = long(blockIdx.x) * TB + threadIdx.x; e < n; e += long(gridDim.x) * TB) {
+    const int jc = int(e / hh);
+    const int q = int(e - long(jc) * hh);
+    const int x = q / hw, gy = q - (q / hw) * hw;
+    const float* re = yf + long(2 * jc) * hh + long(x) * hw;
+    const float* im = re + hh;
+    double sr = 0.0, si = 0.0;
+    for (int y = 0; y < hw; ++y) {
+      double sn, cs;
+      sincospi(2.0 * double((gy * y) % hw) / double(hw), &sn, &cs);
+      sr += re[y] * cs - im[y] * sn;
+      si += re[y] * sn + im[y] * cs;
+    }
+    t[e] = c64{sr, si};
+  }
+}
+
+// inverse DFT along the first axis with the 1/hw^2 of ifft2: band j's vector entry
+// c + 2(gx + hw gy) = sum_x t_jc[x][gy] e^{+2 pi i gx x / hw} / hw^2
+__global__ __launch_bounds__(TB) void k_ws_idft2(const c64* t, int nev, int hw, c64* out) {
+  const long hh = long(hw) * hw, n = 2L * nev * hh;
+  const double sc = 1.0 / double(hh);
+  for (long e = long(blockIdx.x) * TB + threadIdx.x; e < n; e += long(gridDim.x) * TB) {
+    const int jc = int(e / hh);
+    const int q = int(e - long(jc) * hh);
+    const int gx = q / hw, gy = q - (q / hw) * hw;
+    const c64* tc = t + long(jc) * hh + gy;
+    double sr = 0.0, si = 0.0;
+    for (int x = 0; x < hw; ++x) {
+      double sn, cs;
+      sincospi(2.0 * double((gx * x) % hw) / double(hw), &sn, &cs);
+      const c64 v = tc[long(x) * hw];
+      sr += v.x * cs - v.y * sn;
+      si += v.x * sn + v.y * cs;
+    }
+    const int j = jc / 2, c = jc - 2 * (jc / 2);
+    out[long(j) * 2 * hh + c + 2 * (long(gx) + long(hw) * gy)] = c64{sr * sc, si * sc};
+  }
+}
+
+// unit norm per band (one block per band, fixed-order reduction)
+__global__ __launch_bounds__(TB) void k_ws_normalise(c64* v, long per) {
+  c64* b = v + long(blockIdx.x) * per;
+  double s = 0.0;
+  for (long e = threadIdx.x; e < per; e += TB) s += b[e].x * b[e].x + b[e].y * b[e].y;
+  __shared__ double red[TB / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < TB / 64; ++w) t += red[w];
+  const double f = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
+  for (long e = threadIdx.x; e < per; e += TB) b[e] = c64{b[e].x * f, b[e].y * f};
+}
+
 unsigned grid_for(long n) { return unsigned(std::max<long>(1, std::min<long>((n + TB - 1) / TB, 4096))); }
 
 }  // namespace
@@ -532,6 +615,51 @@ double WarmNet::loss_grad(int B, const float* x, const float* tf, const float* t
   if (grad) OMX_HIP_CHECK(hipMemcpyAsync(grad, G_.as<void>(), sizeof(float) * nparams_, hipMemcpyDeviceToHost, s_));
   OMX_HIP_CHECK(hipStreamSynchronize(s_));
   return loss;
+}
+
+const c64* WarmNet::guess_device(const double* eps6, int nx, int ny, double omega, float* k_host) {
+  if (nx % hw_ != 0 || ny % hw_ != 0)
+    throw Error(OMX_ERR_BAD_SHAPE, "the grid is not a multiple of the warm-start map size");
+  OMX_HIP_CHECK(hipSetDevice(device_));
+  ensure_batch(1);
+  const long n7 = 7L * hw_ * hw_, per = 2L * hw_ * hw_, nt = long(nev_) * per;
+  hipLaunchKernelGGL(k_ws_features, dim3(grid_for(n7)), dim3(TB), 0, s_, eps6, nx, ny, hw_, omega, x_.as<float>());
+  fwd(1);
+  dft_.ensure(sizeof(c64) * size_t(nt));
+  low_.ensure(sizeof(c64) * size_t(nt));
+  c64* lowres = low_.as<c64>();
+  hipLaunchKernelGGL(k_ws_idft1, dim3(grid_for(nt)), dim3(TB), 0, s_, y_.as<float>(), nev_, hw_, dft_.as<c64>());
+  hipLaunchKernelGGL(k_ws_idft2, dim3(grid_for(nt)), dim3(TB), 0, s_, dft_.as<c64>(), nev_, hw_, lowres);
+  hipLaunchKernelGGL(k_ws_normalise, dim3(nev_), dim3(TB), 0, s_, lowres, per);
+  OMX_HIP_CHECK(hipGetLastError());
+  OMX_HIP_CHECK(hipMemcpyAsync(k_host, k_.as<void>(), sizeof(float) * nev_, hipMemcpyDeviceToHost, s_));
+  OMX_HIP_CHECK(hipStreamSynchronize(s_));
+  return lowres;
+}
+
+void solve_k_learned(Solver& S, WarmNet& net, double omega, int nev, double k0_scale, double k_tol, double eig_tol,
+                     int max_eigsolves, int maxiter, double* kmags, double* evecs, omx_info* info) {
+  S.require_2d("the learned start");
+  if (net.device() != S.device) throw Error(OMX_ERR_INVALID_ARG, "the network and the solver are on different devices");
+  if (nev < 1 || nev > net.nev()) throw Error(OMX_ERR_INVALID_ARG, "nev must be in 1..(the network's band count)");
+  if (!(k0_scale > 0) || !std::isfinite(k0_scale) || !(omega > 0) || !std::isfinite(omega))
+    throw Error(OMX_ERR_INVALID_ARG, "omega and k0_scale must be positive and finite");
+  std::vector<float> kh(net.nev());
+  const c64* low = net.guess_device(S.p64.plan.eps6, S.nx, S.ny, omega, kh.data());
+  const double k0 = std::max(1e-3, k0_scale * omega * (WS_N_SHIFT + WS_N_SCALE * double(kh[0])));
+  struct Restore {
+    Solver& s;
+    int hw;
+    ~Restore() {
+      s.opt.guess_lowres = hw;
+      s.guess_dev = nullptr;
+    }
+  } restore{S, S.opt.guess_lowres};
+  S.opt.guess_lowres = net.hw();
+  S.guess_dev = low;
+  // Hguess is only tested for presence: the vectors come from guess_dev
+  S.solve_k(omega, nev, k_tol, eig_tol, max_eigsolves, maxiter, &k0, reinterpret_cast<const double*>(low), nev, kmags,
+            evecs, info);
 }
 
 void WarmNet::train(int n, const float* x, const float* tf, const float* tk, float wk, int epochs, int batch,

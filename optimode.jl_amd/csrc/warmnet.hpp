@@ -25,6 +25,14 @@
 
 namespace omx {
 
+// Input and output convention of the warm-start network (the shipped network was trained
+// with it): input channels (eps^-1 block mean - WS_DIAG0) / WS_DIAG_SCALE for xx, yy, zz,
+// block mean / WS_OFF_SCALE for xy, xz, yz, (omega - WS_OMEGA0) / WS_OMEGA_SCALE; effective
+// index of band j = WS_N_SHIFT + WS_N_SCALE * k_j.
+constexpr double WS_DIAG0 = 0.55, WS_DIAG_SCALE = 0.25, WS_OFF_SCALE = 0.05;
+constexpr double WS_OMEGA0 = 0.65, WS_OMEGA_SCALE = 0.05;
+constexpr double WS_N_SHIFT = 1.8, WS_N_SCALE = 0.2;
+
 class WarmNet {
  public:
   static constexpr int NIN = 7;
@@ -45,6 +53,13 @@ class WarmNet {
   // hist[e] = mean minibatch loss of epoch e
   void train(int n, const float* x, const float* tf, const float* tk, float wk, int epochs, int batch, float lr,
              uint64_t seed, double* hist);
+  int device() const { return device_; }
+  // The guess for one 2D cell on the device: features from eps6 (FP64 per pixel xx yy zz xy
+  // xz yz, pixel ix + nx iy), one forward, the band outputs to k_host (nev floats), and the
+  // start vectors' hw x hw lowest plane waves (inverse 2D DFT of the field maps, unit norm per
+  // band; entry c + 2(gx + hw gy) of band j at j 2 hw^2), held in the network's buffer whose
+  // device address is returned (valid until the next call).
+  const c64* guess_device(const double* eps6, int nx, int ny, double omega, float* k_host);
 
  private:
   struct Conv {
@@ -60,6 +75,7 @@ class WarmNet {
   int B_ = 0;               // batch capacity of the activation buffers
   DevMem x_, a1_, p1_, a2_, p2_, a3_, g_, k_, cat4_, a4_, cat5_, a5_, y_;
   DevMem dy_, da5_, dcat5_, da4_, dcat4_, da3_, dp2_, da2_, dp1_, da1_, dk_, dg_, part_, lossd_;
+  DevMem dft_, low_;  // inverse-DFT intermediate and result of guess_device
   long adam_t_ = 0;
 
   void ensure_batch(int B);
@@ -70,5 +86,12 @@ class WarmNet {
   void conv(const Conv& c, const float* in, float* out, int B, bool relu);
   void conv_bwd(const Conv& c, const float* in, const float* dout, float* din, int B);
 };
+
+// solve_k from the network's guess, all of it on the device but nev floats: the shared first
+// eigensolve starts at k0_scale times the predicted band-1 k from the predicted block (its
+// hw x hw lowest plane waves scattered into the LOBPCG block); stopping rules and results
+// are those of Solver::solve_k.
+void solve_k_learned(Solver& S, WarmNet& net, double omega, int nev, double k0_scale, double k_tol, double eig_tol,
+                     int max_eigsolves, int maxiter, double* kmags, double* evecs, omx_info* info);
 
 }  // namespace omx

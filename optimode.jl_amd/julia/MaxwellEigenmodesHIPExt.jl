@@ -12,7 +12,6 @@
 module MaxwellEigenmodesHIPExt
 
 using LinearAlgebra
-using FFTW: ifft
 using Logging: AbstractLogger, NullLogger
 using ChainRulesCore
 using ChainRulesCore: NoTangent, AbstractZero
@@ -452,57 +451,25 @@ function hip_warmnet_train!(net::HIPWarmNet, x, tf, tk; wk::Real=1.0, epochs::In
     return hist
 end
 
-# network input of one (ε⁻¹, ω): block means of ε⁻¹ xx yy zz xy xz yz, then ω
-function _ws_features(ε⁻¹::AbstractArray{<:Real,4}, grid::Grid{2}, ω::Real, hw::Int)
-    fx, fy = grid.Nx ÷ hw, grid.Ny ÷ hw
-    (fx * hw == grid.Nx && fy * hw == grid.Ny) || error("grid is not a multiple of the $hw map")
-    X = Array{Float32}(undef, hw, hw, 7, 1)
-    comps = ((1, 1), (2, 2), (3, 3), (1, 2), (1, 3), (2, 3))
-    for (c, (a, b)) in enumerate(comps), ix in 1:hw, iy in 1:hw
-        m = sum(@view ε⁻¹[a, b, (ix-1)*fx+1:ix*fx, (iy-1)*fy+1:iy*fy]) / (fx * fy)
-        X[iy, ix, c, 1] = c <= 3 ? (m - WS_DIAG[1]) / WS_DIAG[2] : m / WS_OFF
-    end
-    X[:, :, 7, 1] .= (ω - WS_OMEGA[1]) / WS_OMEGA[2]
-    return X
-end
-
 """
     hip_solve_k_learned(ω, ε⁻¹, grid, solver, net; nev=net.nev, k0_scale=1.03, kwargs...)
 
-`solve_k` whose shared first eigensolve starts at `k0_scale` times the network's band-1 k
-from the network's start block (given by its hw x hw lowest plane waves and scattered on
-the GPU); the stopping rules and the converged result are those of `solve_k`.
+`solve_k` from the network's guess, built on the GPU from the solver's ε⁻¹ (features, one
+forward, inverse DFT into the start block's lowest plane waves; `omx_solve_k_learned`): the
+shared first eigensolve starts at `k0_scale` times the predicted band-1 k; the stopping rules
+and the converged result are those of `solve_k`.
 """
 function hip_solve_k_learned(ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64}, solver::HIPSolver,
                              net::HIPWarmNet; nev::Integer=net.nev, k0_scale::Real=1.03, max_eigsolves=60,
                              maxiter=100, k_tol=1e-8, eig_tol=1e-8)
-    nev <= net.nev || error("the network predicts $(net.nev) bands")
-    hw = net.hw
-    Y, K = hip_warmnet_forward(net, _ws_features(ε⁻¹, grid, ω, hw))
-    k0 = max(k0_scale * ω * (WS_N[1] + WS_N[2] * K[1, 1]), 1e-3)
-    L = Array{ComplexF64}(undef, 2, hw, hw, nev)       # entry c + 2(gx + hw gy) of each vector
-    for j in 1:nev
-        for c in 1:2
-            u = [complex(Float64(Y[iy, ix, 4(j-1)+2c-1, 1]), Float64(Y[iy, ix, 4(j-1)+2c, 1])) for ix in 1:hw, iy in 1:hw]
-            L[c, :, :, j] .= ifft(u)
-        end
-        v = @view L[:, :, :, j]
-        v ./= max(norm(v), eps())
-    end
     h = _handle!(solver, grid, ε⁻¹)
     kmags = Vector{Float64}(undef, nev)
     evecs = Matrix{ComplexF64}(undef, 2 * N(grid), nev)
     info = Ref(OmxInfo())
-    kg = Float64[k0]
-    _check(ccall((:omx_set_option, libomx), Cint, (Ptr{Cvoid}, Cstring, Cdouble), h, "guess_lowres", hw), h)
-    st = try
-        ccall((:omx_solve_k, libomx), Cint,
-              (Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble, Cint, Cint, Ptr{Cdouble}, Ptr{ComplexF64}, Cint,
-               Ptr{Cdouble}, Ptr{ComplexF64}, Ref{OmxInfo}),
-              h, ω, nev, k_tol, eig_tol, max_eigsolves, maxiter, kg, L, nev, kmags, evecs, info)
-    finally
-        ccall((:omx_set_option, libomx), Cint, (Ptr{Cvoid}, Cstring, Cdouble), h, "guess_lowres", 0.0)
-    end
+    st = ccall((:omx_solve_k_learned, libomx), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble, Cdouble, Cint, Cint, Ptr{Cdouble},
+                Ptr{ComplexF64}, Ref{OmxInfo}),
+               h, net.handle, ω, nev, k0_scale, k_tol, eig_tol, max_eigsolves, maxiter, kmags, evecs, info)
     _check(st, h)
     return kmags, [evecs[:, j] for j in 1:nev]
 end

@@ -213,8 +213,18 @@ class LearnedStart:
         return cls(net)
 
 
+def device_guess(start, epsinv, grid, omega, solver):
+    """The guess as ``omx_solve_k_learned`` builds it on the GPU from the handle's ε⁻¹:
+    ``(k outputs (nev,), low-plane-wave vectors (nev, 2 hw^2))``."""
+    h = solver.handle(grid, epsinv)
+    k = np.empty(start.nev, dtype=np.float32)
+    low = np.empty((start.nev, 2 * start.hw * start.hw), dtype=np.complex128)
+    _lib.check(_lib.lib().omx_warmnet_guess(start.net.h, h.h, float(omega), _lib.fptr(k), _lib.dptr(low)), h.h)
+    return k, low
+
+
 def solve_k_learned(omega, epsinv, grid, solver, start, nev=None, mode="seed", k0_scale=1.03, return_info=False,
-                    **kw):
+                    on_device=True, **kw):
     """``solve_k`` started from the network's guesses; the stopping rules and the converged
     result are those of ``solve_k``.  ``mode="seed"``: the shared first eigensolve starts at
     ``k0_scale`` times the predicted k of band 1 from the predicted block (instead of k_guess
@@ -227,6 +237,22 @@ def solve_k_learned(omega, epsinv, grid, solver, start, nev=None, mode="seed", k
         raise OmxError(_lib.OMX_ERR_INVALID_ARG, f"the network predicts {start.nev} bands, {nev} requested")
     if mode not in ("seed", "band"):
         raise OmxError(_lib.OMX_ERR_INVALID_ARG, "mode must be 'seed' or 'band'")
+    if mode == "seed" and on_device:
+        # features, forward and start block on the GPU (omx_solve_k_learned)
+        from . import OmxInfo
+        h = solver.handle(grid, epsinv)
+        km = np.empty(nev)
+        fetch = kw.pop("fetch_evecs", True)
+        ev = np.empty((nev, 2 * grid.N), dtype=np.complex128) if fetch else None
+        info = OmxInfo()
+        _lib.check(_lib.lib().omx_solve_k_learned(
+            h.h, start.net.h, float(omega), nev, float(k0_scale), float(kw.pop("k_tol", 1e-8)),
+            float(kw.pop("eig_tol", 1e-8)), int(kw.pop("max_eigsolves", 60)), int(kw.pop("maxiter", 100)),
+            _lib.dptr(km), _lib.dptr(ev), ctypes.byref(info)), h.h)
+        if kw:
+            raise OmxError(_lib.OMX_ERR_INVALID_ARG, f"unexpected arguments {sorted(kw)}")
+        evecs = [ev[j].copy() for j in range(nev)] if fetch else None
+        return (km, evecs, info.as_dict()) if return_info else (km, evecs)
     kg, H = start.guess(epsinv, grid, omega, lowres=True)
     kg = np.maximum(kg[:nev], 1e-3)
     if mode == "band":

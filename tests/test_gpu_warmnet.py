@@ -94,3 +94,37 @@ def test_lowres_guess_equals_full_vector_guess():
         kb, _, ib = omx.solve_k(prm["omega"], ei, grid, solver, kguess=kg, Hguess=low, band_guess=band, guess_hw=16,
                                 **kw)
         assert np.array_equal(ka, kb) and ia["lobpcg_iters_fp32"] == ib["lobpcg_iters_fp32"]
+
+
+def test_device_guess_matches_host_guess():
+    # features, forward and inverse 64^2 DFT on the GPU (omx_warmnet_guess) against the host
+    # maps of omx.warmstart (features, forward, numpy ifft2)
+    grid = omx.Grid(8.0, 4.0, 256, 256)
+    st = WS.LearnedStart.load(WS.DATA_DIR + "/warmnet_tfln256.npz")
+    solver = omx.HIPSolver()
+    for seed in (30001, 30004):
+        prm = WS.random_rib(np.random.default_rng(seed))
+        ei = WS.rib_epsinv(grid, prm)
+        kd, lowd = WS.device_guess(st, ei, grid, prm["omega"], solver)
+        y, kh = st.net.forward(WS.features(ei, grid, prm["omega"], st.hw)[None])
+        lowh = WS.lowres_vectors(y[0], st.hw)
+        assert np.max(np.abs(kd - kh[0])) < 1e-5
+        assert np.max(np.abs(lowd - lowh)) < 1e-5 * np.max(np.abs(lowh))
+        assert np.allclose(np.linalg.norm(lowd, axis=1), 1.0, atol=1e-12)
+
+
+def test_solve_k_learned_on_device_matches_host_path():
+    grid = omx.Grid(8.0, 4.0, 256, 256)
+    st = WS.LearnedStart.load(WS.DATA_DIR + "/warmnet_tfln256.npz")
+    solver = omx.HIPSolver()
+    prm = WS.random_rib(np.random.default_rng(30006))
+    ei = WS.rib_epsinv(grid, prm)
+    kw = dict(nev=8, k_tol=1e-10, eig_tol=1e-8, return_info=True)
+    kd, evd, idv = WS.solve_k_learned(prm["omega"], ei, grid, solver, st, on_device=True, **kw)
+    kh, evh, ihv = WS.solve_k_learned(prm["omega"], ei, grid, solver, st, on_device=False, **kw)
+    assert idv["converged"] and ihv["converged"]
+    assert np.max(np.abs(kd - kh) / kh) < 1e-9
+    for a, b in zip(evd, evh):
+        assert abs(np.vdot(a, b)) > 1 - 1e-6
+    with pytest.raises(omx.OmxError):  # more bands than the network predicts
+        WS.solve_k_learned(prm["omega"], ei, grid, solver, st, nev=9, on_device=True)
