@@ -346,12 +346,13 @@ template <typename T> class Batch {
   Batch& operator=(const Batch&) = delete;
 
   // Row partitioning of the Gram products and block updates for `na` instances with work
-  // (an upper bound is enough: the kernels skip instances without columns): about 512
-  // Gram blocks and 1024 update blocks in total, whatever the number of instances left.
+  // (an upper bound is enough: the kernels skip instances without columns): about
+  // gram_blocks Gram blocks and combine_blocks update blocks in total (1024 and 2048 by
+  // default), whatever the number of instances left.
   void set_active(int na) {
     na = std::max(1, std::min(na, ninst));
-    nchunk = std::max(1, std::min(nchunk_max, 512 / na));
-    nblk = std::max(1, std::min(nblk_max, 1024 / na));
+    nchunk = std::max(1, std::min(nchunk_max, S.opt.gram_blocks / na));
+    nblk = std::max(1, std::min(nblk_max, S.opt.combine_blocks / na));
     dl.nblk = nblk;
   }
 
@@ -689,6 +690,10 @@ void Solver::set_option(const std::string& key, double v) {
     opt.graphs = v != 0 ? 1 : 0;
   else if (key == "jac_stop32")
     opt.jac_stop32 = v;
+  else if (key == "gram_blocks")
+    opt.gram_blocks = std::max(1, int(v));
+  else if (key == "combine_blocks")
+    opt.combine_blocks = std::max(1, int(v));
   else if (key == "reg_ops" || key == "slab3d") {
     (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
     select_operator(*this, p32);
