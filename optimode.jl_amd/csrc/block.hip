@@ -431,8 +431,14 @@ __global__ __launch_bounds__(BLK) void k_convert(const cplx<TS>* src, long lds, 
 
 // Copy columns [off_i, off_i + cnt_i) of instance i from src to dst (same positions).
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_copy_cols(const cplx<T>* src, cplx<T>* dst, long ld, long istride,
-                                                   const int* off, const int* cnt, long K) {
+__global__ __launch_bounds__(BLK) void k_copy_cols(const cplx<T>* src, cplx<T>* dst, const cplx<T>* src2,
+                                                   cplx<T>* dst2, long ld, long istride, const int* off,
+                                                   const int* cnt, long K) {
+  // blockIdx.z selects the pair: (src, dst) or (src2, dst2)
+  if (blockIdx.z == 1) {
+    src = src2;
+    dst = dst2;
+  }
   const int inst = blockIdx.y;
   const int c0 = off ? off[inst] : 0;
   const int nc = cnt[inst];
@@ -540,14 +546,16 @@ void launch_convert(const cplx<TS>* src, long lds, long istrs, cplx<TD>* dst, lo
 
 template <typename T>
 void launch_copy_cols(const cplx<T>* src, cplx<T>* dst, long ld, long istride, int ninst, const int* off,
-                      const int* cnt, long K, hipStream_t stream) {
-  dim3 grid(128, ninst);
-  hipLaunchKernelGGL((k_copy_cols<T>), grid, dim3(BLK), 0, stream, src, dst, ld, istride, off, cnt, K);
+                      const int* cnt, long K, hipStream_t stream, const cplx<T>* src2, cplx<T>* dst2) {
+  dim3 grid(128, ninst, src2 ? 2 : 1);
+  hipLaunchKernelGGL((k_copy_cols<T>), grid, dim3(BLK), 0, stream, src, dst, src2, dst2, ld, istride, off, cnt, K);
   OMX_HIP_CHECK(hipGetLastError());
 }
 
-template void launch_copy_cols<float>(const cplx<float>*, cplx<float>*, long, long, int, const int*, const int*, long, hipStream_t);
-template void launch_copy_cols<double>(const cplx<double>*, cplx<double>*, long, long, int, const int*, const int*, long, hipStream_t);
+template void launch_copy_cols<float>(const cplx<float>*, cplx<float>*, long, long, int, const int*, const int*, long,
+                                      hipStream_t, const cplx<float>*, cplx<float>*);
+template void launch_copy_cols<double>(const cplx<double>*, cplx<double>*, long, long, int, const int*, const int*, long,
+                                       hipStream_t, const cplx<double>*, cplx<double>*);
 template void launch_gram<float>(const ColList<float>&, const ColList<float>&, int, long, int, int, c64*, hipStream_t, int, int,
                                  bool, bool);
 template void launch_gram<double>(const ColList<double>&, const ColList<double>&, int, long, int, int, c64*, hipStream_t,

@@ -87,9 +87,11 @@ template <typename TS, typename TD>
 void launch_convert(const cplx<TS>* src, long lds, long istrs, cplx<TD>* dst, long ldd, long istrd, int ninst,
                     int ncol, long K, hipStream_t stream);
 
-// Copy columns [off_i, off_i + cnt_i) of every instance from src to dst.
+// Copy columns [off_i, off_i + cnt_i) of every instance from src to dst, and with the same
+// columns from src2 to dst2 in the same launch when src2 is given.
 template <typename T>
 void launch_copy_cols(const cplx<T>* src, cplx<T>* dst, long ld, long istride, int ninst, const int* off,
-                      const int* cnt, long K, hipStream_t stream);
+                      const int* cnt, long K, hipStream_t stream, const cplx<T>* src2 = nullptr,
+                      cplx<T>* dst2 = nullptr);
 
 }  // namespace omx

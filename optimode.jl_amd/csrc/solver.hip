@@ -487,8 +487,7 @@ template <typename T> class Batch {
       if (it > 0)
         launch_combine<T>(CMB_TWO, Y, AY, coef(Cp, f(F_NNEW)), qmax, out(P[n], nullptr, 0), out(AP[n], nullptr, 0),
                           out(AP[n], nullptr, 0), ninst, K, nblk, nullptr, sm);
-      launch_copy_cols<T>(p(X[n]), p(X[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm);
-      launch_copy_cols<T>(p(AX[n]), p(AX[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm);
+      launch_copy_cols<T>(p(X[n]), p(X[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm, p(AX[n]), p(AX[c]));
       ortho_r(X[n], P[n], R[n]);
       launch_advance(dl, st.as<LState>(), shape.as<int>(), sm);
       cur = n;
