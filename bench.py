@@ -44,7 +44,7 @@ def cpu_baseline(seconds_cap=60.0):
         ncores = os.cpu_count() or 1
     ncores = max(1, min(16, ncores))
     os.environ["ORACLE_FFT_WORKERS"] = str(ncores)
-    import oracle as O  # the checker: timed here as the CPU baseline only
+    import oracle as O  # the oracle, timed here as the CPU baseline only
     from omx import geometry as G
     g = O.Grid(8.0, 4.0, 256, 256)
     ei = G.tfln_rib(g)
@@ -140,7 +140,7 @@ PER_NEV = 4
 def periodic_cpu_baseline(ncores):
     """The FP64 oracle on the host: the fundamental Bloch mode of the same 3D cell."""
     os.environ["ORACLE_FFT_WORKERS"] = str(ncores)
-    import oracle as O  # the checker: timed here as the CPU baseline only
+    import oracle as O  # the oracle, timed here as the CPU baseline only
     from oracle import periodic as P
     from omx import geometry as G
     g = P.Grid3(*PER_GRID)
@@ -321,7 +321,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every solve here fits one MI355X and "
-              "multi-GPU sharding is by independent instances, which this build does not launch.", file=sys.stderr)
+              "multi-GPU sharding is by independent instances, which this library does not launch.", file=sys.stderr)
         sys.exit(2)
     if a.workload == "eme":
         eme_workload(a)

@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  This package is a NumPy restatement of the reference
 (OptiMode.jl, ``lib/MaxwellEigenmodes``) algorithms that the HIP library replaces.  It
-is the checker: only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
-leg of ``bench.py`` may import it.  The product path (``optimode.jl_amd``) never
+is the reference point of the tests: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it.  The product path (``optimode.jl_amd``) never
 imports, calls or falls back to anything in here.
 
 Every function cites the reference ``file:line`` it restates.  How each one is pinned

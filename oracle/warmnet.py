@@ -1,5 +1,5 @@
 """CPU restatement (NumPy, FP64) of the learned warm-start network: test infrastructure, the
-checker of the GPU forward/backward kernels and nothing else.
+reference point of the GPU forward/backward kernels in the tests and nothing else.
 
 The network has no counterpart in the reference (OptiMode starts every eigensolve from random
 vectors and ``k_guess``, ``src/MaxwellEigenmodes.jl:63``), so nothing in the reference pins
