@@ -100,6 +100,67 @@ __global__ __launch_bounds__(AB) void k_lincomb(Lin<T> L, long K, const int* act
   }
 }
 
+// out_i = sum_j c_j in_j_i as k_lincomb, plus the partial sums of Re <d_i, out_i> over this
+// block's rows into part[i gridDim.x + blockIdx.x] (the layout of k_vdot)
+template <typename T, int NIN>
+__global__ __launch_bounds__(AB) void k_lincomb_dot(Lin<T> L, long K, const int* act, const cplx<T>* D,
+                                                    double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double acc = 0;
+  if (act[i]) {
+    c64 cs[NIN];
+    const cplx<T>* in[NIN];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) {
+      cs[j] = L.coef[4 * i + j];
+      in[j] = L.in[j] + i * L.ld;
+    }
+    cplx<T>* out = L.out + i * L.ld;
+    const cplx<T>* d = D + i * L.ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      cplx<T> v[NIN];
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) v[j] = in[j][r];
+      double sx = 0, sy = 0;
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) {
+        sx += cs[j].x * double(v[j].x) - cs[j].y * double(v[j].y);
+        sy += cs[j].x * double(v[j].y) + cs[j].y * double(v[j].x);
+      }
+      const cplx<T> o{T(sx), T(sy)};
+      out[r] = o;
+      acc += double(d[r].x) * double(o.x) + double(d[r].y) * double(o.y);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// MINRES direction and solution updates in one pass: w_new = c0 v + c1 w1 + c2 w2 (into
+// wout) and x += c3 w_new, with real coefficients c = coef[4 i ..]
+template <typename T>
+__global__ __launch_bounds__(AB) void k_minres_wx(const cplx<T>* v, const cplx<T>* w1, const cplx<T>* w2,
+                                                  cplx<T>* wout, cplx<T>* x, long ld, long K, const c64* coef,
+                                                  const int* act) {
+  const int i = blockIdx.y;
+  if (!act[i]) return;
+  const double c0 = coef[4 * i].x, c1 = coef[4 * i + 1].x, c2 = coef[4 * i + 2].x, c3 = coef[4 * i + 3].x;
+  const long o = i * ld;
+  for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+    const cplx<T> a = v[o + r], b = w1[o + r], c = w2[o + r], xr = x[o + r];
+    const double wx = c0 * double(a.x) + c1 * double(b.x) + c2 * double(c.x);
+    const double wy = c0 * double(a.y) + c1 * double(b.y) + c2 * double(c.y);
+    wout[o + r] = cplx<T>{T(wx), T(wy)};
+    x[o + r] = cplx<T>{T(double(xr.x) + c3 * wx), T(double(xr.y) + c3 * wy)};
+  }
+}
+
 struct MinresState {
   double beta1, beta, oldb, dbar, epsln, phibar, cs, sn, alfa, phi, delta, oldeps, denom, rtol, shift;
   int itn, done, active, pad;
@@ -110,6 +171,9 @@ struct MinresState {
 // step of the iteration they follow.  Coefficients for the next lincomb go to coef.
 // one block per instance: the block sums the instance's partials (tree), then its first
 // thread runs the recurrence
+// Coefficient slots (4 complex per instance each, slot s at coef + 4 ninst s): 0 for the
+// y updates of stages 1 and 2, 1 for the fused w/x update of stage 3, 2 for the next
+// v = y / beta (stages 0 and 3); slot 3 belongs to the projections.
 __global__ __launch_bounds__(AB) void k_minres_scalars(MinresState* ms, const double* part, int nb, c64* coef,
                                                        int stage, int ninst, const int* act) {
   const int i = blockIdx.x;
@@ -124,8 +188,11 @@ __global__ __launch_bounds__(AB) void k_minres_scalars(MinresState* ms, const do
   }
   if (threadIdx.x == 0) {
     MinresState& m = ms[i];
-    c64* c = coef + 4 * i;
+    c64* c = coef + 4 * i + (stage == 0 ? 8 * ninst : (stage == 3 ? 4 * ninst : 0));
+    c64* cv = coef + 4 * i + 8 * ninst;  // slot 2: next v
     for (int j = 0; j < 4; ++j) c[j] = c64{0, 0};
+    if (stage == 3)
+      for (int j = 0; j < 4; ++j) cv[j] = c64{0, 0};
     if (!act[i]) return;
     const double d = red[0];
     if (stage == 0) {
@@ -169,18 +236,14 @@ __global__ __launch_bounds__(AB) void k_minres_scalars(MinresState* ms, const do
       m.phi = m.cs * m.phibar;
       m.phibar = m.sn * m.phibar;
       m.denom = 1.0 / gamma;
-      // w_new = (v - oldeps w1 - delta w2) denom ; inputs (v, w1, w2)
+      // w_new = (v - oldeps w1 - delta w2) denom and x += phi w_new ; inputs (v, w1, w2)
       c[0] = c64{m.denom, 0};
       c[1] = c64{-m.oldeps * m.denom, 0};
       c[2] = c64{-m.delta * m.denom, 0};
-      if (m.phibar <= m.rtol * m.beta1 || m.beta == 0) m.done = 1;
-    } else if (stage == 4) {
-      // x += phi w ; inputs (x, w)
-      c[0] = c64{1, 0};
-      c[1] = c64{m.phi, 0};
-    } else if (stage == 5) {
+      c[3] = c64{m.phi, 0};
       // next v = y / beta
-      c[0] = c64{m.beta > 0 ? 1.0 / m.beta : 0.0, 0};
+      cv[0] = c64{m.beta > 0 ? 1.0 / m.beta : 0.0, 0};
+      if (m.phibar <= m.rtol * m.beta1 || m.beta == 0) m.done = 1;
     }
   }
 }
@@ -226,20 +289,38 @@ VecOpsT<T>::VecOpsT(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const
     : s(s_), nb(nb_), ld(ld_), K(K_), nblk(int(std::min<long>(256, (K_ + AB - 1) / AB))), EV(EV_),
       part(S.vec_part), coef(S.vec_coef) {
   part.ensure(sizeof(c64) * size_t(nb_) * 256);
-  coef.ensure(sizeof(c64) * 4 * size_t(nb_));
+  coef.ensure(sizeof(c64) * 4 * 4 * size_t(nb_));  // four coefficient slots
+}
+
+template <typename T>
+void VecOpsT<T>::lincomb_dot(std::initializer_list<const C*> ins, C* outp, const int* act, const C* d, int slot) {
+  Lin<T> L{};
+  int j = 0;
+  for (auto p : ins) L.in[j++] = p;
+  L.nin = j;
+  L.out = outp;
+  L.coef = coef.template as<c64>() + 4 * size_t(nb) * slot;
+  L.ld = ld;
+  const dim3 g(nblk, nb);
+  switch (L.nin) {
+    case 2: hipLaunchKernelGGL((k_lincomb_dot<T, 2>), g, dim3(AB), 0, s, L, K, act, d, part.template as<double>()); break;
+    case 3: hipLaunchKernelGGL((k_lincomb_dot<T, 3>), g, dim3(AB), 0, s, L, K, act, d, part.template as<double>()); break;
+    default: throw Error(OMX_ERR_INTERNAL, "lincomb_dot takes 2 or 3 inputs");
+  }
 }
 
 template <typename T> void VecOpsT<T>::vdot(const C* A, const C* B, const int* act) {
   hipLaunchKernelGGL(k_vdot<T>, dim3(nblk, nb), dim3(AB), 0, s, A, B, ld, K, act, part.as<double>());
 }
 
-template <typename T> void VecOpsT<T>::lincomb(std::initializer_list<const C*> ins, C* outp, const int* act) {
+template <typename T>
+void VecOpsT<T>::lincomb(std::initializer_list<const C*> ins, C* outp, const int* act, int slot) {
   Lin<T> L{};
   int j = 0;
   for (auto p : ins) L.in[j++] = p;
   L.nin = j;
   L.out = outp;
-  L.coef = coef.as<c64>();
+  L.coef = coef.template as<c64>() + 4 * size_t(nb) * slot;
   L.ld = ld;
   const dim3 g(nblk, nb);
   switch (L.nin) {
@@ -250,10 +331,14 @@ template <typename T> void VecOpsT<T>::lincomb(std::initializer_list<const C*> i
   }
 }
 
-template <typename T> void VecOpsT<T>::project(C* x, const int* act) {
-  hipLaunchKernelGGL(k_cdot<T>, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.as<c64>());
-  hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(AB), 0, s, part.as<c64>(), nblk, coef.as<c64>());
-  lincomb({x, EV}, x, act);
+template <typename T> void VecOpsT<T>::project(C* x, const int* act, const C* dot_with) {
+  hipLaunchKernelGGL(k_cdot<T>, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.template as<c64>());
+  hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(AB), 0, s, part.template as<c64>(), nblk,
+                     coef.template as<c64>() + 4 * size_t(nb) * 3);
+  if (dot_with)
+    lincomb_dot({x, EV}, x, act, dot_with, 3);
+  else
+    lincomb({x, EV}, x, act, 3);
 }
 
 template <typename T>
@@ -301,8 +386,7 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   const OpPlan<T>& plan = S.pd<T>().plan;
   // y = P r1 (projected)
   launch_operator<T>(OP_P, plan, R1v, Yv, dm, sel, dk, nullptr, 0, s);
-  V.project(y, act);
-  V.vdot(r1, y, act);
+  V.project(y, act, r1);  // and the partials of <r1, y>
   hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 0, nb, act);
   OMX_HIP_CHECK(hipMemcpyAsync(r2, r1, vb, hipMemcpyDeviceToDevice, s));
   std::vector<int> live(want);
@@ -310,35 +394,31 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   const int* la = livep;
   // one MINRES iteration; its buffer rotations (r1 <- r2 <- y <- r1 and w <- w1 <- w2 <- w)
   // are 3-cycles, so three iterations leave every pointer where it was
+  const dim3 vg(nblk, nb);
   auto iteration = [&]() {
-    // v = y / beta (coefficients left by the previous scalar step)
-    V.lincomb({y}, v, la);
+    // v = y / beta (slot 2, left by the previous scalar step)
+    V.lincomb({y}, v, la, 2);
     BView<T> Vv{v, ld, ld};
     launch_operator<T>(OP_M, plan, Vv, AVv, dm, sel, dk, nullptr, 0, s);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 1, nb, la);
-    V.lincomb({avp, v, r1}, y, la);
-    V.vdot(v, y, la);
+    V.lincomb_dot({avp, v, r1}, y, la, v, 0);  // and the partials of <v, y>
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 2, nb, la);
     V.lincomb({y, r2}, y, la);
     std::swap(r1, r2);  // r1 <- r2
     std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
     BView<T> R2b{r2, ld, ld}, Yb{y, ld, ld};
     launch_operator<T>(OP_P, plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
-    V.project(y, la);
-    V.vdot(r2, y, la);
+    V.project(y, la, r2);  // and the partials of <r2, y>
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 3, nb, la);
     // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
-    // w1 <- w2, w2 <- w); written into the buffer of the discarded w1
-    V.lincomb({v, w2, w}, w1, la);
+    // w1 <- w2, w2 <- w), written into the buffer of the discarded w1, and x += phi w_new
+    hipLaunchKernelGGL((k_minres_wx<T>), vg, dim3(AB), 0, s, v, w2, w, w1, x, ld, V.K, coef + 4 * nb, la);
     {
       C* wn = w1;
       w1 = w2;
       w2 = w;
       w = wn;
     }
-    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 4, nb, la);
-    V.lincomb({x, w}, x, la);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 5, nb, la);
   };
   // Groups of three iterations, replayed from one captured HIP graph (option "graphs"):
   // the kernels are short and the host's launch rate, not the GPU, set their pace.  The

@@ -27,10 +27,14 @@ template <typename T> struct VecOpsT {
   VecOpsT(Solver& S, hipStream_t s_, int nb_, long ld_, long K_, const C* EV_);
   // part[i nblk + b] = partial sums of Re <A_i, B_i>
   void vdot(const C* A, const C* B, const int* act);
-  // out_i = sum_j coef[4 i + j] in_j_i (out may alias an input)
-  void lincomb(std::initializer_list<const C*> ins, C* outp, const int* act);
-  // x_i <- x_i - ev_i <ev_i, x_i>
-  void project(C* x, const int* act);
+  // out_i = sum_j coef[slot][4 i + j] in_j_i (out may alias an input); coefficient slot
+  // s (of four) at coef + 4 nb s
+  void lincomb(std::initializer_list<const C*> ins, C* outp, const int* act, int slot = 0);
+  // the same, also leaving the partials of Re <d_i, out_i> in part (as vdot)
+  void lincomb_dot(std::initializer_list<const C*> ins, C* outp, const int* act, const C* d, int slot = 0);
+  // x_i <- x_i - ev_i <ev_i, x_i> (coefficient slot 3); with dot_with, also the partials of
+  // Re <dot_with_i, x_i> of the projected x in part
+  void project(C* x, const int* act, const C* dot_with = nullptr);
 };
 using VecOps = VecOpsT<double>;
 
