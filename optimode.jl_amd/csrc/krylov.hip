@@ -142,6 +142,86 @@ __global__ __launch_bounds__(AB) void k_lincomb_dot(Lin<T> L, long K, const int*
   if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
 }
 
+// Block-wide fixed-order sum of n partials (every block of a launch gets the same value)
+__device__ __forceinline__ double block_sum(const double* p, int n, double* red) {
+  double a = 0;
+  for (int b = threadIdx.x; b < n; b += AB) a += p[b];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+struct MinresState;
+
+// Stage-1 update fused with its dot: y <- A v - shift v - (beta/oldb) r1 (A v in av) and the
+// partials of Re <v, y>; the coefficients come from the MINRES state (read only)
+template <typename T>
+__global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* av, const cplx<T>* v,
+                                                  const cplx<T>* r1, cplx<T>* y, long ld, long K, const int* act,
+                                                  double* part);
+
+// Stage-2 update fused with its scalar step: alfa = sum of the <v, y> partials (every block
+// the same, block 0 stores it), y <- y - (alfa/beta) r2
+template <typename T>
+__global__ __launch_bounds__(AB) void k_minres_y2(MinresState* ms, const double* vpart, int nvp,
+                                                  const cplx<T>* r2, cplx<T>* y, long ld, long K, const int* act);
+
+// Projection fused with its coefficient and a dot: x <- x - ev <ev, x> from the complex
+// partials of <ev, x> (each block sums them), and the partials of Re <d, x_new>
+template <typename T>
+__global__ __launch_bounds__(AB) void k_project_dot(const c64* cpart, int ncp, const cplx<T>* ev, cplx<T>* x,
+                                                    const cplx<T>* d, long ld, long K, const int* act, double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double acc = 0;
+  if (act[i]) {
+    // <ev, x> in fixed order: real parts, then imaginary parts (the partials interleave them)
+    const double* cp = reinterpret_cast<const double*>(cpart + long(i) * ncp);
+    double sr = 0, si = 0;
+    for (int b = threadIdx.x; b < ncp; b += AB) sr += cp[2 * b];
+    red[threadIdx.x] = sr;
+    __syncthreads();
+    for (int t = AB / 2; t > 0; t >>= 1) {
+      if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+      __syncthreads();
+    }
+    sr = red[0];
+    __syncthreads();
+    for (int b = threadIdx.x; b < ncp; b += AB) si += cp[2 * b + 1];
+    red[threadIdx.x] = si;
+    __syncthreads();
+    for (int t = AB / 2; t > 0; t >>= 1) {
+      if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+      __syncthreads();
+    }
+    si = red[0];
+    __syncthreads();
+    const long o = i * ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      const cplx<T> e = ev[o + r], xr = x[o + r];
+      // x - ev (sr + i si)
+      const double nx = double(xr.x) - (double(e.x) * sr - double(e.y) * si);
+      const double ny = double(xr.y) - (double(e.x) * si + double(e.y) * sr);
+      const cplx<T> xn{T(nx), T(ny)};
+      x[o + r] = xn;
+      acc += double(d[o + r].x) * double(xn.x) + double(d[o + r].y) * double(xn.y);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
 // MINRES direction and solution updates in one pass: w_new = c0 v + c1 w1 + c2 w2 (into
 // wout) and x += c3 w_new, with real coefficients c = coef[4 i ..]
 template <typename T>
@@ -171,6 +251,51 @@ struct MinresState {
 // step of the iteration they follow.  Coefficients for the next lincomb go to coef.
 // one block per instance: the block sums the instance's partials (tree), then its first
 // thread runs the recurrence
+template <typename T>
+__global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* av, const cplx<T>* v,
+                                                  const cplx<T>* r1, cplx<T>* y, long ld, long K, const int* act,
+                                                  double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double acc = 0;
+  if (act[i]) {
+    const MinresState& m = ms[i];
+    const double c1 = -m.shift, c2 = m.itn >= 1 ? -m.beta / m.oldb : 0.0;
+    const long o = i * ld;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      const cplx<T> a = av[o + r], b = v[o + r], c = r1[o + r];
+      const cplx<T> yn{T(double(a.x) + c1 * double(b.x) + c2 * double(c.x)),
+                       T(double(a.y) + c1 * double(b.y) + c2 * double(c.y))};
+      y[o + r] = yn;
+      acc += double(b.x) * double(yn.x) + double(b.y) * double(yn.y);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(AB) void k_minres_y2(MinresState* ms, const double* vpart, int nvp,
+                                                  const cplx<T>* r2, cplx<T>* y, long ld, long K, const int* act) {
+  const int i = blockIdx.y;
+  if (!act[i]) return;
+  __shared__ double red[AB];
+  const double alfa = block_sum(vpart + long(i) * nvp, nvp, red);
+  // every block read the state it needs before block 0 writes alfa, which no block reads
+  const double f = -alfa / ms[i].beta;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ms[i].alfa = alfa;
+  const long o = i * ld;
+  for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+    const cplx<T> a = y[o + r], c = r2[o + r];
+    y[o + r] = cplx<T>{T(double(a.x) + f * double(c.x)), T(double(a.y) + f * double(c.y))};
+  }
+}
+
 // Coefficient slots (4 complex per instance each, slot s at coef + 4 ninst s): 0 for the
 // y updates of stages 1 and 2, 1 for the fused w/x update of stage 3, 2 for the next
 // v = y / beta (stages 0 and 3); slot 3 belongs to the projections.
@@ -352,7 +477,8 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   // instance, the MINRES states and two flag arrays
   const size_t vb = sizeof(C) * nb * ld;
   const int iblk = ((nb + 63) / 64) * 64;  // ints per flag array
-  const size_t need = 8 * vb + sizeof(C) * nb * n3 + sizeof(MinresState) * nb + 2 * sizeof(int) * iblk;
+  const size_t need = 8 * vb + sizeof(C) * nb * n3 + sizeof(MinresState) * nb + 2 * sizeof(int) * iblk +
+                      sizeof(double) * nb * V.nblk;
   S.krylov_ws.ensure(need);
   char* wp = S.krylov_ws.as<char>();
   C *x = reinterpret_cast<C*>(wp), *r1 = r, *r2 = reinterpret_cast<C*>(wp + vb),
@@ -363,6 +489,8 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   MinresState* ms = reinterpret_cast<MinresState*>(wp + 8 * vb + sizeof(C) * nb * n3);
   int* actp = reinterpret_cast<int*>(reinterpret_cast<char*>(ms) + sizeof(MinresState) * nb);
   int* livep = actp + iblk;
+  // real dot partials of the fused projection (its complex input partials live in V.part)
+  double* cpart2 = reinterpret_cast<double*>(livep + iblk);
   OMX_HIP_CHECK(hipMemcpyAsync(actp, want.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
   const int* act = actp;
   OMX_HIP_CHECK(hipMemsetAsync(x, 0, vb, s));
@@ -400,16 +528,19 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
     V.lincomb({y}, v, la, 2);
     BView<T> Vv{v, ld, ld};
     launch_operator<T>(OP_M, plan, Vv, AVv, dm, sel, dk, nullptr, 0, s);
-    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 1, nb, la);
-    V.lincomb_dot({avp, v, r1}, y, la, v, 0);  // and the partials of <v, y>
-    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 2, nb, la);
-    V.lincomb({y, r2}, y, la);
+    // y <- A v - shift v - (beta/oldb) r1 with the partials of <v, y> (stage 1 fused)
+    hipLaunchKernelGGL((k_minres_y1<T>), vg, dim3(AB), 0, s, ms, avp, v, r1, y, ld, V.K, la, part);
+    // alfa = <v, y>; y <- y - (alfa/beta) r2 (stage 2 fused)
+    hipLaunchKernelGGL((k_minres_y2<T>), vg, dim3(AB), 0, s, ms, part, nblk, r2, y, ld, V.K, la);
     std::swap(r1, r2);  // r1 <- r2
     std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
     BView<T> R2b{r2, ld, ld}, Yb{y, ld, ld};
     launch_operator<T>(OP_P, plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
-    V.project(y, la, r2);  // and the partials of <r2, y>
-    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, part, nblk, coef, 3, nb, la);
+    // y <- y - ev <ev, y> with the partials of <r2, y>
+    hipLaunchKernelGGL((k_cdot<T>), vg, dim3(AB), 0, s, V.EV, y, ld, V.K, la, V.part.template as<c64>());
+    hipLaunchKernelGGL((k_project_dot<T>), vg, dim3(AB), 0, s, V.part.template as<c64>(), nblk, V.EV, y, r2, ld,
+                       V.K, la, cpart2);
+    hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, cpart2, nblk, coef, 3, nb, la);
     // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
     // w1 <- w2, w2 <- w), written into the buffer of the discarded w1, and x += phi w_new
     hipLaunchKernelGGL((k_minres_wx<T>), vg, dim3(AB), 0, s, v, w2, w, w1, x, ld, V.K, coef + 4 * nb, la);
