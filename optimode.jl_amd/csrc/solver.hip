@@ -347,7 +347,7 @@ template <typename T> class Batch {
 
   // Row partitioning of the Gram products and block updates for `na` instances with work
   // (an upper bound is enough: the kernels skip instances without columns): about
-  // gram_blocks Gram blocks and combine_blocks update blocks in total (1024 and 2048 by
+  // gram_blocks Gram blocks and combine_blocks update blocks in total (512 and 1024 by
   // default), whatever the number of instances left.
   void set_active(int na) {
     na = std::max(1, std::min(na, ninst));
