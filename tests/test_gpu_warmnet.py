@@ -128,3 +128,22 @@ def test_solve_k_learned_on_device_matches_host_path():
         assert abs(np.vdot(a, b)) > 1 - 1e-6
     with pytest.raises(omx.OmxError):  # more bands than the network predicts
         WS.solve_k_learned(prm["omega"], ei, grid, solver, st, nev=9, on_device=True)
+
+
+def test_solve_k_learned_rejects_what_it_cannot_serve():
+    st = WS.LearnedStart.load(WS.DATA_DIR + "/warmnet_tfln256.npz")
+    solver = omx.HIPSolver()
+    # a grid that is not a multiple of the network's 64 x 64 map
+    g = omx.Grid(8.0, 4.0, 200, 192)
+    ei = WS.rib_epsinv(g, WS.random_rib(np.random.default_rng(1)))
+    with pytest.raises(omx.OmxError) as e:
+        WS.solve_k_learned(0.65, ei, g, solver, st, nev=4)
+    assert e.value.status == omx._lib.OMX_ERR_BAD_SHAPE
+    # a 3D cell
+    from oracle import periodic as P
+    og3 = P.Grid3(2.4, 2.0, 0.45, 64, 64, 4)
+    g3 = omx.Grid3(2.4, 2.0, 0.45, 64, 64, 4)
+    ei3 = P.bragg_wg_epsi(og3, eps_core=4.0, eps_bg=2.1, w0=0.5, h0=0.4, mod=0.5)
+    with pytest.raises(omx.OmxError) as e:
+        WS.solve_k_learned(0.5, ei3, g3, solver, st, nev=2)
+    assert e.value.status == omx._lib.OMX_ERR_UNSUPPORTED
