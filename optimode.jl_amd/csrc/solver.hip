@@ -347,12 +347,15 @@ template <typename T> class Batch {
 
   // Row partitioning of the Gram products and block updates for `na` instances with work
   // (an upper bound is enough: the kernels skip instances without columns): about
-  // gram_blocks Gram blocks and combine_blocks update blocks in total (512 and 1024 by
-  // default), whatever the number of instances left.
+  // gram_blocks Gram blocks and combine_blocks update blocks in total (by default 1024 and
+  // 2048 on 2D cells, 512 and 1024 on 3D cells), whatever the number of instances left.
   void set_active(int na) {
     na = std::max(1, std::min(na, ninst));
-    nchunk = std::max(1, std::min(nchunk_max, S.opt.gram_blocks / na));
-    nblk = std::max(1, std::min(nblk_max, S.opt.combine_blocks / na));
+    const bool flat = S.nz <= 1;
+    const int gb = S.opt.gram_blocks > 0 ? S.opt.gram_blocks : (flat ? 1024 : 512);
+    const int cb = S.opt.combine_blocks > 0 ? S.opt.combine_blocks : (flat ? 2048 : 1024);
+    nchunk = std::max(1, std::min(nchunk_max, gb / na));
+    nblk = std::max(1, std::min(nblk_max, cb / na));
     dl.nblk = nblk;
   }
 
@@ -690,9 +693,9 @@ void Solver::set_option(const std::string& key, double v) {
   else if (key == "jac_stop32")
     opt.jac_stop32 = v;
   else if (key == "gram_blocks")
-    opt.gram_blocks = std::max(1, int(v));
+    opt.gram_blocks = std::max(0, int(v));
   else if (key == "combine_blocks")
-    opt.combine_blocks = std::max(1, int(v));
+    opt.combine_blocks = std::max(0, int(v));
   else if (key == "reg_ops" || key == "slab3d") {
     (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
     select_operator(*this, p32);

@@ -163,8 +163,10 @@ struct Options {
   int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
   double jac_stop32 = 1e-4;     // Jacobi stop (max |sin| of a sweep) of the FP32 batches' Rayleigh-Ritz
                                 // (off-diagonal left ~1e-8 |A|, below FP32 data; FP64 batches: 1e-9)
-  int gram_blocks = 512;        // Gram products: about this many row-chunk blocks over the active instances
-  int combine_blocks = 1024;    // block updates: about this many row blocks over the active instances
+  int gram_blocks = 0;          // Gram products: about this many row-chunk blocks over the active
+                                // instances (0: 1024 on 2D cells, 512 on 3D cells, as measured)
+  int combine_blocks = 0;       // block updates: about this many row blocks (0: 2048 on 2D cells,
+                                // 1024 on 3D cells)
   int graphs = 0;               // MINRES iterations replayed from a captured HIP graph (groups of 3;
                                 // same results, no measured gain: the gaps are GPU-side)
 };
