@@ -275,16 +275,40 @@ def test_adjoint_epsinv_directional_fd():
     assert np.sum(eibar2 * d) == pytest.approx(fd2, rel=1e-4)
 
 
-def test_epsinv_bar_convention():
-    rng = np.random.default_rng(8)
-    d, ld = _rand_c(rng, 3, 2, 2), _rand_c(rng, 3, 2, 2)
-    e = O.epsinv_bar(d, ld)
-    # grads/solve.jl:498-511 literally
-    assert np.allclose(e[0, 0], -np.real(ld[0] * np.conj(d[0])))
-    assert np.allclose(e[0, 1], np.real(-np.conj(ld[1]) * d[0] - ld[0] * np.conj(d[1])))
-    assert np.allclose(e[1, 2], np.real(-np.conj(ld[2]) * d[1] - ld[1] * np.conj(d[2])))
-    assert np.allclose(e[0, 2], np.real(-np.conj(ld[2]) * d[0] - ld[0] * np.conj(d[2])))
-    assert np.allclose(e, np.transpose(e, (1, 0, 2, 3)))
+@pytest.mark.parametrize("pair", [(0, 1), (0, 2), (1, 2), (0, 0), (2, 2)])
+def test_epsinv_bar_full_tensor_directional_fd(pair):
+    """The 2D ε̄⁻¹ convention (grads/solve.jl:498-511) pinned by finite differences on a
+    full-tensor ε⁻¹: off-diagonal (i, j) and (j, i) both hold the derivative with respect to
+    the symmetric pair.  So a perturbation δ·(e_ij + e_ji) per pixel changes k (k̄ channel)
+    and |<ev, c>|² (ēv channel) by Σ_pixels ε̄⁻¹[i, j]·δ = ½ <ε̄⁻¹, D> -- half the full
+    contraction.  A diagonal direction e_ii changes them by <ε̄⁻¹, D>.  A dropped mirror term,
+    a factor of two or a transposed index fails one of the two channels."""
+    i, j = pair
+    g = O.Grid(4.0, 3.0, 16, 12)
+    ei = _aniso_epsinv(g)
+    rng = np.random.default_rng(20 + 3 * i + j)
+    r = rng.standard_normal(g.shape) * 0.02
+    D = np.zeros_like(ei)
+    D[i, j] = r
+    D[j, i] = r
+    km, evs = O.solve_k(W0, ei, g, nev=1, k_tol=1e-13, eig_tol=1e-12, seed=11)
+    c = _rand_c(rng, 2 * g.N)
+    evbar = 2 * c * np.vdot(c, evs[0])
+    _, eb_k = O.solve_k_pullback(W0, ei, g, km, evs, [1.0], None)
+    _, eb_v = O.solve_k_pullback(W0, ei, g, km, evs, [None], [evbar])
+    assert np.allclose(eb_k, np.transpose(eb_k, (1, 0, 2, 3)))
+    pred_k = np.sum(eb_k[i, j] * r)
+    pred_v = np.sum(eb_v[i, j] * r)
+    if i != j:
+        assert pred_k == pytest.approx(0.5 * np.sum(eb_k * D), rel=1e-12)
+
+    def solve(e):
+        k, ev = O.solve_k(W0, e, g, nev=1, k_tol=1e-13, eig_tol=1e-12, seed=11)
+        return k[0], abs(np.vdot(ev[0], c)) ** 2
+    h = 1e-3
+    (kp, jp), (kn, jn) = solve(ei + h * D), solve(ei - h * D)
+    assert (kp - kn) / (2 * h) == pytest.approx(pred_k, rel=1e-6)
+    assert (jp - jn) / (2 * h) == pytest.approx(pred_v, rel=1e-4)
 
 
 def test_sliceinv_pullback_fd():
