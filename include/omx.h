@@ -100,6 +100,19 @@ const char* omx_last_error(const omx_solver* h);
    each at c + 2(gx + hw gy) in FFT order, and are scattered into the start block on the GPU */
 int omx_set_option(omx_solver* h, const char* key, double value);
 
+/* Autotuner (the reference's counterpart is FFTW's PATIENT planning when a ModeSolver is built,
+   lib/MaxwellEigenmodes/src/maxwell.jl:940-943): timed trials on this handle's grid of the
+   kernels of an nev-mode omx_solve_k, with the shapes its FP32 phase launches them with --
+   the operator path (register FFT or LDS Stockham), the Stockham FFT radix variant and tile
+   shape (rows / columns per block), on 3D cells the yz-slab or five-pass form, and the Gram
+   and block-update row partitions.  A Stockham candidate is used only if its output equals
+   the default plan's to 1e-5.  The fastest of each becomes the handle's option; the same
+   choice can be applied to another handle of the grid with omx_set_option ("reg_ops",
+   "fft_variant32", "tile_rows32", "tile_cols32", "gram_blocks", "combine_blocks", "slab3d").
+   choice_out: NULL or 7 ints in that order; us_out: NULL or 3 doubles, microseconds per launch
+   of the chosen operator application, Gram and block update. */
+int omx_autotune(omx_solver* h, int nev, int* choice_out, double* us_out);
+
 /* HelmholtzMap * H for m vectors at k_vec = (0,0,k): _M! (maxwell.jl:471-481) */
 int omx_apply_helmholtz(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision);
 /* HelmholtzPreconditioner * H: _P! (maxwell.jl:459-469) */

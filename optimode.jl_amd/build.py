@@ -55,15 +55,6 @@ def build(jobs=8, verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, OUT)
-    # calibration copy kernel for the rocprofv3 byte counters (bench traffic figures)
-    calib_src = os.path.join(HERE, "measure", "copy_calib.hip")
-    if os.path.exists(calib_src):
-        r = subprocess.run([HIPCC, "-O2", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
-                            "-packed-fp32-ops", calib_src, "-o", os.path.join(HERE, "measure", "copy_calib")],
-                           capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError("copy_calib build failed")
     return OUT
 
 

@@ -731,6 +731,30 @@ template void launch_operator<float>(OpKind, const OpPlan<float>&, BView<float>,
 template void launch_operator<double>(OpKind, const OpPlan<double>&, BView<double>, BView<double>,
                                       cplx<double>*, const VecSel&, const double*, double*, int, hipStream_t);
 
+bool fft_factor_radix4(int L, FFTFactors& f) {
+  int rem = L, p2 = 0;
+  while (rem % 2 == 0) {
+    rem /= 2;
+    ++p2;
+  }
+  if (p2 < 2 || p2 % 2 != 0 || p2 / 2 > FFT_MAX_RADICES) return fft_factor(L, f);
+  int others[FFT_MAX_RADICES];
+  int no = 0;
+  for (int r : {3, 5}) {
+    while (rem % r == 0) {
+      if (no >= FFT_MAX_RADICES) return fft_factor(L, f);
+      others[no++] = r;
+      rem /= r;
+    }
+  }
+  if (rem != 1 || p2 / 2 + no > FFT_MAX_RADICES) return fft_factor(L, f);
+  f.L = L;
+  f.n = 0;
+  for (int i = 0; i < p2 / 2; ++i) f.r[f.n++] = 4;
+  for (int i = 0; i < no; ++i) f.r[f.n++] = others[i];
+  return true;
+}
+
 bool fft_factor(int L, FFTFactors& f) {
   f.L = L;
   f.n = 0;

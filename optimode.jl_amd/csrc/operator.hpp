@@ -152,5 +152,8 @@ void launch_fields3d(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D,
 
 // Host helper: factor L into radices {8,4,3,5,2} (no lone factor 2 unless L is 2*odd).
 bool fft_factor(int L, FFTFactors& f);
+// The same length in radix-4 passes (then 3s and 5s) when its power of two is even and at
+// least 4; otherwise as fft_factor.
+bool fft_factor_radix4(int L, FFTFactors& f);
 
 }  // namespace omx

@@ -173,6 +173,11 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
   if (!fft_factor(S.nx, pl.fx) || !fft_factor(S.ny, pl.fy))
     throw Error(OMX_ERR_BAD_SHAPE, "grid sizes must factor into 2, 3 and 5 (got " + std::to_string(S.nx) + " x " +
                                        std::to_string(S.ny) + ")");
+  const bool tuned = sizeof(T) == 4 && S.nz == 1;  // options of the FP32 plan of 2D cells
+  if (tuned && S.opt.fft_variant32 == 1) {
+    fft_factor_radix4(S.nx, pl.fx);
+    fft_factor_radix4(S.ny, pl.fy);
+  }
   auto twiddles = [](int L) {
     std::vector<cplx<T>> tw(L);
     for (int t = 0; t < L; ++t) {
@@ -216,6 +221,12 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
       break;
     }
   }
+  // tuned tile shape: any divisor that stays within the LDS budget (the FFT bound is
+  // checked below with the rest of the plan)
+  if (tuned && S.opt.tile_rows32 > 0 && S.ny % S.opt.tile_rows32 == 0 && 3 * S.opt.tile_rows32 * S.nx <= budget)
+    pl.rows_per_block = S.opt.tile_rows32;
+  if (tuned && S.opt.tile_cols32 > 0 && S.nx % S.opt.tile_cols32 == 0 && 3 * S.opt.tile_cols32 * S.ny <= budget)
+    pl.cols_per_block = S.opt.tile_cols32;
   if (S.nz > 1)
     for (int tz : {16, 8, 4, 2}) {
       if (S.nx % tz == 0 && 3 * tz * S.nz <= budget) {
@@ -696,8 +707,24 @@ void Solver::set_option(const std::string& key, double v) {
     opt.gram_blocks = std::max(0, int(v));
   else if (key == "combine_blocks")
     opt.combine_blocks = std::max(0, int(v));
-  else if (key == "reg_ops" || key == "slab3d") {
-    (key == "reg_ops" ? opt.reg_ops : opt.slab3d) = v != 0 ? 1 : 0;
+  else if (key == "fft_variant32" || key == "tile_rows32" || key == "tile_cols32") {
+    int& o = key == "fft_variant32" ? opt.fft_variant32 : (key == "tile_rows32" ? opt.tile_rows32 : opt.tile_cols32);
+    const int nv = key == "fft_variant32" ? (v != 0 ? 1 : 0) : std::max(0, int(v));
+    if (o == nv) return;
+    const int old = o;
+    o = nv;
+    try {
+      build_plan<float>(*this, p32);
+    } catch (...) {
+      o = old;
+      build_plan<float>(*this, p32);
+      throw;
+    }
+    workspaces.clear();  // batches size their dot partials by the operator in use
+  } else if (key == "reg_ops" || key == "slab3d") {
+    int& o = key == "reg_ops" ? opt.reg_ops : opt.slab3d;
+    if (o == (v != 0 ? 1 : 0)) return;  // unchanged: keep the cached workspaces
+    o = v != 0 ? 1 : 0;
     select_operator(*this, p32);
     select_operator(*this, p64);
     workspaces.clear();  // batches size their dot partials by the operator in use
@@ -709,6 +736,210 @@ void Solver::set_option(const std::string& key, double v) {
   }
   else
     throw Error(OMX_ERR_INVALID_ARG, "unknown option '" + key + "'");
+}
+
+// Autotuner (the counterpart of FFTW's PATIENT planning at ModeSolver construction,
+// maxwell.jl:940-943): timed trials, on this handle's grid, of the kernels an nev-mode
+// solve_k spends its FP32 phase in, with the shapes that phase launches them with: nev
+// band instances of nev columns, the operator on half of each block, the Hermitian Gram of
+// [X R P] against [AX AR AP] and the residual block update.  Trials:
+//   * operator (2D cells): the register-FFT path where the grid admits it, and the LDS
+//     Stockham path for each FFT radix variant and tile shape (rows per block of passes 1/3,
+//     columns per block of pass 2) that fits; a Stockham candidate is kept only if its
+//     output equals the default plan's to 1e-5; 3D cells: the yz-slab and five-pass forms;
+//   * the Gram row partition (gram_blocks) and the block-update partition (combine_blocks).
+// The fastest of each becomes the handle's option.  choice_out (7 ints): reg_ops,
+// fft_variant32, tile_rows32, tile_cols32, gram_blocks, combine_blocks, slab3d; us_out (3):
+// microseconds per launch of the chosen operator application, Gram and update.
+void Solver::autotune(int nev, int* choice_out, double* us_out) {
+  if (nev < 1 || nev > 16) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..16");
+  if (3 * nev > K) throw Error(OMX_ERR_BAD_SHAPE, "problem too small for the requested block size");
+  OMX_HIP_CHECK(hipSetDevice(device));
+  const Options saved = opt;
+  // on any error the handle keeps its options and plans as they were
+  struct Restore {
+    Solver& S;
+    const Options& o;
+    bool armed = true;
+    ~Restore() {
+      if (!armed) return;
+      S.opt = o;
+      try {
+        build_plan<float>(S, S.p32);
+        select_operator(S, S.p64);
+      } catch (...) {
+      }
+    }
+  } restore{*this, saved};
+  const int M = nev, ninst = nev;
+  Batch<float> b(*this, ninst, M, 2);  // not from the cache: freed when tuning ends
+  hipStream_t s = stream;
+  for (int q = 0; q < 2; ++q) {
+    launch_randn<float>(b.p(b.X[q]), ld, b.istride, ninst, M, K, 101 + q, s);
+    launch_randn<float>(b.p(b.AX[q]), ld, b.istride, ninst, M, K, 201 + q, s);
+    launch_randn<float>(b.p(b.R[q]), ld, b.istride, ninst, M, K, 301 + q, s);
+    launch_randn<float>(b.p(b.P[q]), ld, b.istride, ninst, M, K, 401 + q, s);
+    launch_randn<float>(b.p(b.AP[q]), ld, b.istride, ninst, M, K, 501 + q, s);
+  }
+  launch_randn<float>(b.p(b.AR), ld, b.istride, ninst, M, K, 601, s);
+  // every instance in the middle of an iteration with all M columns active
+  std::vector<int> shp(size_t(F_NFIELDS) * ninst, 0);
+  for (int i = 0; i < ninst; ++i) {
+    shp[F_NACT * ninst + i] = M;
+    shp[F_NP * ninst + i] = M;
+    shp[F_NNEW * ninst + i] = M;
+    shp[F_M * ninst + i] = M;
+    shp[F_ONE * ninst + i] = 1;
+    shp[F_NPN * ninst + i] = M;
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(b.shape.as<void>(), shp.data(), sizeof(int) * shp.size(), hipMemcpyHostToDevice, s));
+  std::vector<double> coef(size_t(2) * ninst * b.pm * b.pm), lam(size_t(ninst) * b.pm), kh(ninst);
+  uint64_t z = 0x1234567;
+  for (auto& c : coef) {
+    z = z * 6364136223846793005ull + 1442695040888963407ull;
+    c = (double(z >> 11) * (1.0 / 9007199254740992.0) - 0.5) * 0.2;
+  }
+  for (size_t j = 0; j < lam.size(); ++j) lam[j] = 0.4 + 1e-3 * double(j % 16);
+  for (int i = 0; i < ninst; ++i) kh[i] = 1.2 + 0.01 * i;
+  OMX_HIP_CHECK(hipMemcpyAsync(b.Cx.as<void>(), coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(b.lam.as<void>(), lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(b.kd.as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
+  // operator vectors: the leading half of every instance's block
+  const int nop = std::max(1, M / 2);
+  std::vector<int> c0(ninst, 0), nc(ninst, nop);
+  OMX_HIP_CHECK(hipMemcpyAsync(b.selc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(b.selnc.as<void>(), nc.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  hipEvent_t ea, eb;
+  OMX_HIP_CHECK(hipEventCreate(&ea));
+  OMX_HIP_CHECK(hipEventCreate(&eb));
+  auto time_us = [&](const auto& launch) {
+    launch();  // warm-up
+    const int reps = 5;
+    OMX_HIP_CHECK(hipEventRecord(ea, s));
+    for (int r = 0; r < reps; ++r) launch();
+    OMX_HIP_CHECK(hipEventRecord(eb, s));
+    OMX_HIP_CHECK(hipEventSynchronize(eb));
+    float ms = 0;
+    OMX_HIP_CHECK(hipEventElapsedTime(&ms, ea, eb));
+    return 1e3 * double(ms) / reps;
+  };
+  auto op_launch = [&] { b.apply(OP_M, b.X[0], b.AX[0], b.selc0.as<int>(), b.selnc.as<int>()); };
+  // instance 0's operator output on the host, to check a candidate plan against the default
+  const size_t nchk = size_t(nop) * ld;
+  auto op_out0 = [&] {
+    std::vector<c32> h(nchk);
+    op_launch();
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), b.p(b.AX[0]), sizeof(c32) * nchk, hipMemcpyDeviceToHost, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    return h;
+  };
+  auto rel_diff = [&](const std::vector<c32>& a, const std::vector<c32>& r) {
+    double d = 0, n = 0;
+    for (int c = 0; c < nop; ++c)
+      for (long e = 0; e < K; ++e) {
+        const c32 x = a[size_t(c) * ld + e], y = r[size_t(c) * ld + e];
+        d += double(x.x - y.x) * (x.x - y.x) + double(x.y - y.y) * (x.y - y.y);
+        n += double(y.x) * y.x + double(y.y) * y.y;
+      }
+    return std::sqrt(d / std::max(n, 1e-300));
+  };
+  struct OpChoice {
+    int reg, variant, rows, cols, slab;
+  };
+  OpChoice best{saved.reg_ops, saved.fft_variant32, saved.tile_rows32, saved.tile_cols32, saved.slab3d};
+  double best_op = 1e300;
+  auto try_plan = [&](const OpChoice& c, const std::vector<c32>* ref) {
+    opt.reg_ops = c.reg;
+    opt.slab3d = c.slab;
+    opt.fft_variant32 = c.variant;
+    opt.tile_rows32 = c.rows;
+    opt.tile_cols32 = c.cols;
+    try {
+      build_plan<float>(*this, p32);
+    } catch (const Error&) {
+      return;  // the tile or radix variant does not fit this grid's FFT bound
+    }
+    if (ref && rel_diff(op_out0(), *ref) > 1e-5) return;
+    const double t = time_us(op_launch);
+    if (t < best_op) {
+      best_op = t;
+      best = c;
+    }
+  };
+  opt.reg_ops = 0;
+  opt.fft_variant32 = opt.tile_rows32 = opt.tile_cols32 = 0;
+  build_plan<float>(*this, p32);
+  const std::vector<c32> ref = op_out0();  // default Stockham plan (or the 3D default form)
+  if (nz == 1) {
+    if (p32.reg_ok) try_plan(OpChoice{1, 0, 0, 0, saved.slab3d}, &ref);
+    const int budget = 3072;
+    for (int variant : {0, 1})
+      for (int rows : {8, 4, 2, 1}) {
+        if (ny % rows != 0 || 3 * rows * nx > budget) continue;
+        for (int cols : {16, 8, 4, 2, 1}) {
+          if (nx % cols != 0 || 3 * cols * ny > budget) continue;
+          try_plan(OpChoice{0, variant, rows, cols, saved.slab3d}, &ref);
+        }
+      }
+  } else {
+    if (p32.slab_ok) try_plan(OpChoice{saved.reg_ops, 0, 0, 0, 1}, &ref);
+    try_plan(OpChoice{saved.reg_ops, 0, 0, 0, 0}, &ref);
+  }
+  if (best_op >= 1e300) throw Error(OMX_ERR_INTERNAL, "autotune: no operator plan passed its check");
+  opt.reg_ops = best.reg;
+  opt.slab3d = best.slab;
+  opt.fft_variant32 = best.variant;
+  opt.tile_rows32 = best.rows;
+  opt.tile_cols32 = best.cols;
+  build_plan<float>(*this, p32);
+  select_operator(*this, p64);
+  // partitions, with the chosen operator plan in place
+  ColList<float> Y = Batch<float>::list({b.part(b.X[0], b.f(F_XOFF), b.f(F_NACT)), b.part(b.R[0], b.f(F_ROFF), b.f(F_NACT)),
+                                         b.part(b.P[0], nullptr, b.f(F_NP))});
+  ColList<float> AY = Batch<float>::list({b.part(b.AX[0], b.f(F_XOFF), b.f(F_NACT)),
+                                          b.part(b.AR, b.f(F_ROFF), b.f(F_NACT)), b.part(b.AP[0], nullptr, b.f(F_NP))});
+  int best_g = 0, best_c = 0;
+  double tg = 1e300, tc = 1e300;
+  const int gdef = nz > 1 ? 512 : 1024, cdef = nz > 1 ? 1024 : 2048;
+  for (int gb : {128, 256, 512, 1024, 2048, 4096}) {
+    opt.gram_blocks = gb;
+    b.set_active(ninst);
+    const double t = time_us([&] { b.gram(Y, AY, 3 * M, 3 * M, -1, false, true); });
+    if (t < tg * 0.98 || (gb == gdef && t < tg * 1.02)) {  // the default wins near-ties
+      tg = t;
+      best_g = gb;
+    }
+  }
+  for (int cb : {256, 512, 1024, 2048, 4096, 8192}) {
+    opt.combine_blocks = cb;
+    b.set_active(ninst);
+    const double t = time_us([&] {
+      launch_combine<float>(CMB_RESID, Y, AY, b.coef(b.Cx, b.f(F_NACT), b.lam.as<double>()), b.qmax,
+                            b.out(b.X[1], b.f(F_XOFF)), b.out(b.AX[1], b.f(F_XOFF)), b.out(b.R[1], nullptr, 0), ninst,
+                            K, b.nblk, b.normp.as<double>(), s);
+    });
+    if (t < tc * 0.98 || (cb == cdef && t < tc * 1.02)) {
+      tc = t;
+      best_c = cb;
+    }
+  }
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  opt.gram_blocks = best_g;
+  opt.combine_blocks = best_c;
+  workspaces.clear();  // batches size their dot partials by the operator in use
+  restore.armed = false;
+  if (choice_out) {
+    const int c[7] = {opt.reg_ops, opt.fft_variant32, opt.tile_rows32, opt.tile_cols32, opt.gram_blocks,
+                      opt.combine_blocks, opt.slab3d};
+    std::memcpy(choice_out, c, sizeof(c));
+  }
+  if (us_out) {
+    us_out[0] = best_op;
+    us_out[1] = tg;
+    us_out[2] = tc;
+  }
 }
 
 // Host FP64 blocks (m vectors of K complex, contiguous) -> operator -> host.

@@ -167,6 +167,11 @@ struct Options {
                                 // instances (0: 1024 on 2D cells, 512 on 3D cells, as measured)
   int combine_blocks = 0;       // block updates: about this many row blocks (0: 2048 on 2D cells,
                                 // 1024 on 3D cells)
+  // FP32 operator plan of 2D cells on the LDS Stockham path (picked by omx_autotune):
+  int fft_variant32 = 0;        // 0: radix-8 passes first (8, 8, 4 for 256); 1: radix-4 passes
+                                // where the power of two is even (4, 4, 4, 4 for 256)
+  int tile_rows32 = 0;          // rows per block of passes 1 and 3 (0: the largest that fits)
+  int tile_cols32 = 0;          // columns per block of pass 2 (0: the largest that fits)
   int graphs = 0;               // MINRES iterations replayed from a captured HIP graph (groups of 3;
                                 // same results, no measured gain: the gaps are GPU-side)
 };
@@ -215,6 +220,9 @@ class Solver {
   void overlap(int slot_a, int slot_b, int conjugate, double* out);
   void get_mode_fields(int slot, double* E_out, double* H_out);
   void set_option(const std::string& key, double value);
+  // timed trials of the operator path, FFT radix variant, Stockham tile shape and the Gram /
+  // block-update partitions for nev-mode solves; the fastest become the handle's options
+  void autotune(int nev, int* choice_out, double* us_out);
   // per mode j: dw2dk[j] = 2 HMkH(ev_j) and dw2dL[j] = 2 <ev_j| dM/dDz |ev_j> (3D cells)
   void period_derivatives(int nev, const double* kmags, const double* evecs, double* dw2dk, double* dw2dL);
   void require_2d(const char* what) const;
