@@ -115,6 +115,17 @@ int omx_autotune(omx_solver* h, int nev, int* choice_out, double* us_out);
 
 /* HelmholtzMap * H for m vectors at k_vec = (0,0,k): _M! (maxwell.jl:471-481) */
 int omx_apply_helmholtz(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision);
+/* omx_apply_helmholtz that allocates no device memory.  No counterpart in the reference: it is
+   for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Same arguments and results as omx_apply_helmholtz; every
+   device buffer of the call (k, the FP64 staging of H_in / H_out, the FP32 copies and the
+   operator's 3-component intermediate) lives in a fixed 64 MiB array per device that the
+   library holds until the process ends.  A call needs 256 + 112 N m bytes (FP64) or
+   256 + 88 N m bytes (FP32), N = Nx Ny (Nx Ny Nz on 3D handles): at 256x256, m <= 9 in FP64 and
+   m <= 11 in FP32; larger calls return OMX_ERR_BAD_SHAPE.  Calls on one device share the array
+   (it keeps the last call's contents) and are serialised by a per-device lock, since
+   overlapping calls would race on it. */
+int omx_apply_helmholtz_noalloc(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision);
 /* HelmholtzPreconditioner * H: _P! (maxwell.jl:459-469) */
 int omx_apply_preconditioner(omx_solver* h, double k, const double* H_in, double* H_out, int m,
                              int precision);

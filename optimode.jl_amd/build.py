@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "omx", "libomx.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["operator.hip", "operator3d.hip", "block.hip", "dense.hip", "fields.hip", "solver.hip", "adjoint.hip", "krylov.hip", "refine.hip", "smooth.hip", "tensor.hip", "eme.hip",
-           "warmnet.hip", "batch.cpp", "capi.cpp"]
+           "warmnet.hip", "noalloc.hip", "batch.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled on purpose: the compiler must not form

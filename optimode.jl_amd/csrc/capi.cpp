@@ -210,6 +210,10 @@ int omx_apply_helmholtz(omx_solver* h, double k, const double* H_in, double* H_o
   return guarded(h, [&](omx::Solver& s) { s.apply(omx::OP_M, precision, k, H_in, H_out, m, nullptr); });
 }
 
+int omx_apply_helmholtz_noalloc(omx_solver* h, double k, const double* H_in, double* H_out, int m, int precision) {
+  return guarded(h, [&](omx::Solver& s) { omx::apply_helmholtz_noalloc(s, k, H_in, H_out, m, precision); });
+}
+
 int omx_autotune(omx_solver* h, int nev, int* choice_out, double* us_out) {
   return guarded(h, [&](omx::Solver& s) { s.autotune(nev, choice_out, us_out); });
 }

@@ -629,6 +629,7 @@ Solver::Solver(int dev, int nx_, int ny_, int nz_, double Dx_, double Dy_, doubl
     throw Error(OMX_ERR_INVALID_ARG, "cell size must be positive and finite");
   if (nz > 1 && !(Dz > 0 && std::isfinite(Dz))) throw Error(OMX_ERR_INVALID_ARG, "the period Dz must be positive");
   OMX_HIP_CHECK(hipSetDevice(dev));
+  noalloc_init(dev);
   OMX_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   N = long(nx) * ny * nz;
   K = 2 * N;

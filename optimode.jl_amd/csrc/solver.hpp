@@ -284,6 +284,12 @@ void check_symmetric(long N, const double* E, const char* what);
 // per pixel, out = sym(L M R) for column-major 3x3 fields
 void pixel_product3(long N, const double* L, const double* M, const double* R, double* out);
 
+// Allocation-free M(k) H (noalloc.hip): every device buffer in a fixed per-device array whose
+// address noalloc_init looks up once per device (called when a handle is created there).
+void noalloc_init(int device);
+size_t noalloc_bytes(long N, int m, int prec);
+void apply_helmholtz_noalloc(Solver& s, double k, const double* Hin, double* Hout, int m, int prec);
+
 // Many solve_k problems of uneven grid, omega and mode count (batch.cpp).
 void solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only);
 
