@@ -163,13 +163,16 @@ int omx_ng_gvd(omx_solver* h, double omega, int nev, const double* kmags, const 
                double* E_out);
 
 /* Batched solve_k (no single reference counterpart: the 64-instance mixed-grid batch of
-   independent solve_k calls, src/solve.jl:182-196, that sweeps and EME run one by one).
-   Problems may differ in grid, ω and nev; they are taken largest first from a shared queue
-   by min(nstreams, 8) host workers, each with its own HIP stream and handles, so that
-   several solves occupy the GPU at once.  Per-problem results and status go into probs;
-   the call returns OMX_OK when every problem succeeded, else the first failing status.
-   fp64_only != 0 runs every solve in FP64. */
-int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only);
+   independent solve_k calls, src/solve.jl:182-196, that sweeps and EME run one by one,
+   lib/ModeSweeps/src/batch.jl:256-265).  Problems may differ in grid, eps^-1, ω, nev and
+   tolerances.  They are bucketed by grid; a bucket runs on one handle and one HIP stream in
+   launch groups of as many problems as fit half the free device memory (at most max_group
+   problems when max_group > 0): every band of every problem of a group is an instance of the
+   same LOBPCG, Newton, refinement and phase launches.  Per-problem results and status go into
+   probs (info.lobpcg_iters_fp32 and converged are the problem's own; the FP64 step count,
+   matvecs and device_ms are its launch group's); the call returns OMX_OK when every problem
+   succeeded, else the first failing status.  fp64_only != 0 runs every solve in FP64. */
+int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int max_group, int fp64_only);
 
 /* EME per-slice mode set (lib/EigenmodeExpansion/src/modes.jl:105-122, mode_fields and
    build_mode): E = i ε⁻¹ fft(kx_tc(ev_j)) and H = fft(tc(ev_j)) of nev <= 16 modes at

@@ -179,9 +179,9 @@ int omx_sliceinv_3x3_pullback(int device, int64_t npix, const double* Y, const d
   }
 }
 
-int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only) {
+int omx_solve_k_batch(int device, int nprob, omx_problem* probs, int max_group, int fp64_only) {
   try {
-    omx::solve_k_batch(device, nprob, probs, nstreams, fp64_only);
+    omx::solve_k_batch(device, nprob, probs, max_group, fp64_only);
     return OMX_OK;
   } catch (const omx::Error& e) {
     g_error = e.what();

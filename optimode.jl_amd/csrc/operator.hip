@@ -119,7 +119,7 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass2(OpPlan<T> plan, cplx<T>* D, 
   for (int e = threadIdx.x; e < ny * TX; e += blockDim.x) {
     const int iy = e / TX;
     const int t = e - iy * TX;
-    const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+    const T* ep = eps_rowm(plan, inst) + (long(iy) * nx + ix0 + t) * 6;
     const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
     cplx<T>* b0 = buf + (t * 3 + 0) * ny + iy;
     cplx<T>* b1 = buf + (t * 3 + 1) * ny + iy;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass2_fwd(OpPlan<T> plan, cplx<T>*
     const cplx<T> d0 = buf[(t * 3 + 0) * ny + iy], d1 = buf[(t * 3 + 1) * ny + iy], d2 = buf[(t * 3 + 2) * ny + iy];
     const long o = (long(iy) * nx + ix0 + t) * 3;
     if (EPS) {
-      const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+      const T* ep = eps_rowm(plan, inst) + (long(iy) * nx + ix0 + t) * 6;
       const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
       Dv[o + 0] = (d0 * exx + d1 * exy + d2 * exz) * scale;
       Dv[o + 1] = (d0 * exy + d1 * eyy + d2 * eyz) * scale;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass2_inv(OpPlan<T> plan, cplx<T>*
     const int t = e - iy * TX;
     const long o = (long(iy) * nx + ix0 + t) * 3;
     const cplx<T> d0 = Dv[o], d1 = Dv[o + 1], d2 = Dv[o + 2];
-    const T* ep = plan.eps6 + (long(iy) * nx + ix0 + t) * 6;
+    const T* ep = eps_rowm(plan, inst) + (long(iy) * nx + ix0 + t) * 6;
     const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
     buf[(t * 3 + 0) * ny + iy] = (d0 * exx + d1 * exy + d2 * exz) * scale;
     buf[(t * 3 + 1) * ny + iy] = (d0 * exy + d1 * eyy + d2 * eyz) * scale;
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(REG_COL_THREADS) void k_rpass2(OpPlan<T> plan, cplx
   reg_fft<T, Dirs<OP>::y1, LY>(a0, g, xs, tw);
   reg_fft<T, Dirs<OP>::y1, LY>(a1, g, xs, tw);
   reg_fft<T, Dirs<OP>::y1, LY>(a2, g, xs, tw);
-  const T* ep = plan.eps6c + long(ix) * LY * 6;
+  const T* ep = eps_colm(plan, inst) + long(ix) * LY * 6;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const T* e = ep + (g + GY * q) * 6;

@@ -87,6 +87,10 @@ template <typename T> struct OpPlan {
   const cplx<T>* twy;             // device: exp(-2 pi i t/ny)
   const T* eps6;                  // device: (ny, nx, 6) = xx, yy, zz, xy, xz, yz
   const T* eps6c;                 // device: (nx, ny, 6), column-major copy (register path)
+  // several eps^-1 fields on one grid (problems batched into one launch, 2D cells): vector v
+  // of instance i reads the field at eps6 / eps6c + eps_slot[i] * eps_stride; null: one field
+  const int* eps_slot;            // device, per instance, may be null
+  long eps_stride;                // 6 N
   int rows_per_block;             // Stockham path: pass 1/3
   int cols_per_block;             // Stockham path: pass 2
   int vpt;                        // Stockham path: values per thread bound (8, 16, 32)
@@ -104,6 +108,14 @@ template <typename T> struct OpPlan {
   // x-major [v][ix][iz][iy][c]), x-columns per block; 0: five passes
   int slab_cols;
 };
+
+// eps^-1 of instance `inst` in the two device layouts
+template <typename T> __device__ __forceinline__ const T* eps_rowm(const OpPlan<T>& p, int inst) {
+  return p.eps_slot ? p.eps6 + long(p.eps_slot[inst]) * p.eps_stride : p.eps6;
+}
+template <typename T> __device__ __forceinline__ const T* eps_colm(const OpPlan<T>& p, int inst) {
+  return p.eps_slot ? p.eps6c + long(p.eps_slot[inst]) * p.eps_stride : p.eps6c;
+}
 
 // Register path geometry: rows per block of passes 1 and 3 (the column-major intermediate
 // is written / read in runs of that many iy: whole 128-byte lines), threads per block of

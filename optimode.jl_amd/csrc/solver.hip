@@ -588,6 +588,25 @@ template <typename T> class Batch {
     return out;
   }
 
+  // HMkH of the first ncol[i] columns of every instance i (out[i * M + j]), summed on the host.
+  std::vector<double> hmkh_multi(const std::vector<int>& ncol) {
+    std::vector<int> c0(ninst, 0);
+    OMX_HIP_CHECK(hipMemcpyAsync(selc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(selnc.as<void>(), ncol.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
+    apply(OP_MKH, X[cur], X[cur], selc0.as<int>(), selnc.as<int>(), dotp.as<double>());
+    std::vector<double> h(size_t(ninst) * M * dps);
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, S.stream));
+    OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+    std::vector<double> out(size_t(ninst) * M, 0.0);
+    for (int i = 0; i < ninst; ++i)
+      for (int j = 0; j < ncol[i]; ++j) {
+        double acc = 0;
+        for (int q = 0; q < dps; ++q) acc += h[(size_t(i) * M + j) * dps + q];
+        out[size_t(i) * M + j] = acc;
+      }
+    return out;
+  }
+
   // Rayleigh quotients diag(X^H A X) of all M columns of every instance (final_retval).
   std::vector<double> rayleigh() {
     ColList<T> a = list({part(X[cur], nullptr, nullptr, 0, M)});
@@ -1202,8 +1221,9 @@ namespace {
 // Newton driver over a batch of band instances (instance i follows band i).
 template <typename T>
 int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
-                 const std::vector<int>& nchk, const std::vector<int>& msize, double tol_floor, int maxiter,
-                 int max_steps, SolveStats& stats, int& newton_total, bool strict, bool inexact = true) {
+                 const std::vector<int>& nchk, const std::vector<int>& msize, const std::vector<double>& floors,
+                 int maxiter, int max_steps, SolveStats& stats, int& newton_total, bool strict, bool inexact = true,
+                 std::vector<int>* inst_iters = nullptr) {
   const int ninst = b.ninst;
   std::vector<int> act(ninst);
   std::vector<double> tols(ninst);
@@ -1216,10 +1236,13 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
       // inexact Newton: the eigenvalue error ~ resid^2 / gap only has to stay well below
       // the current |Δω²|, so the locking threshold follows it down to the floor
       const double f = std::fabs(hns[i].f);
-      tols[i] = (inexact && hns[i].nsolves > 0 && std::isfinite(f)) ? std::max(tol_floor, 0.1 * f) : tol_floor;
+      tols[i] = (inexact && hns[i].nsolves > 0 && std::isfinite(f)) ? std::max(floors[i], 0.1 * f) : floors[i];
     }
     if (!any) break;
     iters += b.run(act, bands, nchk, msize, tols, maxiter, stats);
+    if (inst_iters)
+      for (int i = 0; i < ninst; ++i)
+        if (act[i]) (*inst_iters)[i] += b.hst[i].niter;
     b.newton(hns);
     ++newton_total;
     if (S.trace) {
@@ -1232,12 +1255,20 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
     // a band may only finish on an eigenpair converged to the requested tolerance: when
     // |Δω²| met k_tol on a looser (inexact-Newton) eigensolve, re-solve at the same k
     for (int i = 0; i < ninst && strict; ++i)
-      if (act[i] && hns[i].done == 1 && tols[i] > tol_floor) {
+      if (act[i] && hns[i].done == 1 && tols[i] > floors[i]) {
         hns[i].done = 0;
         hns[i].f = 0.0;
       }
   }
   return iters;
+}
+
+template <typename T>
+int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
+                 const std::vector<int>& nchk, const std::vector<int>& msize, double tol_floor, int maxiter,
+                 int max_steps, SolveStats& stats, int& newton_total, bool strict, bool inexact = true) {
+  return newton_phase(S, b, hns, bands, nchk, msize, std::vector<double>(size_t(b.ninst), tol_floor), maxiter,
+                      max_steps, stats, newton_total, strict, inexact);
 }
 
 using cd = std::complex<double>;
@@ -1386,8 +1417,8 @@ bool dense_geig(int n, const c64* Gp, const c64* Bp, std::vector<double>& lam, s
 // until |M x - rho x| <= tol; then the Newton update of k from rho and HMkH(x).  The
 // band's Ritz value is the bands[i]-th of the block, as LOBPCG's locking order gives it.
 int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, const std::vector<int>& bands,
-                 const std::vector<int>& msize, double tol, int max_steps, SolveStats& stats, int& newton_total,
-                 double& max_resid) {
+                 const std::vector<int>& msize, const std::vector<double>& tols, int max_steps, SolveStats& stats,
+                 int& newton_total, double& max_resid, std::vector<double>* resid_inst = nullptr) {
   const int ninst = b.ninst;
   const long K = S.K, ld = S.ld, istride = b.istride;
   hipStream_t s = S.stream;
@@ -1493,7 +1524,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       band_resid();
       bool need = false;
       for (int i = 0; i < ninst; ++i) {
-        conv[i] = n1h[i] > 0 && rn[i] <= tol;
+        conv[i] = n1h[i] > 0 && rn[i] <= tols[i];
         jdh[i] = (n1h[i] > 0 && !conv[i]) ? 1 : 0;
         need = need || jdh[i];
       }
@@ -1591,7 +1622,8 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
         ns.slope = slope;
         ns.nsolves += 1;
         max_resid = std::max(max_resid, rn[i]);
-        if (rn[i] <= tol && std::fabs(ns.f) <= ns.k_tol) {
+        if (resid_inst) (*resid_inst)[i] = rn[i];
+        if (rn[i] <= tols[i] && std::fabs(ns.f) <= ns.k_tol) {
           ns.done = 1;
         } else if (ns.nsolves >= ns.max_solves) {
           ns.done = 2;
@@ -1823,7 +1855,8 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
   if (!opt.plain) {
     const int left = std::max(1, max_eigsolves - newton_total);
     if (opt.refine && !opt.fp64_only)
-      stats.it64 += refine_phase(*this, b64, hns, bands, msize, eig_tol, left, stats, newton_total, max_resid);
+      stats.it64 += refine_phase(*this, b64, hns, bands, msize, std::vector<double>(size_t(ninst), eig_tol), left,
+                                 stats, newton_total, max_resid);
     else
       stats.it64 +=
           newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
@@ -1875,6 +1908,257 @@ void Solver::get_evecs(double* evecs) {
     OMX_HIP_CHECK(hipMemcpyAsync(evecs + 2 * size_t(j) * K, last_evecs.as<c64>() + size_t(j) * ld, sizeof(c64) * K,
                                  hipMemcpyDeviceToHost, stream));
   OMX_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+namespace {
+
+// Host conversion of a (3,3,Nx,Ny) column-major field to the operator's two 6-entry layouts,
+// written to device memory at rowm / colm (2D cells).
+template <typename T> void put_tensor6(const Solver& S, const double* E, T* rowm, T* colm, hipStream_t s) {
+  const long N = S.N;
+  std::vector<T> e6(size_t(6) * N), e6c(size_t(6) * N);
+  for (long p = 0; p < N; ++p) {
+    const double* e = E + 9 * p;
+    T* o = e6.data() + 6 * p;
+    o[0] = T(e[0]);
+    o[1] = T(e[4]);
+    o[2] = T(e[8]);
+    o[3] = T(e[3]);
+    o[4] = T(e[6]);
+    o[5] = T(e[7]);
+  }
+  for (int ix = 0; ix < S.nx; ++ix)
+    for (int iy = 0; iy < S.ny; ++iy)
+      for (int a = 0; a < 6; ++a) e6c[6 * (iy + size_t(S.ny) * ix) + a] = e6[6 * (ix + size_t(S.nx) * iy) + a];
+  OMX_HIP_CHECK(hipMemcpyAsync(rowm, e6.data(), sizeof(T) * e6.size(), hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipMemcpyAsync(colm, e6c.data(), sizeof(T) * e6c.size(), hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));  // the host arrays go out of scope
+}
+
+}  // namespace
+
+// Several solve_k problems on this grid in one batch (no reference counterpart as one call:
+// the reference sweeps run solve_k, src/solve.jl:182-196, one problem per call or process,
+// lib/ModeSweeps/src/batch.jl:256-265).  Problem p's band j is instance off_p + j; the block
+// size is the largest nev, each instance using min(nev_p, j + 1 + guard) columns.  Every
+// launch of the FP32 phase (shared first eigensolve per problem, inexact Newton), of the
+// FP64 refinement and of the phase canonicalisation covers all instances; the operator
+// reads instance i's eps^-1 through plan.eps_slot.  Per problem the results are those of
+// omx_solve_k on that problem alone, up to rounding.
+void Solver::solve_k_multi(std::vector<MultiProblem>& probs) {
+  require_2d("the batched solve_k");
+  const int P = int(probs.size());
+  if (P == 0) return;
+  int ninst = 0, M = 0, maxiter = 1, max_eigsolves = 1;
+  for (auto& p : probs) {
+    if (p.nev < 1 || p.nev > 16) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..16");
+    if (3 * p.nev > K) throw Error(OMX_ERR_BAD_SHAPE, "problem too small for the requested block size");
+    if (!p.kmags || !p.epsinv) throw Error(OMX_ERR_INVALID_ARG, "NULL epsinv or kmags_out");
+    if (!(p.omega > 0) || !std::isfinite(p.omega) || !(p.k_tol > 0) || !(p.eig_tol > 0) || p.max_eigsolves < 1 ||
+        p.maxiter < 1)
+      throw Error(OMX_ERR_INVALID_ARG, "bad omega/k_tol/eig_tol/max_eigsolves/maxiter");
+    check_symmetric(N, p.epsinv, "epsinv");
+    ninst += p.nev;
+    M = std::max(M, p.nev);
+    maxiter = std::max(maxiter, p.maxiter);
+    max_eigsolves = std::max(max_eigsolves, p.max_eigsolves);
+  }
+  OMX_HIP_CHECK(hipSetDevice(device));
+  hipStream_t s = stream;
+  hipEvent_t e0, e1;
+  OMX_HIP_CHECK(hipEventCreate(&e0));
+  OMX_HIP_CHECK(hipEventCreate(&e1));
+  OMX_HIP_CHECK(hipEventRecord(e0, s));
+  // the problems' eps^-1, and which one each instance reads
+  const size_t n6 = size_t(6) * N;
+  multi_e32r.ensure(sizeof(float) * n6 * P);
+  multi_e32c.ensure(sizeof(float) * n6 * P);
+  multi_e64r.ensure(sizeof(double) * n6 * P);
+  multi_e64c.ensure(sizeof(double) * n6 * P);
+  multi_slot.ensure(sizeof(int) * ninst);
+  std::vector<int> off(P), slot(ninst);
+  std::vector<double> k0(P);
+  for (int p = 0, o = 0; p < P; ++p) {
+    off[p] = o;
+    for (int j = 0; j < probs[p].nev; ++j) slot[o + j] = p;
+    o += probs[p].nev;
+    put_tensor6<float>(*this, probs[p].epsinv, multi_e32r.as<float>() + n6 * p, multi_e32c.as<float>() + n6 * p, s);
+    put_tensor6<double>(*this, probs[p].epsinv, multi_e64r.as<double>() + n6 * p, multi_e64c.as<double>() + n6 * p, s);
+    double mn = 1e300;  // k_guess (MaxwellEigenmodes.jl:63)
+    for (long q = 0; q < N; ++q)
+      for (int a = 0; a < 3; ++a) mn = std::min(mn, probs[p].epsinv[9 * q + 4 * a]);
+    k0[p] = probs[p].omega * std::sqrt(1.0 / mn);
+  }
+  OMX_HIP_CHECK(hipMemcpyAsync(multi_slot.as<void>(), slot.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  // the plans read the stacked fields for the rest of this call
+  struct PlanSwap {
+    Solver& S;
+    OpPlan<float> a;
+    OpPlan<double> b;
+    ~PlanSwap() {
+      S.p32.plan.eps6 = a.eps6;
+      S.p32.plan.eps6c = a.eps6c;
+      S.p32.plan.eps_slot = nullptr;
+      S.p32.plan.eps_stride = 0;
+      S.p64.plan.eps6 = b.eps6;
+      S.p64.plan.eps6c = b.eps6c;
+      S.p64.plan.eps_slot = nullptr;
+      S.p64.plan.eps_stride = 0;
+    }
+  } swap{*this, p32.plan, p64.plan};
+  p32.plan.eps6 = multi_e32r.as<float>();
+  p32.plan.eps6c = multi_e32c.as<float>();
+  p32.plan.eps_slot = multi_slot.as<int>();
+  p32.plan.eps_stride = long(n6);
+  p64.plan.eps6 = multi_e64r.as<double>();
+  p64.plan.eps6c = multi_e64c.as<double>();
+  p64.plan.eps_slot = multi_slot.as<int>();
+  p64.plan.eps_stride = long(n6);
+
+  SolveStats stats;
+  int newton_total = 0;
+  std::vector<NewtonState> hns(ninst);
+  std::vector<int> bands(ninst), nchk(ninst), msize(ninst), prob_of(ninst);
+  std::vector<double> w2(ninst), tol32(ninst), tol_seed(ninst), eig_tol(ninst), kinit(ninst);
+  for (int p = 0; p < P; ++p)
+    for (int j = 0; j < probs[p].nev; ++j) {
+      const int i = off[p] + j;
+      const MultiProblem& q = probs[p];
+      NewtonState& st = hns[i];
+      std::memset(&st, 0, sizeof(st));
+      st.k = k0[p];
+      st.omega2 = q.omega * q.omega;
+      st.band = j;
+      st.max_solves = q.max_eigsolves;
+      prob_of[i] = p;
+      bands[i] = j;
+      nchk[i] = j + 1;
+      msize[i] = opt.guard < 0 ? q.nev : std::min(q.nev, j + 1 + opt.guard);
+      w2[i] = st.omega2;
+      tol32[i] = std::max(q.eig_tol, opt.fp32_tol_rel * w2[i]);
+      eig_tol[i] = q.eig_tol;
+      kinit[i] = k0[p];
+    }
+  const int maxit64 = std::max(maxiter, opt.maxiter_fp64);
+  const auto b64p = workspace<double>(*this, ninst, M, maxit64);
+  Batch<double>& b64 = *b64p;
+  std::vector<int> it32_inst(ninst, 0);
+
+  // the shared first eigensolve of every problem (instance off_p, all nev_p pairs at k0_p);
+  // each band then starts Newton from its first-order estimate with that block
+  auto seed = [&](auto& b, const std::vector<double>& tfloor, int mi) {
+    using TT = typename std::remove_reference<decltype(b)>::type::value_type;
+    std::vector<int> act(ninst, 0), n0(ninst, M), m0(ninst, M), ncol(ninst, 0);
+    std::vector<double> t0(ninst, 1.0);
+    for (int p = 0; p < P; ++p) {
+      const int i0 = off[p];
+      act[i0] = 1;
+      n0[i0] = m0[i0] = ncol[i0] = probs[p].nev;
+      t0[i0] = std::max(tfloor[i0], 3e-3 * w2[i0]);
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kinit.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
+    const int it = b.run(act, bands, n0, m0, t0, mi, stats);
+    for (int p = 0; p < P; ++p) it32_inst[off[p]] += sizeof(TT) == 4 ? b.hst[off[p]].niter : 0;
+    const auto lam = b.rayleigh();
+    const auto hk = b.hmkh_multi(ncol);
+    std::vector<double> kh(ninst);
+    for (int p = 0; p < P; ++p) {
+      const int i0 = off[p];
+      for (int j = 0; j < probs[p].nev; ++j) {
+        NewtonState& st = hns[i0 + j];
+        const double l = lam[size_t(i0) * M + j], f = l - st.omega2, slope = 2.0 * hk[size_t(i0) * M + j];
+        st.f = f;
+        st.lam = l;
+        st.slope = slope;
+        st.nsolves = 1;
+        if (std::fabs(f) <= st.k_tol) {
+          st.done = 1;
+        } else {
+          const double kn = k0[p] - f / slope;
+          st.k = std::isfinite(kn) && kn > 0 ? kn : k0[p];
+        }
+        kh[i0 + j] = st.k;
+      }
+      for (int j = 1; j < probs[p].nev; ++j)
+        OMX_HIP_CHECK(hipMemcpyAsync(b.p(b.X[b.cur]) + (i0 + j) * b.istride, b.p(b.X[b.cur]) + i0 * b.istride,
+                                     sizeof(cplx<TT>) * b.istride, hipMemcpyDeviceToDevice, s));
+    }
+    OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kh.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
+    ++newton_total;
+    return it;
+  };
+
+  if (!opt.fp64_only) {
+    const auto b32p = workspace<float>(*this, ninst, M, maxiter);
+    Batch<float>& b32 = *b32p;
+    for (int i = 0; i < ninst; ++i) hns[i].k_tol = std::max(probs[prob_of[i]].k_tol, opt.fp32_switch * w2[i]);
+    launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, ninst, M, K, opt.seed, s);
+    stats.it32 += seed(b32, tol32, maxiter);
+    if (opt.fp32_newton)
+      stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
+                                 newton_total, false, true, &it32_inst);
+    launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
+                                  K, s);
+    OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice, s));
+    OMX_HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < ninst; ++i) {
+      if (hns[i].done == 1) hns[i].done = 0;  // refine in FP64
+      hns[i].k_tol = probs[prob_of[i]].k_tol;
+    }
+  } else {
+    for (int i = 0; i < ninst; ++i) hns[i].k_tol = probs[prob_of[i]].k_tol;
+    launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, ninst, M, K, opt.seed, s);
+    stats.it64 += seed(b64, eig_tol, maxit64);
+  }
+  double max_resid = 0;
+  std::vector<double> resid(ninst, 0.0);
+  const int left = std::max(1, max_eigsolves - newton_total);
+  if (opt.refine && !opt.fp64_only)
+    stats.it64 += refine_phase(*this, b64, hns, bands, msize, eig_tol, left, stats, newton_total, max_resid, &resid);
+  else
+    stats.it64 += newton_phase(*this, b64, hns, bands, nchk, msize, eig_tol, maxit64, left, stats, newton_total, true);
+  // band eigenvectors of every instance, phase-canonicalised together (fields.jl:258-260)
+  multi_evecs.ensure(sizeof(c64) * size_t(ninst) * ld);
+  for (int i = 0; i < ninst; ++i)
+    OMX_HIP_CHECK(hipMemcpyAsync(multi_evecs.as<c64>() + size_t(i) * ld,
+                                 b64.p(b64.X[b64.cur]) + i * b64.istride + long(bands[i]) * ld, sizeof(c64) * K,
+                                 hipMemcpyDeviceToDevice, s));
+  canonicalize_phase(b64.kd.as<double>(), multi_evecs.as<c64>(), ld, ld, ninst, 1);
+  OMX_HIP_CHECK(hipEventRecord(e1, s));
+  for (int p = 0; p < P; ++p)
+    if (probs[p].evecs)
+      for (int j = 0; j < probs[p].nev; ++j)
+        OMX_HIP_CHECK(hipMemcpyAsync(probs[p].evecs + 2 * size_t(j) * K, multi_evecs.as<c64>() + size_t(off[p] + j) * ld,
+                                     sizeof(c64) * K, hipMemcpyDeviceToHost, s));
+  OMX_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  OMX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int p = 0; p < P; ++p) {
+    MultiProblem& q = probs[p];
+    bool ok = true;
+    int newton = 0, it32 = 0;
+    double rmax = 0;
+    for (int j = 0; j < q.nev; ++j) {
+      const int i = off[p] + j;
+      q.kmags[j] = hns[i].k;
+      ok = ok && hns[i].done == 1;
+      newton = std::max(newton, hns[i].nsolves);
+      it32 += it32_inst[i];
+      rmax = std::max(rmax, resid[i]);
+    }
+    q.info.converged = ok ? 1 : 0;
+    q.info.n_newton = newton;
+    q.info.lobpcg_iters_fp32 = it32;
+    q.info.lobpcg_iters_fp64 = stats.it64;  // the batch's refinement steps
+    q.info.n_matvec = stats.matvec;         // the batch's operator applications
+    q.info.n_precon = stats.precon;
+    q.info.max_residual = opt.refine && !opt.fp64_only ? rmax : 0.0;
+    q.info.device_ms = ms;                  // the batch's device time
+    q.status = ok ? OMX_OK : OMX_ERR_NOT_CONVERGED;
+  }
 }
 
 void Solver::require_2d(const char* what) const {

@@ -193,6 +193,19 @@ struct RefineWorkspace {
   DevMem XBf, RBf, Tf;  // FP32 copies for the correction solves (option jd_fp32)
 };
 
+// One problem of a batched solve on a handle's grid (Solver::solve_k_multi).
+struct MultiProblem {
+  const double* epsinv;  // (3,3,Nx,Ny) column-major
+  double omega;
+  int nev;
+  double k_tol, eig_tol;
+  int max_eigsolves, maxiter;
+  double* kmags;         // out: nev
+  double* evecs;         // out: NULL or nev vectors of 2N complex
+  omx_info info;         // out
+  int status;            // out
+};
+
 class Solver {
  public:
   Solver(int device, int nx, int ny, double Dx, double Dy, const double* epsinv);
@@ -206,6 +219,10 @@ class Solver {
   void solve_k(double omega, int nev, double k_tol, double eig_tol, int max_eigsolves, int maxiter,
                const double* kguess, const double* Hguess, int nguess, double* kmags, double* evecs, omx_info* info);
   void get_evecs(double* evecs);
+  // solve_k for several problems on this handle's grid (own eps^-1, omega, nev and tolerances
+  // each) as one batch: every band of every problem is an instance of the same LOBPCG,
+  // Newton and refinement launches (2D cells)
+  void solve_k_multi(std::vector<MultiProblem>& probs);
   void hmh(double k, const double* A, const double* H, int m, double* out);
   void group_index(double omega, int nev, const double* kmags, const double* evecs, const double* deps_domega,
                    double* ng_out);
@@ -256,6 +273,9 @@ class Solver {
   DevMem vec_part, vec_coef;          // partial sums and coefficients of the vector helpers
   RefineWorkspace refine_ws;          // buffers of the FP64 refinement phase
   DevMem field_scratch, argmax_v, argmax_i;  // canonicalize_phase
+  // solve_k_multi: the problems' eps^-1 stacked per precision and layout, the instance ->
+  // field map, and the band eigenvectors of all instances
+  DevMem multi_e32r, multi_e32c, multi_e64r, multi_e64c, multi_slot, multi_evecs;
   Profiler prof;
 
   // LOBPCG workspaces kept between solves (a solve allocates nothing once its shape has
@@ -291,7 +311,7 @@ size_t noalloc_bytes(long N, int m, int prec);
 void apply_helmholtz_noalloc(Solver& s, double k, const double* Hin, double* Hout, int m, int prec);
 
 // Many solve_k problems of uneven grid, omega and mode count (batch.cpp).
-void solve_k_batch(int device, int nprob, omx_problem* probs, int nstreams, int fp64_only);
+void solve_k_batch(int device, int nprob, omx_problem* probs, int max_group, int fp64_only);
 
 // Kottke-smoothed eps^-1 of convex polygons (smooth.hip); no handle needed.
 void smooth_epsinv(int device, int nx, int ny, double Dx, double Dy, int nshape, const int* nverts,

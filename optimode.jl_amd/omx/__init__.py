@@ -341,13 +341,15 @@ def solve_k(omega, epsinv, grid, solver, nev=1, max_eigsolves=60, maxiter=100, k
     return km, evecs
 
 
-def solve_k_batch(problems, nstreams=4, precision="mixed", device=0, fetch_evecs=False):
+def solve_k_batch(problems, precision="mixed", device=0, fetch_evecs=False, max_group=0):
     """Many independent ``solve_k`` problems in one call (``omx_solve_k_batch``): each problem
     is a dict with ``grid``, ``epsinv``, ``omega``, ``nev`` and optionally ``k_tol``,
     ``eig_tol`` (1e-8), ``max_eigsolves`` (60), ``maxiter`` (100).  Problems may differ in
-    grid, ω and nev; ``nstreams`` host workers, each with its own HIP stream, run them
-    concurrently.  Returns one dict per problem: ``kmags``, ``evecs`` (when ``fetch_evecs``),
-    ``info`` and ``status`` (0 = OMX_OK)."""
+    grid, ε⁻¹, ω and nev.  Problems on one grid run as launch groups (``max_group`` problems at
+    most when positive, else as many as fit half the free device memory): every band of
+    every problem of a group is an instance of the same kernel launches.  Returns one dict
+    per problem: ``kmags``, ``evecs`` (when ``fetch_evecs``), ``info`` and ``status``
+    (0 = OMX_OK)."""
     n = len(problems)
     arr = (_lib.OmxProblem * max(1, n))()
     keep = []
@@ -371,7 +373,7 @@ def solve_k_batch(problems, nstreams=4, precision="mixed", device=0, fetch_evecs
         q.evecs_out = _lib.dptr(ev)
         outs.append((km, ev))
     L = _lib.lib()
-    st = L.omx_solve_k_batch(int(device), n, arr, int(nstreams), 1 if precision == "fp64" else 0)
+    st = L.omx_solve_k_batch(int(device), n, arr, int(max_group), 1 if precision == "fp64" else 0)
     if st in (_lib.OMX_ERR_NO_DEVICE, _lib.OMX_ERR_INVALID_ARG) and all(arr[j].status == _lib.OMX_ERR_INTERNAL
                                                                           for j in range(n)):
         _lib.check(st, None)
