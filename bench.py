@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the comparison solves from the other start")
     ap.add_argument("--start", default="auto", choices=["auto", "learned", "plain"],
                     help="start of the timed solves: the shipped warm-start network (auto: when it loads) or plain")
+    ap.add_argument("--tune", default="off", choices=["off", "on"],
+                    help="on: omx_autotune's plan and partitions for the grid (wisdom file) before timing")
     ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "periodic", "batch"],
                     help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain; "
                          "periodic: solve_k on a 128x96x24 3D Bragg-grating cell; batch: the 64-instance "
@@ -356,6 +358,10 @@ def main():
     k_ref, slope_ref = gold["kmags"], gold["dw2dk"]
 
     solver = omx.HIPSolver(precision=a.precision, profile=True)
+    tuned = None
+    if a.tune == "on":
+        from omx import tuning
+        tuned = tuning.autotune(solver, grid, ei, NEV)  # outside the timed region, as FFTW planning
     kw = dict(nev=NEV, k_tol=1e-10, eig_tol=1e-8, max_eigsolves=60, maxiter=200, fetch_evecs=False,
               return_info=True)
     # the timed solves start from the shipped warm-start network (the guess is inside the
@@ -452,6 +458,7 @@ def main():
             "newton_steps_per_step": float(np.mean([i["n_newton"] for i in infos])),
             "matvecs_per_step": float(np.mean([i["n_matvec"] for i in infos])),
             "parallelism": "single GPU; bands batched as independent Newton instances",
+            "autotuned": tuned,
             "start": "learned (omx/data/warmnet_tfln256.npz, guess inside the timed region)" if st is not None
                      else "plain (k_guess, random vectors)",
         },

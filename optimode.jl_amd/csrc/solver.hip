@@ -298,8 +298,10 @@ template <typename T> class Batch {
   int cur = 0;
   int maxiter_alloc;
   std::vector<LState> hst;
+  hipStream_t sm;  // the stream every launch and copy of this batch goes to
 
-  Batch(Solver& s, int ninst_, int M_, int maxiter) : S(s), ninst(ninst_), M(M_), maxiter_alloc(maxiter) {
+  Batch(Solver& s, int ninst_, int M_, int maxiter, hipStream_t stream_ = nullptr)
+      : S(s), ninst(ninst_), M(M_), maxiter_alloc(maxiter), sm(stream_ ? stream_ : s.stream) {
     K = S.K;
     ld = S.ld;
     istride = long(M) * ld;
@@ -389,16 +391,16 @@ template <typename T> class Batch {
   void apply(OpKind op, DevMem& in, DevMem& outm, const int* col0, const int* ncol, double* dots = nullptr) {
     VecSel sel{ninst, M, col0, ncol};
     BView<T> Xv{p(in), ld, istride}, Yv{p(outm), ld, istride};
-    launch_operator<T>(op, S.pd<T>().plan, Xv, Yv, p(D), sel, kd.as<double>(), dots, dps, S.stream);
+    launch_operator<T>(op, S.pd<T>().plan, Xv, Yv, p(D), sel, kd.as<double>(), dots, dps, sm);
   }
   // red_i = A_i^H B_i (pmv x pmv per instance, default pm); ca, cb: upper bounds of the
   // column counts of A and B; bpre: B is the leading columns of A
   void gram(const ColList<T>& A, const ColList<T>& B, int ca, int cb, int pmv = -1, bool bpre = false,
             bool herm = false) {
     const int pg = pmv < 0 ? pm : pmv;
-    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), S.stream, ca, cb, bpre, herm);
+    launch_gram<T>(A, B, ninst, K, nchunk, pg, gramp.as<c64>(), sm, ca, cb, bpre, herm);
     // instances whose A has no columns were skipped by the Gram: skip their reduction too
-    launch_reduce(gramp.as<c64>(), ninst, nchunk, pg * pg, red.as<c64>(), S.stream, A.p[0].s.cnt);
+    launch_reduce(gramp.as<c64>(), ninst, nchunk, pg * pg, red.as<c64>(), sm, A.p[0].s.cnt);
   }
   // R <- R U^-1 twice (CholeskyQR2), R = list of one part with offset/count fields
   void cholqr(DevMem& Rm, const int* off, int cnt_field) {
@@ -406,9 +408,9 @@ template <typename T> class Batch {
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
       gram(Rl, Rl, M, M, -1, true);
-      launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, red.as<c64>(), Cw.as<double>(), S.stream);
+      launch_dense_chol(dl, st.as<LState>(), shape.as<int>(), cnt_field, red.as<c64>(), Cw.as<double>(), sm);
       launch_combine<T>(CMB_PLAIN, Rl, none, coef(Cw, f(cnt_field)), qmax, out(Rm, off), out(Rm, off), out(Rm, off),
-                        ninst, K, nblk, nullptr, S.stream);
+                        ninst, K, nblk, nullptr, sm);
     }
   }
   // ortho!(R, [X P]) (dftk.jl:487-495) as two passes of R <- (R - Z Z^H R) U^{-1}, each
@@ -421,23 +423,43 @@ template <typename T> class Batch {
     ColList<T> none{};
     for (int pass = 0; pass < 2; ++pass) {
       gram(RZ, Rl, 3 * M, M, -1, M <= 16);
-      launch_dense_orth(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), pm, Cw.as<double>(), S.stream);
+      launch_dense_orth(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), pm, Cw.as<double>(), sm);
       launch_combine<T>(CMB_PLAIN, RZ, none, coef(Cw, f(F_NNEW)), qmax, out(Rm, f(F_NEWLY)), out(Rm, f(F_NEWLY)),
-                        out(Rm, f(F_NEWLY)), ninst, K, nblk, nullptr, S.stream);
+                        out(Rm, f(F_NEWLY)), ninst, K, nblk, nullptr, sm);
     }
   }
 
   // Run LOBPCG on every active instance until all finished.  Returns iterations summed
   // over instances; per-instance status in hst.
   // msize[i]: block size of instance i (<= M); tols[i]: its locking threshold.
-  int run(const std::vector<int>& act, const std::vector<int>& bands, const std::vector<int>& ncheck,
-          const std::vector<int>& msize, const std::vector<double>& tols, int maxiter, SolveStats& stats) {
+  // The host reads the per-instance state one iteration late: iteration it is enqueued
+  // before the state after iteration it - 1 is examined, so the device never waits for the
+  // host.  An iteration enqueued after every instance has finished does nothing (every
+  // kernel skips instances with no active columns).  run_begin / run_step / run_end are the
+  // pieces of run, so that two batches on two streams can advance in lockstep (run2).
+  struct OpMark {
+    hipEvent_t a, b;
+  };
+  struct RunState {
+    std::vector<int> act, nact;  // nact: active columns entering the next examined iteration
+    std::vector<LState> init;     // host source of the initial states (kept until the copy ran)
+    std::vector<OpMark> marks;    // operator intervals, bytes attributed on examination
+    int it = 0;
+    int maxiter = 0;
+    bool done = false;
+  };
+  void run_begin(RunState& r, const std::vector<int>& act, const std::vector<int>& bands,
+                 const std::vector<int>& ncheck, const std::vector<int>& msize, const std::vector<double>& tols,
+                 int maxiter, SolveStats& stats) {
     if (maxiter > maxiter_alloc) maxiter = maxiter_alloc;
+    r = RunState{};
+    r.act = act;
+    r.maxiter = maxiter;
     dl.maxiter = maxiter;
     dl.jac_stop = sizeof(T) == 4 ? S.opt.jac_stop32 : 1e-9;
-    std::vector<LState> h(ninst);
+    r.init.assign(ninst, LState{});
     for (int i = 0; i < ninst; ++i) {
-      LState& s = h[i];
+      LState& s = r.init[i];
       std::memset(&s, 0, sizeof(s));
       s.M = msize[i];
       s.ncheck = ncheck[i];
@@ -445,8 +467,7 @@ template <typename T> class Batch {
       s.eps = eps_of<T>();
       s.ortho_tol = 2 * eps_of<T>();
     }
-    hipStream_t sm = S.stream;
-    OMX_HIP_CHECK(hipMemcpyAsync(st.as<void>(), h.data(), sizeof(LState) * ninst, hipMemcpyHostToDevice, sm));
+    OMX_HIP_CHECK(hipMemcpyAsync(st.as<void>(), r.init.data(), sizeof(LState) * ninst, hipMemcpyHostToDevice, sm));
     OMX_HIP_CHECK(hipMemcpyAsync(active.as<void>(), act.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
     OMX_HIP_CHECK(hipMemcpyAsync(band.as<void>(), bands.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
     launch_lobpcg_init(dl, st.as<LState>(), shape.as<int>(), active.as<int>(), band.as<int>(), sm);
@@ -456,107 +477,117 @@ template <typename T> class Batch {
     cholqr(X[cur], nullptr, F_M);
     apply(OP_M, X[cur], AX[cur], f(F_XOFF), f(F_M));
     stats.matvec += ncols_active;
-    ColList<T> none{};
-    // The host reads the per-instance state one iteration late: iteration it is enqueued
-    // before the state after iteration it - 1 is examined, so the device never waits for
-    // the host.  An iteration enqueued after every instance has finished does nothing
-    // (every kernel skips instances with no active columns).
-    std::vector<int> nact(ninst);  // active columns entering the next examined iteration
-    for (int i = 0; i < ninst; ++i) nact[i] = act[i] ? msize[i] : 0;
-    struct OpMark {
-      hipEvent_t a, b;
-    };
-    std::vector<OpMark> marks;  // operator intervals of iterations 1.., bytes attributed on examination
+    r.nact.assign(ninst, 0);
+    for (int i = 0; i < ninst; ++i) r.nact[i] = act[i] ? msize[i] : 0;
+    int na = 0;
+    for (int i = 0; i < ninst; ++i) na += act[i] ? 1 : 0;
+    set_active(na);
+    r.it = 0;
+    r.done = false;
+  }
+
+  // Enqueue iteration r.it, then (from the second on) examine the state after the one
+  // before it.  Sets r.done once every instance has finished.
+  void run_step(RunState& r, SolveStats& stats) {
+    const int it = r.it;
     const bool prof = S.prof.enabled;
+    const int c = cur, n = cur ^ 1;
+    if (it > 0) {
+      hipEvent_t a = prof ? S.prof.begin(sm) : nullptr;
+      apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
+      if (prof) r.marks.push_back(OpMark{a, S.prof.begin(sm)});
+    }
+    ColList<T> Y, AY;
+    if (it == 0) {
+      Y = list({part(X[c], f(F_XOFF), f(F_NACT))});
+      AY = list({part(AX[c], f(F_XOFF), f(F_NACT))});
+    } else {
+      Y = list({part(X[c], f(F_XOFF), f(F_NACT)), part(R[c], f(F_ROFF), f(F_NACT)), part(P[c], nullptr, f(F_NP))});
+      AY = list({part(AX[c], f(F_XOFF), f(F_NACT)), part(AR, f(F_ROFF), f(F_NACT)), part(AP[c], nullptr, f(F_NP))});
+    }
+    // Y^H A Y is Hermitian: upper tiles only (the Rayleigh-Ritz step mirrors them)
+    gram(Y, AY, it == 0 ? M : 3 * M, it == 0 ? M : 3 * M, -1, false, true);
+    launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), Cx.as<double>(), lam.as<double>(),
+                    it == 0 ? 1 : 0, sm);
+    launch_combine<T>(CMB_RESID, Y, AY, coef(Cx, f(F_NACT), lam.as<double>()), qmax, out(X[n], f(F_XOFF)),
+                      out(AX[n], f(F_XOFF)), out(R[n], nullptr, 0), ninst, K, nblk, normp.as<double>(), sm);
+    launch_dense_lock(dl, st.as<LState>(), shape.as<int>(), normp.as<double>(), Cx.as<double>(), Cp.as<double>(),
+                      rhist.as<double>(), sm);
+    apply(OP_P, R[n], R[n], f(F_NEWLY), f(F_NNEW));
+    if (it > 0)
+      launch_combine<T>(CMB_TWO, Y, AY, coef(Cp, f(F_NNEW)), qmax, out(P[n], nullptr, 0), out(AP[n], nullptr, 0),
+                        out(AP[n], nullptr, 0), ninst, K, nblk, nullptr, sm);
+    launch_copy_cols<T>(p(X[n]), p(X[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm, p(AX[n]), p(AX[c]));
+    ortho_r(X[n], P[n], R[n]);
+    launch_advance(dl, st.as<LState>(), shape.as<int>(), sm);
+    cur = n;
+    OMX_HIP_CHECK(hipMemcpyAsync(snap[it & 1].as<void>(), st.as<void>(), sizeof(LState) * ninst,
+                                 hipMemcpyDeviceToHost, sm));
+    OMX_HIP_CHECK(hipEventRecord(snap_ev[it & 1], sm));
+    r.it = it + 1;
+    if (it == 0) return;
+    // examine the state after iteration it - 1
+    OMX_HIP_CHECK(hipEventSynchronize(snap_ev[(it - 1) & 1]));
+    const LState* hs = snap[(it - 1) & 1].as<LState>();
+    bool all = true;
+    long m_next = 0;
+    for (int i = 0; i < ninst; ++i) {
+      if (!r.act[i]) continue;
+      if (r.nact[i] > 0) {
+        stats.precon += r.nact[i];
+        if (it - 1 > 0) stats.matvec += r.nact[i];
+      }
+      if (!hs[i].finished) all = false;
+      r.nact[i] = hs[i].finished ? 0 : hs[i].nact;
+      m_next += r.nact[i];
+    }
     {
       int na = 0;
-      for (int i = 0; i < ninst; ++i) na += act[i] ? 1 : 0;
+      for (int i = 0; i < ninst; ++i) na += r.nact[i] > 0 ? 1 : 0;
       set_active(na);
     }
-    int it = 0;
-    for (;; ++it) {
-      const int c = cur, n = cur ^ 1;
-      if (it > 0) {
-        hipEvent_t a = prof ? S.prof.begin(sm) : nullptr;
-        apply(OP_M, R[c], AR, f(F_ROFF), f(F_NACT));
-        if (prof) marks.push_back(OpMark{a, S.prof.begin(sm)});
-      }
-      ColList<T> Y, AY;
-      if (it == 0) {
-        Y = list({part(X[c], f(F_XOFF), f(F_NACT))});
-        AY = list({part(AX[c], f(F_XOFF), f(F_NACT))});
-      } else {
-        Y = list({part(X[c], f(F_XOFF), f(F_NACT)), part(R[c], f(F_ROFF), f(F_NACT)), part(P[c], nullptr, f(F_NP))});
-        AY = list({part(AX[c], f(F_XOFF), f(F_NACT)), part(AR, f(F_ROFF), f(F_NACT)), part(AP[c], nullptr, f(F_NP))});
-      }
-      // Y^H A Y is Hermitian: upper tiles only (the Rayleigh-Ritz step mirrors them)
-      gram(Y, AY, it == 0 ? M : 3 * M, it == 0 ? M : 3 * M, -1, false, true);
-      launch_dense_rr(dl, st.as<LState>(), shape.as<int>(), red.as<c64>(), Cx.as<double>(), lam.as<double>(),
-                      it == 0 ? 1 : 0, sm);
-      launch_combine<T>(CMB_RESID, Y, AY, coef(Cx, f(F_NACT), lam.as<double>()), qmax, out(X[n], f(F_XOFF)),
-                        out(AX[n], f(F_XOFF)), out(R[n], nullptr, 0), ninst, K, nblk, normp.as<double>(), sm);
-      launch_dense_lock(dl, st.as<LState>(), shape.as<int>(), normp.as<double>(), Cx.as<double>(), Cp.as<double>(),
-                        rhist.as<double>(), sm);
-      apply(OP_P, R[n], R[n], f(F_NEWLY), f(F_NNEW));
-      if (it > 0)
-        launch_combine<T>(CMB_TWO, Y, AY, coef(Cp, f(F_NNEW)), qmax, out(P[n], nullptr, 0), out(AP[n], nullptr, 0),
-                          out(AP[n], nullptr, 0), ninst, K, nblk, nullptr, sm);
-      launch_copy_cols<T>(p(X[n]), p(X[c]), ld, istride, ninst, f(F_XOFF), f(F_NCOPY), K, sm, p(AX[n]), p(AX[c]));
-      ortho_r(X[n], P[n], R[n]);
-      launch_advance(dl, st.as<LState>(), shape.as<int>(), sm);
-      cur = n;
-      OMX_HIP_CHECK(hipMemcpyAsync(snap[it & 1].as<void>(), st.as<void>(), sizeof(LState) * ninst,
-                                   hipMemcpyDeviceToHost, sm));
-      OMX_HIP_CHECK(hipEventRecord(snap_ev[it & 1], sm));
-      if (it == 0) continue;
-      // examine the state after iteration it - 1
-      OMX_HIP_CHECK(hipEventSynchronize(snap_ev[(it - 1) & 1]));
-      const LState* hs = snap[(it - 1) & 1].as<LState>();
-      bool all = true;
-      long m_next = 0;
-      for (int i = 0; i < ninst; ++i) {
-        if (!act[i]) continue;
-        if (nact[i] > 0) {
-          stats.precon += nact[i];
-          if (it - 1 > 0) stats.matvec += nact[i];
-        }
-        if (!hs[i].finished) all = false;
-        nact[i] = hs[i].finished ? 0 : hs[i].nact;
-        m_next += nact[i];
-      }
-      {
-        int na = 0;
-        for (int i = 0; i < ninst; ++i) na += nact[i] > 0 ? 1 : 0;
-        set_active(na);
-      }
-      if (prof) {
-        // iteration it applied the operator to the columns active after iteration it - 1
-        const double w = double(sizeof(T)) / 4.0;
-        const OpMark& mk = marks.back();
-        if (m_next > 0)
-          S.prof.commit(mk.a, mk.b, sizeof(T) == 4 ? &S.prof.op32 : &S.prof.op64,
-                        w * (32.0 * S.N * m_next + 24.0 * S.N));
-        else
-          S.prof.recycle(mk.a, mk.b);
-      }
-      if (all) break;
-      if (it > maxiter + 2) throw Error(OMX_ERR_INTERNAL, "LOBPCG driver did not terminate");
+    if (prof) {
+      // iteration it applied the operator to the columns active after iteration it - 1
+      const double w = double(sizeof(T)) / 4.0;
+      const OpMark& mk = r.marks.back();
+      if (m_next > 0)
+        S.prof.commit(mk.a, mk.b, sizeof(T) == 4 ? &S.prof.op32 : &S.prof.op64,
+                      w * (32.0 * S.N * m_next + 24.0 * S.N));
+      else
+        S.prof.recycle(mk.a, mk.b);
     }
-    // iteration it changed nothing: the state after it equals the state after it - 1
-    OMX_HIP_CHECK(hipEventSynchronize(snap_ev[it & 1]));
-    std::memcpy(hst.data(), snap[it & 1].as<LState>(), sizeof(LState) * ninst);
+    if (all) {
+      r.done = true;
+      return;
+    }
+    if (it > r.maxiter + 2) throw Error(OMX_ERR_INTERNAL, "LOBPCG driver did not terminate");
+  }
+
+  // The final states (the iteration enqueued last changed nothing); returns iterations
+  // summed over the active instances.
+  int run_end(RunState& r) {
+    const int last = r.it - 1;
+    OMX_HIP_CHECK(hipEventSynchronize(snap_ev[last & 1]));
+    std::memcpy(hst.data(), snap[last & 1].as<LState>(), sizeof(LState) * ninst);
     int tot = 0;
     for (int i = 0; i < ninst; ++i)
-      if (act[i]) tot += hst[i].niter;
+      if (r.act[i]) tot += hst[i].niter;
     for (int i = 0; i < ninst; ++i)
-      if (act[i] && hst[i].status == OMX_ERR_CHOLESKY)
+      if (r.act[i] && hst[i].status == OMX_ERR_CHOLESKY)
         throw Error(OMX_ERR_CHOLESKY, "Cholesky orthogonalisation failed even after shifting");
     return tot;
   }
 
+  int run(const std::vector<int>& act, const std::vector<int>& bands, const std::vector<int>& ncheck,
+          const std::vector<int>& msize, const std::vector<double>& tols, int maxiter, SolveStats& stats) {
+    RunState r;
+    run_begin(r, act, bands, ncheck, msize, tols, maxiter, stats);
+    while (!r.done) run_step(r, stats);
+    return run_end(r);
+  }
+
   // One Newton update for every active instance from its band's Rayleigh quotient and HMkH.
   void newton(std::vector<NewtonState>& hns) {
-    hipStream_t sm = S.stream;
     OMX_HIP_CHECK(hipMemcpyAsync(ns.as<void>(), hns.data(), sizeof(NewtonState) * ninst, hipMemcpyHostToDevice, sm));
     ColList<T> a = list({part(X[cur], f(F_EIG), f(F_ONE))});
     ColList<T> b = list({part(AX[cur], f(F_EIG), f(F_ONE))});
@@ -573,12 +604,12 @@ template <typename T> class Batch {
     std::vector<int> c0(ninst, 0), nc(ninst, 0);
     nc[inst] = ncol;
     DevMem &dc0 = selc0, &dnc = selnc;
-    OMX_HIP_CHECK(hipMemcpyAsync(dc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
-    OMX_HIP_CHECK(hipMemcpyAsync(dnc.as<void>(), nc.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(dc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
+    OMX_HIP_CHECK(hipMemcpyAsync(dnc.as<void>(), nc.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
     apply(OP_MKH, X[cur], X[cur], dc0.as<int>(), dnc.as<int>(), dotp.as<double>());
     std::vector<double> h(size_t(ninst) * M * dps);
-    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, S.stream));
-    OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, sm));
+    OMX_HIP_CHECK(hipStreamSynchronize(sm));
     std::vector<double> out(ncol);
     for (int j = 0; j < ncol; ++j) {
       double acc = 0;
@@ -591,12 +622,12 @@ template <typename T> class Batch {
   // HMkH of the first ncol[i] columns of every instance i (out[i * M + j]), summed on the host.
   std::vector<double> hmkh_multi(const std::vector<int>& ncol) {
     std::vector<int> c0(ninst, 0);
-    OMX_HIP_CHECK(hipMemcpyAsync(selc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
-    OMX_HIP_CHECK(hipMemcpyAsync(selnc.as<void>(), ncol.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(selc0.as<void>(), c0.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
+    OMX_HIP_CHECK(hipMemcpyAsync(selnc.as<void>(), ncol.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, sm));
     apply(OP_MKH, X[cur], X[cur], selc0.as<int>(), selnc.as<int>(), dotp.as<double>());
     std::vector<double> h(size_t(ninst) * M * dps);
-    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, S.stream));
-    OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), dotp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, sm));
+    OMX_HIP_CHECK(hipStreamSynchronize(sm));
     std::vector<double> out(size_t(ninst) * M, 0.0);
     for (int i = 0; i < ninst; ++i)
       for (int j = 0; j < ncol[i]; ++j) {
@@ -613,8 +644,8 @@ template <typename T> class Batch {
     ColList<T> b = list({part(AX[cur], nullptr, nullptr, 0, M)});
     gram(a, b, M, M);
     std::vector<c64> h(size_t(ninst) * pm * pm);
-    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), red.as<void>(), sizeof(c64) * h.size(), hipMemcpyDeviceToHost, S.stream));
-    OMX_HIP_CHECK(hipStreamSynchronize(S.stream));
+    OMX_HIP_CHECK(hipMemcpyAsync(h.data(), red.as<void>(), sizeof(c64) * h.size(), hipMemcpyDeviceToHost, sm));
+    OMX_HIP_CHECK(hipStreamSynchronize(sm));
     std::vector<double> out(size_t(ninst) * M);
     for (int i = 0; i < ninst; ++i)
       for (int j = 0; j < M; ++j) out[size_t(i) * M + j] = h[size_t(i) * pm * pm + j * pm + j].x;
@@ -650,6 +681,7 @@ Solver::Solver(int dev, int nx_, int ny_, int nz_, double Dx_, double Dy_, doubl
   OMX_HIP_CHECK(hipSetDevice(dev));
   noalloc_init(dev);
   OMX_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  OMX_HIP_CHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
   N = long(nx) * ny * nz;
   K = 2 * N;
   ld = (K + 63) / 64 * 64;
@@ -659,6 +691,11 @@ Solver::Solver(int dev, int nx_, int ny_, int nz_, double Dx_, double Dy_, doubl
 }
 
 Solver::~Solver() {
+  workspaces.clear();  // batches hold events on the streams
+  if (stream2) {
+    (void)hipStreamSynchronize(stream2);
+    (void)hipStreamDestroy(stream2);
+  }
   if (stream) {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
@@ -719,6 +756,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_maxit = int(v);
   else if (key == "jd_fp32")
     opt.jd_fp32 = v != 0 ? 1 : 0;
+  else if (key == "streams2")
+    opt.streams2 = v != 0 ? 1 : 0;
   else if (key == "graphs")
     opt.graphs = v != 0 ? 1 : 0;
   else if (key == "jac_stop32")
@@ -1074,11 +1113,11 @@ namespace {
 // The caller holds the returned pointer for as long as it uses the batch: a later request
 // in the same solve may drop the entry from the cache, and the batch then lives until the
 // caller lets it go.
-template <typename T> std::shared_ptr<Batch<T>> workspace(Solver& S, int ninst, int M, int maxiter) {
+template <typename T> std::shared_ptr<Batch<T>> workspace(Solver& S, int ninst, int M, int maxiter, int tag = 0) {
   const int tb = int(sizeof(T));
   for (size_t j = 0; j < S.workspaces.size(); ++j) {
     const Solver::Workspace& w = S.workspaces[j];
-    if (w.tbytes == tb && w.ninst == ninst && w.M == M && w.maxiter >= maxiter) {
+    if (w.tbytes == tb && w.ninst == ninst && w.M == M && w.maxiter >= maxiter && w.tag == tag) {
       Solver::Workspace hit = w;
       S.workspaces.erase(S.workspaces.begin() + long(j));
       S.workspaces.push_back(hit);
@@ -1088,9 +1127,10 @@ template <typename T> std::shared_ptr<Batch<T>> workspace(Solver& S, int ninst, 
     }
   }
   if (S.workspaces.size() >= 4) S.workspaces.erase(S.workspaces.begin());
-  std::shared_ptr<void> p(new Batch<T>(S, ninst, M, maxiter), [](void* q) { delete static_cast<Batch<T>*>(q); });
+  std::shared_ptr<void> p(new Batch<T>(S, ninst, M, maxiter, tag == 1 ? S.stream2 : nullptr),
+                          [](void* q) { delete static_cast<Batch<T>*>(q); });
   const size_t bytes = size_t(2 * tb) * size_t(ninst) * M * (11 * size_t(S.ld) + 3 * size_t(S.N));
-  S.workspaces.push_back(Solver::Workspace{tb, ninst, M, maxiter, p, bytes});
+  S.workspaces.push_back(Solver::Workspace{tb, ninst, M, maxiter, tag, p, bytes});
   return std::shared_ptr<Batch<T>>(p, static_cast<Batch<T>*>(p.get()));
 }
 
@@ -1269,6 +1309,61 @@ int newton_phase(Solver& S, Batch<T>& b, std::vector<NewtonState>& hns, const st
                  int max_steps, SolveStats& stats, int& newton_total, bool strict, bool inexact = true) {
   return newton_phase(S, b, hns, bands, nchk, msize, std::vector<double>(size_t(b.ninst), tol_floor), maxiter,
                       max_steps, stats, newton_total, strict, inexact);
+}
+
+// newton_phase over the bands split into two batches on their own streams (instances
+// [0, A.ninst) in A, the rest in B): the two LOBPCG runs of a Newton step advance in
+// lockstep, iteration by iteration, so that the latency-bound dense steps of one batch
+// overlap the memory-bound kernels of the other.  Same updates as newton_phase.
+template <typename T>
+int newton_phase2(Solver& S, Batch<T>& A, Batch<T>& B, std::vector<NewtonState>& hns, const std::vector<int>& bands,
+                  const std::vector<int>& nchk, const std::vector<int>& msize, const std::vector<double>& floors,
+                  int maxiter, int max_steps, SolveStats& stats, int& newton_total) {
+  const int na = A.ninst, nb = B.ninst;
+  auto part = [](const auto& v, int off, int n) {
+    return std::vector<typename std::decay<decltype(v[0])>::type>(v.begin() + off, v.begin() + off + n);
+  };
+  const auto bA = part(bands, 0, na), bB = part(bands, na, nb), cA = part(nchk, 0, na), cB = part(nchk, na, nb);
+  const auto mA = part(msize, 0, na), mB = part(msize, na, nb);
+  int iters = 0;
+  for (int step = 0; step < max_steps; ++step) {
+    std::vector<int> act(na + nb);
+    std::vector<double> tols(na + nb);
+    bool any = false;
+    for (int i = 0; i < na + nb; ++i) {
+      act[i] = hns[i].done ? 0 : 1;
+      any = any || act[i];
+      const double f = std::fabs(hns[i].f);
+      tols[i] = (hns[i].nsolves > 0 && std::isfinite(f)) ? std::max(floors[i], 0.1 * f) : floors[i];
+    }
+    if (!any) break;
+    const auto aA = part(act, 0, na), aB = part(act, na, nb);
+    const bool runA = std::count(aA.begin(), aA.end(), 1) > 0, runB = std::count(aB.begin(), aB.end(), 1) > 0;
+    typename Batch<T>::RunState ra, rb;
+    if (runA) A.run_begin(ra, aA, bA, cA, mA, part(tols, 0, na), maxiter, stats);
+    if (runB) B.run_begin(rb, aB, bB, cB, mB, part(tols, na, nb), maxiter, stats);
+    while ((runA && !ra.done) || (runB && !rb.done)) {
+      if (runA && !ra.done) A.run_step(ra, stats);
+      if (runB && !rb.done) B.run_step(rb, stats);
+    }
+    if (runA) iters += A.run_end(ra);
+    if (runB) iters += B.run_end(rb);
+    std::vector<NewtonState> hA = part(hns, 0, na), hB = part(hns, na, nb);
+    if (runA) A.newton(hA);
+    if (runB) B.newton(hB);
+    std::copy(hA.begin(), hA.end(), hns.begin());
+    std::copy(hB.begin(), hB.end(), hns.begin() + na);
+    ++newton_total;
+    if (S.trace) {
+      std::fprintf(stderr, "[omx] fp32 newton step %d (two streams):", step);
+      for (int i = 0; i < na + nb; ++i)
+        if (act[i])
+          std::fprintf(stderr, " b%d(it %d tol %.0e f %.1e)", i, (i < na ? A.hst[i] : B.hst[i - na]).niter, tols[i],
+                       hns[i].f);
+      std::fprintf(stderr, "\n");
+    }
+  }
+  return iters;
 }
 
 using cd = std::complex<double>;
@@ -1814,25 +1909,82 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
         if (i != b) hns[i] = keep[i];
     }
   } else if (!opt.fp64_only) {
-    const auto b32p = workspace<float>(*this, ninst, M, maxiter);
+    // two half-batches of bands on two streams (the seed eigensolve runs in the first)
+    const int na = (ninst + 1) / 2, nb = ninst - na;
+    const bool split = opt.streams2 && nb > 0 && !warm && !banded && opt.fp32_newton && stream2;
+    const auto b32p = workspace<float>(*this, split ? na : ninst, M, maxiter);
     Batch<float>& b32 = *b32p;
+    const std::shared_ptr<Batch<float>> b32q = split ? workspace<float>(*this, nb, M, maxiter, 1) : nullptr;
     for (auto& s : hns) s.k_tol = std::max(k_tol, opt.fp32_switch * w2);
     if (warm) {
       warm_block(b32);
     } else if (banded) {
       guided_block(b32);
+    } else if (split) {
+      Batch<float>& B = *b32q;
+      launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
+      if (nguess_used) put_guess(b32, 0, 1);
+      // the shared first eigensolve in instance 0 of the first half, as seed_bands
+      std::vector<int> act(na, 0), n0(na, M), m0(na, M), bA(bands.begin(), bands.begin() + na);
+      act[0] = 1;
+      std::vector<double> t0(na, std::max(tol32, 3e-3 * w2)), kh(ninst, k0);
+      OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * na, hipMemcpyHostToDevice, stream));
+      stats.it32 += b32.run(act, bA, n0, m0, t0, maxiter, stats);
+      const auto lam = b32.rayleigh();
+      const auto hk = b32.hmkh_cols(0, M);
+      for (int i = 0; i < ninst; ++i) {
+        const double f = lam[i] - w2, slope = 2.0 * hk[i];
+        hns[i].f = f;
+        hns[i].lam = lam[i];
+        hns[i].slope = slope;
+        hns[i].nsolves = 1;
+        if (std::fabs(f) <= hns[i].k_tol) {
+          hns[i].done = 1;
+        } else {
+          const double kn = k0 - f / slope;
+          hns[i].k = std::isfinite(kn) && kn > 0 ? kn : k0;
+        }
+        kh[i] = hns[i].k;
+      }
+      const cplx<float>* x0 = b32.p(b32.X[b32.cur]);
+      for (int i = 1; i < na; ++i)
+        OMX_HIP_CHECK(hipMemcpyAsync(b32.p(b32.X[b32.cur]) + i * b32.istride, x0, sizeof(c32) * b32.istride,
+                                     hipMemcpyDeviceToDevice, stream));
+      B.cur = 0;
+      for (int i = 0; i < nb; ++i)
+        OMX_HIP_CHECK(hipMemcpyAsync(B.p(B.X[0]) + i * B.istride, x0, sizeof(c32) * B.istride, hipMemcpyDeviceToDevice,
+                                     stream));
+      OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * na, hipMemcpyHostToDevice, stream));
+      OMX_HIP_CHECK(hipMemcpyAsync(B.kd.as<void>(), kh.data() + na, sizeof(double) * nb, hipMemcpyHostToDevice, stream));
+      OMX_HIP_CHECK(hipStreamSynchronize(stream));  // the second stream starts from these
+      ++newton_total;
     } else {
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) put_guess(b32, 0, 1);
       stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
     }
-    if (opt.fp32_newton)
-      stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
-                                 newton_total, false);
-    launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
-                                  K, stream);
-    OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,
-                                 stream));
+    if (split) {
+      stats.it32 += newton_phase2(*this, b32, *b32q, hns, bands, nchk, msize, std::vector<double>(ninst, tol32), maxiter,
+                                  max_eigsolves - 1, stats, newton_total);
+      OMX_HIP_CHECK(hipStreamSynchronize(stream2));
+      Batch<float>& B = *b32q;
+      launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, na, M, K,
+                                    stream);
+      launch_convert<float, double>(B.p(B.X[B.cur]), ld, B.istride, b64.p(b64.X[0]) + na * b64.istride, ld, b64.istride,
+                                    nb, M, K, stream);
+      OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * na, hipMemcpyDeviceToDevice,
+                                   stream));
+      OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<double>() + na, B.kd.as<void>(), sizeof(double) * nb,
+                                   hipMemcpyDeviceToDevice, stream));
+    } else {
+      if (opt.fp32_newton)
+        stats.it32 += newton_phase(*this, b32, hns, bands, nchk, msize, tol32, maxiter, max_eigsolves - 1, stats,
+                                   newton_total, false);
+      launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, ninst, M,
+                                    K, stream);
+      OMX_HIP_CHECK(hipMemcpyAsync(b64.kd.as<void>(), b32.kd.as<void>(), sizeof(double) * ninst, hipMemcpyDeviceToDevice,
+                                   stream));
+    }
     OMX_HIP_CHECK(hipStreamSynchronize(stream));
     t_fp32_end = std::chrono::steady_clock::now();
     for (auto& s : hns) {

@@ -172,6 +172,9 @@ struct Options {
                                 // where the power of two is even (4, 4, 4, 4 for 256)
   int tile_rows32 = 0;          // rows per block of passes 1 and 3 (0: the largest that fits)
   int tile_cols32 = 0;          // columns per block of pass 2 (0: the largest that fits)
+  int streams2 = 1;             // FP32 Newton phase of solve_k as two half-batches of bands on two
+                                // streams in lockstep (one half's latency-bound dense steps overlap
+                                // the other half's memory-bound kernels)
   int graphs = 0;               // MINRES iterations replayed from a captured HIP graph (groups of 3;
                                 // same results, no measured gain: the gaps are GPU-side)
 };
@@ -250,6 +253,7 @@ class Solver {
   // shared state, public for the templated engine
   int device;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // second stream of the two-stream FP32 Newton phase
   int nx, ny;
   double Dx, Dy;
   int nz = 1;       // planes along z (1: a 2D cell)
@@ -281,7 +285,7 @@ class Solver {
   // LOBPCG workspaces kept between solves (a solve allocates nothing once its shape has
   // been seen): type-erased batches keyed by (precision bytes, instances, block, maxiter).
   struct Workspace {
-    int tbytes, ninst, M, maxiter;
+    int tbytes, ninst, M, maxiter, tag;  // tag 1: a batch on stream2
     std::shared_ptr<void> batch;
     size_t bytes;  // device memory the batch holds
   };
