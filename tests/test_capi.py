@@ -88,8 +88,8 @@ def test_tensor_field_round_trip():
     assert np.array_equal(omx._tensor_back(flat, shape), e)
 
 
-@pytest.mark.parametrize("args", [["--gpus", "2"], ["--workload", "batch"]])
-def test_bench_refuses_multi_gpu_and_disabled_workloads(args):
+@pytest.mark.parametrize("args", [["--gpus", "2"], ["--gpus", "8"]])
+def test_bench_refuses_multi_gpu(args):
     import subprocess
     import sys
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, capture_output=True, text=True,
