@@ -188,6 +188,24 @@ int omx_build_modes(omx_solver* h, int slot, int nev, const double* kmags, const
    conjugated when conjugate != 0; na x nb complex, column-major. */
 int omx_overlap_matrix(omx_solver* h, int slot_a, int slot_b, int conjugate, double* out);
 
+/* Pull-back of overlap_matrix (modes.jl:156-170, the reverse-mode rule the reference takes
+   from Zygote for it): given the cotangent obar (na x nb complex, column-major) of
+   O = overlap_matrix(set slot_a, set slot_b; conjugate), add the field cotangents to the sets'
+   device accumulators -- of set a's E when which & 1, of set b's H when which & 2.  Cotangent
+   convention: a real scalar L of a complex array Z changes by Re sum conj(Zbar) dZ.  The
+   accumulators start at zero after omx_build_modes and after omx_mode_pullback. */
+int omx_overlap_pullback(omx_solver* h, int slot_a, int slot_b, int conjugate, const double* obar, int which);
+
+/* Pull-back of build_mode (modes.jl:116-122: mode_fields :105-113 and the power
+   normalisation) for the nev modes of a set, from its accumulated field cotangents (which
+   are then cleared): kbar_out[j] (through the plane-wave basis' dependence on k_j),
+   evbar_out (nev vectors of 2N complex: the eigenvector cotangents omx_solve_k_adjoint
+   takes) and epsinvbar_out ((3,3,Nx,Ny) column-major: the direct dependence of
+   E = i eps^-1 fft(kx_tc(ev)) on eps^-1, summed over the modes, each entry its own
+   derivative as the contraction eps^-1_dot's rule gives it).  The handle's eps^-1 must be
+   the one the set was built with. */
+int omx_mode_pullback(omx_solver* h, int slot, double* kbar_out, double* evbar_out, double* epsinvbar_out);
+
 /* The fields of mode set `slot`: E_out, H_out (either may be NULL) receive nev fields of
    3N complex each, (3,Nx,Ny) column-major, as Mode.E / Mode.H hold them. */
 int omx_get_mode_fields(omx_solver* h, int slot, double* E_out, double* H_out);

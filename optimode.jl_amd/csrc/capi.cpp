@@ -252,6 +252,14 @@ int omx_overlap_matrix(omx_solver* h, int slot_a, int slot_b, int conjugate, dou
   return guarded(h, [&](omx::Solver& s) { s.overlap(slot_a, slot_b, conjugate, out); });
 }
 
+int omx_overlap_pullback(omx_solver* h, int slot_a, int slot_b, int conjugate, const double* obar, int which) {
+  return guarded(h, [&](omx::Solver& s) { s.overlap_pullback(slot_a, slot_b, conjugate, obar, which); });
+}
+
+int omx_mode_pullback(omx_solver* h, int slot, double* kbar_out, double* evbar_out, double* epsinvbar_out) {
+  return guarded(h, [&](omx::Solver& s) { s.mode_pullback(slot, kbar_out, evbar_out, epsinvbar_out); });
+}
+
 int omx_get_mode_fields(omx_solver* h, int slot, double* E_out, double* H_out) {
   return guarded(h, [&](omx::Solver& s) { s.get_mode_fields(slot, E_out, H_out); });
 }

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass2_fwd(OpPlan<T> plan, cplx<T>*
 
 // Inverse path of _d2ẽ! (fields.jl:88-93): e = bfft2(eps^-1 . d) * scale, in place in D.
 // Column pass: eps^-1 . d and inverse FFT along y.
-template <typename T, int VPT>
+template <typename T, int VPT, bool EPS>
 __global__ __launch_bounds__(OP_BLOCK) void k_pass2_inv(OpPlan<T> plan, cplx<T>* D, VecSel sel, T scale) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -284,11 +284,17 @@ __global__ __launch_bounds__(OP_BLOCK) void k_pass2_inv(OpPlan<T> plan, cplx<T>*
     const int t = e - iy * TX;
     const long o = (long(iy) * nx + ix0 + t) * 3;
     const cplx<T> d0 = Dv[o], d1 = Dv[o + 1], d2 = Dv[o + 2];
-    const T* ep = eps_rowm(plan, inst) + (long(iy) * nx + ix0 + t) * 6;
-    const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
-    buf[(t * 3 + 0) * ny + iy] = (d0 * exx + d1 * exy + d2 * exz) * scale;
-    buf[(t * 3 + 1) * ny + iy] = (d0 * exy + d1 * eyy + d2 * eyz) * scale;
-    buf[(t * 3 + 2) * ny + iy] = (d0 * exz + d1 * eyz + d2 * ezz) * scale;
+    if (EPS) {
+      const T* ep = eps_rowm(plan, inst) + (long(iy) * nx + ix0 + t) * 6;
+      const T exx = ep[0], eyy = ep[1], ezz = ep[2], exy = ep[3], exz = ep[4], eyz = ep[5];
+      buf[(t * 3 + 0) * ny + iy] = (d0 * exx + d1 * exy + d2 * exz) * scale;
+      buf[(t * 3 + 1) * ny + iy] = (d0 * exy + d1 * eyy + d2 * eyz) * scale;
+      buf[(t * 3 + 2) * ny + iy] = (d0 * exz + d1 * eyz + d2 * ezz) * scale;
+    } else {
+      buf[(t * 3 + 0) * ny + iy] = d0 * scale;
+      buf[(t * 3 + 1) * ny + iy] = d1 * scale;
+      buf[(t * 3 + 2) * ny + iy] = d2 * scale;
+    }
   }
   __syncthreads();
   fft_lds<T, +1, VPT>(buf, 3 * TX, ny, 1, plan.fy, tw);
@@ -655,7 +661,10 @@ void launch_fields_vpt(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* 
     else
       hipLaunchKernelGGL((k_pass2_fwd<T, VPT, false>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
   } else {
-    hipLaunchKernelGGL((k_pass2_inv<T, VPT>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
+    if (fop == FIELD_EINV)
+      hipLaunchKernelGGL((k_pass2_inv<T, VPT, true>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
+    else
+      hipLaunchKernelGGL((k_pass2_inv<T, VPT, false>), g2, dim3(OP_BLOCK), lds2, stream, plan, D, sel, scale);
     hipLaunchKernelGGL((k_pass3_inv<T, VPT>), g1, dim3(OP_BLOCK), lds13, stream, plan, D, sel);
   }
 }

@@ -141,9 +141,10 @@ void launch_operator(OpKind op, const OpPlan<T>& plan, BView<T> X, BView<T> Y, c
 //   FIELD_DK: D = fft2(d kx_tc/dk (H)) * scale = -fft2(zx_tc(H)) * scale
 //   FIELD_DKK: D = fft2(d^2 kx_tc/dk^2 (H)) * scale = fft2(-H1 n |G|^2 / |k+G|^3) * scale
 //   FIELD_H: D = fft2(tc(H)) * scale = fft2(H1 m + H2 n) * scale   (modes.jl:112)
+//   FIELD_BFFT: D <- bfft2(D) * scale, X unused (2D cells)
 // (the k-derivatives are taken at fixed transverse coefficients H, with the basis
 // m(k), n, |k+G|(k) of basis_at)
-enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2, FIELD_DK = 3, FIELD_DKK = 4, FIELD_H = 5 };
+enum FieldOp : int { FIELD_E = 0, FIELD_D = 1, FIELD_EINV = 2, FIELD_DK = 3, FIELD_DKK = 4, FIELD_H = 5, FIELD_BFFT = 6 };
 template <typename T>
 void launch_fields(FieldOp fop, const OpPlan<T>& plan, BView<T> X, cplx<T>* D, const VecSel& sel,
                    const double* kinst, double scale, hipStream_t stream);

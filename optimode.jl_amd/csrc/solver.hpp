@@ -185,6 +185,14 @@ constexpr int MODE_SLOTS = 4;
 struct ModeSlot {
   int nev = 0;
   DevMem E, H;
+  // for the pull-back (omx_mode_pullback): the eigenvectors (nev columns of ld), their k,
+  // the norms the fields were divided by, the inner-product convention, and the field
+  // cotangents accumulated by omx_overlap_pullback (planar as E and H; bar_live = 0: zero)
+  DevMem ev, kd, Ebar, Hbar;
+  std::vector<c64> nrm;
+  int conj = 0;
+  int normalized = 1;
+  int bar_live = 0;
 };
 
 // Buffers of the FP64 eigenpair refinement, kept on the handle between solves (grown on
@@ -239,6 +247,9 @@ class Solver {
                    double* norms_out);
   void overlap(int slot_a, int slot_b, int conjugate, double* out);
   void get_mode_fields(int slot, double* E_out, double* H_out);
+  // pull-backs of overlap_matrix and build_mode (eme.hip)
+  void overlap_pullback(int slot_a, int slot_b, int conjugate, const double* obar, int which);
+  void mode_pullback(int slot, double* kbar_out, double* evbar_out, double* epsinvbar_out);
   void set_option(const std::string& key, double value);
   // timed trials of the operator path, FFT radix variant, Stockham tile shape and the Gram /
   // block-update partitions for nev-mode solves; the fastest become the handle's options

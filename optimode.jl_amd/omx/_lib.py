@@ -75,6 +75,8 @@ SIGNATURES = [
     ("omx_last_error", ctypes.c_char_p, [_vp]),
     ("omx_set_option", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
     ("omx_apply_helmholtz", ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+    ("omx_overlap_pullback", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]),
+    ("omx_mode_pullback", ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
     ("omx_apply_helmholtz_noalloc", ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int]),
     ("omx_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
     ("omx_apply_preconditioner", ctypes.c_int, [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int]),

@@ -16,7 +16,8 @@ import numpy as np
 from . import _lib, solve_k, OmxError
 
 __all__ = ["build_modes", "overlap_matrix", "mode_fields", "SMat", "reg_solve", "enforce_passivity",
-           "interface_smatrix", "propagation_smatrix", "star", "cascade", "EMEResult", "eme", "power_coupling"]
+           "interface_smatrix", "propagation_smatrix", "star", "cascade", "EMEResult", "eme", "power_coupling",
+           "assemble_pullback", "overlap_pullback", "mode_pullback", "power_coupling_objective", "eme_gradient"]
 
 
 # ---------------------------------------------------------------------------------------
@@ -213,3 +214,294 @@ def power_coupling(result, in_mode=1, out_mode=1):
     """``|S₂₁[out, in]|²`` (``eme.jl:140-149``), 1-based mode numbers as in the reference."""
     S = result.S if isinstance(result, EMEResult) else result
     return float(np.abs(S.transmission()[out_mode - 1, in_mode - 1]) ** 2)
+
+
+# ---------------------------------------------------------------------------------------
+# reverse mode of the S-matrix algebra (host)
+# ---------------------------------------------------------------------------------------
+# The reference differentiates the cascade with Zygote (eme.jl:80-88: every cell's solve_k
+# rrule, then plain differentiable linear algebra).  Here the nev x nev algebra records its
+# operations on a small tape and replays them backwards, with Zygote's convention for
+# complex cotangents: a real scalar L of a complex array Z changes by Re Σ conj(Z̄) dZ.
+
+class _Tape:
+    """Reverse-mode recorder for complex matrix expressions (products, solves, sums,
+    transposes, blocks, elementwise phases)."""
+
+    def __init__(self):
+        self.ops = []      # (output id, backward fn)
+        self.vals = []
+        self.grads = {}
+
+    def var(self, value):
+        self.vals.append(np.asarray(value, dtype=np.complex128))
+        return len(self.vals) - 1
+
+    def _op(self, value, back):
+        i = self.var(value)
+        self.ops.append((i, back))
+        return i
+
+    def v(self, i):
+        return self.vals[i]
+
+    def acc(self, i, g):
+        self.grads[i] = self.grads[i] + g if i in self.grads else np.array(g, dtype=np.complex128)
+
+    # operations
+    def matmul(self, a, b):
+        A, B = self.vals[a], self.vals[b]
+        def back(g):
+            self.acc(a, g @ B.conj().T)
+            self.acc(b, A.conj().T @ g)
+        return self._op(A @ B, back)
+
+    def solve(self, a, b):
+        A, B = self.vals[a], self.vals[b]
+        X = np.linalg.solve(A, B)
+        def back(g):
+            gb = np.linalg.solve(A.conj().T, g)
+            self.acc(b, gb)
+            self.acc(a, -gb @ X.conj().T)
+        return self._op(X, back)
+
+    def add(self, a, b, sb=1.0):
+        def back(g):
+            self.acc(a, g)
+            self.acc(b, np.conj(sb) * g)
+        return self._op(self.vals[a] + sb * self.vals[b], back)
+
+    def scale(self, a, s):
+        return self._op(s * self.vals[a], lambda g: self.acc(a, np.conj(s) * g))
+
+    def const_add(self, a, C):
+        return self._op(self.vals[a] + C, lambda g: self.acc(a, g))
+
+    def T(self, a):
+        return self._op(self.vals[a].T, lambda g: self.acc(a, g.T))
+
+    def H(self, a):
+        return self._op(self.vals[a].conj().T, lambda g: self.acc(a, g.conj().T))
+
+    def block(self, parts):
+        """2 x 2 block matrix of ids [[a, b], [c, d]]."""
+        rows = [[self.vals[p] for p in r] for r in parts]
+        out = np.block(rows)
+        r0, c0 = rows[0][0].shape
+
+        def back(g):
+            self.acc(parts[0][0], g[:r0, :c0])
+            self.acc(parts[0][1], g[:r0, c0:])
+            self.acc(parts[1][0], g[r0:, :c0])
+            self.acc(parts[1][1], g[r0:, c0:])
+        return self._op(out, back)
+
+    def sub(self, a, r0, r1, c0, c1):
+        A = self.vals[a]
+
+        def back(g):
+            z = np.zeros_like(A)
+            z[r0:r1, c0:c1] = g
+            self.acc(a, z)
+        return self._op(A[r0:r1, c0:c1], back)
+
+    def reg_solve(self, a, b, reg):
+        """``reg_solve`` (``smatrix.jl:47-52``) on the tape: (AᴴA + μI)⁻¹ AᴴB with
+        μ = reg (Re tr(AᴴA)/n + eps)."""
+        A = self.vals[a]
+        n = A.shape[1]
+        Ah = self.H(a)
+        AhA = self.matmul(Ah, a)
+        mu = reg * (np.real(np.trace(self.vals[AhA])) / max(n, 1) + np.finfo(float).eps)
+        # μ I as a function of A: dμ = (reg/n) 2 Re tr(Aᴴ dA)
+        G = self.const_add(AhA, mu * np.eye(n))
+        gi = len(self.vals) - 1
+
+        def back_mu(g):
+            self.acc(a, (2.0 * reg / max(n, 1)) * np.real(np.trace(g)) * A)
+        self.ops.append((gi, back_mu))
+        return self.solve(G, self.matmul(Ah, b))
+
+    def backward(self, out, seed):
+        self.grads = {out: np.asarray(seed, dtype=np.complex128)}
+        for i, back in reversed(self.ops):
+            if i in self.grads:
+                back(self.grads[i])
+        return self.grads
+
+    def grad(self, i):
+        return self.grads.get(i, np.zeros_like(self.vals[i]))
+
+
+def _t_interface(tp, oLR, oRL, conjugate, reg, reciprocity, passivity):
+    """``interface_smatrix`` on the tape; the passivity projection is differentiated only
+    where it leaves the matrix unchanged (every singular value <= 1)."""
+    NL, NR = tp.v(oLR).shape
+    oRLt = tp.H(oRL) if conjugate else tp.T(oRL)
+    oLRt = tp.H(oLR) if conjugate else tp.T(oLR)
+    IL, IR = np.eye(NL, dtype=complex), np.eye(NR, dtype=complex)
+    TLR = tp.reg_solve(tp.add(oLR, oRLt), tp.var(2.0 * IL), reg)
+    TRL = tp.reg_solve(tp.add(oRL, oLRt), tp.var(2.0 * IR), reg)
+    # R_LL = ½ (O_RLᵀ T_LR − O_LR T_LR), R_RR likewise (the identities cancel)
+    RLL = tp.scale(tp.add(tp.matmul(oRLt, TLR), tp.matmul(oLR, TLR), -1.0), 0.5)
+    RRR = tp.scale(tp.add(tp.matmul(oLRt, TRL), tp.matmul(oRL, TRL), -1.0), 0.5)
+    S = tp.block([[RLL, TRL], [TLR, RRR]])
+    if passivity != "none":
+        sv = np.linalg.svd(tp.v(S), compute_uv=False)
+        if sv.max() > 1.0:
+            raise NotImplementedError("pull-back through an active passivity projection (singular value "
+                                      f"{sv.max():.6g} > 1); use passivity='none'")
+    if reciprocity:
+        S = tp.scale(tp.add(S, tp.T(S)), 0.5)
+    return S
+
+
+def _t_propagation(tp, k, length):
+    """``propagation_smatrix`` on the tape from a real k vector (its id)."""
+    kv = np.real(tp.v(k))
+    ph = np.exp(2j * np.pi * kv * float(length))
+    n = len(kv)
+    Z = np.zeros((n, n), dtype=complex)
+    out = np.block([[Z, np.diag(ph)], [np.diag(ph), Z]])
+
+    def back(g):
+        gd = np.diag(g[:n, n:]) + np.diag(g[n:, :n])
+        # dP = 2πi L ph dk: dL = Re Σ conj(ḡ) 2πi L ph dk
+        tp.acc(k, np.real(np.conj(gd) * 2j * np.pi * float(length) * ph).astype(complex))
+    return tp._op(out, back)
+
+
+def _t_star(tp, A, nAl, B, nBl):
+    """Redheffer ``star`` (``smatrix.jl:156-172``) on the tape; ids of the two S matrices
+    with their left port counts."""
+    a, b = tp.v(A), tp.v(B)
+    nAr, nBr = a.shape[0] - nAl, b.shape[0] - nBl
+    A11, A12 = tp.sub(A, 0, nAl, 0, nAl), tp.sub(A, 0, nAl, nAl, nAl + nAr)
+    A21, A22 = tp.sub(A, nAl, nAl + nAr, 0, nAl), tp.sub(A, nAl, nAl + nAr, nAl, nAl + nAr)
+    B11, B12 = tp.sub(B, 0, nBl, 0, nBl), tp.sub(B, 0, nBl, nBl, nBl + nBr)
+    B21, B22 = tp.sub(B, nBl, nBl + nBr, 0, nBl), tp.sub(B, nBl, nBl + nBr, nBl, nBl + nBr)
+    I = tp.var(np.eye(nAr, dtype=complex))
+    L1 = tp.add(I, tp.matmul(B11, A22), -1.0)
+    L2 = tp.add(I, tp.matmul(A22, B11), -1.0)
+    C11 = tp.add(A11, tp.matmul(A12, tp.solve(L1, tp.matmul(B11, A21))))
+    C12 = tp.matmul(A12, tp.solve(L1, B12))
+    C21 = tp.matmul(B21, tp.solve(L2, A21))
+    C22 = tp.add(B22, tp.matmul(B21, tp.solve(L2, tp.matmul(A22, B12))))
+    return tp.block([[C11, C12], [C21, C22]])
+
+
+def assemble_pullback(overlaps, kmags, lengths, conjugate=False, reg=1e-9, reciprocity=True, passivity="invert"):
+    """``_assemble`` (``eme.jl:125-136``) recorded for its pull-back: from the interface
+    overlaps ``[(O_LR, O_RL), ...]`` and the per-cell propagation constants, the device S
+    matrix and a function mapping a cotangent ``Sbar`` of ``S.S`` to the cotangents of every
+    overlap matrix and of every cell's k: ``back(Sbar) -> ([(Ō_LR, Ō_RL), ...], [k̄, ...])``."""
+    tp = _Tape()
+    kid = [tp.var(np.asarray(k, dtype=float)) for k in kmags]
+    oid = [(tp.var(a), tp.var(b)) for a, b in overlaps]
+    acc = _t_propagation(tp, kid[0], lengths[0])
+    nl = len(kmags[0])
+    for i in range(len(kmags) - 1):
+        iface = _t_interface(tp, oid[i][0], oid[i][1], conjugate, reg, reciprocity, passivity)
+        prop = _t_propagation(tp, kid[i + 1], lengths[i + 1])
+        acc = _t_star(tp, acc, nl, iface, len(kmags[i]))
+        acc = _t_star(tp, acc, nl, prop, len(kmags[i + 1]))
+
+    def back(Sbar):
+        tp.backward(acc, Sbar)
+        return [(tp.grad(a), tp.grad(b)) for a, b in oid], [np.real(tp.grad(k)) for k in kid]
+    return SMat(tp.v(acc), nl, len(kmags[-1])), back
+
+
+def overlap_pullback(solver, slot_a, slot_b, obar, conjugate=False, which=3):
+    """``omx_overlap_pullback``: add the field cotangents of ``O = overlap_matrix(set a,
+    set b)`` for the cotangent ``obar`` to the device accumulators of set a's E (which & 1)
+    and set b's H (which & 2)."""
+    ob = np.asfortranarray(np.asarray(obar, dtype=np.complex128))
+    _lib.check(_lib.lib().omx_overlap_pullback(solver._h.h, int(slot_a), int(slot_b), int(bool(conjugate)),
+                                               _lib.dptr(ob.reshape(-1, order="F").copy()), int(which)), solver._h.h)
+
+
+def mode_pullback(solver, slot, nev, grid):
+    """``omx_mode_pullback``: ``(k̄[nev], [ēv_j], ε̄⁻¹ (3,3,Nx,Ny))`` of a mode set from its
+    accumulated field cotangents (the direct ε⁻¹ dependence of E, not symmetrised)."""
+    kb = np.empty(nev)
+    eb = np.empty((nev, 2 * grid.N), dtype=np.complex128)
+    ec = np.empty((grid.Ny, grid.Nx, 3, 3))
+    _lib.check(_lib.lib().omx_mode_pullback(solver._h.h, int(slot), _lib.dptr(kb), _lib.dptr(eb), _lib.dptr(ec)),
+               solver._h.h)
+    return kb, [eb[j].copy() for j in range(nev)], np.ascontiguousarray(ec.transpose(3, 2, 1, 0))
+
+
+def power_coupling_objective(in_mode=1, out_mode=1):
+    """``power_coupling`` (``eme.jl:140-149``) as an objective for ``eme_gradient``:
+    ``S -> (|T[out, in]|², S̄)``."""
+    def f(S):
+        i, o = in_mode - 1, out_mode - 1
+        t = S.S[S.nl + o, i]
+        Sbar = np.zeros_like(S.S)
+        Sbar[S.nl + o, i] = 2.0 * t
+        return float(abs(t) ** 2), Sbar
+    return f
+
+
+def eme_gradient(epsinvs, lengths, omega, grid, solver, objective, nev=2, conjugate=False, reg=1e-9,
+                 reciprocity=True, passivity="invert", warmstart=True, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=60,
+                 maxiter=100, return_parts=False):
+    """Value and gradient of a scalar of the EME device S matrix with respect to every cell's
+    ε⁻¹ and ω: what ``Zygote.gradient`` gives through ``eme(ε⁻¹s, ∂ε_∂ωs, lengths, ω, grid,
+    solver)`` (``eme.jl:80-104``).  ``objective(S) -> (value, S̄)``.
+
+    Forward: the cell chain of ``eme`` (solve, mode sets, overlaps), keeping each cell's k and
+    eigenvectors.  Backward: the host S-matrix algebra's reverse pass gives the cotangents of
+    the overlap matrices and of the propagation constants; per cell, the GPU rebuilds its mode
+    set and its neighbours', pulls the overlap cotangents back to field cotangents
+    (``omx_overlap_pullback``) and those to (k̄, ēv) and the direct ε̄⁻¹ of E
+    (``omx_mode_pullback``), then runs the ``solve_k`` pull-back (``omx_solve_k_adjoint``).
+    Returns ``(value, [ε̄⁻¹ per cell], ω̄)``; each cell's ε̄⁻¹ is the sum of the ``solve_k``
+    rule's (mirrored off-diagonals) and the field rule's (each entry its own derivative), as
+    the two rules add up in the reference; ``return_parts=True`` appends the two parts per
+    cell, ``[(solve_k part, field part), ...]``."""
+    from . import _adjoint
+    n = len(epsinvs)
+    eis = [e() if callable(e) else np.asarray(e, dtype=np.float64) for e in epsinvs]
+    kms, evs, overlaps = [], [], []
+    for i in range(n):
+        solver.options["warm_start"] = 1.0 if (warmstart and i > 0) else 0.0
+        km, ev = solve_k(omega, eis[i], grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
+                         max_eigsolves=max_eigsolves, maxiter=maxiter)
+        slot = i % 2
+        build_modes(km, ev, eis[i], grid, solver, slot=slot, conjugate=conjugate)
+        if i > 0:
+            overlaps.append((overlap_matrix(solver, 1 - slot, slot, nev, nev, conjugate),
+                             overlap_matrix(solver, slot, 1 - slot, nev, nev, conjugate)))
+        kms.append(km)
+        evs.append(ev)
+    solver.options["warm_start"] = 0.0
+    S, back = assemble_pullback(overlaps, kms, lengths, conjugate, reg, reciprocity, passivity)
+    value, Sbar = objective(S)
+    obar, kbar_prop = back(Sbar)
+    grads, parts, wbar = [], [], 0.0
+    for s in range(n):
+        # neighbours first (their fields use their own ε⁻¹), the cell itself last
+        if s > 0:
+            build_modes(kms[s - 1], evs[s - 1], eis[s - 1], grid, solver, slot=1, conjugate=conjugate)
+        if s < n - 1:
+            build_modes(kms[s + 1], evs[s + 1], eis[s + 1], grid, solver, slot=2, conjugate=conjugate)
+        build_modes(kms[s], evs[s], eis[s], grid, solver, slot=0, conjugate=conjugate)
+        if s > 0:
+            oLR, oRL = obar[s - 1]          # O_LR = overlap(cell s-1, cell s), O_RL = overlap(s, s-1)
+            overlap_pullback(solver, 1, 0, oLR, conjugate, which=2)
+            overlap_pullback(solver, 0, 1, oRL, conjugate, which=1)
+        if s < n - 1:
+            oLR, oRL = obar[s]              # O_LR = overlap(s, s+1), O_RL = overlap(s+1, s)
+            overlap_pullback(solver, 0, 2, oLR, conjugate, which=1)
+            overlap_pullback(solver, 2, 0, oRL, conjugate, which=2)
+        kb, evb, direct = mode_pullback(solver, 0, nev, grid)
+        h = solver.handle(grid, eis[s])
+        wb, eib, _ = _adjoint(h, grid, omega, nev, kms[s], evs[s], (list(kb + kbar_prop[s]), evb), period=False)
+        grads.append(eib + direct)
+        parts.append((eib, direct))
+        wbar += wb
+    if return_parts:
+        return value, grads, wbar, parts
+    return value, grads, wbar

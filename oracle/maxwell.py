@@ -18,6 +18,11 @@ import numpy as np
 import scipy.fft as _sfft
 
 _WORKERS = int(os.environ.get("ORACLE_FFT_WORKERS", str(os.cpu_count() or 1)))
+_SMALL = 3 * 96 * 96  # below this many elements one thread is faster than starting several
+
+
+def _workers(a):
+    return _WORKERS if np.size(a) >= _SMALL else 1
 
 __all__ = [
     "Grid", "my_fftfreq", "gvec", "k_guess", "mag_mn", "kx_tc", "kx_ct", "zx_tc",
@@ -119,16 +124,16 @@ def to_flat(Hg):
 
 
 def fft2(a):
-    return _sfft.fft2(a, axes=(-2, -1), workers=_WORKERS)
+    return _sfft.fft2(a, axes=(-2, -1), workers=_workers(a))
 
 
 def bfft2(a):
     """Unnormalised inverse transform (FFTW ``bfft``)."""
-    return _sfft.ifft2(a, axes=(-2, -1), norm="forward", workers=_WORKERS)
+    return _sfft.ifft2(a, axes=(-2, -1), norm="forward", workers=_workers(a))
 
 
 def ifft2(a):
-    return _sfft.ifft2(a, axes=(-2, -1), workers=_WORKERS)
+    return _sfft.ifft2(a, axes=(-2, -1), workers=_workers(a))
 
 
 def kx_tc(H, m, n, mag):

@@ -15,7 +15,7 @@ Layouts follow the reference's column-major arrays: ``epsinv[a, b, ix, iy, iz]``
 import numpy as np
 import scipy.fft as _sfft
 
-from .maxwell import _WORKERS, my_fftfreq
+from .maxwell import _workers, my_fftfreq
 
 __all__ = [
     "Grid3", "gvec3", "gz_field", "period_weight", "grid_with_period", "mag_mn3", "to_grid3", "to_flat3",
@@ -107,15 +107,15 @@ def to_flat3(Hg):
 
 
 def _fft3(a):
-    return _sfft.fftn(a, axes=(-3, -2, -1), workers=_WORKERS)
+    return _sfft.fftn(a, axes=(-3, -2, -1), workers=_workers(a))
 
 
 def _bfft3(a):
-    return _sfft.ifftn(a, axes=(-3, -2, -1), norm="forward", workers=_WORKERS)
+    return _sfft.ifftn(a, axes=(-3, -2, -1), norm="forward", workers=_workers(a))
 
 
 def _ifft3(a):
-    return _sfft.ifftn(a, axes=(-3, -2, -1), workers=_WORKERS)
+    return _sfft.ifftn(a, axes=(-3, -2, -1), workers=_workers(a))
 
 
 def _cols(Hflat):

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# the oracle's dense algebra works on small matrices in these tests: one BLAS thread is
+# faster than several (and keeps parallel test workers from oversubscribing the host)
+try:
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(limits=1, user_api="blas")
+except ImportError:  # pragma: no cover
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "optimode.jl_amd")):
     if p not in sys.path:

@@ -73,3 +73,58 @@ def test_reg_solve_is_a_pseudo_inverse():
 def test_power_coupling_reads_the_transmission_block():
     S = X.SMat(np.arange(16).reshape(4, 4) * (1 + 0j), 2, 2)
     assert X.power_coupling(S, in_mode=1, out_mode=2) == abs(S.S[3, 0]) ** 2
+
+
+def test_assemble_pullback_matches_finite_differences():
+    """The reverse pass of the host S-matrix algebra (interfaces with the regularised solve,
+    propagation phases, Redheffer stars) against central differences of the same algebra,
+    for complex directions of every overlap matrix and real directions of every k."""
+    rng = np.random.default_rng(0)
+    n = 3
+    ks = [rng.uniform(1, 2, n) for _ in range(3)]
+    ov = []
+    for _ in range(2):
+        A = np.eye(n) + 0.2 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        ov.append((A, A.T + 0.1 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))))
+    L = [0.7, 1.3, 0.4]
+
+    def fwd(ov, ks):
+        acc = X.propagation_smatrix(ks[0], L[0])
+        for i in range(2):
+            acc = X.star(X.star(acc, X.interface_smatrix(ov[i][0], ov[i][1], passivity="none")),
+                         X.propagation_smatrix(ks[i + 1], L[i + 1]))
+        return acc.S
+
+    S0 = fwd(ov, ks)
+    W = rng.standard_normal(S0.shape) + 1j * rng.standard_normal(S0.shape)
+
+    def f(S):
+        return np.real(np.sum(np.conj(W) * S)) + abs(S[n, 0]) ** 2
+    Sbar = W.copy()
+    Sbar[n, 0] += 2 * S0[n, 0]
+    S, back = X.assemble_pullback(ov, ks, L, passivity="none")
+    assert np.abs(S.S - S0).max() == 0
+    ob, kb = back(Sbar)
+    h = 1e-6
+    for (i, which, r, c) in [(0, 0, 1, 2), (1, 1, 0, 1), (0, 1, 2, 2), (1, 0, 2, 0)]:
+        for dz in (1.0, 1j):
+            ovp = [(a.copy(), b.copy()) for a, b in ov]
+            ovm = [(a.copy(), b.copy()) for a, b in ov]
+            ovp[i][which][r, c] += h * dz
+            ovm[i][which][r, c] -= h * dz
+            fd = (f(fwd(ovp, ks)) - f(fwd(ovm, ks))) / (2 * h)
+            assert np.real(np.conj(ob[i][which][r, c]) * dz) == pytest.approx(fd, rel=1e-6, abs=1e-9)
+    for j in range(3):
+        kp = [k.copy() for k in ks]
+        km = [k.copy() for k in ks]
+        kp[j][1] += h
+        km[j][1] -= h
+        fd = (f(fwd(ov, kp)) - f(fwd(ov, km))) / (2 * h)
+        assert kb[j][1] == pytest.approx(fd, rel=1e-6)
+
+
+def test_assemble_pullback_refuses_an_active_passivity_projection():
+    n = 2
+    A = 0.3 * np.eye(n, dtype=complex)   # overlaps far from unit: interface singular values > 1
+    with pytest.raises(NotImplementedError):
+        X.assemble_pullback([(A, 0.2 * A)], [np.ones(n), np.ones(n)], [1.0, 1.0], passivity="invert")

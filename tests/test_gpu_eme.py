@@ -132,3 +132,95 @@ def test_eme_slices_at_scale_512x256_16_modes():
         print("  min_i max_j |O_LR|", best.min(), flush=True)
         assert best.min() > 0.9
     assert np.all(np.abs(res.S.transmission()) <= 1 + 1e-6)
+
+
+# ---------------------------------------------------------------------------------------
+# pull-backs of the EME chain (the reverse mode the reference gets from Zygote, eme.jl:80-88)
+# ---------------------------------------------------------------------------------------
+
+from oracle.eme import (mode_fields_pullback as o_fields_pb, normalization_pullback as o_norm_pb,  # noqa: E402
+                        overlap_pullback as o_overlap_pb)
+
+
+@pytest.mark.parametrize("conj", [False, True])
+def test_mode_set_pullbacks_match_oracle(conj):
+    """omx_overlap_pullback + omx_mode_pullback on the oracle's own eigenvectors against the
+    oracle's overlap, normalisation and mode-field pull-backs (k̄, ēv, direct ε̄⁻¹), for seeded
+    overlap cotangents on both sides of an interface."""
+    print(f"eme pullbacks vs oracle conj={conj}", flush=True)
+    og, g = O.Grid(6.0, 4.0, 32, 24), omx.Grid(6.0, 4.0, 32, 24)
+    eis = [O.gaussian_wg_epsi([4.2, 2.1, 1.0, 0.6], og), O.gaussian_wg_epsi([4.0, 2.1, 1.2, 0.55], og)]
+    sol = [O.solve_k(W0, e, og, nev=3, k_tol=1e-12, eig_tol=1e-10, maxiter=300, seed=3) for e in eis]
+    solver = omx.HIPSolver(precision="fp64")
+    rng = np.random.default_rng(12)
+    obLR = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+    obRL = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+    X.build_modes(sol[1][0], sol[1][1], eis[1], g, solver, slot=1, conjugate=conj)
+    X.build_modes(sol[0][0], sol[0][1], eis[0], g, solver, slot=0, conjugate=conj)
+    # cell 0 on the left: O_LR = overlap(0, 1) reaches its E, O_RL = overlap(1, 0) its H
+    X.overlap_pullback(solver, 0, 1, obLR, conj, which=1)
+    X.overlap_pullback(solver, 1, 0, obRL, conj, which=2)
+    kb, evb, direct = X.mode_pullback(solver, 0, 3, g)
+    dA = og.dV()
+    b0 = [o_build_mode(k, v, eis[0], og, conjugate=conj) for k, v in zip(*sol[0])]
+    b1 = [o_build_mode(k, v, eis[1], og, conjugate=conj) for k, v in zip(*sol[1])]
+    m0, m1 = [(b[0], b[1]) for b in b0], [(b[0], b[1]) for b in b1]
+    ca, _ = o_overlap_pb(m0, m1, dA, obLR, conj)
+    _, cb = o_overlap_pb(m1, m0, dA, obRL, conj)
+    ref_direct = np.zeros_like(eis[0])
+    for j, (E, H, nrm) in enumerate(b0):
+        Erb, Hrb = o_norm_pb(E, H, nrm, dA, ca[j][0] + cb[j][0], ca[j][1] + cb[j][1], conj)
+        kr, er, dr = o_fields_pb(sol[0][0][j], sol[0][1][j], eis[0], og, Erb, Hrb)
+        ref_direct += dr
+        print(f"  mode {j}: kbar {kb[j]:.12g} oracle {kr:.12g}", flush=True)
+        assert kb[j] == pytest.approx(kr, rel=1e-9, abs=1e-12 * abs(kr) + 1e-14)
+        assert np.linalg.norm(evb[j] - er) <= 1e-9 * np.linalg.norm(er)
+    assert np.abs(direct - ref_direct).max() <= 1e-9 * np.abs(ref_direct).max()
+
+
+def test_eme_gradient_matches_finite_differences():
+    """eme_gradient (the cascade's power coupling |T11|² through three asymmetric coupler
+    cells, two modes each) against central differences of the GPU EME forward along symmetric
+    ε⁻¹ directions of the middle cell (a diagonal and an off-diagonal entry) and in ω.  The
+    cells are well conditioned (|T11|² ≈ 1.06, overlap matrices far from singular); with the
+    symmetric coupler's three modes the overlaps are near singular, |T11|² reaches 130 without
+    the passivity projection and central differences of the forward lose their digits (the
+    oracle's own differences agree with its adjoint to 1.6% there)."""
+    print("eme gradient vs fd", flush=True)
+    g = omx.Grid(6.0, 4.0, 64, 48)
+    eis = [_coupler(g, gap, w2=0.7) for gap in (0.35, 0.5, 0.7)]
+    lengths = [1.0, 2.5, 1.0]
+    kw = dict(nev=2, passivity="none", k_tol=1e-12, eig_tol=1e-10, warmstart=False)
+    solver = omx.HIPSolver(precision="fp64")
+    obj = X.power_coupling_objective(1, 1)
+    val, grads, wbar, parts = X.eme_gradient(eis, lengths, W0, g, solver, obj, return_parts=True, **kw)
+
+    def value(e, w=W0):
+        S = X.eme(e, lengths, w, g, omx.HIPSolver(precision="fp64"), nev=2, passivity="none", k_tol=1e-12,
+                  eig_tol=1e-10, warmstart=False).S
+        return obj(S)[0]
+    assert val == pytest.approx(value(eis), rel=1e-9)
+    rng = np.random.default_rng(4)
+    for (a, b) in ((1, 1), (0, 1)):
+        r = rng.standard_normal((g.Nx, g.Ny)) * 0.1
+        D = np.zeros_like(eis[1])
+        D[a, b] = r
+        D[b, a] = r
+        h = 1e-3
+        ep = [e.copy() for e in eis]
+        em = [e.copy() for e in eis]
+        ep[1] = ep[1] + h * D
+        em[1] = em[1] - h * D
+        fd = (value(ep) - value(em)) / (2 * h)
+        # the solve_k rule mirrors off-diagonals, the field rule does not: the directional
+        # derivative along the symmetric D is the field part's contraction with D plus one
+        # entry of each mirrored pair of the solve_k part
+        sk, direct = parts[1]
+        pred = np.sum(direct * D) + np.sum(sk[a, b] * r)
+        assert np.array_equal(grads[1], sk + direct)
+        print(f"  ({a},{b}) pred {pred:.10g} fd {fd:.10g}", flush=True)
+        assert pred == pytest.approx(fd, rel=1e-3, abs=1e-8)
+    hw = 1e-4
+    fdw = (value(eis, W0 + hw) - value(eis, W0 - hw)) / (2 * hw)
+    print(f"  omega bar {wbar:.10g} fd {fdw:.10g}", flush=True)
+    assert wbar == pytest.approx(fdw, rel=1e-3, abs=1e-8)
