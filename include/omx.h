@@ -94,7 +94,8 @@ int omx_destroy(omx_solver* h);
 /* update_ε⁻¹ (maxwell.jl:1114-1136) */
 int omx_set_epsinv(omx_solver* h, const double* epsinv);
 const char* omx_last_error(const omx_solver* h);
-/* tuning knobs: "fp32_tol", "fp32_switch", "maxiter_fp64", "guard", "seed"; start modes
+/* tuning knobs: "fp32_tol" (solve_k's FP32 locking threshold / omega^2, 1e-3), "fp32_tol_omega2"
+   (solve_omega2's, 1e-5), "fp32_switch", "jd_rtol", "maxiter_fp64", "guard", "seed"; start modes
    "warm_start" and "band_guess" (omx_solve_k); "guess_lowres" = hw > 0 (even, <= Nx, Ny):
    omx_solve_k's Hguess vectors hold only their hw x hw lowest plane waves, 2 hw^2 complex
    each at c + 2(gx + hw gy) in FFT order, and are scattered into the start block on the GPU */
@@ -103,13 +104,14 @@ int omx_set_option(omx_solver* h, const char* key, double value);
 /* Autotuner (the reference's counterpart is FFTW's PATIENT planning when a ModeSolver is built,
    lib/MaxwellEigenmodes/src/maxwell.jl:940-943): timed trials on this handle's grid of the
    kernels of an nev-mode omx_solve_k, with the shapes its FP32 phase launches them with --
-   the operator path (register FFT or LDS Stockham), the Stockham FFT radix variant and tile
-   shape (rows / columns per block), on 3D cells the yz-slab or five-pass form, and the Gram
-   and block-update row partitions.  A Stockham candidate is used only if its output equals
-   the default plan's to 1e-5.  The fastest of each becomes the handle's option; the same
-   choice can be applied to another handle of the grid with omx_set_option ("reg_ops",
-   "fft_variant32", "tile_rows32", "tile_cols32", "gram_blocks", "combine_blocks", "slab3d").
-   choice_out: NULL or 7 ints in that order; us_out: NULL or 3 doubles, microseconds per launch
+   the operator path (register FFT, with its rows per block of passes 1/3 and threads of pass
+   2, or LDS Stockham), the Stockham FFT radix variant and tile shape (rows / columns per
+   block), on 3D cells the yz-slab or five-pass form, and the Gram and block-update row
+   partitions.  A candidate plan is used only if its output equals the default plan's to 1e-5.
+   The fastest of each becomes the handle's option; the same choice can be applied to another
+   handle of the grid with omx_set_option ("reg_ops", "fft_variant32", "tile_rows32",
+   "tile_cols32", "gram_blocks", "combine_blocks", "slab3d", "reg_rows32", "reg_nt2").
+   choice_out: NULL or 9 ints in that order; us_out: NULL or 3 doubles, microseconds per launch
    of the chosen operator application, Gram and block update. */
 int omx_autotune(omx_solver* h, int nev, int* choice_out, double* us_out);
 

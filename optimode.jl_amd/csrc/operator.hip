@@ -370,10 +370,10 @@ __device__ __forceinline__ Basis<T> basis_fast(int ix, int iy, int nx, int ny, d
 // pass 1: rows iy0 .. iy0 + RB - 1 of one vector; the LX/8 lanes of group r hold row
 // iy0 + r (lane g: ix = g + (LX/8) q, q < 8).  Curl in k-space, FFT along x, then the rows
 // are written column-major through an LDS tile (RB consecutive iy per ix: whole lines).
-template <typename T, int OP, int LX>
-__global__ __launch_bounds__((reg_row_threads<T, LX>())) void k_rpass1(OpPlan<T> plan, BView<T> X, cplx<T>* D,
-                                                                    VecSel sel, const double* kinst) {
-  constexpr int NT = reg_row_threads<T, LX>(), RB = reg_rows<T>(), GX = LX / 8;
+template <typename T, int OP, int LX, int RB>
+__global__ __launch_bounds__(RB*(LX / 8)) void k_rpass1(OpPlan<T> plan, BView<T> X, cplx<T>* D, VecSel sel,
+                                                         const double* kinst) {
+  constexpr int NT = RB * (LX / 8), GX = LX / 8;
   constexpr int SG = reg_fft_scratch<LX>(), TS = RB + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -436,9 +436,9 @@ __global__ __launch_bounds__((reg_row_threads<T, LX>())) void k_rpass1(OpPlan<T>
 // pass 2: one column ix per group of LY/8 lanes (lane g holds iy = g + (LY/8) q): FFT along
 // y, eps^-1 (or eps_ave) per pixel, inverse FFT along y, in place.  No block barrier after
 // the twiddles are staged.
-template <typename T, int OP, int LY>
-__global__ __launch_bounds__(REG_COL_THREADS) void k_rpass2(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
-  constexpr int NT = REG_COL_THREADS, GY = LY / 8, CB = NT / GY;
+template <typename T, int OP, int LY, int NT>
+__global__ __launch_bounds__(NT) void k_rpass2(OpPlan<T> plan, cplx<T>* D, VecSel sel) {
+  constexpr int GY = LY / 8, CB = NT / GY;
   constexpr int SG = reg_fft_scratch<LY>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -498,11 +498,11 @@ __global__ __launch_bounds__(REG_COL_THREADS) void k_rpass2(OpPlan<T> plan, cplx
 // pass 3: rows iy0 .. iy0 + RB - 1 read column-major through an LDS tile (the next
 // component's lines are fetched while the current one is unpacked), inverse FFT along x,
 // curl back onto (m, n) scaled by 1/N; for OP_MKH dotted with the input.
-template <typename T, int OP, int LX>
-__global__ __launch_bounds__((reg_row_threads<T, LX>())) void k_rpass3(OpPlan<T> plan, BView<T> Hin, BView<T> Y,
-                                                                    cplx<T>* D, VecSel sel, const double* kinst,
-                                                                    double* dot_partial, int dps) {
-  constexpr int NT = reg_row_threads<T, LX>(), RB = reg_rows<T>(), GX = LX / 8;
+template <typename T, int OP, int LX, int RB>
+__global__ __launch_bounds__(RB*(LX / 8)) void k_rpass3(OpPlan<T> plan, BView<T> Hin, BView<T> Y, cplx<T>* D,
+                                                         VecSel sel, const double* kinst, double* dot_partial,
+                                                         int dps) {
+  constexpr int NT = RB * (LX / 8), GX = LX / 8;
   constexpr int SG = reg_fft_scratch<LX>(), TS = RB + 1, PER = LX * RB / NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -590,26 +590,50 @@ __global__ __launch_bounds__((reg_row_threads<T, LX>())) void k_rpass3(OpPlan<T>
   }
 }
 
-template <typename T, int OP, int LX>
-void rop_rows(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
-              const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
-  constexpr int NT = reg_row_threads<T, LX>(), RB = reg_rows<T>();
+template <typename T, int OP, int LX, int RB>
+void rop_rows_rb(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+                 const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  constexpr int NT = RB * (LX / 8);
   constexpr int SG = reg_fft_scratch<LX>(), TS = RB + 1;
   const size_t lds = sizeof(cplx<T>) * (LX + std::max(RB * SG, LX * TS));
   dim3 grid(plan.ny / RB, sel.ninst * sel.mper);
   if (pass == 1)
-    hipLaunchKernelGGL((k_rpass1<T, OP, LX>), grid, dim3(NT), lds, stream, plan, X, D, sel, kinst);
+    hipLaunchKernelGGL((k_rpass1<T, OP, LX, RB>), grid, dim3(NT), lds, stream, plan, X, D, sel, kinst);
   else
-    hipLaunchKernelGGL((k_rpass3<T, OP, LX>), grid, dim3(NT), lds, stream, plan, X, Y, D, sel, kinst, dot_partial,
-                       dps);
+    hipLaunchKernelGGL((k_rpass3<T, OP, LX, RB>), grid, dim3(NT), lds, stream, plan, X, Y, D, sel, kinst,
+                       dot_partial, dps);
 }
 
-template <typename T, int OP, int LY>
-void rop_cols(const OpPlan<T>& plan, cplx<T>* D, const VecSel& sel, hipStream_t stream) {
-  constexpr int NT = REG_COL_THREADS, CB = NT / (LY / 8);
+// rows per block: reg_rows<T>() or half of it (plan.reg_rb)
+template <typename T, int OP, int LX>
+void rop_rows(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
+              const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
+  constexpr int R0 = reg_rows<T>();
+  if (reg_rows_of(plan) == R0)
+    rop_rows_rb<T, OP, LX, R0>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream);
+  else if (reg_rows_of(plan) == R0 / 2)
+    rop_rows_rb<T, OP, LX, R0 / 2>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream);
+  else
+    throw Error(OMX_ERR_INTERNAL, "register FFT path: unsupported rows per block");
+}
+
+template <typename T, int OP, int LY, int NT>
+void rop_cols_nt(const OpPlan<T>& plan, cplx<T>* D, const VecSel& sel, hipStream_t stream) {
+  constexpr int CB = NT / (LY / 8);
   const size_t lds = sizeof(cplx<T>) * (LY + CB * reg_fft_scratch<LY>());
   dim3 grid(plan.nx / CB, sel.ninst * sel.mper);
-  hipLaunchKernelGGL((k_rpass2<T, OP, LY>), grid, dim3(NT), lds, stream, plan, D, sel);
+  hipLaunchKernelGGL((k_rpass2<T, OP, LY, NT>), grid, dim3(NT), lds, stream, plan, D, sel);
+}
+
+// threads per block of pass 2: REG_COL_THREADS or half of it (plan.reg_nt2)
+template <typename T, int OP, int LY>
+void rop_cols(const OpPlan<T>& plan, cplx<T>* D, const VecSel& sel, hipStream_t stream) {
+  if (plan.reg_nt2 == 0 || plan.reg_nt2 == REG_COL_THREADS)
+    rop_cols_nt<T, OP, LY, REG_COL_THREADS>(plan, D, sel, stream);
+  else if (plan.reg_nt2 == REG_COL_THREADS / 2)
+    rop_cols_nt<T, OP, LY, REG_COL_THREADS / 2>(plan, D, sel, stream);
+  else
+    throw Error(OMX_ERR_INTERNAL, "register FFT path: unsupported pass-2 block");
 }
 
 template <typename T, int OP>

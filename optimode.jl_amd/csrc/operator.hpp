@@ -95,6 +95,8 @@ template <typename T> struct OpPlan {
   int cols_per_block;             // Stockham path: pass 2
   int vpt;                        // Stockham path: values per thread bound (8, 16, 32)
   int reg;                        // 1: the operator uses the register-FFT path
+  int reg_rb;                     // register path: rows per block of passes 1/3 (0: reg_rows<T>())
+  int reg_nt2;                    // register path: threads per block of pass 2 (0: REG_COL_THREADS)
   int dot_blocks;                 // OP_MKH dot partials per vector (blocks of pass 3)
   // 3D cells periodic along z (nz > 1; 2D: nz = 1): plane iz has g_z = fft_index(iz)/Dz,
   // vectors are nz consecutive planes of 2 nx ny values, the intermediate field is
@@ -120,9 +122,12 @@ template <typename T> __device__ __forceinline__ const T* eps_colm(const OpPlan<
 // Register path geometry: rows per block of passes 1 and 3 (the column-major intermediate
 // is written / read in runs of that many iy: whole 128-byte lines), threads per block of
 // pass 1 / 3 (a group of L/8 lanes per row) and of pass 2 (a group per column).
+// Each precision has a second, halved row count (more, smaller blocks for launches of few
+// vectors) and pass 2 a halved thread count; plan.reg_rb / plan.reg_nt2 pick them.
 template <typename T> constexpr int reg_rows() { return sizeof(T) == 4 ? 16 : 8; }
 template <typename T, int LX> constexpr int reg_row_threads() { return reg_rows<T>() * (LX / 8); }
 constexpr int REG_COL_THREADS = 256;
+template <typename T> inline int reg_rows_of(const OpPlan<T>& p) { return p.reg_rb > 0 ? p.reg_rb : reg_rows<T>(); }
 
 // Launch one operator application on `stream`.
 //   X: input block, Y: output block (OP_M/OP_P), D: scratch of 3*N complex per vector

@@ -269,8 +269,10 @@ template <typename T> void build_plan(Solver& S, PrecisionData<T>& P) {
 template <typename T> void select_operator(Solver& S, PrecisionData<T>& P) {
   OpPlan<T>& pl = P.plan;
   pl.reg = (P.reg_ok && S.opt.reg_ops) ? 1 : 0;
+  pl.reg_rb = sizeof(T) == 4 ? S.opt.reg_rows32 : S.opt.reg_rows64;
+  pl.reg_nt2 = S.opt.reg_nt2;
   pl.slab_cols = S.opt.slab3d ? P.slab_ok : 0;
-  pl.dot_blocks = S.nz * (S.ny / (pl.reg ? reg_rows<T>() : pl.rows_per_block));
+  pl.dot_blocks = S.nz * (S.ny / (pl.reg ? reg_rows_of(pl) : pl.rows_per_block));
 }
 
 template <typename T> void upload_eps(Solver& S, PrecisionData<T>& P) {
@@ -447,6 +449,7 @@ template <typename T> class Batch {
     int it = 0;
     int maxiter = 0;
     bool done = false;
+    double host_enqueue_ms = 0, host_wait_ms = 0;  // host time launching / waiting (OMX_TRACE)
   };
   void run_begin(RunState& r, const std::vector<int>& act, const std::vector<int>& bands,
                  const std::vector<int>& ncheck, const std::vector<int>& msize, const std::vector<double>& tols,
@@ -489,6 +492,8 @@ template <typename T> class Batch {
   // Enqueue iteration r.it, then (from the second on) examine the state after the one
   // before it.  Sets r.done once every instance has finished.
   void run_step(RunState& r, SolveStats& stats) {
+    using clk = std::chrono::steady_clock;
+    const auto t_enq = clk::now();
     const int it = r.it;
     const bool prof = S.prof.enabled;
     const int c = cur, n = cur ^ 1;
@@ -525,9 +530,12 @@ template <typename T> class Batch {
                                  hipMemcpyDeviceToHost, sm));
     OMX_HIP_CHECK(hipEventRecord(snap_ev[it & 1], sm));
     r.it = it + 1;
+    const auto t_wait = clk::now();
+    r.host_enqueue_ms += std::chrono::duration<double, std::milli>(t_wait - t_enq).count();
     if (it == 0) return;
     // examine the state after iteration it - 1
     OMX_HIP_CHECK(hipEventSynchronize(snap_ev[(it - 1) & 1]));
+    r.host_wait_ms += std::chrono::duration<double, std::milli>(clk::now() - t_wait).count();
     const LState* hs = snap[(it - 1) & 1].as<LState>();
     bool all = true;
     long m_next = 0;
@@ -572,6 +580,9 @@ template <typename T> class Batch {
     int tot = 0;
     for (int i = 0; i < ninst; ++i)
       if (r.act[i]) tot += hst[i].niter;
+    if (S.trace)
+      std::fprintf(stderr, "[omx] %s lobpcg run: %d rounds, host enqueue %.2f ms, host wait %.2f ms\n",
+                   sizeof(T) == 4 ? "fp32" : "fp64", r.it, r.host_enqueue_ms, r.host_wait_ms);
     for (int i = 0; i < ninst; ++i)
       if (r.act[i] && hst[i].status == OMX_ERR_CHOLESKY)
         throw Error(OMX_ERR_CHOLESKY, "Cholesky orthogonalisation failed even after shifting");
@@ -730,8 +741,12 @@ void Solver::set_epsinv(const double* epsinv) {
 void Solver::set_option(const std::string& key, double v) {
   if (key == "fp32_tol")
     opt.fp32_tol_rel = v;
+  else if (key == "fp32_tol_omega2")
+    opt.fp32_tol_omega2 = v;
   else if (key == "fp32_switch")
     opt.fp32_switch = v;
+  else if (key == "seed_tol")
+    opt.seed_tol = v;
   else if (key == "maxiter_fp64")
     opt.maxiter_fp64 = int(v);
   else if (key == "guard")
@@ -787,6 +802,19 @@ void Solver::set_option(const std::string& key, double v) {
     select_operator(*this, p32);
     select_operator(*this, p64);
     workspaces.clear();  // batches size their dot partials by the operator in use
+  } else if (key == "reg_rows32" || key == "reg_rows64" || key == "reg_nt2") {
+    // register-path block shapes: the default or half of it
+    const int nv = int(v);
+    const int full = key == "reg_rows32" ? 16 : (key == "reg_rows64" ? 8 : REG_COL_THREADS);
+    if (nv != 0 && nv != full && nv != full / 2)
+      throw Error(OMX_ERR_INVALID_ARG, "option '" + key + "' must be 0, " + std::to_string(full) + " or " +
+                                           std::to_string(full / 2));
+    int& o = key == "reg_rows32" ? opt.reg_rows32 : (key == "reg_rows64" ? opt.reg_rows64 : opt.reg_nt2);
+    if (o == nv) return;
+    o = nv;
+    select_operator(*this, p32);
+    select_operator(*this, p64);
+    workspaces.clear();  // batches size their dot partials by the operator in use
   } else if (key == "fp64_only")
     opt.fp64_only = int(v);
   else if (key == "profile") {
@@ -807,8 +835,10 @@ void Solver::set_option(const std::string& key, double v) {
 //     columns per block of pass 2) that fits; a Stockham candidate is kept only if its
 //     output equals the default plan's to 1e-5; 3D cells: the yz-slab and five-pass forms;
 //   * the Gram row partition (gram_blocks) and the block-update partition (combine_blocks).
-// The fastest of each becomes the handle's option.  choice_out (7 ints): reg_ops,
-// fft_variant32, tile_rows32, tile_cols32, gram_blocks, combine_blocks, slab3d; us_out (3):
+// On the register path, its block shapes (rows per block of passes 1/3, threads of pass 2)
+// are trials too.  The fastest of each becomes the handle's option.  choice_out (9 ints):
+// reg_ops, fft_variant32, tile_rows32, tile_cols32, gram_blocks, combine_blocks, slab3d,
+// reg_rows32, reg_nt2; us_out (3):
 // microseconds per launch of the chosen operator application, Gram and update.
 void Solver::autotune(int nev, int* choice_out, double* us_out) {
   if (nev < 1 || nev > 16) throw Error(OMX_ERR_UNSUPPORTED, "nev must be in 1..16");
@@ -904,9 +934,10 @@ void Solver::autotune(int nev, int* choice_out, double* us_out) {
     return std::sqrt(d / std::max(n, 1e-300));
   };
   struct OpChoice {
-    int reg, variant, rows, cols, slab;
+    int reg, variant, rows, cols, slab, rrows = 0, nt2 = 0;
   };
-  OpChoice best{saved.reg_ops, saved.fft_variant32, saved.tile_rows32, saved.tile_cols32, saved.slab3d};
+  OpChoice best{saved.reg_ops, saved.fft_variant32, saved.tile_rows32, saved.tile_cols32, saved.slab3d,
+                saved.reg_rows32, saved.reg_nt2};
   double best_op = 1e300;
   auto try_plan = [&](const OpChoice& c, const std::vector<c32>* ref) {
     opt.reg_ops = c.reg;
@@ -914,6 +945,8 @@ void Solver::autotune(int nev, int* choice_out, double* us_out) {
     opt.fft_variant32 = c.variant;
     opt.tile_rows32 = c.rows;
     opt.tile_cols32 = c.cols;
+    opt.reg_rows32 = c.rrows;
+    opt.reg_nt2 = c.nt2;
     try {
       build_plan<float>(*this, p32);
     } catch (const Error&) {
@@ -931,7 +964,9 @@ void Solver::autotune(int nev, int* choice_out, double* us_out) {
   build_plan<float>(*this, p32);
   const std::vector<c32> ref = op_out0();  // default Stockham plan (or the 3D default form)
   if (nz == 1) {
-    if (p32.reg_ok) try_plan(OpChoice{1, 0, 0, 0, saved.slab3d}, &ref);
+    if (p32.reg_ok)
+      for (int rr : {16, 8})
+        for (int nt2 : {REG_COL_THREADS, REG_COL_THREADS / 2}) try_plan(OpChoice{1, 0, 0, 0, saved.slab3d, rr, nt2}, &ref);
     const int budget = 3072;
     for (int variant : {0, 1})
       for (int rows : {8, 4, 2, 1}) {
@@ -951,6 +986,8 @@ void Solver::autotune(int nev, int* choice_out, double* us_out) {
   opt.fft_variant32 = best.variant;
   opt.tile_rows32 = best.rows;
   opt.tile_cols32 = best.cols;
+  opt.reg_rows32 = best.rrows;
+  opt.reg_nt2 = best.nt2;
   build_plan<float>(*this, p32);
   select_operator(*this, p64);
   // partitions, with the chosen operator plan in place
@@ -990,8 +1027,8 @@ void Solver::autotune(int nev, int* choice_out, double* us_out) {
   workspaces.clear();  // batches size their dot partials by the operator in use
   restore.armed = false;
   if (choice_out) {
-    const int c[7] = {opt.reg_ops, opt.fft_variant32, opt.tile_rows32, opt.tile_cols32, opt.gram_blocks,
-                      opt.combine_blocks, opt.slab3d};
+    const int c[9] = {opt.reg_ops, opt.fft_variant32, opt.tile_rows32, opt.tile_cols32, opt.gram_blocks,
+                      opt.combine_blocks, opt.slab3d, opt.reg_rows32, opt.reg_nt2};
     std::memcpy(choice_out, c, sizeof(c));
   }
   if (us_out) {
@@ -1214,7 +1251,7 @@ void Solver::solve_omega2(double k, int nev, double tol, int maxiter, double* H_
     else
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, nev, K, opt.seed, stream);
     // FP32 phase: locking threshold at the FP32 accuracy floor (relative to the Ritz values)
-    const double tol32 = std::max(tol, opt.fp32_tol_rel * std::max(1e-300, std::fabs(k * k)));
+    const double tol32 = std::max(tol, opt.fp32_tol_omega2 * std::max(1e-300, std::fabs(k * k)));
     stats.it32 += b32.run(act, bands, nchk, {nev}, {tol32}, maxiter, stats);
     launch_convert<float, double>(b32.p(b32.X[b32.cur]), ld, b32.istride, b64.p(b64.X[0]), ld, b64.istride, 1, nev, K,
                                   stream);
@@ -1517,6 +1554,8 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
   const int ninst = b.ninst;
   const long K = S.K, ld = S.ld, istride = b.istride;
   hipStream_t s = S.stream;
+  const auto t_start = std::chrono::steady_clock::now();
+  int jd_total = 0, minres_total = 0;
   const int nblk_g = int(std::max<long>(1, std::min<long>((K + 31) / 32, std::max(16, 1024 / ninst))));
   const int nblk_r = int(std::min<long>(256, (K + 255) / 256));
   RefineWorkspace& W = S.refine_ws;
@@ -1628,7 +1667,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       VecOps V(S, s, ninst, ld, K, XB.as<c64>());
       int its;
       if (S.opt.jd_fp32) {
-        // the correction only needs jd_rtol (1e-3) relative accuracy: FP32 vectors and
+        // the correction only needs jd_rtol (5e-3) relative accuracy: FP32 vectors and
         // operator, FP64 recurrences; the residual that decides convergence stays FP64
         using cf = cplx<float>;
         launch_convert<double, float>(XB.as<c64>(), ld, ld, W.XBf.as<cf>(), ld, ld, ninst, 1, K, s);
@@ -1645,6 +1684,8 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
         stats.matvec += long(its) * jdh[i];
         stats.precon += long(its) * jdh[i];
       }
+      ++jd_total;
+      minres_total += its;
       // t <- t - X (X^H t), twice, then unit norm (instances with jdh only)
       std::vector<int> n1j(ninst);
       for (int i = 0; i < ninst; ++i) n1j[i] = jdh[i] ? msize[i] : 0;
@@ -1738,6 +1779,11 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       std::fprintf(stderr, "\n");
     }
   }
+  if (S.trace)
+    std::fprintf(stderr, "[omx] fp64 refine: %.2f ms host wall, %d Rayleigh-Ritz steps, %d correction solves, %d MINRES "
+                 "iterations\n",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count(), outer,
+                 jd_total, minres_total);
   return outer;
 }
 
@@ -1806,7 +1852,9 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     else
       upload_block<TT>(*this, b, Hguess, nguess_used, from, to);
   };
-  const double tol32 = std::max(eig_tol, opt.fp32_tol_rel * w2);
+  // FP32 locking threshold: loose where the FP64 refinement finishes the vectors, at the FP32
+  // accuracy floor where an FP64 LOBPCG does
+  const double tol32 = std::max(eig_tol, (opt.refine ? opt.fp32_tol_rel : opt.fp32_tol_omega2) * w2);
 
   // first eigensolve, shared by every band: the lowest nev pairs at k0 (instance 0);
   // each band then starts Newton from its own first-order estimate of k with this block
@@ -1927,7 +1975,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       // the shared first eigensolve in instance 0 of the first half, as seed_bands
       std::vector<int> act(na, 0), n0(na, M), m0(na, M), bA(bands.begin(), bands.begin() + na);
       act[0] = 1;
-      std::vector<double> t0(na, std::max(tol32, 3e-3 * w2)), kh(ninst, k0);
+      std::vector<double> t0(na, std::max(tol32, opt.seed_tol * w2)), kh(ninst, k0);
       OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * na, hipMemcpyHostToDevice, stream));
       stats.it32 += b32.run(act, bA, n0, m0, t0, maxiter, stats);
       const auto lam = b32.rayleigh();
@@ -1961,7 +2009,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     } else {
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) put_guess(b32, 0, 1);
-      stats.it32 += seed_bands(b32, std::max(tol32, 3e-3 * w2));
+      stats.it32 += seed_bands(b32, std::max(tol32, opt.seed_tol * w2));
     }
     if (split) {
       stats.it32 += newton_phase2(*this, b32, *b32q, hns, bands, nchk, msize, std::vector<double>(ninst, tol32), maxiter,
@@ -2000,7 +2048,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     } else {
       launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) put_guess(b64, 0, 1);
-      stats.it64 += seed_bands(b64, std::max(eig_tol, 3e-3 * w2));
+      stats.it64 += seed_bands(b64, std::max(eig_tol, opt.seed_tol * w2));
     }
   }
   double max_resid = 0;
@@ -2188,7 +2236,7 @@ void Solver::solve_k_multi(std::vector<MultiProblem>& probs) {
       nchk[i] = j + 1;
       msize[i] = opt.guard < 0 ? q.nev : std::min(q.nev, j + 1 + opt.guard);
       w2[i] = st.omega2;
-      tol32[i] = std::max(q.eig_tol, opt.fp32_tol_rel * w2[i]);
+      tol32[i] = std::max(q.eig_tol, (opt.refine ? opt.fp32_tol_rel : opt.fp32_tol_omega2) * w2[i]);
       eig_tol[i] = q.eig_tol;
       kinit[i] = k0[p];
     }
@@ -2207,7 +2255,7 @@ void Solver::solve_k_multi(std::vector<MultiProblem>& probs) {
       const int i0 = off[p];
       act[i0] = 1;
       n0[i0] = m0[i0] = ncol[i0] = probs[p].nev;
-      t0[i0] = std::max(tfloor[i0], 3e-3 * w2[i0]);
+      t0[i0] = std::max(tfloor[i0], opt.seed_tol * w2[i0]);
     }
     OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kinit.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
     const int it = b.run(act, bands, n0, m0, t0, mi, stats);

@@ -137,13 +137,19 @@ class Profiler {
 };
 
 struct Options {
-  double fp32_tol_rel = 1e-5;   // FP32 LOBPCG locking threshold, relative to omega^2
+  double fp32_tol_rel = 1e-3;   // solve_k: FP32 LOBPCG locking threshold of the Newton phase, relative
+                                // to omega^2 (the FP64 refinement takes the vectors from there)
+  double fp32_tol_omega2 = 1e-5;  // solve_omega2: FP32 locking threshold before its FP64 LOBPCG
   double fp32_switch = 1e-6;    // |Δω²|/ω² below which Newton moves to the FP64 phase
+  double seed_tol = 5e-2;       // locking threshold / omega^2 of the shared first eigensolve
   int maxiter_fp64 = 200;       // LOBPCG iteration cap in the FP64 phase
   int guard = 2;                // block columns of band i: min(nev, i + 1 + guard); < 0: nev
   uint64_t seed = 0x5eed;
   int fp64_only = 0;            // skip the FP32 phase
   int reg_ops = 1;              // register-FFT operator where the grid admits it
+  int reg_rows32 = 0;           // register path, FP32: rows per block of passes 1/3 (0: 16; or 8)
+  int reg_rows64 = 0;           // register path, FP64: rows per block of passes 1/3 (0: 8; or 4)
+  int reg_nt2 = 0;              // register path: threads per block of pass 2 (0: 256; or 128)
   int slab3d = 1;               // 3D cells: the yz-slab operator where the cell admits it
   int plain = 0;                // the reference's algorithm: FP64 only, nev-column block per band,
                                 // random start at k_guess per band, every eigensolve to eig_tol
@@ -158,7 +164,7 @@ struct Options {
                                 // band's vector (0: FP64 LOBPCG on the whole block)
   int fp32_newton = 1;          // FP32 Newton steps before the FP64 phase (0: the FP64 phase starts
                                 // from the shared first FP32 eigensolve)
-  double jd_rtol = 1e-3;        // relative tolerance of the refinement's correction solves
+  double jd_rtol = 5e-3;        // relative tolerance of the refinement's correction solves
   int jd_maxit = 60;            // iteration cap of one correction solve
   int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
   double jac_stop32 = 1e-4;     // Jacobi stop (max |sin| of a sweep) of the FP32 batches' Rayleigh-Ritz

@@ -1,7 +1,8 @@
 """Autotuner wisdom for the HIP backend.
 
 ``omx_autotune`` times, on one handle's grid, the kernels of an nev-mode ``solve_k``: the
-operator path (register FFT or LDS Stockham), the Stockham FFT radix variant and tile shape,
+operator path (register FFT, with its block shapes, or LDS Stockham), the Stockham FFT radix
+variant and tile shape,
 on 3D cells the yz-slab or five-pass form, and the Gram and block-update row partitions, and
 makes the fastest the handle's options.  This module keeps those choices per (device, grid
 shape, nev) in a JSON file, as FFTW keeps its wisdom (the reference plans with
@@ -18,7 +19,8 @@ import numpy as np
 
 from . import _lib
 
-KEYS = ("reg_ops", "fft_variant32", "tile_rows32", "tile_cols32", "gram_blocks", "combine_blocks", "slab3d")
+KEYS = ("reg_ops", "fft_variant32", "tile_rows32", "tile_cols32", "gram_blocks", "combine_blocks", "slab3d",
+        "reg_rows32", "reg_nt2")
 
 
 def default_wisdom_path():

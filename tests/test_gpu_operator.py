@@ -101,3 +101,40 @@ def test_fp32_register_and_stockham_operators_agree(size):
     assert _relerr(out[True][0], out[False][0]) < 2e-6
     assert _relerr(out[True][1], out[False][1]) < 2e-6
     assert np.allclose(out[True][2], out[False][2], rtol=2e-5, atol=0)
+
+
+@pytest.mark.parametrize("size", [(4.0, 4.0, 64, 64), (6.0, 4.0, 128, 256), (8.0, 4.0, 256, 256),
+                                  (8.0, 4.0, 512, 256)], ids=lambda s: f"{s[2]}x{s[3]}")
+def test_register_path_block_shapes_match_oracle(size):
+    """The register path with half the rows per block in passes 1/3 (8 in FP32, 4 in FP64)
+    and half the threads per block in pass 2 (options reg_rows32, reg_rows64, reg_nt2, which
+    the autotuner may pick): M, P and HMkH against the oracle, including the HMkH partial
+    sums, whose count follows the rows per block."""
+    print(f"register block shapes {size}", flush=True)
+    Dx, Dy, nx, ny = size
+    og, g = O.Grid(Dx, Dy, nx, ny), omx.Grid(Dx, Dy, nx, ny)
+    ei = _aniso(og)
+    k = 1.21
+    rng = np.random.default_rng(nx + 3 * ny)
+    H = _rand(rng, 2 * og.N, 3)
+    ref = O.HelmholtzOracle(k, ei, og)
+    M_ref, P_ref = ref.M(H), ref.P(H)
+    hk_ref = np.array([ref.HMkH(H[:, j]) for j in range(3)])
+    for rows32, rows64, nt2 in ((8, 4, 128), (16, 8, 128), (8, 4, 256)):
+        solver = omx.HIPSolver()
+        solver.options.update(reg_rows32=float(rows32), reg_rows64=float(rows64), reg_nt2=float(nt2))
+        for prec, tol in ((omx.OMX_PREC_FP64, 1e-12), (omx.OMX_PREC_FP32, 2e-6)):
+            Mg = omx.HelmholtzMap(k, ei, g, solver, precision=prec)
+            assert _relerr(Mg * H, M_ref) < tol, (rows32, rows64, nt2, prec)
+            assert _relerr(Mg.precondition(H), P_ref) < tol, (rows32, rows64, nt2, prec)
+            assert np.allclose(Mg.HMkH(H), hk_ref, rtol=10 * tol, atol=0), (rows32, rows64, nt2, prec)
+
+
+def test_register_block_shape_options_are_checked():
+    g = omx.Grid(4.0, 4.0, 64, 64)
+    ei = _aniso(O.Grid(4.0, 4.0, 64, 64))
+    solver = omx.HIPSolver()
+    h = solver.handle(g, ei)
+    L = omx._lib.lib()
+    for key, bad in (("reg_rows32", 4), ("reg_rows64", 16), ("reg_nt2", 64)):
+        assert L.omx_set_option(h.h, key.encode(), float(bad)) == omx._lib.OMX_ERR_INVALID_ARG

@@ -1,6 +1,7 @@
 """Host logic of the autotuner's wisdom file (no GPU): keys, persistence and the options
 a stored choice sets."""
 import json
+import os
 
 from omx import tuning
 
@@ -28,6 +29,13 @@ def test_wisdom_keys_separate_grids_nev_and_devices():
 
 
 def test_choice_keys_are_solver_options():
-    # every stored key is an option omx_set_option accepts (the C side names them alike)
-    assert set(tuning.KEYS) == {"reg_ops", "fft_variant32", "tile_rows32", "tile_cols32", "gram_blocks",
-                                "combine_blocks", "slab3d"}
+    # every stored key is an option omx_set_option accepts, in the order omx_autotune's
+    # choice array reports them (the header lists that order)
+    assert tuning.KEYS == ("reg_ops", "fft_variant32", "tile_rows32", "tile_cols32", "gram_blocks",
+                           "combine_blocks", "slab3d", "reg_rows32", "reg_nt2")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = open(os.path.join(root, "optimode.jl_amd", "csrc", "solver.hip")).read()
+    hdr = open(os.path.join(root, "include", "omx.h")).read()
+    for k in tuning.KEYS:
+        assert f'"{k}"' in src and f'"{k}"' in hdr, k
+    assert "9 ints in that order" in hdr
