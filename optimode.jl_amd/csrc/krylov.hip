@@ -401,6 +401,123 @@ __global__ __launch_bounds__(AB) void k_proj_coef(const c64* part, int nblk, c64
   }
 }
 
+// Block projection: partials of <Q_c, x> (c < nq[i], complex, FP64) over this block's
+// rows into part[(i gridDim.x + blockIdx.x) QM + c] (zero for c >= nq[i])
+template <typename T, int QM>
+__global__ __launch_bounds__(AB) void k_qdot(const cplx<T>* Q, long qld, long qis, const int* nq, const cplx<T>* X,
+                                             long ld, long K, const int* act, c64* part) {
+  const int i = blockIdx.y;
+  const int n = act[i] ? nq[i] : 0;
+  __shared__ double wr[AB / 64][QM], wi[AB / 64][QM];
+  double sr[QM], si[QM];
+#pragma unroll
+  for (int c = 0; c < QM; ++c) sr[c] = si[c] = 0.0;
+  if (n > 0) {
+    const cplx<T>* x = X + i * ld;
+    const cplx<T>* q = Q + i * qis;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      const double xx = x[r].x, xy = x[r].y;
+#pragma unroll
+      for (int c = 0; c < QM; ++c)
+        if (c < n) {
+          const cplx<T> e = q[c * qld + r];
+          sr[c] += double(e.x) * xx + double(e.y) * xy;
+          si[c] += double(e.x) * xy - double(e.y) * xx;
+        }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < QM; ++c) {
+    double a = sr[c], b = si[c];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off, 64);
+      b += __shfl_down(b, off, 64);
+    }
+    if (lane == 0) {
+      wr[wv][c] = a;
+      wi[wv][c] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < QM) {
+    const int c = threadIdx.x;
+    double a = 0, b = 0;
+    for (int w = 0; w < AB / 64; ++w) {
+      a += wr[w][c];
+      b += wi[w][c];
+    }
+    part[(long(i) * gridDim.x + blockIdx.x) * QM + c] = c64{a, b};
+  }
+}
+
+// coef[i QM + c] = -<Q_c, x_i>: the fixed-order sum of instance i's nblk partials
+template <int QM>
+__global__ __launch_bounds__(AB) void k_qcoef(const c64* part, int nblk, c64* coef) {
+  constexpr int G = AB / QM;  // partial groups per coefficient
+  const int i = blockIdx.x;
+  const int c = threadIdx.x % QM, j = threadIdx.x / QM;
+  __shared__ double rr[G][QM], ri[G][QM];
+  double a = 0, b = 0;
+  for (int q = j; q < nblk; q += G) {
+    const c64 v = part[(long(i) * nblk + q) * QM + c];
+    a += v.x;
+    b += v.y;
+  }
+  rr[j][c] = a;
+  ri[j][c] = b;
+  __syncthreads();
+  if (threadIdx.x < QM) {
+    double sa = 0, sb = 0;
+    for (int g = 0; g < G; ++g) {
+      sa += rr[g][threadIdx.x];
+      sb += ri[g][threadIdx.x];
+    }
+    coef[long(i) * QM + threadIdx.x] = c64{-sa, -sb};
+  }
+}
+
+// x_i <- x_i + sum_c coef[i QM + c] Q_c (the block projection), and with d the partials of
+// Re <d_i, x_i new> into part[i gridDim.x + blockIdx.x]
+template <typename T, int QM>
+__global__ __launch_bounds__(AB) void k_qproject(const c64* coef, const cplx<T>* Q, long qld, long qis, const int* nq,
+                                                 cplx<T>* X, const cplx<T>* d, long ld, long K, const int* act,
+                                                 double* part) {
+  const int i = blockIdx.y;
+  __shared__ double red[AB];
+  double acc = 0;
+  if (act[i]) {
+    const int n = nq[i];
+    c64 cf[QM];
+#pragma unroll
+    for (int c = 0; c < QM; ++c) cf[c] = c < n ? coef[long(i) * QM + c] : c64{0, 0};
+    cplx<T>* x = X + i * ld;
+    const cplx<T>* q = Q + i * qis;
+    const cplx<T>* dv = d ? d + i * ld : nullptr;
+    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      double nx = x[r].x, ny = x[r].y;
+#pragma unroll
+      for (int c = 0; c < QM; ++c)
+        if (c < n) {
+          const cplx<T> e = q[c * qld + r];
+          nx += cf[c].x * double(e.x) - cf[c].y * double(e.y);
+          ny += cf[c].x * double(e.y) + cf[c].y * double(e.x);
+        }
+      const cplx<T> xn{T(nx), T(ny)};
+      x[r] = xn;
+      if (dv) acc += double(dv[r].x) * double(xn.x) + double(dv[r].y) * double(xn.y);
+    }
+  }
+  if (!part) return;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = AB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[long(i) * gridDim.x + blockIdx.x] = red[0];
+}
+
 }  // namespace
 
 double sum_part(const double* p, int nb) {
@@ -456,7 +573,36 @@ void VecOpsT<T>::lincomb(std::initializer_list<const C*> ins, C* outp, const int
   }
 }
 
+template <typename T> void VecOpsT<T>::set_block(const C* Qb, long qis, const int* nq) {
+  // block partials (nb nblk QMAX complex), then the coefficients (nb QMAX complex)
+  part.ensure(sizeof(c64) * size_t(nb) * (size_t(nblk) * QMAX + QMAX));
+  Q = Qb;
+  qistride = qis;
+  qn = nq;
+}
+
+template <typename T> void VecOpsT<T>::project_dot_to(C* x, const int* act, const C* dot_with, double* out) {
+  const dim3 g(nblk, nb);
+  if (Q) {
+    c64* qp = part.template as<c64>();
+    c64* qc = qp + size_t(nb) * nblk * QMAX;
+    hipLaunchKernelGGL((k_qdot<T, QMAX>), g, dim3(AB), 0, s, Q, ld, qistride, qn, x, ld, K, act, qp);
+    hipLaunchKernelGGL((k_qcoef<QMAX>), dim3(nb), dim3(AB), 0, s, qp, nblk, qc);
+    hipLaunchKernelGGL((k_qproject<T, QMAX>), g, dim3(AB), 0, s, qc, Q, ld, qistride, qn, x, dot_with, ld, K, act,
+                       out);
+    return;
+  }
+  hipLaunchKernelGGL(k_cdot<T>, g, dim3(AB), 0, s, EV, x, ld, K, act, part.template as<c64>());
+  hipLaunchKernelGGL((k_project_dot<T>), g, dim3(AB), 0, s, part.template as<c64>(), nblk, EV, x, dot_with, ld, K,
+                     act, out);
+}
+
 template <typename T> void VecOpsT<T>::project(C* x, const int* act, const C* dot_with) {
+  if (Q) {
+    // the real partials land at the start of part, past which the block coefficients sit
+    project_dot_to(x, act, dot_with, dot_with ? part.template as<double>() : nullptr);
+    return;
+  }
   hipLaunchKernelGGL(k_cdot<T>, dim3(nblk, nb), dim3(AB), 0, s, EV, x, ld, K, act, part.template as<c64>());
   hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(AB), 0, s, part.template as<c64>(), nblk,
                      coef.template as<c64>() + 4 * size_t(nb) * 3);
@@ -536,10 +682,8 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
     std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
     BView<T> R2b{r2, ld, ld}, Yb{y, ld, ld};
     launch_operator<T>(OP_P, plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
-    // y <- y - ev <ev, y> with the partials of <r2, y>
-    hipLaunchKernelGGL((k_cdot<T>), vg, dim3(AB), 0, s, V.EV, y, ld, V.K, la, V.part.template as<c64>());
-    hipLaunchKernelGGL((k_project_dot<T>), vg, dim3(AB), 0, s, V.part.template as<c64>(), nblk, V.EV, y, r2, ld,
-                       V.K, la, cpart2);
+    // y <- y - ev <ev, y> (or the block projection) with the partials of <r2, y>
+    V.project_dot_to(y, la, r2, cpart2);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, cpart2, nblk, coef, 3, nb, la);
     // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
     // w1 <- w2, w2 <- w), written into the buffer of the discarded w1, and x += phi w_new

@@ -35,6 +35,16 @@ template <typename T> struct VecOpsT {
   // x_i <- x_i - ev_i <ev_i, x_i> (coefficient slot 3); with dot_with, also the partials of
   // Re <dot_with_i, x_i> of the projected x in part
   void project(C* x, const int* act, const C* dot_with = nullptr);
+  // the same, the partials of Re <dot_with_i, x_i> into out (layout of part)
+  void project_dot_to(C* x, const int* act, const C* dot_with, double* out);
+  // Block deflation: from here on the projections remove the components of x_i along the
+  // nq[i] orthonormal columns of instance i's block Qb (column c at Qb + i qis + c ld, at
+  // most QMAX columns; nq on the device) instead of along ev_i.
+  static constexpr int QMAX = 16;
+  void set_block(const C* Qb, long qis, const int* nq);
+  const C* Q = nullptr;
+  long qistride = 0;
+  const int* qn = nullptr;
 };
 using VecOps = VecOpsT<double>;
 

@@ -604,14 +604,26 @@ void rop_rows_rb(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T
                        dot_partial, dps);
 }
 
-// rows per block: reg_rows<T>() or half of it (plan.reg_rb)
+// Blocks below which a register-path launch takes the halved block shape when the plan
+// leaves it open (two blocks per CU of a 256-CU device): launches of few vectors (the
+// refinement's correction solves, the shared first eigensolve, the last active bands)
+// otherwise leave CUs idle.
+constexpr int REG_FILL_BLOCKS = 512;
+
+// rows per block: reg_rows<T>() or half of it (plan.reg_rb; 0: half for M and P launches
+// of fewer than REG_FILL_BLOCKS blocks; the HMkH launches keep the plan's count, which
+// fixes their dot-partial layout)
 template <typename T, int OP, int LX>
 void rop_rows(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
               const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
   constexpr int R0 = reg_rows<T>();
-  if (reg_rows_of(plan) == R0)
+  int rb = reg_rows_of(plan);
+  if (plan.reg_rb == 0 && (OP == OP_M || OP == OP_P) &&
+      long(plan.ny / R0) * sel.ninst * sel.mper < REG_FILL_BLOCKS)
+    rb = R0 / 2;
+  if (rb == R0)
     rop_rows_rb<T, OP, LX, R0>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream);
-  else if (reg_rows_of(plan) == R0 / 2)
+  else if (rb == R0 / 2)
     rop_rows_rb<T, OP, LX, R0 / 2>(pass, plan, X, Y, D, sel, kinst, dot_partial, dps, stream);
   else
     throw Error(OMX_ERR_INTERNAL, "register FFT path: unsupported rows per block");
@@ -625,12 +637,16 @@ void rop_cols_nt(const OpPlan<T>& plan, cplx<T>* D, const VecSel& sel, hipStream
   hipLaunchKernelGGL((k_rpass2<T, OP, LY, NT>), grid, dim3(NT), lds, stream, plan, D, sel);
 }
 
-// threads per block of pass 2: REG_COL_THREADS or half of it (plan.reg_nt2)
+// threads per block of pass 2: REG_COL_THREADS or half of it (plan.reg_nt2; 0: half for
+// launches of fewer than REG_FILL_BLOCKS blocks)
 template <typename T, int OP, int LY>
 void rop_cols(const OpPlan<T>& plan, cplx<T>* D, const VecSel& sel, hipStream_t stream) {
-  if (plan.reg_nt2 == 0 || plan.reg_nt2 == REG_COL_THREADS)
+  int nt = plan.reg_nt2 > 0 ? plan.reg_nt2 : REG_COL_THREADS;
+  if (plan.reg_nt2 == 0 && long(plan.nx / (REG_COL_THREADS / (LY / 8))) * sel.ninst * sel.mper < REG_FILL_BLOCKS)
+    nt = REG_COL_THREADS / 2;
+  if (nt == REG_COL_THREADS)
     rop_cols_nt<T, OP, LY, REG_COL_THREADS>(plan, D, sel, stream);
-  else if (plan.reg_nt2 == REG_COL_THREADS / 2)
+  else if (nt == REG_COL_THREADS / 2)
     rop_cols_nt<T, OP, LY, REG_COL_THREADS / 2>(plan, D, sel, stream);
   else
     throw Error(OMX_ERR_INTERNAL, "register FFT path: unsupported pass-2 block");

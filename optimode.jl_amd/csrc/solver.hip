@@ -746,7 +746,7 @@ void Solver::set_option(const std::string& key, double v) {
   else if (key == "fp32_switch")
     opt.fp32_switch = v;
   else if (key == "seed_tol")
-    opt.seed_tol = v;
+    opt.seed_tol = std::max(0.0, v);
   else if (key == "maxiter_fp64")
     opt.maxiter_fp64 = int(v);
   else if (key == "guard")
@@ -771,6 +771,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_maxit = int(v);
   else if (key == "jd_fp32")
     opt.jd_fp32 = v != 0 ? 1 : 0;
+  else if (key == "jd_deflate")
+    opt.jd_deflate = v != 0 ? 1 : 0;
   else if (key == "streams2")
     opt.streams2 = v != 0 ? 1 : 0;
   else if (key == "graphs")
@@ -965,8 +967,9 @@ void Solver::autotune(int nev, int* choice_out, double* us_out) {
   const std::vector<c32> ref = op_out0();  // default Stockham plan (or the 3D default form)
   if (nz == 1) {
     if (p32.reg_ok)
-      for (int rr : {16, 8})
-        for (int nt2 : {REG_COL_THREADS, REG_COL_THREADS / 2}) try_plan(OpChoice{1, 0, 0, 0, saved.slab3d, rr, nt2}, &ref);
+      for (int rr : {0, 16, 8})  // 0: by launch size
+        for (int nt2 : {0, REG_COL_THREADS, REG_COL_THREADS / 2})
+          try_plan(OpChoice{1, 0, 0, 0, saved.slab3d, rr, nt2}, &ref);
     const int budget = 3072;
     for (int variant : {0, 1})
       for (int rows : {8, 4, 2, 1}) {
@@ -1556,6 +1559,12 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
   hipStream_t s = S.stream;
   const auto t_start = std::chrono::steady_clock::now();
   int jd_total = 0, minres_total = 0;
+  // host wall per part (OMX_TRACE): Rayleigh-Ritz, residuals, correction solves, the rest
+  double t_rr = 0, t_res = 0, t_mr = 0;
+  auto tnow = [] { return std::chrono::steady_clock::now(); };
+  auto tms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
   const int nblk_g = int(std::max<long>(1, std::min<long>((K + 31) / 32, std::max(16, 1024 / ninst))));
   const int nblk_r = int(std::min<long>(256, (K + 255) / 256));
   RefineWorkspace& W = S.refine_ws;
@@ -1599,6 +1608,12 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
   // Rayleigh-Ritz of [X | t] (t only where with_t[i]) for the instances with n1h[i] > 0;
   // X, AX <- [X | t] C, [AX | At] C with C the n1h[i] lowest Ritz vectors
   auto rr = [&](const std::vector<int>& with_t) {
+    const auto t0 = tnow();
+    struct Acc {
+      double& t;
+      std::chrono::steady_clock::time_point a;
+      ~Acc() { t += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count(); }
+    } acc{t_rr, t0};
     bool any_t = false;
     for (int i = 0; i < ninst; ++i) any_t = any_t || (n1h[i] > 0 && with_t[i]);
     const RCols Ux{b.p(b.X[b.cur]), ld, istride, any_t ? Tm.as<c64>() : nullptr};
@@ -1630,6 +1645,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
   };
   // band column's x (XB), residual (RBm) and residual norms for the instances in n1h
   auto band_resid = [&]() {
+    const auto t0 = tnow();
     OMX_HIP_CHECK(hipMemcpyAsync(lamd.as<void>(), lam_band.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
     launch_band_resid(b.p(b.X[b.cur]), b.p(b.AX[b.cur]), ld, istride, cold.as<int>(), lamd.as<double>(),
                       n1d.as<int>(), ninst, K, nblk_r, XB.as<c64>(), RBm.as<c64>(), normp.as<double>(), s);
@@ -1637,6 +1653,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
     OMX_HIP_CHECK(hipMemcpyAsync(h.data(), normp.as<void>(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
     OMX_HIP_CHECK(hipStreamSynchronize(s));
     for (int i = 0; i < ninst; ++i) rn[i] = n1h[i] ? std::sqrt(sum_part(h.data() + size_t(i) * nblk_r, nblk_r)) : 0.0;
+    t_res += tms(t0, tnow());
   };
   const int jd_max = 8;
   int outer = 0;
@@ -1666,6 +1683,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       // t: (M - rho) t = r on the complement of x (the sign of t is immaterial)
       VecOps V(S, s, ninst, ld, K, XB.as<c64>());
       int its;
+      const auto t_mr0 = tnow();
       if (S.opt.jd_fp32) {
         // the correction only needs jd_rtol (5e-3) relative accuracy: FP32 vectors and
         // operator, FP64 recurrences; the residual that decides convergence stays FP64
@@ -1673,6 +1691,16 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
         launch_convert<double, float>(XB.as<c64>(), ld, ld, W.XBf.as<cf>(), ld, ld, ninst, 1, K, s);
         launch_convert<double, float>(RBm.as<c64>(), ld, ld, W.RBf.as<cf>(), ld, ld, ninst, 1, K, s);
         VecOpsT<float> V32(S, s, ninst, ld, K, W.XBf.as<cf>());
+        if (S.opt.jd_deflate) {
+          // the block after its Rayleigh-Ritz step: orthonormal Ritz vectors, the band's x
+          // among them, and r orthogonal to all of them
+          const long qis = long(b.M) * ld;
+          W.Xf.ensure(sizeof(cf) * size_t(ninst) * qis);
+          W.nqd.ensure(sizeof(int) * size_t(ninst));
+          launch_convert<double, float>(b.p(b.X[b.cur]), ld, istride, W.Xf.as<cf>(), ld, qis, ninst, b.M, K, s);
+          OMX_HIP_CHECK(hipMemcpyAsync(W.nqd.as<void>(), msize.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
+          V32.set_block(W.Xf.as<cf>(), qis, W.nqd.as<int>());
+        }
         its = minres_complement_t<float>(S, V32, lam_band, b.kd.as<double>(), W.RBf.as<cf>(), W.Tf.as<cf>(), jdh,
                                          S.opt.jd_rtol, S.opt.jd_maxit, false);
         launch_convert<float, double>(W.Tf.as<cf>(), ld, ld, Tm.as<c64>(), ld, ld, ninst, 1, K, s);
@@ -1686,6 +1714,8 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       }
       ++jd_total;
       minres_total += its;
+      OMX_HIP_CHECK(hipStreamSynchronize(s));
+      t_mr += tms(t_mr0, tnow());
       // t <- t - X (X^H t), twice, then unit norm (instances with jdh only)
       std::vector<int> n1j(ninst);
       for (int i = 0; i < ninst; ++i) n1j[i] = jdh[i] ? msize[i] : 0;
@@ -1780,10 +1810,9 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
     }
   }
   if (S.trace)
-    std::fprintf(stderr, "[omx] fp64 refine: %.2f ms host wall, %d Rayleigh-Ritz steps, %d correction solves, %d MINRES "
-                 "iterations\n",
-                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count(), outer,
-                 jd_total, minres_total);
+    std::fprintf(stderr, "[omx] fp64 refine: %.2f ms host wall (Rayleigh-Ritz %.2f, residuals %.2f, correction solves "
+                 "%.2f), %d Rayleigh-Ritz steps, %d correction solves, %d MINRES iterations\n",
+                 tms(t_start, tnow()), t_rr, t_res, t_mr, outer, jd_total, minres_total);
   return outer;
 }
 
@@ -1975,7 +2004,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       // the shared first eigensolve in instance 0 of the first half, as seed_bands
       std::vector<int> act(na, 0), n0(na, M), m0(na, M), bA(bands.begin(), bands.begin() + na);
       act[0] = 1;
-      std::vector<double> t0(na, std::max(tol32, opt.seed_tol * w2)), kh(ninst, k0);
+      std::vector<double> t0(na, std::max(tol32, seed_tol_rel() * w2)), kh(ninst, k0);
       OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * na, hipMemcpyHostToDevice, stream));
       stats.it32 += b32.run(act, bA, n0, m0, t0, maxiter, stats);
       const auto lam = b32.rayleigh();
@@ -2009,7 +2038,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     } else {
       launch_randn<float>(b32.p(b32.X[0]), ld, b32.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) put_guess(b32, 0, 1);
-      stats.it32 += seed_bands(b32, std::max(tol32, opt.seed_tol * w2));
+      stats.it32 += seed_bands(b32, std::max(tol32, seed_tol_rel() * w2));
     }
     if (split) {
       stats.it32 += newton_phase2(*this, b32, *b32q, hns, bands, nchk, msize, std::vector<double>(ninst, tol32), maxiter,
@@ -2048,7 +2077,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
     } else {
       launch_randn<double>(b64.p(b64.X[0]), ld, b64.istride, 1, M, K, opt.seed, stream);
       if (nguess_used) put_guess(b64, 0, 1);
-      stats.it64 += seed_bands(b64, std::max(eig_tol, opt.seed_tol * w2));
+      stats.it64 += seed_bands(b64, std::max(eig_tol, seed_tol_rel() * w2));
     }
   }
   double max_resid = 0;
@@ -2255,7 +2284,7 @@ void Solver::solve_k_multi(std::vector<MultiProblem>& probs) {
       const int i0 = off[p];
       act[i0] = 1;
       n0[i0] = m0[i0] = ncol[i0] = probs[p].nev;
-      t0[i0] = std::max(tfloor[i0], opt.seed_tol * w2[i0]);
+      t0[i0] = std::max(tfloor[i0], seed_tol_rel() * w2[i0]);
     }
     OMX_HIP_CHECK(hipMemcpyAsync(b.kd.template as<void>(), kinit.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, s));
     const int it = b.run(act, bands, n0, m0, t0, mi, stats);

@@ -141,15 +141,17 @@ struct Options {
                                 // to omega^2 (the FP64 refinement takes the vectors from there)
   double fp32_tol_omega2 = 1e-5;  // solve_omega2: FP32 locking threshold before its FP64 LOBPCG
   double fp32_switch = 1e-6;    // |Δω²|/ω² below which Newton moves to the FP64 phase
-  double seed_tol = 5e-2;       // locking threshold / omega^2 of the shared first eigensolve
+  double seed_tol = 0;          // locking threshold / omega^2 of the shared first eigensolve
+                                // (0: 5e-2 on 2D cells, 3e-3 on 3D cells, as measured)
   int maxiter_fp64 = 200;       // LOBPCG iteration cap in the FP64 phase
   int guard = 2;                // block columns of band i: min(nev, i + 1 + guard); < 0: nev
   uint64_t seed = 0x5eed;
   int fp64_only = 0;            // skip the FP32 phase
   int reg_ops = 1;              // register-FFT operator where the grid admits it
-  int reg_rows32 = 0;           // register path, FP32: rows per block of passes 1/3 (0: 16; or 8)
-  int reg_rows64 = 0;           // register path, FP64: rows per block of passes 1/3 (0: 8; or 4)
-  int reg_nt2 = 0;              // register path: threads per block of pass 2 (0: 256; or 128)
+  int reg_rows32 = 0;           // register path, FP32: rows per block of passes 1/3 (16 or 8; 0: by
+                                // launch size, 8 for M/P launches of few vectors, else 16)
+  int reg_rows64 = 0;           // register path, FP64: the same with 8 or 4
+  int reg_nt2 = 0;              // register path: threads per block of pass 2 (256 or 128; 0: by launch size)
   int slab3d = 1;               // 3D cells: the yz-slab operator where the cell admits it
   int plain = 0;                // the reference's algorithm: FP64 only, nev-column block per band,
                                 // random start at k_guess per band, every eigensolve to eig_tol
@@ -167,6 +169,8 @@ struct Options {
   double jd_rtol = 5e-3;        // relative tolerance of the refinement's correction solves
   int jd_maxit = 60;            // iteration cap of one correction solve
   int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
+  int jd_deflate = 1;           // FP32 correction solves on the complement of the band's whole block
+                                // (not only of its vector): the neighbouring bands leave the spectrum
   double jac_stop32 = 1e-4;     // Jacobi stop (max |sin| of a sweep) of the FP32 batches' Rayleigh-Ritz
                                 // (off-diagonal left ~1e-8 |A|, below FP32 data; FP64 batches: 1e-9)
   int gram_blocks = 0;          // Gram products: about this many row-chunk blocks over the active
@@ -208,6 +212,7 @@ struct RefineWorkspace {
   long ld = 0;
   DevMem n1d, zero, cold, lamd, Tm, T2, ATm, XB, RBm, part, Gd, Bd, Cd, normp, scl, dots;
   DevMem XBf, RBf, Tf;  // FP32 copies for the correction solves (option jd_fp32)
+  DevMem Xf, nqd;       // FP32 copy of the block and its column counts (option jd_deflate)
 };
 
 // One problem of a batched solve on a handle's grid (Solver::solve_k_multi).
@@ -260,6 +265,7 @@ class Solver {
   // timed trials of the operator path, FFT radix variant, Stockham tile shape and the Gram /
   // block-update partitions for nev-mode solves; the fastest become the handle's options
   void autotune(int nev, int* choice_out, double* us_out);
+  double seed_tol_rel() const { return opt.seed_tol > 0 ? opt.seed_tol : (nz > 1 ? 3e-3 : 5e-2); }
   // per mode j: dw2dk[j] = 2 HMkH(ev_j) and dw2dL[j] = 2 <ev_j| dM/dDz |ev_j> (3D cells)
   void period_derivatives(int nev, const double* kmags, const double* evecs, double* dw2dk, double* dw2dL);
   void require_2d(const char* what) const;
