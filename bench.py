@@ -334,6 +334,31 @@ def batch_workload(a):
     print(json.dumps(out), flush=True)
 
 
+def copy_workload(a):
+    """A plain device copy of 256 MiB (torch's copy kernel): each dispatch reads and writes
+    exactly 262144 KiB, which calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters, and its
+    rate is the copy bandwidth the box reaches (the practical ceiling of an HBM-bound pass)."""
+    import torch
+    n = 256 << 20
+    src = torch.empty(n // 4, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(max(1, a.warmup)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    gbs = 2 * n / dt / 1e9
+    print(json.dumps({"metric": "device copy bandwidth (read + write bytes / s)", "value": round(gbs, 1),
+                      "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                      "ms_per_step": round(1e3 * dt, 4), "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "copy of 256 MiB (64 Mi float32) on the device", "bytes_per_step": 2 * n},
+                      "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,10 +372,11 @@ def main():
                     help="start of the timed solves: the shipped warm-start network (auto: when it loads) or plain")
     ap.add_argument("--tune", default="off", choices=["off", "on"],
                     help="on: omx_autotune's plan and partitions for the grid (wisdom file) before timing")
-    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "periodic", "batch"],
+    ap.add_argument("--workload", default="solve_k", choices=["solve_k", "eme", "periodic", "batch", "copy"],
                     help="solve_k: the 256x256 TFLN modes/s line (default); eme: the 512x256 EME slice chain; "
                          "periodic: solve_k on a 128x96x24 3D Bragg-grating cell; batch: the 64-instance "
-                         "mixed-grid batch")
+                         "mixed-grid batch; copy: a plain 256 MiB device copy (the HBM counters' calibration "
+                         "and the copy bandwidth the box reaches)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every solve here fits one MI355X and "
@@ -364,6 +390,9 @@ def main():
         return
     if a.workload == "batch":
         batch_workload(a)
+        return
+    if a.workload == "copy":
+        copy_workload(a)
         return
 
     import numpy as np

@@ -573,12 +573,17 @@ void VecOpsT<T>::lincomb(std::initializer_list<const C*> ins, C* outp, const int
   }
 }
 
-template <typename T> void VecOpsT<T>::set_block(const C* Qb, long qis, const int* nq) {
+template <typename T> void VecOpsT<T>::set_block(const C* Qb, long qis, const int* nq, int nq_max) {
+  if (nq_max < 1 || nq_max > QMAX) throw Error(OMX_ERR_INTERNAL, "block projection: 1..16 columns");
   // block partials (nb nblk QMAX complex), then the coefficients (nb QMAX complex)
   part.ensure(sizeof(c64) * size_t(nb) * (size_t(nblk) * QMAX + QMAX));
   Q = Qb;
   qistride = qis;
   qn = nq;
+  qmax = nq_max;
+  // a few thousand rows per block: the per-block reduction of the column dots stays small
+  // beside the rows' loads
+  qblk = std::max(1, std::min(nblk, 64));
 }
 
 template <typename T> void VecOpsT<T>::project_dot_to(C* x, const int* act, const C* dot_with, double* out) {
@@ -586,10 +591,18 @@ template <typename T> void VecOpsT<T>::project_dot_to(C* x, const int* act, cons
   if (Q) {
     c64* qp = part.template as<c64>();
     c64* qc = qp + size_t(nb) * nblk * QMAX;
-    hipLaunchKernelGGL((k_qdot<T, QMAX>), g, dim3(AB), 0, s, Q, ld, qistride, qn, x, ld, K, act, qp);
-    hipLaunchKernelGGL((k_qcoef<QMAX>), dim3(nb), dim3(AB), 0, s, qp, nblk, qc);
-    hipLaunchKernelGGL((k_qproject<T, QMAX>), g, dim3(AB), 0, s, qc, Q, ld, qistride, qn, x, dot_with, ld, K, act,
-                       out);
+    const dim3 gq(qblk, nb);
+    if (qmax <= 8) {
+      hipLaunchKernelGGL((k_qdot<T, 8>), gq, dim3(AB), 0, s, Q, ld, qistride, qn, x, ld, K, act, qp);
+      hipLaunchKernelGGL((k_qcoef<8>), dim3(nb), dim3(AB), 0, s, qp, qblk, qc);
+      hipLaunchKernelGGL((k_qproject<T, 8>), g, dim3(AB), 0, s, qc, Q, ld, qistride, qn, x, dot_with, ld, K, act,
+                         out);
+    } else {
+      hipLaunchKernelGGL((k_qdot<T, QMAX>), gq, dim3(AB), 0, s, Q, ld, qistride, qn, x, ld, K, act, qp);
+      hipLaunchKernelGGL((k_qcoef<QMAX>), dim3(nb), dim3(AB), 0, s, qp, qblk, qc);
+      hipLaunchKernelGGL((k_qproject<T, QMAX>), g, dim3(AB), 0, s, qc, Q, ld, qistride, qn, x, dot_with, ld, K,
+                         act, out);
+    }
     return;
   }
   hipLaunchKernelGGL(k_cdot<T>, g, dim3(AB), 0, s, EV, x, ld, K, act, part.template as<c64>());

@@ -41,10 +41,12 @@ template <typename T> struct VecOpsT {
   // nq[i] orthonormal columns of instance i's block Qb (column c at Qb + i qis + c ld, at
   // most QMAX columns; nq on the device) instead of along ev_i.
   static constexpr int QMAX = 16;
-  void set_block(const C* Qb, long qis, const int* nq);
+  void set_block(const C* Qb, long qis, const int* nq, int nq_max);
   const C* Q = nullptr;
   long qistride = 0;
   const int* qn = nullptr;
+  int qmax = 0;   // the largest nq[i] (host): columns the block kernels are compiled for
+  int qblk = 0;   // row blocks per instance of the block dots
 };
 using VecOps = VecOpsT<double>;
 

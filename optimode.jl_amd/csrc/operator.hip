@@ -610,15 +610,17 @@ void rop_rows_rb(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T
 // otherwise leave CUs idle.
 constexpr int REG_FILL_BLOCKS = 512;
 
-// rows per block: reg_rows<T>() or half of it (plan.reg_rb; 0: half for M and P launches
-// of fewer than REG_FILL_BLOCKS blocks; the HMkH launches keep the plan's count, which
-// fixes their dot-partial layout)
+// rows per block: reg_rows<T>() or half of it (plan.reg_rb; 0: half in pass 1 of M and P
+// launches of fewer than REG_FILL_BLOCKS blocks).  Pass 3 keeps the full count when the plan
+// leaves it open: its column-major reads of half as many rows are 64-byte runs, which fetch
+// whole 128-byte lines (twice the intermediate's bytes, counted); the HMkH launches keep the
+// plan's count, which fixes their dot-partial layout.
 template <typename T, int OP, int LX>
 void rop_rows(int pass, const OpPlan<T>& plan, BView<T> X, BView<T> Y, cplx<T>* D, const VecSel& sel,
               const double* kinst, double* dot_partial, int dps, hipStream_t stream) {
   constexpr int R0 = reg_rows<T>();
   int rb = reg_rows_of(plan);
-  if (plan.reg_rb == 0 && (OP == OP_M || OP == OP_P) &&
+  if (plan.reg_rb == 0 && pass == 1 && (OP == OP_M || OP == OP_P) &&
       long(plan.ny / R0) * sel.ninst * sel.mper < REG_FILL_BLOCKS)
     rb = R0 / 2;
   if (rb == R0)

@@ -1699,7 +1699,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
           W.nqd.ensure(sizeof(int) * size_t(ninst));
           launch_convert<double, float>(b.p(b.X[b.cur]), ld, istride, W.Xf.as<cf>(), ld, qis, ninst, b.M, K, s);
           OMX_HIP_CHECK(hipMemcpyAsync(W.nqd.as<void>(), msize.data(), sizeof(int) * ninst, hipMemcpyHostToDevice, s));
-          V32.set_block(W.Xf.as<cf>(), qis, W.nqd.as<int>());
+          V32.set_block(W.Xf.as<cf>(), qis, W.nqd.as<int>(), *std::max_element(msize.begin(), msize.end()));
         }
         its = minres_complement_t<float>(S, V32, lam_band, b.kd.as<double>(), W.RBf.as<cf>(), W.Tf.as<cf>(), jdh,
                                          S.opt.jd_rtol, S.opt.jd_maxit, false);
