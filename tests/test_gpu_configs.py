@@ -81,9 +81,18 @@ def test_mode_fields_and_group_index_match_golden(name, builder):
               flush=True)
         assert low_def <= 2e-6
         assert overlap_est >= 1 - 1e-6
+    # n_g is first order in the eigenvector error (HMkH is not variational): at the workload's
+    # eig_tol 1e-8 it agrees to about 1e-8; solved to the fixture's own tolerances (eig_tol
+    # 1e-10) it must agree to 1e-8 with room to spare
     ng = omx.group_index(km, evs, float(f["omega"]), ei, None, g, solver)
     print(name, "n_g", ng, "rel err", np.abs(ng / f["ng"] - 1).max(), flush=True)
-    assert np.allclose(ng, f["ng"], rtol=1e-8, atol=0)
+    assert np.allclose(ng, f["ng"], rtol=1e-7, atol=0)
+    tight = omx.HIPSolver()
+    km2, evs2 = omx.solve_k(float(f["omega"]), ei, g, tight, nev=int(f["nev"]), k_tol=float(f["k_tol"]),
+                            eig_tol=float(f["eig_tol"]))
+    ng2 = omx.group_index(km2, evs2, float(f["omega"]), ei, None, g, tight)
+    print(name, "n_g at eig_tol", float(f["eig_tol"]), "rel err", np.abs(ng2 / f["ng"] - 1).max(), flush=True)
+    assert np.allclose(ng2, f["ng"], rtol=1e-8, atol=0)
 
 
 def _rel(a, b):
