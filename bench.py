@@ -28,10 +28,12 @@ OMEGA = 1 / 1.55
 NEV = 8
 REL_TOL = 1e-8
 # HBM bytes per algorithmic byte of the FP32 operator application on this workload, from
-# rocprofv3 FETCH_SIZE + WRITE_SIZE of k_rpass1/2/3<float, OP_M> (FETCH scaled by 2.0, WRITE
-# by 1.0: the factors a 256 MiB copy kernel reads back on the same box); the measurement is
-# kept in profiles/solve_k_tfln256_pmc_traffic.md.  Update when the kernels change.
-OP_TRAFFIC_RATIO = 3.82
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of the k_rpass1/2/3<float, OP_M> launches of the
+# refinement's correction solves (8 vectors, all active, so their algorithmic bytes are
+# exact), FETCH scaled by 2.0 and WRITE by 1.0 (the factors `bench.py --workload copy`, a
+# 256 MiB device copy, reads back on the same box); the measurement is kept in
+# profiles/solve_k_tfln256_pmc_traffic.md.  Update when the kernels change.
+OP_TRAFFIC_RATIO = 3.78
 
 
 def cpu_baseline(seconds_cap=60.0):
