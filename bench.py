@@ -107,8 +107,11 @@ def eme_workload(a):
     nw, ns = a.warmup, a.steps
     cells = [(lambda i=i: eme_cell(grid, i)) for i in range(nw + ns)]
     solver = omx.HIPSolver(precision=a.precision)
+    # the coupler network starts the chain's cold first slice (and any cold restart)
+    from omx import warmstart as WS
+    learned = WS.load_coupler() if a.start != "plain" else None
     # warm-up slices (also the chain's start), then the timed slices continue the same chain
-    X.eme(cells[:nw], [L] * nw, OMEGA, grid, solver, nev=nev, warmstart=True, **kw)
+    X.eme(cells[:nw], [L] * nw, OMEGA, grid, solver, nev=nev, warmstart=True, learned=learned, **kw)
     t0 = time.perf_counter()
     res = X.eme(cells[nw:], [L] * ns, OMEGA, grid, solver, nev=nev, warmstart=True, continue_chain=True, **kw)
     elapsed = time.perf_counter() - t0
@@ -124,6 +127,17 @@ def eme_workload(a):
     tc = time.perf_counter()
     rc = X.eme(cells[nw:nw + cold_n], [L] * cold_n, OMEGA, grid, sc, nev=nev, warmstart=False, **kw)
     cold_ms = 1e3 * (time.perf_counter() - tc) / cold_n
+    # the same cold slices from the coupler network's guess
+    cold_learned = None
+    if learned is not None:
+        tl = time.perf_counter()
+        rl = X.eme(cells[nw:nw + cold_n], [L] * cold_n, OMEGA, grid, sc, nev=nev, warmstart=False, learned=learned,
+                   **kw)
+        cold_learned = dict(ms_per_slice=round(1e3 * (time.perf_counter() - tl) / cold_n, 2),
+                            newton_per_slice=float(np.mean([st["n_newton"] for st in rl.stats])),
+                            lobpcg_iters_fp32_per_slice=float(np.mean([st["lobpcg_iters_fp32"] for st in rl.stats])),
+                            max_rel_diff_k_vs_plain_cold=float(max(np.max(np.abs(x / y - 1))
+                                                                   for x, y in zip(rl.kmags, rc.kmags))))
     out = {
         "metric": "EME slices/s at 512x256, 16 modes per slice (previous-slice warm start)",
         "value": round(ns / elapsed, 3),
@@ -149,6 +163,9 @@ def eme_workload(a):
             "last_slice_max_rel_err_k2_vs_cold_fp64": rel,
             "cold_chain_ms_per_slice": round(cold_ms, 2),
             "cold_newton_per_slice": float(np.mean([st["n_newton"] for st in rc.stats])),
+            "cold_lobpcg_iters_fp32_per_slice": float(np.mean([st["lobpcg_iters_fp32"] for st in rc.stats])),
+            "cold_learned_start": cold_learned,
+            "first_slice_start": "learned (omx/data/warmnet_coupler512x256.npz)" if learned is not None else "plain",
             "power_coupling_1_1": X.power_coupling(res),
         },
     }

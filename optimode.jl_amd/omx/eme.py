@@ -171,26 +171,56 @@ class EMEResult:
 
 def eme(epsinvs, lengths, omega, grid, solver, nev=2, conjugate=False, reg=1e-9, reciprocity=True,
         passivity="invert", warmstart=True, k_tol=1e-8, eig_tol=1e-8, max_eigsolves=60, maxiter=100,
-        continue_chain=False):
+        continue_chain=False, learned=None):
     """``eme(ε⁻¹s, ∂ε_∂ωs, lengths, ω, grid, solver; nev, conjugate, reg, reciprocity,
     passivity)`` (``eme.jl:90-104``) with ``warmstart`` (``eme.jl:58-76``): solve every
     cell, build its power-normalised modes on the GPU, couple neighbours through interface
     S-matrices and cascade ``prop₀ ⋆ iface₀₁ ⋆ prop₁ ⋆ …`` (``_assemble``, ``eme.jl:125-136``).
     ``epsinvs`` may be arrays or zero-argument callables producing them (e.g. GPU
     smoothing of the cell's cross-section).  ``continue_chain=True`` warm-starts the first
-    cell from the solver's previous solve as well (a chain processed in pieces)."""
+    cell from the solver's previous solve as well (a chain processed in pieces).
+    ``learned`` (a ``warmstart.LearnedStart`` predicting at least ``nev`` modes on this grid,
+    e.g. ``warmstart.load_coupler()``) starts the cold cells -- the first one, every cell when
+    ``warmstart`` is off, and a cell whose warm-started solve did not converge, which is solved
+    again cold -- from the network's guess instead of ``k_guess`` and random vectors; the
+    converged modes do not depend on the start."""
     n = len(epsinvs)
     if n < 1 or len(lengths) != n:
         raise ValueError("need one length per cell and at least one cell")
+    if learned is not None and learned.nev < nev:
+        raise ValueError(f"the learned start predicts {learned.nev} modes, {nev} requested")
+
+    def cold(ei):
+        solver.options["warm_start"] = 0.0
+        if learned is None:
+            return solve_k(omega, ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
+                           max_eigsolves=max_eigsolves, maxiter=maxiter, return_info=True, fetch_evecs=False)
+        from .warmstart import solve_k_learned
+        return solve_k_learned(omega, ei, grid, solver, learned, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
+                               max_eigsolves=max_eigsolves, maxiter=maxiter, return_info=True, fetch_evecs=False)
+
     kms, overlaps, stats = [], [], []
     acc = None
     prev_slot = None
     for i in range(n):
         ei = epsinvs[i]() if callable(epsinvs[i]) else epsinvs[i]
-        solver.options["warm_start"] = 1.0 if (warmstart and (i > 0 or continue_chain)) else 0.0
+        warm = warmstart and (i > 0 or continue_chain)
         t0 = time.perf_counter()
-        km, _, info = solve_k(omega, ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
-                              max_eigsolves=max_eigsolves, maxiter=maxiter, return_info=True, fetch_evecs=False)
+        if warm:
+            solver.options["warm_start"] = 1.0
+            try:
+                km, _, info = solve_k(omega, ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
+                                      max_eigsolves=max_eigsolves, maxiter=maxiter, return_info=True,
+                                      fetch_evecs=False)
+            except OmxError as e:
+                if e.status != _lib.OMX_ERR_NOT_CONVERGED:
+                    raise
+                info = None
+            if info is None or not info["converged"]:
+                km, _, info = cold(ei)   # the previous cell's modes were no start for this one
+                info["cold_restart"] = True
+        else:
+            km, _, info = cold(ei)
         slot = i % 2
         build_modes(km, None, ei, grid, solver, slot=slot, conjugate=conjugate)
         info["seconds"] = time.perf_counter() - t0

@@ -278,16 +278,35 @@ def rib_epsinv(grid, prm, backend="gpu"):
                       backend=backend)
 
 
-def generate(seeds, grid, nev, solver, hw=64, k_tol=1e-10, eig_tol=1e-8, log=None):
-    """Inputs and targets for the draws of ``seeds``: ``(X, TF, TK, infos)``."""
+def random_coupler(rng):
+    """One draw of the coupler distribution: cross-sections of two Si3N4 cores in SiO2 around
+    the EME chain's adiabatic coupler (gap 0.3 -> 1.5 um, widths 0.8 <-> 1.2 um, 0.8 um
+    thick, ω = 1/1.55) on its 8 x 4 um 512 x 256 grid."""
+    return dict(gap=rng.uniform(0.2, 1.7), w1=rng.uniform(0.7, 1.3), w2=rng.uniform(0.7, 1.3), h=rng.uniform(0.7, 0.9),
+                omega=rng.uniform(0.60, 0.69))
+
+
+def coupler_epsinv(grid, prm, backend="gpu"):
+    from . import geometry as G
+    return G.coupler_slice(grid, prm["gap"], prm["w1"], prm["w2"], h=prm["h"], backend=backend)
+
+
+# structure families: (parameter draw, ε⁻¹ of a draw)
+FAMILIES = {"rib": (random_rib, rib_epsinv), "coupler": (random_coupler, coupler_epsinv)}
+
+
+def generate(seeds, grid, nev, solver, hw=64, k_tol=1e-10, eig_tol=1e-8, log=None, family="rib"):
+    """Inputs and targets for the draws of ``seeds`` of a structure family: ``(X, TF, TK,
+    infos)``."""
     from . import solve_k
+    draw, build = FAMILIES[family]
     X = np.empty((len(seeds), 7, hw, hw), dtype=np.float32)
     TF = np.empty((len(seeds), 4 * nev, hw, hw), dtype=np.float32)
     TK = np.empty((len(seeds), nev), dtype=np.float32)
     infos = []
     for i, s in enumerate(seeds):
-        prm = random_rib(np.random.default_rng(int(s)))
-        ei = rib_epsinv(grid, prm)
+        prm = draw(np.random.default_rng(int(s)))
+        ei = build(grid, prm)
         km, ev, info = solve_k(prm["omega"], ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol,
                                return_info=True)
         X[i] = features(ei, grid, prm["omega"], hw)
@@ -299,14 +318,15 @@ def generate(seeds, grid, nev, solver, hw=64, k_tol=1e-10, eig_tol=1e-8, log=Non
     return X, TF, TK, infos
 
 
-def compare(seeds, grid, nev, start, solver, k_tol=1e-10, eig_tol=1e-8, mode="seed"):
+def compare(seeds, grid, nev, start, solver, k_tol=1e-10, eig_tol=1e-8, mode="seed", family="rib"):
     """Held-out draws solved from the plain start and from the learned start: per draw the
     two k vectors, LOBPCG iterations and wall times."""
     from . import solve_k
+    draw, build = FAMILIES[family]
     rows = []
     for s in seeds:
-        prm = random_rib(np.random.default_rng(int(s)))
-        ei = rib_epsinv(grid, prm)
+        prm = draw(np.random.default_rng(int(s)))
+        ei = build(grid, prm)
         w = prm["omega"]
         t0 = time.perf_counter()
         kp, _, ip = solve_k(w, ei, grid, solver, nev=nev, k_tol=k_tol, eig_tol=eig_tol, return_info=True,
@@ -342,11 +362,46 @@ def train_default(n=512, epochs=600, out=None, log=print):
     return st, hist
 
 
+COUPLER_GRID = (8.0, 4.0, 512, 256)
+COUPLER_NEV = 16
+COUPLER_FILE = "warmnet_coupler512x256.npz"
+
+
+def train_coupler(n=320, epochs=500, out=None, log=print):
+    """The EME chain's network (``data/warmnet_coupler512x256.npz``): ``n`` coupler draws
+    (seeds 5000..) solved at 512 x 256 with 16 modes, then Adam (lr 2e-3, minibatch 16) on
+    the GPU.  It starts the chain's cold cells (``eme.eme(..., learned=...)``)."""
+    from . import HIPSolver
+    grid = Grid(*COUPLER_GRID)
+    nev, hw = COUPLER_NEV, 64
+    t = time.perf_counter()
+    X, TF, TK, _ = generate(range(5000, 5000 + n), grid, nev, HIPSolver(), hw=hw, log=log, family="coupler")
+    log(f"generated {n} coupler draws in {time.perf_counter() - t:.1f} s")
+    net = WarmNet(hw, nev, seed=0)
+    t = time.perf_counter()
+    hist = net.train(X, TF, TK, wk=1.0, epochs=epochs, batch=16, lr=2e-3, seed=1)
+    log(f"trained {epochs} epochs in {time.perf_counter() - t:.1f} s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    st = LearnedStart(net)
+    if out:
+        st.save(out)
+    return st, hist
+
+
+def load_coupler(device=0):
+    """The shipped coupler network, or None when its file is absent."""
+    path = os.path.join(DATA_DIR, COUPLER_FILE)
+    return LearnedStart.load(path, device) if os.path.exists(path) else None
+
+
 if __name__ == "__main__":
     import argparse
-    ap = argparse.ArgumentParser(description="train the warm-start network on solver-converged rib draws")
-    ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--epochs", type=int, default=600)
-    ap.add_argument("--out", default=os.path.join(DATA_DIR, "warmnet_tfln256.npz"))
+    ap = argparse.ArgumentParser(description="train a warm-start network on solver-converged draws of a family")
+    ap.add_argument("--family", default="rib", choices=sorted(FAMILIES))
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--epochs", type=int, default=None)
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
-    train_default(a.n, a.epochs, a.out)
+    if a.family == "rib":
+        train_default(a.n or 512, a.epochs or 600, a.out or os.path.join(DATA_DIR, "warmnet_tfln256.npz"))
+    else:
+        train_coupler(a.n or 320, a.epochs or 500, a.out or os.path.join(DATA_DIR, COUPLER_FILE))

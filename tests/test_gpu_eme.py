@@ -224,3 +224,27 @@ def test_eme_gradient_matches_finite_differences():
     fdw = (value(eis, W0 + hw) - value(eis, W0 - hw)) / (2 * hw)
     print(f"  omega bar {wbar:.10g} fd {fdw:.10g}", flush=True)
     assert wbar == pytest.approx(fdw, rel=1e-3, abs=1e-8)
+
+
+def test_eme_chain_learned_cold_start_matches_plain():
+    """The shipped coupler network (512x256, 16 modes) starts the cold cells of an EME chain:
+    the same modes and device S-matrix as the plain start (the start never changes the
+    converged result), every cell converged, and the learned guess used (fewer FP32 LOBPCG
+    iterations summed over the cold cells is printed, not asserted: it is a speed property)."""
+    from omx import warmstart as WS
+    print("eme chain learned cold start", flush=True)
+    learned = WS.load_coupler()
+    assert learned is not None and learned.nev >= 16
+    g = omx.Grid(*WS.COUPLER_GRID)
+    cells = [(lambda gp=gp: GEO.coupler_slice(g, gp, 1.0, 1.0, h=0.8, backend="gpu")) for gp in (0.6, 0.9)]
+    kw = dict(nev=16, warmstart=False, k_tol=1e-10, eig_tol=1e-9)
+    rp = X.eme(cells, [1.0, 1.0], W0, g, omx.HIPSolver(), **kw)
+    rl = X.eme(cells, [1.0, 1.0], W0, g, omx.HIPSolver(), learned=learned, **kw)
+    print("  fp32 iterations plain", [s["lobpcg_iters_fp32"] for s in rp.stats], "learned",
+          [s["lobpcg_iters_fp32"] for s in rl.stats], flush=True)
+    assert all(s["converged"] for s in rp.stats + rl.stats)
+    for a, b in zip(rl.kmags, rp.kmags):
+        assert np.allclose(a, b, rtol=1e-9, atol=0)
+    assert np.allclose(np.abs(rl.S.transmission()), np.abs(rp.S.transmission()), atol=1e-6)
+    with pytest.raises(ValueError):
+        X.eme(cells, [1.0, 1.0], W0, g, omx.HIPSolver(), nev=17, learned=learned)

@@ -62,3 +62,30 @@ def test_lowres_vectors_are_the_kept_plane_waves_of_start_vectors():
             for gy in (0, 5, hw - 2):
                 assert np.allclose(low[:, c + 2 * (gx + hw * gy)], full[:, c + 2 * (sx[gx] + 64 * sy[gy])])
     assert np.allclose(np.linalg.norm(low, axis=1), 1.0)
+
+
+def test_structure_families_draw_in_range_and_build_on_the_host():
+    """Both training families: seeded draws are repeatable and inside their ranges, and each
+    builds a symmetric ε⁻¹ of the grid's shape (host smoothing, no GPU)."""
+    assert set(WS.FAMILIES) == {"rib", "coupler"}
+    g = omx.Grid(8.0, 4.0, 64, 32)
+    for fam, (draw, build) in WS.FAMILIES.items():
+        a, b = draw(np.random.default_rng(7)), draw(np.random.default_rng(7))
+        assert a == b
+        assert 0.55 <= a["omega"] <= 0.75
+        ei = build(g, a, backend="host")
+        assert ei.shape == (3, 3, 64, 32)
+        assert np.allclose(ei, ei.transpose(1, 0, 2, 3))
+        assert np.all(np.linalg.eigvalsh(ei.transpose(2, 3, 0, 1)) > 0)
+    c = WS.random_coupler(np.random.default_rng(3))
+    assert 0.2 <= c["gap"] <= 1.7 and 0.7 <= c["w1"] <= 1.3 and 0.7 <= c["w2"] <= 1.3 and 0.7 <= c["h"] <= 0.9
+
+
+def test_shipped_coupler_network_file():
+    """The EME chain's network file: 16 modes on the 64 x 64 map, parameters finite (loaded
+    with the non-executing npz reader; the GPU tests run it)."""
+    import os
+    path = os.path.join(WS.DATA_DIR, WS.COUPLER_FILE)
+    with np.load(path, allow_pickle=False) as z:
+        assert int(z["nev"]) == WS.COUPLER_NEV == 16 and int(z["hw"]) == 64
+        assert np.all(np.isfinite(z["params"])) and z["params"].size > 10000
