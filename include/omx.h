@@ -301,6 +301,10 @@ int omx_warmnet_loss_grad(omx_warmnet* h, int batch, const float* x, const float
    mean minibatch loss of epoch e. */
 int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
                       int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out);
+/* omx_warmnet_train continuing the previous call's Adam moments and step count (a training
+   run in pieces whose targets change between them, e.g. re-aligned to the current output). */
+int omx_warmnet_train_resume(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
+                             int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out);
 
 /* The guess omx_solve_k_learned starts from, for inspection: k_out (nev floats, the
    normalised effective indices) and, unless NULL, lowres_out (nev vectors of 2 hw^2 complex,

@@ -663,7 +663,7 @@ void solve_k_learned(Solver& S, WarmNet& net, double omega, int nev, double k0_s
 }
 
 void WarmNet::train(int n, const float* x, const float* tf, const float* tk, float wk, int epochs, int batch,
-                    float lr, uint64_t seed, double* hist) {
+                    float lr, uint64_t seed, double* hist, bool resume) {
   if (n < 1 || epochs < 1 || batch < 1 || !x || !tf || !tk || !(lr > 0))
     throw Error(OMX_ERR_INVALID_ARG, "bad warm-start training arguments");
   OMX_HIP_CHECK(hipSetDevice(device_));
@@ -687,9 +687,11 @@ void WarmNet::train(int n, const float* x, const float* tf, const float* tk, flo
   OMX_HIP_CHECK(hipMemcpyAsync(dIdx.as<void>(), perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, s_));
   lossd_.ensure(sizeof(double) * epochs);
   OMX_HIP_CHECK(hipMemsetAsync(lossd_.as<void>(), 0, sizeof(double) * epochs, s_));
-  OMX_HIP_CHECK(hipMemsetAsync(M1_.as<void>(), 0, sizeof(float) * nparams_, s_));
-  OMX_HIP_CHECK(hipMemsetAsync(M2_.as<void>(), 0, sizeof(float) * nparams_, s_));
-  adam_t_ = 0;
+  if (!resume || adam_t_ == 0) {
+    OMX_HIP_CHECK(hipMemsetAsync(M1_.as<void>(), 0, sizeof(float) * nparams_, s_));
+    OMX_HIP_CHECK(hipMemsetAsync(M2_.as<void>(), 0, sizeof(float) * nparams_, s_));
+    adam_t_ = 0;
+  }
   const float b1 = 0.9f, b2 = 0.999f;
   for (int e = 0; e < epochs; ++e) {
     for (int m = 0; m < nb; ++m) {

@@ -51,8 +51,9 @@ class WarmNet {
   double loss_grad(int B, const float* x, const float* tf, const float* tk, float wk, float* grad);
   // Adam over n samples held on the device, minibatches of `batch` in a seeded shuffle;
   // hist[e] = mean minibatch loss of epoch e
+  // (resume: continue the previous call's Adam moments and step count instead of resetting them)
   void train(int n, const float* x, const float* tf, const float* tk, float wk, int epochs, int batch, float lr,
-             uint64_t seed, double* hist);
+             uint64_t seed, double* hist, bool resume = false);
   int device() const { return device_; }
   // The guess for one 2D cell on the device: features from eps6 (FP64 per pixel xx yy zz xy
   // xz yz, pixel ix + nx iy), one forward, the band outputs to k_host (nev floats), and the

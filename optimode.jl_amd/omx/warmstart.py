@@ -95,8 +95,9 @@ class WarmNet:
                                                     ctypes.byref(loss), _lib.fptr(g)))
         return loss.value, g
 
-    def train(self, x, tf, tk, wk=1.0, epochs=100, batch=16, lr=1e-3, seed=0):
-        """Adam over the whole set on the device; returns the per-epoch mean loss."""
+    def train(self, x, tf, tk, wk=1.0, epochs=100, batch=16, lr=1e-3, seed=0, resume=False):
+        """Adam over the whole set on the device; returns the per-epoch mean loss.  With
+        ``resume`` the optimiser continues the previous call's moments and step count."""
         x = self._x(x)
         n = x.shape[0]
         tf = np.ascontiguousarray(tf, dtype=np.float32)
@@ -104,9 +105,64 @@ class WarmNet:
         if tf.shape != (n, 4 * self.nev, self.hw, self.hw) or tk.shape != (n, self.nev):
             raise OmxError(_lib.OMX_ERR_BAD_SHAPE, "target shapes do not match the data set")
         hist = np.empty(int(epochs))
-        _lib.check(_lib.lib().omx_warmnet_train(self.h, n, _lib.fptr(x), _lib.fptr(tf), _lib.fptr(tk), float(wk),
-                                                int(epochs), int(batch), float(lr), int(seed), _lib.dptr(hist)))
+        fn = _lib.lib().omx_warmnet_train_resume if resume else _lib.lib().omx_warmnet_train
+        _lib.check(fn(self.h, n, _lib.fptr(x), _lib.fptr(tf), _lib.fptr(tk), float(wk), int(epochs), int(batch),
+                      float(lr), int(seed), _lib.dptr(hist)))
         return hist
+
+    def forward_all(self, x, chunk=64):
+        """``forward`` over a data set in chunks."""
+        ys, ks = [], []
+        for i in range(0, x.shape[0], chunk):
+            y, k = self.forward(x[i:i + chunk])
+            ys.append(y)
+            ks.append(k)
+        return np.concatenate(ys), np.concatenate(ks)
+
+
+def _complex_maps(f, nev):
+    """(n, 4 nev, hw, hw) re/im channels -> (n, nev, 2, hw, hw) complex band maps."""
+    n, _, hw, _ = f.shape
+    g = f.reshape(n, nev, 2, 2, hw, hw)
+    return g[:, :, :, 0] + 1j * g[:, :, :, 1]
+
+
+def _channel_maps(c):
+    n, nev, _, hw, _ = c.shape
+    out = np.empty((n, nev, 2, 2, hw, hw), dtype=np.float32)
+    out[:, :, :, 0] = c.real
+    out[:, :, :, 1] = c.imag
+    return out.reshape(n, 4 * nev, hw, hw)
+
+
+def align_targets(y, tf, nev):
+    """The targets rotated within each sample's band span to best match the outputs: per
+    sample, T W with the unitary W minimising sum_k ||y_k - (T W)_k||^2 (orthogonal
+    Procrustes: W = U V^H from the SVD T^H y = U S V^H).  The field maps are a linear image
+    of the band vectors, so T W are the maps of the rotated vectors, which span the same
+    subspace -- all that the shared first eigensolve takes from the start block."""
+    T, Y = _complex_maps(tf, nev), _complex_maps(y, nev)
+    M = np.einsum("njcab,nkcab->njk", T.conj(), Y)
+    U, _, Vh = np.linalg.svd(M)
+    W = U @ Vh
+    return _channel_maps(np.einsum("njcab,njk->nkcab", T, W))
+
+
+def train_subspace(net, x, tf, tk, wk=1.0, epochs=600, realign=10, batch=16, lr=2e-3, seed=1, log=None):
+    """Training on a loss invariant to rotations of the predicted block within each sample's
+    band span: every ``realign`` epochs the targets are rotated by ``align_targets`` to the
+    current outputs, and Adam continues on them (its state carried across the pieces).  The
+    per-band effective-index outputs keep their band order."""
+    hist = []
+    for e in range(0, int(epochs), int(realign)):
+        y, _ = net.forward_all(x)
+        t_al = align_targets(y, tf, net.nev)
+        h = net.train(x, t_al, tk, wk=wk, epochs=min(int(realign), int(epochs) - e), batch=batch, lr=lr,
+                      seed=seed + e, resume=e > 0)
+        hist.extend(h.tolist())
+        if log and (e // int(realign)) % 10 == 0:
+            log(f"subspace training: epoch {e + len(h)}, loss {h[-1]:.4f}")
+    return np.array(hist)
 
 
 def _blocks(grid, hw):
@@ -342,10 +398,11 @@ def compare(seeds, grid, nev, start, solver, k_tol=1e-10, eig_tol=1e-8, mode="se
     return rows
 
 
-def train_default(n=512, epochs=600, out=None, log=print):
-    """The shipped network (``data/warmnet_tfln256.npz``): 512 rib draws (seeds 1000..1511)
+def train_default(n=1024, epochs=600, out=None, log=print):
+    """The shipped network (``data/warmnet_tfln256.npz``): 1024 rib draws (seeds 1000..2023)
     solved at 256x256 with 8 bands, 600 epochs of Adam (lr 2e-3, minibatch 16, wk 1) on the
-    GPU."""
+    GPU with the rotation-invariant field loss (``train_subspace``, targets re-aligned every
+    10 epochs)."""
     from . import HIPSolver
     grid = Grid(8.0, 4.0, 256, 256)
     nev, hw = 8, 64
@@ -354,7 +411,7 @@ def train_default(n=512, epochs=600, out=None, log=print):
     log(f"generated {n} draws in {time.perf_counter() - t:.1f} s")
     net = WarmNet(hw, nev, seed=0)
     t = time.perf_counter()
-    hist = net.train(X, TF, TK, wk=1.0, epochs=epochs, batch=16, lr=2e-3, seed=1)
+    hist = train_subspace(net, X, TF, TK, wk=1.0, epochs=epochs, realign=10, batch=16, lr=2e-3, seed=1, log=log)
     log(f"trained {epochs} epochs in {time.perf_counter() - t:.1f} s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
     st = LearnedStart(net)
     if out:
@@ -370,7 +427,8 @@ COUPLER_FILE = "warmnet_coupler512x256.npz"
 def train_coupler(n=320, epochs=500, out=None, log=print):
     """The EME chain's network (``data/warmnet_coupler512x256.npz``): ``n`` coupler draws
     (seeds 5000..) solved at 512 x 256 with 16 modes, then Adam (lr 2e-3, minibatch 16) on
-    the GPU.  It starts the chain's cold cells (``eme.eme(..., learned=...)``)."""
+    the GPU with the rotation-invariant field loss.  It starts the chain's cold cells
+    (``eme.eme(..., learned=...)``)."""
     from . import HIPSolver
     grid = Grid(*COUPLER_GRID)
     nev, hw = COUPLER_NEV, 64
@@ -379,7 +437,7 @@ def train_coupler(n=320, epochs=500, out=None, log=print):
     log(f"generated {n} coupler draws in {time.perf_counter() - t:.1f} s")
     net = WarmNet(hw, nev, seed=0)
     t = time.perf_counter()
-    hist = net.train(X, TF, TK, wk=1.0, epochs=epochs, batch=16, lr=2e-3, seed=1)
+    hist = train_subspace(net, X, TF, TK, wk=1.0, epochs=epochs, realign=10, batch=16, lr=2e-3, seed=1, log=log)
     log(f"trained {epochs} epochs in {time.perf_counter() - t:.1f} s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
     st = LearnedStart(net)
     if out:
@@ -402,6 +460,6 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.family == "rib":
-        train_default(a.n or 512, a.epochs or 600, a.out or os.path.join(DATA_DIR, "warmnet_tfln256.npz"))
+        train_default(a.n or 1024, a.epochs or 600, a.out or os.path.join(DATA_DIR, "warmnet_tfln256.npz"))
     else:
         train_coupler(a.n or 320, a.epochs or 500, a.out or os.path.join(DATA_DIR, COUPLER_FILE))

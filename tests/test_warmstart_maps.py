@@ -89,3 +89,25 @@ def test_shipped_coupler_network_file():
     with np.load(path, allow_pickle=False) as z:
         assert int(z["nev"]) == WS.COUPLER_NEV == 16 and int(z["hw"]) == 64
         assert np.all(np.isfinite(z["params"])) and z["params"].size > 10000
+
+
+def test_align_targets_solves_the_procrustes_problem():
+    """Outputs that are the targets rotated by a unitary W0 within each sample's band span (plus
+    a little noise): the aligned targets recover the outputs to the noise level, the aligned
+    loss never exceeds the unaligned one, and the rotation is unitary (norms kept)."""
+    rng = np.random.default_rng(5)
+    n, nev, hw = 3, 4, 8
+    tf = rng.standard_normal((n, 4 * nev, hw, hw)).astype(np.float32)
+    T = WS._complex_maps(tf, nev)
+    Y = np.empty_like(T)
+    for s in range(n):
+        q, _ = np.linalg.qr(rng.standard_normal((nev, nev)) + 1j * rng.standard_normal((nev, nev)))
+        Y[s] = np.einsum("jcab,jk->kcab", T[s], q)
+    y = WS._channel_maps(Y) + 1e-4 * rng.standard_normal(tf.shape).astype(np.float32)
+    al = WS.align_targets(y, tf, nev)
+    assert np.abs(al - y).max() < 1e-2
+    assert np.sum((al - y) ** 2) <= np.sum((tf - y) ** 2)
+    assert np.allclose(np.sum(al.astype(np.float64) ** 2, axis=(1, 2, 3)), np.sum(tf.astype(np.float64) ** 2, axis=(1, 2, 3)),
+                       rtol=1e-5)
+    # round trip of the channel layout
+    assert np.array_equal(WS._channel_maps(WS._complex_maps(tf, nev)), tf)
