@@ -314,7 +314,7 @@ int omx_warmnet_guess(omx_warmnet* net, omx_solver* h, double omega, float* k_ou
 /* omx_solve_k from the network's guess with everything on the device but the nev predicted
    indices: features from the handle's eps^-1, one network forward, the inverse 2D DFT of the
    predicted maps into the start block's hw x hw lowest plane waves; the shared first
-   eigensolve starts at k0_scale times the predicted band-1 k (1.03 is the tested choice).
+   eigensolve starts at k0_scale times the predicted band-1 k (1.0 is the tested choice).
    Arguments and outputs otherwise as omx_solve_k; 2D handles whose Nx, Ny are multiples of
    the network's hw, nev <= the network's band count, both on one device. */
 int omx_solve_k_learned(omx_solver* h, omx_warmnet* net, double omega, int nev, double k0_scale, double k_tol,

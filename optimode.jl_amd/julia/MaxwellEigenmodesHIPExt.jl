@@ -452,7 +452,7 @@ function hip_warmnet_train!(net::HIPWarmNet, x, tf, tk; wk::Real=1.0, epochs::In
 end
 
 """
-    hip_solve_k_learned(ω, ε⁻¹, grid, solver, net; nev=net.nev, k0_scale=1.03, kwargs...)
+    hip_solve_k_learned(ω, ε⁻¹, grid, solver, net; nev=net.nev, k0_scale=1.0, kwargs...)
 
 `solve_k` from the network's guess, built on the GPU from the solver's ε⁻¹ (features, one
 forward, inverse DFT into the start block's lowest plane waves; `omx_solve_k_learned`): the
@@ -460,7 +460,7 @@ shared first eigensolve starts at `k0_scale` times the predicted band-1 k; the s
 and the converged result are those of `solve_k`.
 """
 function hip_solve_k_learned(ω::Real, ε⁻¹::AbstractArray{Float64,4}, grid::Grid{2,Float64}, solver::HIPSolver,
-                             net::HIPWarmNet; nev::Integer=net.nev, k0_scale::Real=1.03, max_eigsolves=60,
+                             net::HIPWarmNet; nev::Integer=net.nev, k0_scale::Real=1.0, max_eigsolves=60,
                              maxiter=100, k_tol=1e-8, eig_tol=1e-8)
     h = _handle!(solver, grid, ε⁻¹)
     kmags = Vector{Float64}(undef, nev)

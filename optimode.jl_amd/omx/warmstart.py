@@ -279,13 +279,13 @@ def device_guess(start, epsinv, grid, omega, solver):
     return k, low
 
 
-def solve_k_learned(omega, epsinv, grid, solver, start, nev=None, mode="seed", k0_scale=1.03, return_info=False,
+def solve_k_learned(omega, epsinv, grid, solver, start, nev=None, mode="seed", k0_scale=1.0, return_info=False,
                     on_device=True, **kw):
     """``solve_k`` started from the network's guesses; the stopping rules and the converged
     result are those of ``solve_k``.  ``mode="seed"``: the shared first eigensolve starts at
     ``k0_scale`` times the predicted k of band 1 from the predicted block (instead of k_guess
-    and random vectors; slightly above band 1 so that every band's first Newton step moves
-    down in k, as from k_guess); ``mode="band"``: every band starts at its own predicted k
+    and random vectors; 1.0 measured best with the rotation-invariant network, ahead of 1.015,
+    1.03 and 1.05); ``mode="band"``: every band starts at its own predicted k
     with the predicted block and the shared first eigensolve is skipped (``band_guess``)."""
     from . import solve_k
     nev = start.nev if nev is None else int(nev)
