@@ -15,7 +15,10 @@ __device__ __forceinline__ const c64* col_ptr(const RCols& C, int i, int c, int 
   return c < n1 ? C.blk + i * C.istride + c * C.ld : C.extra + i * C.ld;
 }
 
-__global__ __launch_bounds__(RB) void k_bgram(RCols U, RCols V, int vmode, const int* n1, long K, c64* part) {
+// Partials of U^H V (and, when part_uu is given, of U^H U from the same staged rows: the
+// Rayleigh-Ritz pair of Gram matrices in one pass over the block)
+__global__ __launch_bounds__(RB) void k_bgram(RCols U, RCols V, int vmode, const int* n1, long K, c64* part,
+                                              c64* part_uu) {
   __shared__ c64 us[RF_MAXC][RQ + 1], vs[RF_MAXC][RQ + 1];
   const int i = blockIdx.y;
   const int m = n1[i];
@@ -23,7 +26,7 @@ __global__ __launch_bounds__(RB) void k_bgram(RCols U, RCols V, int vmode, const
   const int nv1 = vmode == 0 ? m : 0;
   const int nv = nv1 + (V.extra ? 1 : 0);
   const int t = threadIdx.x;
-  double ar[2] = {0, 0}, ai[2] = {0, 0};
+  double ar[2] = {0, 0}, ai[2] = {0, 0}, br[2] = {0, 0}, bi[2] = {0, 0};
   if (m > 0) {
     for (long r0 = long(blockIdx.x) * RQ; r0 < K; r0 += long(gridDim.x) * RQ) {
       for (int e = t; e < RF_MAXC * RQ; e += RB) {
@@ -49,38 +52,60 @@ __global__ __launch_bounds__(RB) void k_bgram(RCols U, RCols V, int vmode, const
           ar[s] += sr;
           ai[s] += si;
         }
+        if (part_uu && p < nu * nu) {
+          const int a = p % nu, b = p / nu;
+          double sr = 0, si = 0;
+#pragma unroll 8
+          for (int q = 0; q < RQ; ++q) {
+            const c64 x = us[a][q], y = us[b][q];
+            sr += x.x * y.x + x.y * y.y;
+            si += x.x * y.y - x.y * y.x;
+          }
+          br[s] += sr;
+          bi[s] += si;
+        }
       }
       __syncthreads();
     }
   }
   c64* pp = part + (long(i) * gridDim.x + blockIdx.x) * RF_PAIRS;
+  c64* pu = part_uu ? part_uu + (long(i) * gridDim.x + blockIdx.x) * RF_PAIRS : nullptr;
   for (int s = 0; s < 2; ++s) {
     const int p = t + s * RB;
-    if (p < RF_PAIRS) {
-      // store in G's layout a + RF_MAXC b
-      if (p < nu * nv) {
-        const int a = p % nu, b = p / nu;
-        pp[a + RF_MAXC * b] = c64{ar[s], ai[s]};
-      }
+    // store in G's layout a + RF_MAXC b
+    if (p < nu * nv) {
+      const int a = p % nu, b = p / nu;
+      pp[a + RF_MAXC * b] = c64{ar[s], ai[s]};
+    }
+    if (pu && p < nu * nu) {
+      const int a = p % nu, b = p / nu;
+      pu[a + RF_MAXC * b] = c64{br[s], bi[s]};
     }
   }
 }
 
+// G entry p of instance i = the sum of its nblk partials: one wavefront per entry, lane l
+// summing partials l, l + 64, ... and a fixed shuffle tree (the same order on every run)
 __global__ __launch_bounds__(RB) void k_bgram_reduce(const c64* part, int nblk, const int* n1, int has_u2, int vmode,
                                                      int has_v2, c64* G) {
   const int i = blockIdx.x;
   const int m = n1[i];
   const int nu = m + has_u2, nv = (vmode == 0 ? m : 0) + has_v2;
-  for (int p = threadIdx.x; p < RF_PAIRS; p += RB) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int p = w; p < RF_PAIRS; p += RB / 64) {
     const int a = p % RF_MAXC, b = p / RF_MAXC;
     double sr = 0, si = 0;
     if (m > 0 && a < nu && b < nv)
-      for (int k = 0; k < nblk; ++k) {
+      for (int k = lane; k < nblk; k += 64) {
         const c64 v = part[(long(i) * nblk + k) * RF_PAIRS + p];
         sr += v.x;
         si += v.y;
       }
-    G[long(i) * RF_PAIRS + p] = c64{sr, si};
+    for (int off = 32; off > 0; off >>= 1) {
+      sr += __shfl_down(sr, off, 64);
+      si += __shfl_down(si, off, 64);
+    }
+    if (lane == 0) G[long(i) * RF_PAIRS + p] = c64{sr, si};
   }
 }
 
@@ -154,10 +179,13 @@ int rows_grid(long K) { return int(std::min<long>(256, (K + RB - 1) / RB)); }
 }  // namespace
 
 void launch_bgram(const RCols& U, const RCols& V, int vmode, const int* n1, int ninst, long K, int nblk, c64* part,
-                  c64* G, hipStream_t s) {
-  hipLaunchKernelGGL(k_bgram, dim3(nblk, ninst), dim3(RB), 0, s, U, V, vmode, n1, K, part);
+                  c64* G, hipStream_t s, c64* part_uu, c64* B) {
+  hipLaunchKernelGGL(k_bgram, dim3(nblk, ninst), dim3(RB), 0, s, U, V, vmode, n1, K, part, B ? part_uu : nullptr);
   hipLaunchKernelGGL(k_bgram_reduce, dim3(ninst), dim3(RB), 0, s, part, nblk, n1, U.extra ? 1 : 0, vmode,
                      V.extra ? 1 : 0, G);
+  if (B)
+    hipLaunchKernelGGL(k_bgram_reduce, dim3(ninst), dim3(RB), 0, s, part_uu, nblk, n1, U.extra ? 1 : 0, 0,
+                       U.extra ? 1 : 0, B);
   OMX_HIP_CHECK(hipGetLastError());
 }
 

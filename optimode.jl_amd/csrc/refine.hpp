@@ -19,9 +19,10 @@ struct RCols {
 
 // G_i[a + RF_MAXC b] = sum_rows conj(U_a) V_b for a < nu_i = n1[i] + (U.extra != 0) and
 // b < nv_i; V's block part has n1[i] columns when vmode = 0 and none when vmode = 1 (V is
-// then its extra column only).  `part` holds nblk * ninst * RF_PAIRS partials.
+// then its extra column only).  `part` holds nblk * ninst * RF_PAIRS partials.  With B (and
+// part_uu, as large as part), B_i = U^H U is formed in the same pass over the rows.
 void launch_bgram(const RCols& U, const RCols& V, int vmode, const int* n1, int ninst, long K, int nblk, c64* part,
-                  c64* G, hipStream_t s);
+                  c64* G, hipStream_t s, c64* part_uu = nullptr, c64* B = nullptr);
 
 // out_j = sum_l in_l C_i[l + RF_MAXC j] with in = U's columns (n1[i] block columns plus the
 // extra column when present); outputs j < n1[i] (omode = 0: a block laid out like U's, at

@@ -1577,7 +1577,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
     W.lamd.alloc(sizeof(double) * n);
     for (DevMem* v : {&W.Tm, &W.T2, &W.ATm, &W.XB, &W.RBm}) v->alloc(sizeof(c64) * size_t(n) * ld);
     for (DevMem* v : {&W.XBf, &W.RBf, &W.Tf}) v->alloc(sizeof(cplx<float>) * size_t(n) * ld);
-    W.part.alloc(sizeof(c64) * size_t(n) * ng * RF_PAIRS);
+    W.part.alloc(2 * sizeof(c64) * size_t(n) * ng * RF_PAIRS);  // U^H V and U^H U partials
     for (DevMem* v : {&W.Gd, &W.Bd, &W.Cd}) v->alloc(sizeof(c64) * size_t(n) * RF_PAIRS);
     W.normp.alloc(sizeof(double) * size_t(n) * nr);
     W.scl.alloc(sizeof(double) * n);
@@ -1618,8 +1618,9 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
     for (int i = 0; i < ninst; ++i) any_t = any_t || (n1h[i] > 0 && with_t[i]);
     const RCols Ux{b.p(b.X[b.cur]), ld, istride, any_t ? Tm.as<c64>() : nullptr};
     const RCols Ua{b.p(b.AX[b.cur]), ld, istride, any_t ? ATm.as<c64>() : nullptr};
-    launch_bgram(Ux, Ua, 0, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Gd.as<c64>(), s);
-    launch_bgram(Ux, Ux, 0, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Bd.as<c64>(), s);
+    // Rayleigh-Ritz pair: U^H (AU) and U^H U in one pass over the rows
+    launch_bgram(Ux, Ua, 0, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Gd.as<c64>(), s,
+                 part.as<c64>() + size_t(W.ninst) * W.nblk_g * RF_PAIRS, Bd.as<c64>());
     OMX_HIP_CHECK(hipMemcpyAsync(hG.data(), Gd.as<void>(), sizeof(c64) * hG.size(), hipMemcpyDeviceToHost, s));
     OMX_HIP_CHECK(hipMemcpyAsync(hB.data(), Bd.as<void>(), sizeof(c64) * hB.size(), hipMemcpyDeviceToHost, s));
     OMX_HIP_CHECK(hipStreamSynchronize(s));
