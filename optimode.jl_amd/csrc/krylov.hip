@@ -451,46 +451,45 @@ __global__ __launch_bounds__(AB) void k_qdot(const cplx<T>* Q, long qld, long qi
   }
 }
 
-// coef[i QM + c] = -<Q_c, x_i>: the fixed-order sum of instance i's nblk partials
-template <int QM>
-__global__ __launch_bounds__(AB) void k_qcoef(const c64* part, int nblk, c64* coef) {
-  constexpr int G = AB / QM;  // partial groups per coefficient
-  const int i = blockIdx.x;
-  const int c = threadIdx.x % QM, j = threadIdx.x / QM;
-  __shared__ double rr[G][QM], ri[G][QM];
-  double a = 0, b = 0;
-  for (int q = j; q < nblk; q += G) {
-    const c64 v = part[(long(i) * nblk + q) * QM + c];
-    a += v.x;
-    b += v.y;
-  }
-  rr[j][c] = a;
-  ri[j][c] = b;
-  __syncthreads();
-  if (threadIdx.x < QM) {
-    double sa = 0, sb = 0;
-    for (int g = 0; g < G; ++g) {
-      sa += rr[g][threadIdx.x];
-      sb += ri[g][threadIdx.x];
-    }
-    coef[long(i) * QM + threadIdx.x] = c64{-sa, -sb};
-  }
-}
-
-// x_i <- x_i + sum_c coef[i QM + c] Q_c (the block projection), and with d the partials of
-// Re <d_i, x_i new> into part[i gridDim.x + blockIdx.x]
+// x_i <- x_i - sum_c <Q_c, x_i> Q_c (the block projection), the coefficients summed by every
+// block from k_qdot's qnblk partials of its instance in one fixed order; with d, the
+// partials of Re <d_i, x_i new> into part[i gridDim.x + blockIdx.x]
 template <typename T, int QM>
-__global__ __launch_bounds__(AB) void k_qproject(const c64* coef, const cplx<T>* Q, long qld, long qis, const int* nq,
-                                                 cplx<T>* X, const cplx<T>* d, long ld, long K, const int* act,
-                                                 double* part) {
+__global__ __launch_bounds__(AB) void k_qproject(const c64* qpart, int qnblk, const cplx<T>* Q, long qld, long qis,
+                                                 const int* nq, cplx<T>* X, const cplx<T>* d, long ld, long K,
+                                                 const int* act, double* part) {
+  constexpr int G = AB / QM;
   const int i = blockIdx.y;
   __shared__ double red[AB];
+  __shared__ double gr[G][QM], gi[G][QM];
+  __shared__ c64 cfs[QM];
   double acc = 0;
   if (act[i]) {
     const int n = nq[i];
+    {
+      const int c = threadIdx.x % QM, j = threadIdx.x / QM;
+      double a = 0, b = 0;
+      for (int q = j; q < qnblk; q += G) {
+        const c64 v = qpart[(long(i) * qnblk + q) * QM + c];
+        a += v.x;
+        b += v.y;
+      }
+      gr[j][c] = a;
+      gi[j][c] = b;
+      __syncthreads();
+      if (threadIdx.x < QM) {
+        double sa = 0, sb = 0;
+        for (int g = 0; g < G; ++g) {
+          sa += gr[g][threadIdx.x];
+          sb += gi[g][threadIdx.x];
+        }
+        cfs[threadIdx.x] = c64{-sa, -sb};
+      }
+      __syncthreads();
+    }
     c64 cf[QM];
 #pragma unroll
-    for (int c = 0; c < QM; ++c) cf[c] = c < n ? coef[long(i) * QM + c] : c64{0, 0};
+    for (int c = 0; c < QM; ++c) cf[c] = c < n ? cfs[c] : c64{0, 0};
     cplx<T>* x = X + i * ld;
     const cplx<T>* q = Q + i * qis;
     const cplx<T>* dv = d ? d + i * ld : nullptr;
@@ -575,8 +574,10 @@ void VecOpsT<T>::lincomb(std::initializer_list<const C*> ins, C* outp, const int
 
 template <typename T> void VecOpsT<T>::set_block(const C* Qb, long qis, const int* nq, int nq_max) {
   if (nq_max < 1 || nq_max > QMAX) throw Error(OMX_ERR_INTERNAL, "block projection: 1..16 columns");
-  // block partials (nb nblk QMAX complex), then the coefficients (nb QMAX complex)
-  part.ensure(sizeof(c64) * size_t(nb) * (size_t(nblk) * QMAX + QMAX));
+  // the real dot partials (nb nblk, at the start, where the MINRES steps read them), then
+  // the block partials (nb nblk QMAX complex): every block of k_qproject reads the latter
+  // while others write the former, so the two never overlap
+  part.ensure(sizeof(c64) * size_t(nb) * (size_t(nblk) + size_t(nblk) * QMAX));
   Q = Qb;
   qistride = qis;
   qn = nq;
@@ -589,18 +590,15 @@ template <typename T> void VecOpsT<T>::set_block(const C* Qb, long qis, const in
 template <typename T> void VecOpsT<T>::project_dot_to(C* x, const int* act, const C* dot_with, double* out) {
   const dim3 g(nblk, nb);
   if (Q) {
-    c64* qp = part.template as<c64>();
-    c64* qc = qp + size_t(nb) * nblk * QMAX;
+    c64* qp = part.template as<c64>() + size_t(nb) * nblk;
     const dim3 gq(qblk, nb);
     if (qmax <= 8) {
       hipLaunchKernelGGL((k_qdot<T, 8>), gq, dim3(AB), 0, s, Q, ld, qistride, qn, x, ld, K, act, qp);
-      hipLaunchKernelGGL((k_qcoef<8>), dim3(nb), dim3(AB), 0, s, qp, qblk, qc);
-      hipLaunchKernelGGL((k_qproject<T, 8>), g, dim3(AB), 0, s, qc, Q, ld, qistride, qn, x, dot_with, ld, K, act,
-                         out);
+      hipLaunchKernelGGL((k_qproject<T, 8>), g, dim3(AB), 0, s, qp, qblk, Q, ld, qistride, qn, x, dot_with, ld, K,
+                         act, out);
     } else {
       hipLaunchKernelGGL((k_qdot<T, QMAX>), gq, dim3(AB), 0, s, Q, ld, qistride, qn, x, ld, K, act, qp);
-      hipLaunchKernelGGL((k_qcoef<QMAX>), dim3(nb), dim3(AB), 0, s, qp, qblk, qc);
-      hipLaunchKernelGGL((k_qproject<T, QMAX>), g, dim3(AB), 0, s, qc, Q, ld, qistride, qn, x, dot_with, ld, K,
+      hipLaunchKernelGGL((k_qproject<T, QMAX>), g, dim3(AB), 0, s, qp, qblk, Q, ld, qistride, qn, x, dot_with, ld, K,
                          act, out);
     }
     return;
