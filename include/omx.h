@@ -301,6 +301,10 @@ int omx_warmnet_loss_grad(omx_warmnet* h, int batch, const float* x, const float
    mean minibatch loss of epoch e. */
 int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
                       int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out);
+/* Convolutions of the network (forward and data gradient) in bf16 on the matrix cores with
+   FP32 accumulation (bf16 != 0) or FP32 on the vector units (0, the default); parameters,
+   weight gradient and Adam stay FP32. */
+int omx_warmnet_set_precision(omx_warmnet* h, int bf16);
 /* omx_warmnet_train continuing the previous call's Adam moments and step count (a training
    run in pieces whose targets change between them, e.g. re-aligned to the current output). */
 int omx_warmnet_train_resume(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,

@@ -343,6 +343,10 @@ int omx_warmnet_train(omx_warmnet* h, int n, const float* x, const float* t_fiel
   return guarded_net(h, [&](omx::WarmNet& w) { w.train(n, x, t_fields, t_k, wk, epochs, batch, lr, seed, loss_hist_out); });
 }
 
+int omx_warmnet_set_precision(omx_warmnet* h, int bf16) {
+  return guarded_net(h, [&](omx::WarmNet& w) { w.set_bf16(bf16 != 0); });
+}
+
 int omx_warmnet_train_resume(omx_warmnet* h, int n, const float* x, const float* t_fields, const float* t_k, float wk,
                              int epochs, int batch, float lr, uint64_t seed, double* loss_hist_out) {
   return guarded_net(h, [&](omx::WarmNet& w) {

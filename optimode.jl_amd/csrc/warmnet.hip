@@ -73,6 +73,95 @@ __global__ __launch_bounds__(TB) void k_conv(const float* in, int cin, const flo
   }
 }
 
+// The same convolution on the matrix cores: an implicit GEMM, out (16 output channels x the
+// 256 pixels of a TS x TS tile) = W (16 x 9 cin) . im2col(in) (9 cin x 256), contracted in
+// steps of 16 with v_mfma_f32_16x16x16bf16_1k: bf16 operands (round to nearest even of the
+// FP32 weights and activations), FP32 accumulation.  The block stages the input halo of all
+// cin channels and its 16 output channels' weights in LDS (FP32); wave w computes tile rows
+// 4w .. 4w + 3, one 16 x 16 MFMA tile (16 channels x the 16 pixels of a row) each.  Operand
+// layout of the instruction: A lane l -> (row l%16, k 4(l/16) + j), B lane l -> (k 4(l/16)
+// + j, column l%16), accumulator register j of lane l -> (row 4(l/16) + j, column l%16).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int CMX_CO = 16;   // output channels per MFMA convolution block
+constexpr int HALO = (TS + 2) * (TS + 2);
+
+__device__ __forceinline__ short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+
+template <bool BWD>
+__global__ __launch_bounds__(TB) void k_conv_mfma(const float* in, int cin, const float* w, const float* bias,
+                                                  float* out, int cout, int res, int relu) {
+  extern __shared__ __align__(16) float smf[];
+  const int kin = cin * 9, kp = ((kin + 15) / 16) * 16;
+  float* tile = smf;                 // [cin][TS + 2][TS + 2]
+  float* ws = smf + long(cin) * HALO;  // [CMX_CO][kp]
+  const int ntx = (res + TS - 1) / TS;
+  const int ty0 = (int(blockIdx.x) / ntx) * TS, tx0 = (int(blockIdx.x) % ntx) * TS;
+  const int o0 = blockIdx.y * CMX_CO;
+  const int b = blockIdx.z;
+  const long plane = long(res) * res;
+  const float* inb = in + long(b) * cin * plane;
+  for (int e = threadIdx.x; e < cin * HALO; e += TB) {
+    const int i = e / HALO, q = e - i * HALO, r = q / (TS + 2), c = q - r * (TS + 2);
+    const int yy = ty0 + r - 1, xx = tx0 + c - 1;
+    tile[e] = (yy >= 0 && yy < res && xx >= 0 && xx < res) ? inb[long(i) * plane + long(yy) * res + xx] : 0.f;
+  }
+  for (int e = threadIdx.x; e < CMX_CO * kp; e += TB) {
+    const int j = e / kp, k = e - j * kp, o = o0 + j;
+    float v = 0.f;
+    if (o < cout && k < kin) {
+      const int i = k / 9, t = k - i * 9;
+      v = BWD ? w[(long(i) * cout + o) * 9 + 8 - t] : w[(long(o) * cin + i) * 9 + t];
+    }
+    ws[e] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  f32x4 acc[4];
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) acc[rr] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < kp; k0 += 16) {
+    bf16x4 a;
+    int off[4];
+    bool ok[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + 4 * lg + j;
+      a[j] = bf16_rne(ws[lr * kp + k]);
+      ok[j] = k < kin;
+      const int i = ok[j] ? k / 9 : 0, t = k - (k / 9) * 9;
+      off[j] = i * HALO + (t / 3) * (TS + 2) + lr + t % 3;  // tap t of pixel (row 0, column lr)
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int ly = wv * 4 + rr;
+      bf16x4 bv;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = ok[j] ? bf16_rne(tile[off[j] + ly * (TS + 2)]) : short(0);
+      acc[rr] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[rr], 0, 0, 0);
+    }
+  }
+  const int x = tx0 + lr;
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    const int y = ty0 + wv * 4 + rr;
+    if (y >= res || x >= res) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int o = o0 + 4 * lg + j;
+      if (o >= cout) continue;
+      float v = acc[rr][j] + (bias ? bias[o] : 0.f);
+      if (relu) v = fmaxf(v, 0.f);
+      out[(long(b) * cout + o) * plane + long(y) * res + x] = v;
+    }
+  }
+}
+
 // Weight gradient: dw[(o cin + i) 9 + t] = sum_b sum_p dout[b][o][p] in[b][i][p + off_t] and
 // (blocks with i = 0) db[o] = sum_b sum_p dout[b][o][p]; block (i, group of WG outputs); each
 // value reduced per wavefront by a fixed butterfly, then over the wavefronts in order.
@@ -486,11 +575,19 @@ void WarmNet::ensure_batch(int B) {
   B_ = B;
 }
 
+static size_t conv_mfma_lds(int cin) {
+  return sizeof(float) * (size_t(cin) * HALO + size_t(CMX_CO) * (((cin * 9 + 15) / 16) * 16));
+}
+
 void WarmNet::conv(const Conv& c, const float* in, float* out, int B, bool relu) {
   const int nt = (c.res + TS - 1) / TS;
   const float* P = P_.as<float>();
-  hipLaunchKernelGGL(k_conv<false>, dim3(nt * nt, (c.co + COB - 1) / COB, B), dim3(TB), 0, s_, in, c.ci, P + c.w,
-                     P + c.b, out, c.co, c.res, relu ? 1 : 0);
+  if (bf16_)
+    hipLaunchKernelGGL(k_conv_mfma<false>, dim3(nt * nt, (c.co + CMX_CO - 1) / CMX_CO, B), dim3(TB),
+                       conv_mfma_lds(c.ci), s_, in, c.ci, P + c.w, P + c.b, out, c.co, c.res, relu ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_conv<false>, dim3(nt * nt, (c.co + COB - 1) / COB, B), dim3(TB), 0, s_, in, c.ci, P + c.w,
+                       P + c.b, out, c.co, c.res, relu ? 1 : 0);
 }
 
 void WarmNet::conv_bwd(const Conv& c, const float* in, const float* dout, float* din, int B) {
@@ -499,8 +596,12 @@ void WarmNet::conv_bwd(const Conv& c, const float* in, const float* dout, float*
                      G + c.w, G + c.b);
   if (din) {
     const int nt = (c.res + TS - 1) / TS;
-    hipLaunchKernelGGL(k_conv<true>, dim3(nt * nt, (c.ci + COB - 1) / COB, B), dim3(TB), 0, s_, dout, c.co,
-                       P_.as<float>() + c.w, nullptr, din, c.ci, c.res, 0);
+    if (bf16_)
+      hipLaunchKernelGGL(k_conv_mfma<true>, dim3(nt * nt, (c.ci + CMX_CO - 1) / CMX_CO, B), dim3(TB),
+                         conv_mfma_lds(c.co), s_, dout, c.co, P_.as<float>() + c.w, nullptr, din, c.ci, c.res, 0);
+    else
+      hipLaunchKernelGGL(k_conv<true>, dim3(nt * nt, (c.ci + COB - 1) / COB, B), dim3(TB), 0, s_, dout, c.co,
+                         P_.as<float>() + c.w, nullptr, din, c.ci, c.res, 0);
   }
 }
 

@@ -4,8 +4,8 @@
 // random vectors and k_guess, src/MaxwellEigenmodes.jl:63); it is trained on the GPU by the
 // kernels in warmnet.hip on modes the solver itself converged.
 //
-// Tensors are FP32, [batch][channel][y][x] with y, x in 0..hw-1.  Network (3x3 convolutions,
-// zero padding, stride 1):
+// Tensors are FP32, [batch][channel][y][x] with y, x in 0..hw-1 (the convolutions may run in
+// bf16 on the matrix cores, set_bf16).  Network (3x3 convolutions, zero padding, stride 1):
 //   c1: 7 -> 16 at hw, ReLU                         (input: 6 eps^-1 components + omega)
 //   pool: 2x2 average                               -> hw/2
 //   c2: 16 -> 32 at hw/2, ReLU
@@ -55,6 +55,11 @@ class WarmNet {
   void train(int n, const float* x, const float* tf, const float* tk, float wk, int epochs, int batch, float lr,
              uint64_t seed, double* hist, bool resume = false);
   int device() const { return device_; }
+  // convolutions (forward and data gradient) in bf16 on the matrix cores with FP32
+  // accumulation (true) or FP32 on the vector units (false); the weight gradient, the
+  // parameters and Adam stay FP32 either way
+  void set_bf16(bool on) { bf16_ = on; }
+  bool bf16() const { return bf16_; }
   // The guess for one 2D cell on the device: features from eps6 (FP64 per pixel xx yy zz xy
   // xz yz, pixel ix + nx iy), one forward, the band outputs to k_host (nev floats), and the
   // start vectors' hw x hw lowest plane waves (inverse 2D DFT of the field maps, unit norm per
@@ -68,6 +73,7 @@ class WarmNet {
     long w, b;  // offsets into the parameter vector
   };
   int device_, hw_, nev_;
+  bool bf16_ = false;
   long nparams_ = 0;
   Conv c_[6];
   long dw_ = 0, db_ = 0;  // dense head

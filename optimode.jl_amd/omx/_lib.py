@@ -111,6 +111,7 @@ SIGNATURES = [
     ("omx_warmnet_loss_grad", ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_float, _dp, _fp]),
     ("omx_warmnet_train", ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_float, ctypes.c_uint64, _dp]),
+    ("omx_warmnet_set_precision", ctypes.c_int, [_vp, ctypes.c_int]),
     ("omx_warmnet_train_resume", ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_float, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _dp]),
     ("omx_warmnet_guess", ctypes.c_int, [_vp, _vp, ctypes.c_double, _fp, _dp]),

@@ -34,7 +34,7 @@ DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 class WarmNet:
     """The network on the GPU (``omx_warmnet_*``): FP32 forward, loss gradient and Adam."""
 
-    def __init__(self, hw=64, nev=8, seed=0, device=0):
+    def __init__(self, hw=64, nev=8, seed=0, device=0, bf16=False):
         self.hw, self.nev = int(hw), int(nev)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().omx_warmnet_create(int(device), self.hw, self.nev, int(seed), ctypes.byref(h)))
@@ -42,6 +42,13 @@ class WarmNet:
         n = ctypes.c_int64()
         _lib.check(_lib.lib().omx_warmnet_num_params(self.h, ctypes.byref(n)))
         self.nparams = int(n.value)
+        self.bf16 = False
+        self.set_bf16(bf16)
+
+    def set_bf16(self, on):
+        """Convolutions in bf16 on the matrix cores (FP32 accumulation) or FP32."""
+        _lib.check(_lib.lib().omx_warmnet_set_precision(self.h, int(bool(on))))
+        self.bf16 = bool(on)
 
     def close(self):
         if self.h:
@@ -259,12 +266,13 @@ class LearnedStart:
         return kg, (lowres_vectors(y[0], self.hw) if lowres else start_vectors(y[0], grid, self.hw))
 
     def save(self, path):
-        np.savez(path, params=self.net.params, hw=self.hw, nev=self.nev)
+        np.savez(path, params=self.net.params, hw=self.hw, nev=self.nev, bf16=int(self.net.bf16))
 
     @classmethod
     def load(cls, path, device=0):
+        """The network of a file, its convolutions in the precision it was trained with."""
         with np.load(path, allow_pickle=False) as z:
-            net = WarmNet(int(z["hw"]), int(z["nev"]), device=device)
+            net = WarmNet(int(z["hw"]), int(z["nev"]), device=device, bf16=bool(int(z["bf16"])) if "bf16" in z else False)
             net.params = z["params"]
         return cls(net)
 
@@ -402,14 +410,15 @@ def train_default(n=1024, epochs=600, out=None, log=print):
     """The shipped network (``data/warmnet_tfln256.npz``): 1024 rib draws (seeds 1000..2023)
     solved at 256x256 with 8 bands, 600 epochs of Adam (lr 2e-3, minibatch 16, wk 1) on the
     GPU with the rotation-invariant field loss (``train_subspace``, targets re-aligned every
-    10 epochs)."""
+    10 epochs) and the convolutions in bf16 on the matrix cores (FP32 accumulation, FP32
+    master weights and weight gradient)."""
     from . import HIPSolver
     grid = Grid(8.0, 4.0, 256, 256)
     nev, hw = 8, 64
     t = time.perf_counter()
     X, TF, TK, _ = generate(range(1000, 1000 + n), grid, nev, HIPSolver(), hw=hw, log=log)
     log(f"generated {n} draws in {time.perf_counter() - t:.1f} s")
-    net = WarmNet(hw, nev, seed=0)
+    net = WarmNet(hw, nev, seed=0, bf16=True)
     t = time.perf_counter()
     hist = train_subspace(net, X, TF, TK, wk=1.0, epochs=epochs, realign=10, batch=16, lr=2e-3, seed=1, log=log)
     log(f"trained {epochs} epochs in {time.perf_counter() - t:.1f} s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
@@ -427,7 +436,7 @@ COUPLER_FILE = "warmnet_coupler512x256.npz"
 def train_coupler(n=320, epochs=500, out=None, log=print):
     """The EME chain's network (``data/warmnet_coupler512x256.npz``): ``n`` coupler draws
     (seeds 5000..) solved at 512 x 256 with 16 modes, then Adam (lr 2e-3, minibatch 16) on
-    the GPU with the rotation-invariant field loss.  It starts the chain's cold cells
+    the GPU with the rotation-invariant field loss and bf16 convolutions.  It starts the chain's cold cells
     (``eme.eme(..., learned=...)``)."""
     from . import HIPSolver
     grid = Grid(*COUPLER_GRID)
@@ -435,7 +444,7 @@ def train_coupler(n=320, epochs=500, out=None, log=print):
     t = time.perf_counter()
     X, TF, TK, _ = generate(range(5000, 5000 + n), grid, nev, HIPSolver(), hw=hw, log=log, family="coupler")
     log(f"generated {n} coupler draws in {time.perf_counter() - t:.1f} s")
-    net = WarmNet(hw, nev, seed=0)
+    net = WarmNet(hw, nev, seed=0, bf16=True)
     t = time.perf_counter()
     hist = train_subspace(net, X, TF, TK, wk=1.0, epochs=epochs, realign=10, batch=16, lr=2e-3, seed=1, log=log)
     log(f"trained {epochs} epochs in {time.perf_counter() - t:.1f} s: loss {hist[0]:.4f} -> {hist[-1]:.4f}")

@@ -40,6 +40,36 @@ def test_forward_and_gradient_match_oracle(hw, nev, B):
     assert np.max(np.abs(g - gr)) < 1e-4 * np.max(np.abs(gr))
 
 
+@pytest.mark.parametrize("hw,nev,B", [(16, 3, 3), (32, 2, 5), (64, 8, 2), (64, 16, 1)])
+def test_bf16_mfma_convolutions_match_oracle_to_bf16(hw, nev, B):
+    """The network with its convolutions (forward and data gradient) in bf16 on the matrix
+    cores, FP32 accumulation: outputs and loss gradient against the FP64 oracle to bf16
+    accuracy (8 significand bits, six layers deep); a wrong operand layout or tap index
+    would give O(1) errors."""
+    print(f"warmnet bf16 parity hw {hw} nev {nev} B {B}", flush=True)
+    p, x, tf, tk = _rand_case(hw, nev, B, 2)
+    net = WS.WarmNet(hw, nev, bf16=True)
+    net.params = p
+    y, k = net.forward(x)
+    yr, kr = W.forward(p.astype(np.float64), x, hw, nev)
+    ey = np.max(np.abs(y - yr)) / np.max(np.abs(yr))
+    ek = np.max(np.abs(k - kr)) / np.max(np.abs(kr))
+    wk = 0.5
+    L, g = net.loss_grad(x, tf, tk, wk)
+    Lr, gr = W.loss_grad(p.astype(np.float64), x, tf, tk, wk, hw, nev)
+    eg = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
+    print(f"  outputs {ey:.2e}, k {ek:.2e}, loss {abs(L - Lr) / abs(Lr):.2e}, gradient {eg:.2e} (cos {cos:.6f})",
+          flush=True)
+    assert ey < 3e-2 and ek < 3e-2
+    assert abs(L - Lr) < 1e-2 * abs(Lr)
+    assert eg < 5e-2 and cos > 0.998
+    # the FP32 path of the same network is unchanged
+    net.set_bf16(False)
+    y32, _ = net.forward(x)
+    assert np.max(np.abs(y32 - yr)) < 1e-5 * np.max(np.abs(yr))
+
+
 def test_training_reduces_loss_and_is_deterministic():
     hw, nev = 16, 2
     rng = np.random.default_rng(4)
