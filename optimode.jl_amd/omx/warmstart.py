@@ -149,10 +149,12 @@ def align_targets(y, tf, nev):
     of the band vectors, so T W are the maps of the rotated vectors, which span the same
     subspace -- all that the shared first eigensolve takes from the start block."""
     T, Y = _complex_maps(tf, nev), _complex_maps(y, nev)
-    M = np.einsum("njcab,nkcab->njk", T.conj(), Y)
+    n, _, _, hw, _ = T.shape
+    Tf, Yf = T.reshape(n, nev, -1), Y.reshape(n, nev, -1)
+    M = np.matmul(Tf.conj(), Yf.transpose(0, 2, 1))    # (n, nev, nev): <T_j, y_k>
     U, _, Vh = np.linalg.svd(M)
     W = U @ Vh
-    return _channel_maps(np.einsum("njcab,njk->nkcab", T, W))
+    return _channel_maps(np.matmul(W.transpose(0, 2, 1), Tf).reshape(n, nev, 2, hw, hw))
 
 
 def train_subspace(net, x, tf, tk, wk=1.0, epochs=600, realign=10, batch=16, lr=2e-3, seed=1, log=None):
