@@ -159,10 +159,11 @@ __device__ __forceinline__ double block_sum(const double* p, int n, double* red)
 
 struct MinresState;
 
-// Stage-1 update fused with its dot: y <- A v - shift v - (beta/oldb) r1 (A v in av) and the
-// partials of Re <v, y>; the coefficients come from the MINRES state (read only)
+// Stage-1 update fused with its dot, with v = u / beta not formed: y <- (A u - shift u) / beta
+// - (beta/oldb) r1 (A u in au) and the partials of Re <v, y> = Re <u, y> / beta; the
+// coefficients come from the MINRES state (read only)
 template <typename T>
-__global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* av, const cplx<T>* v,
+__global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* au, const cplx<T>* u,
                                                   const cplx<T>* r1, cplx<T>* y, long ld, long K, const int* act,
                                                   double* part);
 
@@ -252,7 +253,7 @@ struct MinresState {
 // one block per instance: the block sums the instance's partials (tree), then its first
 // thread runs the recurrence
 template <typename T>
-__global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* av, const cplx<T>* v,
+__global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* au, const cplx<T>* u,
                                                   const cplx<T>* r1, cplx<T>* y, long ld, long K, const int* act,
                                                   double* part) {
   const int i = blockIdx.y;
@@ -260,15 +261,18 @@ __global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const c
   double acc = 0;
   if (act[i]) {
     const MinresState& m = ms[i];
-    const double c1 = -m.shift, c2 = m.itn >= 1 ? -m.beta / m.oldb : 0.0;
+    // beta = 0 ends the instance (its v is zero): the zero scale keeps its updates finite
+    const double ib = m.beta > 0 ? 1.0 / m.beta : 0.0;
+    const double c0 = ib, c1 = -m.shift * ib, c2 = m.itn >= 1 ? -m.beta / m.oldb : 0.0;
     const long o = i * ld;
     for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-      const cplx<T> a = av[o + r], b = v[o + r], c = r1[o + r];
-      const cplx<T> yn{T(double(a.x) + c1 * double(b.x) + c2 * double(c.x)),
-                       T(double(a.y) + c1 * double(b.y) + c2 * double(c.y))};
+      const cplx<T> a = au[o + r], b = u[o + r], c = r1[o + r];
+      const cplx<T> yn{T(c0 * double(a.x) + c1 * double(b.x) + c2 * double(c.x)),
+                       T(c0 * double(a.y) + c1 * double(b.y) + c2 * double(c.y))};
       y[o + r] = yn;
       acc += double(b.x) * double(yn.x) + double(b.y) * double(yn.y);
     }
+    acc *= ib;
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(AB) void k_minres_y2(MinresState* ms, const double*
   __shared__ double red[AB];
   const double alfa = block_sum(vpart + long(i) * nvp, nvp, red);
   // every block read the state it needs before block 0 writes alfa, which no block reads
-  const double f = -alfa / ms[i].beta;
+  const double f = ms[i].beta > 0 ? -alfa / ms[i].beta : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) ms[i].alfa = alfa;
   const long o = i * ld;
   for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
@@ -361,8 +365,9 @@ __global__ __launch_bounds__(AB) void k_minres_scalars(MinresState* ms, const do
       m.phi = m.cs * m.phibar;
       m.phibar = m.sn * m.phibar;
       m.denom = 1.0 / gamma;
-      // w_new = (v - oldeps w1 - delta w2) denom and x += phi w_new ; inputs (v, w1, w2)
-      c[0] = c64{m.denom, 0};
+      // w_new = (v - oldeps w1 - delta w2) denom and x += phi w_new ; inputs (u, w1, w2)
+      // with u = oldb v, the vector the iteration's operator ran on
+      c[0] = c64{m.oldb > 0 ? m.denom / m.oldb : 0.0, 0};
       c[1] = c64{-m.oldeps * m.denom, 0};
       c[2] = c64{-m.delta * m.denom, 0};
       c[3] = c64{m.phi, 0};
@@ -680,37 +685,43 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   // one MINRES iteration; its buffer rotations (r1 <- r2 <- y <- r1 and w <- w1 <- w2 <- w)
   // are 3-cycles, so three iterations leave every pointer where it was
   const dim3 vg(nblk, nb);
+  // One MINRES iteration on u = y (beta v; v itself is never formed).  Its buffer rotations
+  // are a 4-cycle (y <- r1 <- r2 <- v <- y) and a 3-cycle (w <- w1 <- w2 <- w), so twelve
+  // iterations leave every pointer where it was.
   auto iteration = [&]() {
-    // v = y / beta (slot 2, left by the previous scalar step)
-    V.lincomb({y}, v, la, 2);
-    BView<T> Vv{v, ld, ld};
-    launch_operator<T>(OP_M, plan, Vv, AVv, dm, sel, dk, nullptr, 0, s);
-    // y <- A v - shift v - (beta/oldb) r1 with the partials of <v, y> (stage 1 fused)
-    hipLaunchKernelGGL((k_minres_y1<T>), vg, dim3(AB), 0, s, ms, avp, v, r1, y, ld, V.K, la, part);
-    // alfa = <v, y>; y <- y - (alfa/beta) r2 (stage 2 fused)
-    hipLaunchKernelGGL((k_minres_y2<T>), vg, dim3(AB), 0, s, ms, part, nblk, r2, y, ld, V.K, la);
-    std::swap(r1, r2);  // r1 <- r2
-    std::swap(r2, y);   // r2 <- y ; y <- old r1 (free)
-    BView<T> R2b{r2, ld, ld}, Yb{y, ld, ld};
+    BView<T> Uv{y, ld, ld};
+    launch_operator<T>(OP_M, plan, Uv, AVv, dm, sel, dk, nullptr, 0, s);
+    // v-buffer <- (A u - shift u) / beta - (beta/oldb) r1 with the partials of <v, .>
+    hipLaunchKernelGGL((k_minres_y1<T>), vg, dim3(AB), 0, s, ms, avp, y, r1, v, ld, V.K, la, part);
+    // alfa = <v, .>; v-buffer -= (alfa/beta) r2 (stage 2 fused)
+    hipLaunchKernelGGL((k_minres_y2<T>), vg, dim3(AB), 0, s, ms, part, nblk, r2, v, ld, V.K, la);
+    C* u = y;   // read once more by the w update
+    C* yn = r1;  // free: receives P r2 of the new r2
+    r1 = r2;
+    r2 = v;
+    BView<T> R2b{r2, ld, ld}, Yb{yn, ld, ld};
     launch_operator<T>(OP_P, plan, R2b, Yb, dm, sel, dk, nullptr, 0, s);
-    // y <- y - ev <ev, y> (or the block projection) with the partials of <r2, y>
-    V.project_dot_to(y, la, r2, cpart2);
+    // yn <- yn - ev <ev, yn> (or the block projection) with the partials of <r2, yn>
+    V.project_dot_to(yn, la, r2, cpart2);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, cpart2, nblk, coef, 3, nb, la);
     // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
     // w1 <- w2, w2 <- w), written into the buffer of the discarded w1, and x += phi w_new
-    hipLaunchKernelGGL((k_minres_wx<T>), vg, dim3(AB), 0, s, v, w2, w, w1, x, ld, V.K, coef + 4 * nb, la);
+    // (v = u / oldb, folded into the coefficient)
+    hipLaunchKernelGGL((k_minres_wx<T>), vg, dim3(AB), 0, s, u, w2, w, w1, x, ld, V.K, coef + 4 * nb, la);
     {
       C* wn = w1;
       w1 = w2;
       w2 = w;
       w = wn;
     }
+    v = u;
+    y = yn;
   };
-  // Groups of three iterations, replayed from one captured HIP graph (option "graphs"):
-  // the kernels are short and the host's launch rate, not the GPU, set their pace.  The
-  // graph holds kernels only (no memset or copy nodes); the host reads the MINRES states
-  // between groups.
-  constexpr int GROUP = 3;
+  // The host reads the MINRES states every three iterations; with option "graphs" groups
+  // of twelve (the rotations' common period) are replayed from one captured HIP graph
+  // instead: the kernels are short and the host's launch rate, not the GPU, sets their
+  // pace.  The graph holds kernels only (no memset or copy nodes).
+  const int GROUP = S.opt.graphs ? 12 : 3;
   hipGraphExec_t gexec = nullptr;
   for (; it < maxit; it += GROUP) {
     OMX_HIP_CHECK(hipMemcpyAsync(hm.data(), ms, sizeof(MinresState) * nb, hipMemcpyDeviceToHost, s));

@@ -1408,137 +1408,135 @@ int newton_phase2(Solver& S, Batch<T>& A, Batch<T>& B, std::vector<NewtonState>&
 
 using cd = std::complex<double>;
 
-// Real symmetric eigenproblem by cyclic Jacobi (n <= 40): A (row-major n x n) is destroyed;
-// w ascending, V columns the eigenvectors (row-major, V[r * n + c]).
-void jacobi_sym(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& V) {
-  V.assign(size_t(n) * n, 0.0);
-  for (int i = 0; i < n; ++i) V[size_t(i) * n + i] = 1.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0, tot = 0;
-    for (int p = 0; p < n; ++p)
-      for (int q = 0; q < n; ++q) {
-        const double a = A[size_t(p) * n + q] * A[size_t(p) * n + q];
-        tot += a;
-        if (p != q) off += a;
-      }
-    if (off <= 1e-32 * tot) break;
+// Complex products spelled out: std::complex's operator* keeps C's infinity and NaN rules
+// through a library call per product, and the values here are finite.
+inline cd cmulf(cd a, cd b) {
+  return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+inline cd cjmulf(cd a, cd b) {  // conj(a) b
+  return cd(a.real() * b.real() + a.imag() * b.imag(), a.real() * b.imag() - a.imag() * b.real());
+}
+
+// Hermitian eigenproblem by cyclic complex Jacobi (n <= RF_MAXC): A (row-major n x n,
+// Hermitian) is destroyed; w ascending, V's columns the eigenvectors (row-major,
+// V[r * n + c]).  The rotation of pair p < q is U = D R with D = diag(1, conj(e)) on (p, q),
+// e = A[p][q] / |A[p][q]|, and R the real rotation [[c, s], [-s, c]] that zeroes the real
+// 2 x 2 block [[A_pp, |A_pq|], [|A_pq|, A_qq]] (t = tan of the angle, the smaller root of
+// t^2 + 2 theta t - 1 = 0 with theta = (A_qq - A_pp) / (2 |A_pq|)); A <- U^H A U, V <- V U.
+// A pair is left alone once |A_pq| <= 1e-15 sqrt(|A_pp A_qq|) or <= 1e-17 |A|_F (rounding
+// keeps a rotated matrix's off-diagonal entries at about that level), and the sweeps end
+// with the first one that rotates no pair.
+void jacobi_herm(int n, cd* A, double* w, cd* V) {
+  for (int i = 0; i < n * n; ++i) V[i] = 0.0;
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  double fro = 0;
+  for (int i = 0; i < n * n; ++i) fro += std::norm(A[i]);
+  const double floor2 = 1e-34 * fro;  // (1e-17 |A|_F)^2
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    int rotations = 0;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
-        const double apq = A[size_t(p) * n + q];
-        if (std::fabs(apq) < 1e-300) continue;
-        const double app = A[size_t(p) * n + p], aqq = A[size_t(q) * n + q];
-        const double theta = 0.5 * (aqq - app) / apq;
+        const cd apq = A[p * n + q];
+        const double a2 = std::norm(apq);
+        const double app = A[p * n + p].real(), aqq = A[q * n + q].real();
+        if (!(a2 > floor2 && a2 > 1e-30 * std::fabs(app * aqq))) continue;
+        ++rotations;
+        const double a = std::sqrt(a2);
+        const cd e = apq / a, ec = std::conj(e);
+        const double theta = 0.5 * (aqq - app) / a;
         const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
         const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double akp = A[size_t(k) * n + p], akq = A[size_t(k) * n + q];
-          A[size_t(k) * n + p] = c * akp - sn * akq;
-          A[size_t(k) * n + q] = sn * akp + c * akq;
+        const cd sec = sn * ec, cec = c * ec, se = sn * e, ce = c * e;
+        for (int k = 0; k < n; ++k) {  // columns: A <- A U
+          const cd akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - cmulf(sec, akq);
+          A[k * n + q] = sn * akp + cmulf(cec, akq);
         }
-        for (int k = 0; k < n; ++k) {
-          const double apk = A[size_t(p) * n + k], aqk = A[size_t(q) * n + k];
-          A[size_t(p) * n + k] = c * apk - sn * aqk;
-          A[size_t(q) * n + k] = sn * apk + c * aqk;
+        for (int k = 0; k < n; ++k) {  // rows: A <- U^H A
+          const cd apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - cmulf(se, aqk);
+          A[q * n + k] = sn * apk + cmulf(ce, aqk);
         }
+        A[p * n + p] = app - t * a;
+        A[q * n + q] = aqq + t * a;
+        A[p * n + q] = A[q * n + p] = 0.0;
         for (int k = 0; k < n; ++k) {
-          const double vkp = V[size_t(k) * n + p], vkq = V[size_t(k) * n + q];
-          V[size_t(k) * n + p] = c * vkp - sn * vkq;
-          V[size_t(k) * n + q] = sn * vkp + c * vkq;
+          const cd vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - cmulf(sec, vkq);
+          V[k * n + q] = sn * vkp + cmulf(cec, vkq);
         }
       }
+    if (rotations == 0) break;
   }
-  std::vector<int> idx(n);
+  int idx[RF_MAXC];
   for (int i = 0; i < n; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return A[size_t(a) * n + a] < A[size_t(b) * n + b]; });
-  std::vector<double> V2(size_t(n) * n);
-  w.resize(n);
+  std::sort(idx, idx + n, [&](int a, int b) { return A[a * n + a].real() < A[b * n + b].real(); });
+  cd V2[RF_MAXC * RF_MAXC];
   for (int c = 0; c < n; ++c) {
-    w[c] = A[size_t(idx[c]) * n + idx[c]];
-    for (int r = 0; r < n; ++r) V2[size_t(r) * n + c] = V[size_t(r) * n + idx[c]];
+    w[c] = A[idx[c] * n + idx[c]].real();
+    for (int r = 0; r < n; ++r) V2[r * n + c] = V[r * n + idx[c]];
   }
-  V.swap(V2);
+  for (int i = 0; i < n * n; ++i) V[i] = V2[i];
 }
 
 // Generalised Hermitian eigenproblem G c = lam B c for n <= RF_MAXC (G, B column-major
-// with leading dimension RF_MAXC): B = L L^H, H = L^-1 G L^-H, H's eigenvectors from its
-// real 2n x 2n form (each complex pair appears twice; complex Gram-Schmidt keeps one
-// vector per pair), c = L^-H y.  lam ascending, C column-major n x n.  False when B is
-// not numerically positive definite.
+// with leading dimension RF_MAXC, their Hermitian parts used): B = L L^H, H = L^-1 G L^-H,
+// H y = lam y by complex Jacobi, c = L^-H y.  lam ascending, C column-major n x n.  False
+// when B is not numerically positive definite.
 bool dense_geig(int n, const c64* Gp, const c64* Bp, std::vector<double>& lam, std::vector<cd>& C) {
-  auto G = [&](int a, int b) { return 0.5 * (cd(Gp[a + RF_MAXC * b].x, Gp[a + RF_MAXC * b].y) +
-                                             std::conj(cd(Gp[b + RF_MAXC * a].x, Gp[b + RF_MAXC * a].y))); };
-  auto B = [&](int a, int b) { return 0.5 * (cd(Bp[a + RF_MAXC * b].x, Bp[a + RF_MAXC * b].y) +
-                                             std::conj(cd(Bp[b + RF_MAXC * a].x, Bp[b + RF_MAXC * a].y))); };
-  std::vector<cd> L(size_t(n) * n, 0.0);  // column-major lower
-  for (int j = 0; j < n; ++j) {
-    cd d = B(j, j);
-    for (int k = 0; k < j; ++k) d -= L[j + size_t(n) * k] * std::conj(L[j + size_t(n) * k]);
-    if (!(d.real() > 1e-14)) return false;
-    const double ljj = std::sqrt(d.real());
-    L[j + size_t(n) * j] = ljj;
+  if (n < 1 || n > RF_MAXC) return false;
+  auto herm = [](const c64* M, int a, int b) {
+    const c64 u = M[a + RF_MAXC * b], v = M[b + RF_MAXC * a];
+    return cd(0.5 * (u.x + v.x), 0.5 * (u.y - v.y));
+  };
+  cd L[RF_MAXC * RF_MAXC], Wm[RF_MAXC * RF_MAXC], Hm[RF_MAXC * RF_MAXC], V[RF_MAXC * RF_MAXC];
+  double ld[RF_MAXC];  // the real diagonal of L
+  for (int j = 0; j < n; ++j) {  // L column-major lower: B = L L^H
+    double d = herm(Bp, j, j).real();
+    for (int k = 0; k < j; ++k) d -= std::norm(L[j + n * k]);
+    if (!(d > 1e-14)) return false;
+    ld[j] = std::sqrt(d);
+    L[j + n * j] = ld[j];
     for (int i = j + 1; i < n; ++i) {
-      cd v = B(i, j);
-      for (int k = 0; k < j; ++k) v -= L[i + size_t(n) * k] * std::conj(L[j + size_t(n) * k]);
-      L[i + size_t(n) * j] = v / ljj;
+      cd v = herm(Bp, i, j);
+      for (int k = 0; k < j; ++k) v -= cmulf(L[i + n * k], std::conj(L[j + n * k]));
+      L[i + n * j] = v / ld[j];
     }
   }
-  // W = L^-1 G (column by column), H = L^-1 W^H  (H = H^H)
-  std::vector<cd> W(size_t(n) * n), H(size_t(n) * n);
-  auto fsolve = [&](std::vector<cd>& x) {  // x <- L^-1 x
+  auto fsolve = [&](cd* x) {  // x <- L^-1 x
     for (int i = 0; i < n; ++i) {
       cd v = x[i];
-      for (int k = 0; k < i; ++k) v -= L[i + size_t(n) * k] * x[k];
-      x[i] = v / L[i + size_t(n) * i];
+      for (int k = 0; k < i; ++k) v -= cmulf(L[i + n * k], x[k]);
+      x[i] = v / ld[i];
     }
   };
-  std::vector<cd> col(n);
+  // W = L^-1 G (columns), H = L^-1 W^H, held row-major as Jacobi's input (H = H^H)
+  cd col[RF_MAXC];
   for (int j = 0; j < n; ++j) {
-    for (int i = 0; i < n; ++i) col[i] = G(i, j);
+    for (int i = 0; i < n; ++i) col[i] = herm(Gp, i, j);
     fsolve(col);
-    for (int i = 0; i < n; ++i) W[i + size_t(n) * j] = col[i];
+    for (int i = 0; i < n; ++i) Wm[i + n * j] = col[i];
   }
   for (int j = 0; j < n; ++j) {
-    for (int i = 0; i < n; ++i) col[i] = std::conj(W[j + size_t(n) * i]);
+    for (int i = 0; i < n; ++i) col[i] = std::conj(Wm[j + n * i]);
     fsolve(col);
-    for (int i = 0; i < n; ++i) H[i + size_t(n) * j] = col[i];
+    for (int i = 0; i < n; ++i) Hm[i * n + j] = col[i];  // H[i][j]
   }
-  const int n2 = 2 * n;
-  std::vector<double> A(size_t(n2) * n2), w, V;
   for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      const cd h = 0.5 * (H[i + size_t(n) * j] + std::conj(H[j + size_t(n) * i]));
-      A[size_t(i) * n2 + j] = h.real();
-      A[size_t(i + n) * n2 + j + n] = h.real();
-      A[size_t(i) * n2 + j + n] = -h.imag();
-      A[size_t(i + n) * n2 + j] = h.imag();
+    for (int j = i; j < n; ++j) {
+      const cd h = 0.5 * (Hm[i * n + j] + std::conj(Hm[j * n + i]));
+      Hm[i * n + j] = h;
+      Hm[j * n + i] = std::conj(h);
     }
-  jacobi_sym(n2, A, w, V);
-  std::vector<cd> Y;  // accepted complex eigenvectors, n each
-  lam.clear();
-  for (int c = 0; c < n2 && int(lam.size()) < n; ++c) {
-    std::vector<cd> y(n);
-    for (int r = 0; r < n; ++r) y[r] = cd(V[size_t(r) * n2 + c], V[size_t(r + n) * n2 + c]);
-    for (int pass = 0; pass < 2; ++pass)
-      for (size_t q = 0; q < lam.size(); ++q) {
-        cd d = 0;
-        for (int r = 0; r < n; ++r) d += std::conj(Y[q * n + r]) * y[r];
-        for (int r = 0; r < n; ++r) y[r] -= d * Y[q * n + r];
-      }
-    double nr = 0;
-    for (int r = 0; r < n; ++r) nr += std::norm(y[r]);
-    nr = std::sqrt(nr);
-    if (nr < 0.5) continue;  // the other half of a pair already taken
-    for (int r = 0; r < n; ++r) Y.push_back(y[r] / nr);
-    lam.push_back(w[c]);
-  }
-  if (int(lam.size()) < n) return false;
-  // C = L^-H Y: back substitution with L^H (upper)
+  lam.resize(n);
+  jacobi_herm(n, Hm, lam.data(), V);
+  // C = L^-H Y (Y = V's columns): back substitution with L^H (upper)
   C.assign(size_t(n) * n, 0.0);
   for (int j = 0; j < n; ++j) {
     for (int i = n - 1; i >= 0; --i) {
-      cd v = Y[size_t(j) * n + i];
-      for (int k = i + 1; k < n; ++k) v -= std::conj(L[k + size_t(n) * i]) * C[k + size_t(n) * j];
-      C[i + size_t(n) * j] = v / L[i + size_t(n) * i];
+      cd v = V[i * n + j];
+      for (int k = i + 1; k < n; ++k) v -= cjmulf(L[k + n * i], C[k + size_t(n) * j]);
+      C[i + size_t(n) * j] = v / ld[i];
     }
   }
   return true;
@@ -1717,11 +1715,14 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       minres_total += its;
       OMX_HIP_CHECK(hipStreamSynchronize(s));
       t_mr += tms(t_mr0, tnow());
-      // t <- t - X (X^H t), twice, then unit norm (instances with jdh only)
+      // t <- t - X (X^H t), then unit norm (instances with jdh only).  Twice, unless the
+      // correction solve already ran on the complement of the FP32 copy of the whole block:
+      // t is then orthogonal to X up to FP32 rounding, and one FP64 pass removes that.
       std::vector<int> n1j(ninst);
       for (int i = 0; i < ninst; ++i) n1j[i] = jdh[i] ? msize[i] : 0;
       up_n1(n1j);
-      for (int pass = 0; pass < 2; ++pass) {
+      const int orth_passes = (S.opt.jd_fp32 && S.opt.jd_deflate) ? 1 : 2;
+      for (int pass = 0; pass < orth_passes; ++pass) {
         const RCols Xc{b.p(b.X[b.cur]), ld, istride, nullptr}, Tc{nullptr, ld, ld, Tm.as<c64>()};
         launch_bgram(Xc, Tc, 1, n1d.as<int>(), ninst, K, nblk_g, part.as<c64>(), Gd.as<c64>(), s);
         OMX_HIP_CHECK(hipMemcpyAsync(hG.data(), Gd.as<void>(), sizeof(c64) * hG.size(), hipMemcpyDeviceToHost, s));

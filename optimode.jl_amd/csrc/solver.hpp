@@ -185,7 +185,7 @@ struct Options {
   int streams2 = 1;             // FP32 Newton phase of solve_k as two half-batches of bands on two
                                 // streams in lockstep (one half's latency-bound dense steps overlap
                                 // the other half's memory-bound kernels)
-  int graphs = 0;               // MINRES iterations replayed from a captured HIP graph (groups of 3;
+  int graphs = 0;               // MINRES iterations replayed from a captured HIP graph (groups of 12;
                                 // same results, no measured gain: the gaps are GPU-side)
 };
 

@@ -63,6 +63,27 @@ def test_solve_k_matches_golden(name, builder, precision):
         assert abs(np.linalg.norm(v) - 1) < 1e-10
 
 
+def test_minres_graph_replay_matches_golden():
+    """Option ``graphs``: the correction solves' MINRES iterations replayed from a captured
+    HIP graph, twelve per replay (the common period of its buffer rotations), give the same
+    k as the default launches, on the 256² rib where each correction solve runs over twelve
+    iterations."""
+    name, builder = CASES[1]
+    print(f"graph replay {name}", flush=True)
+    f, g, ei = _load(name, builder)
+    w = float(f["omega"])
+    nev = int(f["nev"])
+    base = omx.HIPSolver()
+    km0 = omx.solve_k(w, ei, g, base, nev=nev, k_tol=1e-10, eig_tol=1e-8)[0]
+    solver = omx.HIPSolver()
+    solver.options["graphs"] = 1.0
+    km, evs, info = omx.solve_k(w, ei, g, solver, nev=nev, k_tol=1e-10, eig_tol=1e-8, return_info=True)
+    rel = np.abs(f["dw2dk"] * (km - f["kmags"])) / w ** 2
+    print(name, "graphs max rel err", rel.max(), info, flush=True)
+    assert np.all(rel <= 1e-8), rel
+    assert np.allclose(km, km0, rtol=1e-9, atol=0)
+
+
 @pytest.mark.parametrize("name,builder", CASES, ids=IDS)
 def test_mode_fields_and_group_index_match_golden(name, builder):
     print(f"fields {name}", flush=True)
