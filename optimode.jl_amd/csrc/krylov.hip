@@ -406,6 +406,74 @@ __global__ __launch_bounds__(AB) void k_proj_coef(const c64* part, int nblk, c64
   }
 }
 
+// Column loops of the block projection with the instance's column count NQ fixed at
+// compile time (qdot_cols / qproject_cols pick it from the runtime count): every load of a
+// row is issued before the first use, so a row costs one memory round trip, not one per column.
+template <typename T, int NQ>
+__device__ __forceinline__ void qdot_rows(const cplx<T>* q, long qld, const cplx<T>* x, long r0, long stride, long K,
+                                          double* sr, double* si) {
+  for (long r = r0; r < K; r += stride) {
+    const cplx<T> xv = x[r];
+    cplx<T> e[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) e[c] = q[c * qld + r];
+    const double xx = xv.x, xy = xv.y;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+      sr[c] += double(e[c].x) * xx + double(e[c].y) * xy;
+      si[c] += double(e[c].x) * xy - double(e[c].y) * xx;
+    }
+  }
+}
+
+template <typename T, int QM, int NQ = 1>
+__device__ __forceinline__ void qdot_cols(int n, const cplx<T>* q, long qld, const cplx<T>* x, long r0, long stride,
+                                          long K, double* sr, double* si) {
+  if constexpr (NQ <= QM) {
+    if (n == NQ)
+      qdot_rows<T, NQ>(q, qld, x, r0, stride, K, sr, si);
+    else
+      qdot_cols<T, QM, NQ + 1>(n, q, qld, x, r0, stride, K, sr, si);
+  }
+}
+
+// x <- x + sum_c cf_c Q_c over the rows, and sum of Re <d, x_new> when DOT
+template <typename T, int NQ, bool DOT>
+__device__ __forceinline__ double qproject_rows(const c64* cf, const cplx<T>* q, long qld, cplx<T>* x,
+                                                const cplx<T>* dv, long r0, long stride, long K) {
+  double acc = 0;
+  for (long r = r0; r < K; r += stride) {
+    const cplx<T> xv = x[r];
+    cplx<T> dr{T(0), T(0)};
+    if (DOT) dr = dv[r];
+    cplx<T> e[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) e[c] = q[c * qld + r];
+    double nx = xv.x, ny = xv.y;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+      nx += cf[c].x * double(e[c].x) - cf[c].y * double(e[c].y);
+      ny += cf[c].x * double(e[c].y) + cf[c].y * double(e[c].x);
+    }
+    const cplx<T> xn{T(nx), T(ny)};
+    x[r] = xn;
+    if (DOT) acc += double(dr.x) * double(xn.x) + double(dr.y) * double(xn.y);
+  }
+  return acc;
+}
+
+template <typename T, int QM, int NQ = 1>
+__device__ __forceinline__ double qproject_cols(int n, const c64* cf, const cplx<T>* q, long qld, cplx<T>* x,
+                                                const cplx<T>* dv, long r0, long stride, long K) {
+  if constexpr (NQ <= QM) {
+    if (n == NQ)
+      return dv ? qproject_rows<T, NQ, true>(cf, q, qld, x, dv, r0, stride, K)
+                : qproject_rows<T, NQ, false>(cf, q, qld, x, dv, r0, stride, K);
+    return qproject_cols<T, QM, NQ + 1>(n, cf, q, qld, x, dv, r0, stride, K);
+  }
+  return 0.0;
+}
+
 // Block projection: partials of <Q_c, x> (c < nq[i], complex, FP64) over this block's
 // rows into part[(i gridDim.x + blockIdx.x) QM + c] (zero for c >= nq[i])
 template <typename T, int QM>
@@ -417,20 +485,9 @@ __global__ __launch_bounds__(AB) void k_qdot(const cplx<T>* Q, long qld, long qi
   double sr[QM], si[QM];
 #pragma unroll
   for (int c = 0; c < QM; ++c) sr[c] = si[c] = 0.0;
-  if (n > 0) {
-    const cplx<T>* x = X + i * ld;
-    const cplx<T>* q = Q + i * qis;
-    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-      const double xx = x[r].x, xy = x[r].y;
-#pragma unroll
-      for (int c = 0; c < QM; ++c)
-        if (c < n) {
-          const cplx<T> e = q[c * qld + r];
-          sr[c] += double(e.x) * xx + double(e.y) * xy;
-          si[c] += double(e.x) * xy - double(e.y) * xx;
-        }
-    }
-  }
+  if (n > 0)
+    qdot_cols<T, QM>(n, Q + i * qis, qld, X + i * ld, long(blockIdx.x) * AB + threadIdx.x, long(gridDim.x) * AB, K,
+                     sr, si);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < QM; ++c) {
@@ -495,22 +552,9 @@ __global__ __launch_bounds__(AB) void k_qproject(const c64* qpart, int qnblk, co
     c64 cf[QM];
 #pragma unroll
     for (int c = 0; c < QM; ++c) cf[c] = c < n ? cfs[c] : c64{0, 0};
-    cplx<T>* x = X + i * ld;
-    const cplx<T>* q = Q + i * qis;
-    const cplx<T>* dv = d ? d + i * ld : nullptr;
-    for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
-      double nx = x[r].x, ny = x[r].y;
-#pragma unroll
-      for (int c = 0; c < QM; ++c)
-        if (c < n) {
-          const cplx<T> e = q[c * qld + r];
-          nx += cf[c].x * double(e.x) - cf[c].y * double(e.y);
-          ny += cf[c].x * double(e.y) + cf[c].y * double(e.x);
-        }
-      const cplx<T> xn{T(nx), T(ny)};
-      x[r] = xn;
-      if (dv) acc += double(dv[r].x) * double(xn.x) + double(dv[r].y) * double(xn.y);
-    }
+    if (n > 0)
+      acc = qproject_cols<T, QM>(n, cf, Q + i * qis, qld, X + i * ld, d ? d + i * ld : nullptr,
+                                 long(blockIdx.x) * AB + threadIdx.x, long(gridDim.x) * AB, K);
   }
   if (!part) return;
   red[threadIdx.x] = acc;

@@ -28,15 +28,40 @@ __global__ __launch_bounds__(RB) void k_bgram(RCols U, RCols V, int vmode, const
   const int t = threadIdx.x;
   double ar[2] = {0, 0}, ai[2] = {0, 0}, br[2] = {0, 0}, bi[2] = {0, 0};
   if (m > 0) {
-    for (long r0 = long(blockIdx.x) * RQ; r0 < K; r0 += long(gridDim.x) * RQ) {
-      for (int e = t; e < RF_MAXC * RQ; e += RB) {
-        const int c = e / RQ, q = e % RQ;
-        const long r = r0 + q;
-        const c64 z{0, 0};
-        us[c][q] = (c < nu && r < K) ? col_ptr(U, i, c, m)[r] : z;
-        vs[c][q] = (c < nv && r < K) ? col_ptr(V, i, c, nv1)[r] : z;
+    // a chunk's entries are fetched into registers one chunk ahead (every load of a chunk in
+    // flight together, and during the previous chunk's products), then staged in LDS.  Entries
+    // outside the columns or rows load a valid address (row 0 of the first column) and are
+    // replaced by zero.
+    constexpr int NE = (RF_MAXC * RQ + RB - 1) / RB;
+    const c64* u0 = U.blk + i * U.istride;
+    const c64* v0 = nv1 > 0 ? V.blk + i * V.istride : V.extra + i * V.ld;
+    c64 ur[NE], vr[NE];
+    bool ok_u[NE], ok_v[NE];
+    const long step = long(gridDim.x) * RQ;
+    long r0 = long(blockIdx.x) * RQ;
+    auto fetch = [&](long rb) {
+#pragma unroll
+      for (int k = 0; k < NE; ++k) {
+        const int e = t + k * RB, c = e / RQ, q = e % RQ;
+        const long r = rb + q;
+        ok_u[k] = e < RF_MAXC * RQ && c < nu && r < K;
+        ok_v[k] = e < RF_MAXC * RQ && c < nv && r < K;
+        ur[k] = *(ok_u[k] ? col_ptr(U, i, c, m) + r : u0);
+        vr[k] = *(ok_v[k] ? col_ptr(V, i, c, nv1) + r : v0);
+      }
+    };
+    fetch(r0);
+    for (; r0 < K; r0 += step) {
+#pragma unroll
+      for (int k = 0; k < NE; ++k) {
+        const int e = t + k * RB;
+        if (e < RF_MAXC * RQ) {
+          us[e / RQ][e % RQ] = ok_u[k] ? ur[k] : c64{0, 0};
+          vs[e / RQ][e % RQ] = ok_v[k] ? vr[k] : c64{0, 0};
+        }
       }
       __syncthreads();
+      if (r0 + step < K) fetch(r0 + step);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int p = t + s * RB;
