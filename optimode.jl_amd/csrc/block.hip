@@ -244,6 +244,8 @@ __global__ __launch_bounds__(BLK) void k_reduce(const c64* partial, int nchunk, 
   double ax = 0.0, ay = 0.0;
   if (e < esz) {
     const c64* p = partial + long(inst) * nchunk * esz + e;
+    // unrolled: the loads of eight chunks are in flight together (the sums keep their order)
+#pragma unroll 8
     for (int c = grp; c < nchunk; c += 16) {
       const c64 v = p[long(c) * esz];
       ax += v.x;

@@ -507,6 +507,9 @@ __global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st,
   LState& s = st[i];
   const int nact = shp(shape, F_NACT, ninst, i);
   if (nact == 0) return;
+  // one read of the state (its fields are dependent loads otherwise, each after the
+  // stores that might alias it); thread 0 stores the fields it changes
+  const LState s0 = s;
   // residual norms from the per-block partials: lane l sums blocks l, l + 64, ... of every
   // column (independent loads), then one wavefront reduction per column (fixed order)
   {
@@ -514,10 +517,15 @@ __global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st,
 #pragma unroll
     for (int q = 0; q < 16; ++q) part[q] = 0.0;
     for (int b = threadIdx.x; b < dl.nblk; b += DBLK) {
+      // every column's load issued before the sums (columns past the row's qmax re-read its
+      // last entry and are not summed)
       const double* np = npart + (long(i) * dl.nblk + b) * dl.qmax;
+      double v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = np[q < dl.qmax ? q : dl.qmax - 1];
 #pragma unroll
       for (int q = 0; q < 16; ++q)
-        if (q < nact) part[q] += np[q];
+        if (q < nact) part[q] += v[q];
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -529,38 +537,39 @@ __global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int M = s.M;
-    const int prev = s.prev_nlocked;
-    if (s.niter <= dl.maxiter)
-      for (int q = 0; q < nact; ++q) rhist[(long(i) * (dl.maxiter + 1) + s.niter) * 16 + prev + q] = nrm[q];
+    const int M = s0.M;
+    const int prev = s0.prev_nlocked;
+    if (s0.niter <= dl.maxiter)
+      for (int q = 0; q < nact; ++q) rhist[(long(i) * (dl.maxiter + 1) + s0.niter) * 16 + prev + q] = nrm[q];
     int nl = prev;
-    if (s.niter >= 1) {
+    if (s0.niter >= 1) {
       for (int q = 0; q < nact; ++q) {
-        if (nrm[q] < s.tol)
+        if (nrm[q] < s0.tol)
           ++nl;
         else
           break;
       }
     }
+    const int conv = nl >= s0.ncheck ? 1 : 0;
+    const int finished = conv || s0.niter >= dl.maxiter;
     s.nlocked = nl;
     s.newly_locked = nl - prev;
-    s.converged = nl >= s.ncheck ? 1 : 0;
-    const int finished = s.converged || s.niter >= dl.maxiter;
-    if (finished && !s.converged) s.status = OMX_ERR_NOT_CONVERGED;
+    s.converged = conv;
+    if (finished && !conv) s.status = OMX_ERR_NOT_CONVERGED;
     s.finished = finished;
     shape[F_NLOCK * ninst + i] = nl;
     shape[F_NEWLY * ninst + i] = nl - prev;
     shape[F_NNEW * ninst + i] = finished ? 0 : M - nl;
     shape[F_NCOPY * ninst + i] = finished ? nact : nl - prev;
-    shape[F_NPN * ninst + i] = (finished || s.niter == 0) ? 0 : M - nl;
+    shape[F_NPN * ninst + i] = (finished || s0.niter == 0) ? 0 : M - nl;
     s_newly = nl - prev;
     s_finished = finished;
   }
   __syncthreads();
   const int newly = s_newly;
-  if (s_finished || s.niter == 0) return;
+  if (s_finished || s0.niter == 0) return;
   // cP = (cX - e)[:, xn], xn = newly .. nact-1 (dftk.jl:426-438)
-  const int np = s.np;
+  const int np = s0.np;
   const int py = 2 * nact + np;
   const int nn = nact - newly;
   const long esz = long(pm) * pm;
@@ -578,8 +587,8 @@ __global__ __launch_bounds__(DBLK) void k_dense_lock(DenseLaunch dl, LState* st,
     cP[a * SLD + jp] = v;
   }
   __syncthreads();
-  const int rc = d_ortho_against(cP, py, nn, cX, nact, s.eps, s.ortho_tol, B, O, W, Wr,
-                                 uint64_t(i) * 1000003ull + uint64_t(s.niter));
+  const int rc = d_ortho_against(cP, py, nn, cX, nact, s0.eps, s0.ortho_tol, B, O, W, Wr,
+                                 uint64_t(i) * 1000003ull + uint64_t(s0.niter));
   if (rc != 0 && threadIdx.x == 0) s.status = rc;
   double* Pi = Cp + 2 * long(i) * esz;
   for (int e = threadIdx.x; e < py * nn; e += blockDim.x) {
