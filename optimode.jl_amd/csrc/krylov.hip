@@ -162,10 +162,15 @@ struct MinresState;
 // Stage-1 update fused with its dot, with v = u / beta not formed: y <- (A u - shift u) / beta
 // - (beta/oldb) r1 (A u in au) and the partials of Re <v, y> = Re <u, y> / beta; the
 // coefficients come from the MINRES state (read only)
-template <typename T>
+//
+// With WX the previous iteration's direction and solution updates come first, row by row:
+// w_new = c0 v' + c1 w1' + c2 w2' (into wout) and x += c3 w_new with v' the previous
+// iteration's u, which the output buffer y still holds until this row is written.
+template <typename T, bool WX>
 __global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* au, const cplx<T>* u,
                                                   const cplx<T>* r1, cplx<T>* y, long ld, long K, const int* act,
-                                                  double* part);
+                                                  double* part, const cplx<T>* w1p, const cplx<T>* w2p,
+                                                  cplx<T>* wout, cplx<T>* x, const c64* wcoef);
 
 // Stage-2 update fused with its scalar step: alfa = sum of the <v, y> partials (every block
 // the same, block 0 stores it), y <- y - (alfa/beta) r2
@@ -252,10 +257,11 @@ struct MinresState {
 // step of the iteration they follow.  Coefficients for the next lincomb go to coef.
 // one block per instance: the block sums the instance's partials (tree), then its first
 // thread runs the recurrence
-template <typename T>
+template <typename T, bool WX>
 __global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const cplx<T>* au, const cplx<T>* u,
                                                   const cplx<T>* r1, cplx<T>* y, long ld, long K, const int* act,
-                                                  double* part) {
+                                                  double* part, const cplx<T>* w1p, const cplx<T>* w2p,
+                                                  cplx<T>* wout, cplx<T>* x, const c64* wcoef) {
   const int i = blockIdx.y;
   __shared__ double red[AB];
   double acc = 0;
@@ -264,8 +270,22 @@ __global__ __launch_bounds__(AB) void k_minres_y1(const MinresState* ms, const c
     // beta = 0 ends the instance (its v is zero): the zero scale keeps its updates finite
     const double ib = m.beta > 0 ? 1.0 / m.beta : 0.0;
     const double c0 = ib, c1 = -m.shift * ib, c2 = m.itn >= 1 ? -m.beta / m.oldb : 0.0;
+    double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+    if (WX) {
+      d0 = wcoef[4 * i].x;
+      d1 = wcoef[4 * i + 1].x;
+      d2 = wcoef[4 * i + 2].x;
+      d3 = wcoef[4 * i + 3].x;
+    }
     const long o = i * ld;
     for (long r = long(blockIdx.x) * AB + threadIdx.x; r < K; r += long(gridDim.x) * AB) {
+      if (WX) {
+        const cplx<T> pv = y[o + r], pb = w1p[o + r], pc = w2p[o + r], xr = x[o + r];
+        const double wx = d0 * double(pv.x) + d1 * double(pb.x) + d2 * double(pc.x);
+        const double wy = d0 * double(pv.y) + d1 * double(pb.y) + d2 * double(pc.y);
+        wout[o + r] = cplx<T>{T(wx), T(wy)};
+        x[o + r] = cplx<T>{T(double(xr.x) + d3 * wx), T(double(xr.y) + d3 * wy)};
+      }
       const cplx<T> a = au[o + r], b = u[o + r], c = r1[o + r];
       const cplx<T> yn{T(c0 * double(a.x) + c1 * double(b.x) + c2 * double(c.x)),
                        T(c0 * double(a.y) + c1 * double(b.y) + c2 * double(c.y))};
@@ -732,14 +752,25 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   // One MINRES iteration on u = y (beta v; v itself is never formed).  Its buffer rotations
   // are a 4-cycle (y <- r1 <- r2 <- v <- y) and a 3-cycle (w <- w1 <- w2 <- w), so twelve
   // iterations leave every pointer where it was.
-  auto iteration = [&]() {
+  // The direction and solution updates of an iteration (w_new = (v - oldeps w1' - delta w2')
+  // denom with w1' = w2 and w2' = w, MINRES's w1 <- w2, w2 <- w, written into the buffer of
+  // the discarded w1, and x += phi w_new, with v = u / oldb folded into the coefficient)
+  // wait, pending, for the next iteration's first update, which runs them row by row before
+  // its own (k_minres_y1<T, true>), or for flush at the end of a group.
+  C *pw1 = nullptr, *pw2 = nullptr, *pwo = nullptr;
+  auto iteration = [&](bool fuse) {
     BView<T> Uv{y, ld, ld};
     launch_operator<T>(OP_M, plan, Uv, AVv, dm, sel, dk, nullptr, 0, s);
     // v-buffer <- (A u - shift u) / beta - (beta/oldb) r1 with the partials of <v, .>
-    hipLaunchKernelGGL((k_minres_y1<T>), vg, dim3(AB), 0, s, ms, avp, y, r1, v, ld, V.K, la, part);
+    if (fuse)
+      hipLaunchKernelGGL((k_minres_y1<T, true>), vg, dim3(AB), 0, s, ms, avp, y, r1, v, ld, V.K, la, part, pw1, pw2,
+                         pwo, x, coef + 4 * nb);
+    else
+      hipLaunchKernelGGL((k_minres_y1<T, false>), vg, dim3(AB), 0, s, ms, avp, y, r1, v, ld, V.K, la, part, nullptr,
+                         nullptr, nullptr, nullptr, nullptr);
     // alfa = <v, .>; v-buffer -= (alfa/beta) r2 (stage 2 fused)
     hipLaunchKernelGGL((k_minres_y2<T>), vg, dim3(AB), 0, s, ms, part, nblk, r2, v, ld, V.K, la);
-    C* u = y;   // read once more by the w update
+    C* u = y;   // the next iteration's v-buffer, read first by this iteration's w/x update
     C* yn = r1;  // free: receives P r2 of the new r2
     r1 = r2;
     r2 = v;
@@ -748,10 +779,10 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
     // yn <- yn - ev <ev, yn> (or the block projection) with the partials of <r2, yn>
     V.project_dot_to(yn, la, r2, cpart2);
     hipLaunchKernelGGL(k_minres_scalars, dim3(nb), dim3(AB), 0, s, ms, cpart2, nblk, coef, 3, nb, la);
-    // w_new = (v - oldeps w1' - delta w2') denom with w1' = w2 and w2' = w (MINRES's
-    // w1 <- w2, w2 <- w), written into the buffer of the discarded w1, and x += phi w_new
-    // (v = u / oldb, folded into the coefficient)
-    hipLaunchKernelGGL((k_minres_wx<T>), vg, dim3(AB), 0, s, u, w2, w, w1, x, ld, V.K, coef + 4 * nb, la);
+    // this iteration's w/x update, pending (its v is u, the next iteration's v-buffer)
+    pw1 = w2;
+    pw2 = w;
+    pwo = w1;
     {
       C* wn = w1;
       w1 = w2;
@@ -760,6 +791,9 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
     }
     v = u;
     y = yn;
+  };
+  auto flush = [&]() {
+    hipLaunchKernelGGL((k_minres_wx<T>), vg, dim3(AB), 0, s, v, pw1, pw2, pwo, x, ld, V.K, coef + 4 * nb, la);
   };
   // The host reads the MINRES states every three iterations; with option "graphs" groups
   // of twelve (the rotations' common period) are replayed from one captured HIP graph
@@ -782,7 +816,8 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
       if (!gexec) {
         hipGraph_t g = nullptr;
         OMX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int u = 0; u < GROUP; ++u) iteration();
+        for (int u = 0; u < GROUP; ++u) iteration(u > 0);
+        flush();
         OMX_HIP_CHECK(hipStreamEndCapture(s, &g));
         const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -790,7 +825,8 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
       }
       OMX_HIP_CHECK(hipGraphLaunch(gexec, s));
     } else {
-      for (int u = 0; u < nit; ++u) iteration();
+      for (int u = 0; u < nit; ++u) iteration(u > 0);
+      flush();
     }
   }
   if (gexec) (void)hipGraphExecDestroy(gexec);
