@@ -773,6 +773,8 @@ void Solver::set_option(const std::string& key, double v) {
     opt.jd_fp32 = v != 0 ? 1 : 0;
   else if (key == "jd_deflate")
     opt.jd_deflate = v != 0 ? 1 : 0;
+  else if (key == "jd_first_tol")
+    opt.jd_first_tol = std::max(0.0, v);
   else if (key == "streams2")
     opt.streams2 = v != 0 ? 1 : 0;
   else if (key == "graphs")
@@ -1670,13 +1672,24 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
     rr(std::vector<int>(ninst, 0));
     ++outer;
     std::vector<int> conv(ninst, 0);
+    // the first step corrects only until rho is good enough for a Newton update of k (the
+    // residual at the new k is then of the size of the k change and is corrected next step)
+    std::vector<double> step_tol(tols);
+    if (step == 0 && S.opt.jd_first_tol > 0)
+      for (int i = 0; i < ninst; ++i) step_tol[i] = std::max(tols[i], S.opt.jd_first_tol * hns[i].omega2);
     for (int jd = 0;; ++jd) {
       band_resid();
       bool need = false;
       for (int i = 0; i < ninst; ++i) {
-        conv[i] = n1h[i] > 0 && rn[i] <= tols[i];
+        conv[i] = n1h[i] > 0 && rn[i] <= step_tol[i];
         jdh[i] = (n1h[i] > 0 && !conv[i]) ? 1 : 0;
         need = need || jdh[i];
+      }
+      if (S.trace) {
+        std::fprintf(stderr, "[omx] fp64 refine step %d, correction %d, residuals:", step, jd);
+        for (int i = 0; i < ninst; ++i)
+          if (n1h[i]) std::fprintf(stderr, " %.1e", rn[i]);
+        std::fprintf(stderr, "\n");
       }
       if (!need || jd >= jd_max) break;
       // t: (M - rho) t = r on the complement of x (the sign of t is immaterial)

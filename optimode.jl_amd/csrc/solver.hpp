@@ -171,6 +171,10 @@ struct Options {
   int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
   int jd_deflate = 1;           // FP32 correction solves on the complement of the band's whole block
                                 // (not only of its vector): the neighbouring bands leave the spectrum
+  double jd_first_tol = 1e-5;   // residual (relative to omega^2) at which the refinement's first
+                                // Newton step stops correcting and moves k (0: at the eigensolve
+                                // tolerance, as the later steps): Newton needs rho, whose error is
+                                // the residual squared over the band gap
   double jac_stop32 = 1e-4;     // Jacobi stop (max |sin| of a sweep) of the FP32 batches' Rayleigh-Ritz
                                 // (off-diagonal left ~1e-8 |A|, below FP32 data; FP64 batches: 1e-9)
   int gram_blocks = 0;          // Gram products: about this many row-chunk blocks over the active
