@@ -94,8 +94,9 @@ int omx_destroy(omx_solver* h);
 /* update_ε⁻¹ (maxwell.jl:1114-1136) */
 int omx_set_epsinv(omx_solver* h, const double* epsinv);
 const char* omx_last_error(const omx_solver* h);
-/* tuning knobs: "fp32_tol" (solve_k's FP32 locking threshold / omega^2, 1e-3), "fp32_tol_omega2"
-   (solve_omega2's, 1e-5), "fp32_switch", "jd_rtol", "maxiter_fp64", "guard", "seed"; start modes
+/* tuning knobs: "fp32_tol" (solve_k's FP32 locking threshold / omega^2, 2e-3), "fp32_tol_omega2"
+   (solve_omega2's, 1e-5), "fp32_switch", "jd_rtol", "jd_first_tol" (residual / omega^2 at which
+   the FP64 refinement's first Newton step moves k, 3e-5), "maxiter_fp64", "guard", "seed"; start modes
    "warm_start" and "band_guess" (omx_solve_k); "guess_lowres" = hw > 0 (even, <= Nx, Ny):
    omx_solve_k's Hguess vectors hold only their hw x hw lowest plane waves, 2 hw^2 complex
    each at c + 2(gx + hw gy) in FFT order, and are scattered into the start block on the GPU */

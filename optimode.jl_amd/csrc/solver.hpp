@@ -137,7 +137,7 @@ class Profiler {
 };
 
 struct Options {
-  double fp32_tol_rel = 1e-3;   // solve_k: FP32 LOBPCG locking threshold of the Newton phase, relative
+  double fp32_tol_rel = 2e-3;   // solve_k: FP32 LOBPCG locking threshold of the Newton phase, relative
                                 // to omega^2 (the FP64 refinement takes the vectors from there)
   double fp32_tol_omega2 = 1e-5;  // solve_omega2: FP32 locking threshold before its FP64 LOBPCG
   double fp32_switch = 1e-6;    // |Δω²|/ω² below which Newton moves to the FP64 phase
@@ -171,7 +171,7 @@ struct Options {
   int jd_fp32 = 1;              // correction solves with FP32 vectors and operator (FP64 recurrences)
   int jd_deflate = 1;           // FP32 correction solves on the complement of the band's whole block
                                 // (not only of its vector): the neighbouring bands leave the spectrum
-  double jd_first_tol = 1e-5;   // residual (relative to omega^2) at which the refinement's first
+  double jd_first_tol = 3e-5;   // residual (relative to omega^2) at which the refinement's first
                                 // Newton step stops correcting and moves k (0: at the eigensolve
                                 // tolerance, as the later steps): Newton needs rho, whose error is
                                 // the residual squared over the band gap

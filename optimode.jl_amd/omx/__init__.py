@@ -127,7 +127,7 @@ class HIPSolver:
     ``precision`` is the policy of the iterative phase: ``"mixed"`` (default: FP32
     operator applications, then an FP64 refinement phase) or ``"fp64"``.  Tolerance
     knobs of the FP32 phase are relative to ω²: ``fp32_tol`` (``solve_k``'s LOBPCG locking
-    threshold, 1e-3: the FP64 refinement finishes the vectors) and ``fp32_switch``
+    threshold, 2e-3: the FP64 refinement finishes the vectors) and ``fp32_switch``
     (|Δω²|/ω² at which Newton moves to FP64).  ``reg_ops=False`` keeps
     the FP32 operator on the LDS Stockham FFT even where the register-FFT kernels apply
     (power-of-two grids 64 to 512).  ``plain=True`` runs the reference's algorithm on the same
