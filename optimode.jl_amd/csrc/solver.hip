@@ -2019,7 +2019,7 @@ void Solver::solve_k(double omega, int nev, double k_tol, double eig_tol, int ma
       // the shared first eigensolve in instance 0 of the first half, as seed_bands
       std::vector<int> act(na, 0), n0(na, M), m0(na, M), bA(bands.begin(), bands.begin() + na);
       act[0] = 1;
-      std::vector<double> t0(na, std::max(tol32, seed_tol_rel() * w2)), kh(ninst, k0);
+      std::vector<double> t0(na, std::max(tol32, seed_tol_rel(true) * w2)), kh(ninst, k0);
       OMX_HIP_CHECK(hipMemcpyAsync(b32.kd.as<void>(), kh.data(), sizeof(double) * na, hipMemcpyHostToDevice, stream));
       stats.it32 += b32.run(act, bA, n0, m0, t0, maxiter, stats);
       const auto lam = b32.rayleigh();

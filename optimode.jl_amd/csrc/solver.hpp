@@ -142,7 +142,9 @@ struct Options {
   double fp32_tol_omega2 = 1e-5;  // solve_omega2: FP32 locking threshold before its FP64 LOBPCG
   double fp32_switch = 1e-6;    // |Δω²|/ω² below which Newton moves to the FP64 phase
   double seed_tol = 0;          // locking threshold / omega^2 of the shared first eigensolve
-                                // (0: 5e-2 on 2D cells, 3e-3 on 3D cells, as measured)
+                                // (0: 1e-1 on 2D cells in the two-stream FP32 phase of a single
+                                // solve, 5e-2 in batches and the other paths, 3e-3 on 3D cells,
+                                // as measured)
   int maxiter_fp64 = 200;       // LOBPCG iteration cap in the FP64 phase
   int guard = 2;                // block columns of band i: min(nev, i + 1 + guard); < 0: nev
   uint64_t seed = 0x5eed;
@@ -269,7 +271,11 @@ class Solver {
   // timed trials of the operator path, FFT radix variant, Stockham tile shape and the Gram /
   // block-update partitions for nev-mode solves; the fastest become the handle's options
   void autotune(int nev, int* choice_out, double* us_out);
-  double seed_tol_rel() const { return opt.seed_tol > 0 ? opt.seed_tol : (nz > 1 ? 3e-3 : 5e-2); }
+  // halves: the single solve's FP32 phase on two streams (its Newton steps make up for a
+  // looser seed; the batched launch groups wait for their slowest problem and do not)
+  double seed_tol_rel(bool halves = false) const {
+    return opt.seed_tol > 0 ? opt.seed_tol : (nz > 1 ? 3e-3 : (halves ? 1e-1 : 5e-2));
+  }
   // per mode j: dw2dk[j] = 2 HMkH(ev_j) and dw2dL[j] = 2 <ev_j| dM/dDz |ev_j> (3D cells)
   void period_derivatives(int nev, const double* kmags, const double* evecs, double* dw2dk, double* dw2dL);
   void require_2d(const char* what) const;
