@@ -1658,6 +1658,9 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
   };
   const int jd_max = 8;
   int outer = 0;
+  // each band's residual at its last Newton update (the first step's is only the looser
+  // jd_first_tol one); their maximum is the solve's final residual
+  std::vector<double> last_rn(ninst, 0.0);
   for (int step = 0; step < max_steps; ++step) {
     bool any = false;
     for (int i = 0; i < ninst; ++i) {
@@ -1802,7 +1805,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
         ns.f = lam_band[i] - ns.omega2;
         ns.slope = slope;
         ns.nsolves += 1;
-        max_resid = std::max(max_resid, rn[i]);
+        last_rn[i] = rn[i];
         if (resid_inst) (*resid_inst)[i] = rn[i];
         if (rn[i] <= tols[i] && std::fabs(ns.f) <= ns.k_tol) {
           ns.done = 1;
@@ -1824,6 +1827,7 @@ int refine_phase(Solver& S, Batch<double>& b, std::vector<NewtonState>& hns, con
       std::fprintf(stderr, "\n");
     }
   }
+  for (int i = 0; i < ninst; ++i) max_resid = std::max(max_resid, last_rn[i]);
   if (S.trace)
     std::fprintf(stderr, "[omx] fp64 refine: %.2f ms host wall (Rayleigh-Ritz %.2f, residuals %.2f, correction solves "
                  "%.2f), %d Rayleigh-Ritz steps, %d correction solves, %d MINRES iterations\n",

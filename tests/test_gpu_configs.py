@@ -85,6 +85,31 @@ def test_minres_graph_replay_matches_golden():
 
 
 @pytest.mark.parametrize("name,builder", CASES, ids=IDS)
+def test_hand_over_tolerances_keep_k(name, builder):
+    """The FP32 hand-over (fp32_tol), the shared first eigensolve's lock (seed_tol) and the
+    refinement's first Newton step (jd_first_tol: 0 corrects to the eigensolve tolerance
+    before k moves) change the path, not the answer: both settings give the golden k."""
+    print(f"hand-over tolerances {name}", flush=True)
+    f, g, ei = _load(name, builder)
+    w = float(f["omega"])
+    nev = int(f["nev"])
+    res = {}
+    for label, opts in (("default", {}), ("tight", {"fp32_tol": 1e-3, "seed_tol": 5e-2, "jd_first_tol": 0.0})):
+        solver = omx.HIPSolver()
+        solver.options.update(opts)
+        km, _, info = omx.solve_k(w, ei, g, solver, nev=nev, k_tol=1e-10, eig_tol=1e-8, return_info=True,
+                                  fetch_evecs=False)
+        rel = np.abs(f["dw2dk"] * (km - f["kmags"])) / w ** 2
+        print(name, label, "max rel err", rel.max(), info, flush=True)
+        assert info["converged"] == 1
+        # the reported residual is every band's final one (omx_info.max_residual), within eig_tol
+        assert 0 < info["max_residual"] <= 1e-8
+        assert np.all(rel <= 1e-8), rel
+        res[label] = km
+    assert np.allclose(res["default"], res["tight"], rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("name,builder", CASES, ids=IDS)
 def test_mode_fields_and_group_index_match_golden(name, builder):
     print(f"fields {name}", flush=True)
     f, g, ei, solver, km, evs, _ = _mixed_solve(name, builder)
