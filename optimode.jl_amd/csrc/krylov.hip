@@ -746,8 +746,6 @@ int minres_complement_t(Solver& S, VecOpsT<T>& V, const std::vector<double>& shi
   std::vector<int> live(want);
   int it = 0;
   const int* la = livep;
-  // one MINRES iteration; its buffer rotations (r1 <- r2 <- y <- r1 and w <- w1 <- w2 <- w)
-  // are 3-cycles, so three iterations leave every pointer where it was
   const dim3 vg(nblk, nb);
   // One MINRES iteration on u = y (beta v; v itself is never formed).  Its buffer rotations
   // are a 4-cycle (y <- r1 <- r2 <- v <- y) and a 3-cycle (w <- w1 <- w2 <- w), so twelve
